@@ -1,0 +1,342 @@
+"""MI355X (gfx950) s-step Chebyshev-basis CG: Python binding of libcacg.so.
+
+The product is the C-ABI library cacg_amd/libcacg.so (include/cacg.h,
+include/cacg_kernels.h): HIP kernels for gfx950 behind a C11 host driver.
+This module only loads it with ctypes and moves numpy arrays across; there is
+no Python or CPU compute path.  If the library is missing, every call raises.
+
+    import cacg_amd, numpy as np
+    from cacg_amd import problems
+    A = problems.laplacian_2d(128)
+    B = problems.rhs_uniform(A.n, 4, seed=0)
+    out = cacg_amd.bcbcg(A, B, np.zeros_like(B), s=4, tol=1e-8)
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcacg.so")
+
+STATUS = {0: "ok", 1: "maxiter", 2: "breakdown", 3: "einval", 4: "enomem", 5: "edevice"}
+STOP_REF_COL0_ABS, STOP_REL_PER_COLUMN = 0, 1
+ORDER_REFERENCE, ORDER_FAST = 0, 1
+MODE = {"reference": 0, "fast": 1}
+EPI = {"none": 0, "resid": 1, "cheb1": 2, "cheb": 3}
+PHASES = ["setup", "basis", "gram", "solve", "qupdate", "aq", "xrupdate", "other"]
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_lp = ctypes.POINTER(ctypes.c_long)
+_vp = ctypes.c_void_p
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [("stop_mode", ctypes.c_int), ("eps", ctypes.c_double), ("tol", ctypes.c_double),
+                ("maxiter", ctypes.c_long), ("order", ctypes.c_int), ("precision", ctypes.c_int),
+                ("print_period", ctypes.c_int)]
+
+
+class Report(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("iters", ctypes.c_long),
+                ("bounds", ctypes.c_double * 2), ("res_hist", _dp), ("res_hist_cap", ctypes.c_long),
+                ("seconds", ctypes.c_double), ("phase_ms", ctypes.c_double * 8),
+                ("phase_calls", ctypes.c_long * 8)]
+
+
+class DCsr(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("rowptr", _vp), ("col", _vp), ("val", _vp)]
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libcacg.so (built by cacg_amd.build); raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"libcacg.so not built: {LIB_PATH} (run python -m cacg_amd.build)")
+    L = ctypes.CDLL(LIB_PATH)
+    c_int, c_long, c_double = ctypes.c_int, ctypes.c_long, ctypes.c_double
+    sig = {
+        "cacg_last_status": ([], c_int),
+        "cacg_last_iterations": ([], c_long),
+        "cacg_status_string": ([c_int], ctypes.c_char_p),
+        "cacg_default_opts": ([ctypes.POINTER(Opts)], None),
+        "cacg_bcbcg": ([c_long, _lp, _lp, _dp, c_long, c_long, _dp, _dp, ctypes.POINTER(Opts),
+                        ctypes.POINTER(Report)], c_int),
+        "cacg_solver_create": ([ctypes.POINTER(_vp), c_long, _lp, _lp, _dp, c_long, c_long,
+                                ctypes.POINTER(Opts)], c_int),
+        "cacg_solver_solve_device": ([_vp, _vp, _vp, ctypes.POINTER(Report)], c_int),
+        "cacg_solver_destroy": ([_vp], c_int),
+        "cacg_solver_footprint": ([_vp], c_long),
+        "cacg_device_count": ([], c_int),
+        "cacg_dev_alloc": ([ctypes.POINTER(_vp), c_long], c_int),
+        "cacg_dev_free": ([_vp], c_int),
+        "cacg_h2d": ([_vp, _vp, c_long], c_int),
+        "cacg_d2h": ([_vp, _vp, c_long], c_int),
+        "cacg_dev_sync": ([], c_int),
+        "cacg_k_gerschgorin": ([ctypes.POINTER(DCsr), _vp, _vp, _vp], c_int),
+        "cacg_k_spmm": ([ctypes.POINTER(DCsr), c_int, _vp, c_long, _vp, c_long, c_int, c_double,
+                         c_double, _vp, c_long, _vp, c_long, _vp, c_long, _vp], c_int),
+        "cacg_k_gram_ref": ([c_int, _vp, c_long, c_int, _vp, c_long, c_int, _vp, c_long, c_int,
+                             _vp, _vp, _vp], c_int),
+        "cacg_gram_workspace": ([c_int, c_int], c_long),
+        "cacg_k_gram_fast": ([c_int, _vp, c_long, c_int, _vp, c_long, c_int, _vp, c_long, c_int,
+                              _vp, _vp, _vp, _vp], c_int),
+        "cacg_k_lu_solve": ([c_int, _vp, c_int, _vp, _vp, _vp, _vp, c_int, _vp], c_int),
+        "cacg_k_qupdate": ([c_int, c_int, _vp, _vp, _vp, c_int, _vp], c_int),
+        "cacg_xrupdate_workspace": ([c_int], c_long),
+        "cacg_k_xrupdate": ([c_int, c_int, c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, c_int, _vp],
+                            c_int),
+        "cacg_k_colnorm2_ref": ([c_int, c_int, _vp, c_long, _vp, _vp], c_int),
+        "cacg_k_fill_i32": ([_vp, c_long, c_int, _vp], c_int),
+        "cacg_k_fill_f64": ([_vp, c_long, c_double, _vp], c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed with code {rc}")
+
+
+class DeviceBuffer:
+    """A block of device memory (HBM) allocated through the library."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = _vp()
+        _check(lib().cacg_dev_alloc(ctypes.byref(p), max(self.nbytes, 8)), "cacg_dev_alloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_array(cls, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        b.upload(a)
+        return b
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        _check(lib().cacg_h2d(self.ptr, a.ctypes.data, a.nbytes), "cacg_h2d")
+
+    def download(self, shape, dtype=np.float64) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        _check(lib().cacg_d2h(out.ctypes.data, self.ptr, out.nbytes), "cacg_d2h")
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().cacg_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class DeviceCSR:
+    """The int32-index device copy of a problems.CSR (as the solver stores it)."""
+
+    def __init__(self, A):
+        self.n = A.n
+        self.rowptr = DeviceBuffer.from_array(np.asarray(A.rowptr, dtype=np.int32))
+        self.col = DeviceBuffer.from_array(np.asarray(A.col, dtype=np.int32))
+        self.val = DeviceBuffer.from_array(np.asarray(A.val, dtype=np.float64))
+        self.s = DCsr(A.n, self.rowptr.ptr, self.col.ptr, self.val.ptr)
+
+
+# ---- kernel-level wrappers (host arrays in, host arrays out; for tests) -------------
+
+
+def k_gerschgorin(A) -> np.ndarray:
+    dA = DeviceCSR(A)
+    work, out = DeviceBuffer(8 * 2 * 1024), DeviceBuffer(16)
+    _check(lib().cacg_k_gerschgorin(ctypes.byref(dA.s), work.ptr, out.ptr, None), "gerschgorin")
+    return out.download(2)
+
+
+def k_spmm(A, X, epi="none", a=0.0, b=0.0, p1=None, p2=None, want_copy0=False):
+    """Y = epilogue(A X) for X n x k; returns Y (and copy0 for 'cheb1')."""
+    dA = DeviceCSR(A)
+    X = np.ascontiguousarray(X, dtype=np.float64).reshape(A.n, -1)
+    k = X.shape[1]
+    dX = DeviceBuffer.from_array(X)
+    dY = DeviceBuffer(X.nbytes)
+    d1 = DeviceBuffer.from_array(np.ascontiguousarray(p1, dtype=np.float64)) if p1 is not None else None
+    d2 = DeviceBuffer.from_array(np.ascontiguousarray(p2, dtype=np.float64)) if p2 is not None else None
+    dC = DeviceBuffer(X.nbytes) if epi == "cheb1" else None
+    rc = lib().cacg_k_spmm(ctypes.byref(dA.s), k, dX.ptr, k, dY.ptr, k, EPI[epi], a, b,
+                           d1.ptr if d1 else None, k, d2.ptr if d2 else None, k,
+                           dC.ptr if dC else None, k, None)
+    _check(rc, "spmm")
+    Y = dY.download((A.n, k))
+    if want_copy0:
+        return Y, dC.download((A.n, k))
+    return Y
+
+
+def k_gram(L, R1, R2=None, mode="reference"):
+    L = np.ascontiguousarray(L, dtype=np.float64)
+    R1 = np.ascontiguousarray(R1, dtype=np.float64)
+    n, kl = L.shape
+    kr1 = R1.shape[1]
+    kr2 = 0 if R2 is None else R2.shape[1]
+    dL, d1 = DeviceBuffer.from_array(L), DeviceBuffer.from_array(R1)
+    d2 = DeviceBuffer.from_array(np.ascontiguousarray(R2, dtype=np.float64)) if R2 is not None else None
+    g1, g2 = DeviceBuffer(8 * kl * kr1), DeviceBuffer(8 * kl * max(kr2, 1))
+    if mode == "reference":
+        rc = lib().cacg_k_gram_ref(n, dL.ptr, kl, kl, d1.ptr, kr1, kr1, d2.ptr if d2 else None,
+                                   max(kr2, 1), kr2, g1.ptr, g2.ptr, None)
+    else:
+        ws = DeviceBuffer(8 * lib().cacg_gram_workspace(kl, kr1 + kr2))
+        rc = lib().cacg_k_gram_fast(n, dL.ptr, kl, kl, d1.ptr, kr1, kr1, d2.ptr if d2 else None,
+                                    max(kr2, 1), kr2, g1.ptr, g2.ptr, ws.ptr, None)
+    _check(rc, "gram")
+    G1 = g1.download((kl, kr1))
+    if R2 is None:
+        return G1
+    return G1, g2.download((kl, kr2))
+
+
+def k_lu_solve(lhs, rhs, mode="reference"):
+    lhs = np.ascontiguousarray(lhs, dtype=np.float64)
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    N, k = rhs.shape
+    dl, dr, dx = DeviceBuffer.from_array(lhs), DeviceBuffer.from_array(rhs), DeviceBuffer(rhs.nbytes)
+    work = DeviceBuffer(8 * N * N)
+    flag = DeviceBuffer.from_array(np.zeros(1, dtype=np.int32))
+    _check(lib().cacg_k_lu_solve(N, dl.ptr, k, dr.ptr, dx.ptr, work.ptr, flag.ptr, MODE[mode], None),
+           "lu_solve")
+    return dx.download((N, k)), int(flag.download(1, np.int32)[0])
+
+
+def k_qupdate(S, Q, beta, mode="reference"):
+    S = np.ascontiguousarray(S, dtype=np.float64)
+    n, sm = S.shape
+    dS, dQ, db = DeviceBuffer.from_array(S), DeviceBuffer.from_array(Q), DeviceBuffer.from_array(beta)
+    _check(lib().cacg_k_qupdate(n, sm, dS.ptr, dQ.ptr, db.ptr, MODE[mode], None), "qupdate")
+    return dQ.download((n, sm))
+
+
+def k_xrupdate(Q, AQ, alpha, X, R, mode="reference"):
+    Q = np.ascontiguousarray(Q, dtype=np.float64)
+    n, sm = Q.shape
+    m = alpha.shape[1]
+    dQ, dAQ, da = DeviceBuffer.from_array(Q), DeviceBuffer.from_array(AQ), DeviceBuffer.from_array(alpha)
+    dX, dR = DeviceBuffer.from_array(X), DeviceBuffer.from_array(R)
+    dn = DeviceBuffer(8 * m)
+    ws = DeviceBuffer(8 * lib().cacg_xrupdate_workspace(m))
+    _check(lib().cacg_k_xrupdate(n, sm, m, dQ.ptr, dAQ.ptr, da.ptr, dX.ptr, dR.ptr, dn.ptr, ws.ptr,
+                                 MODE[mode], None), "xrupdate")
+    return dX.download((n, m)), dR.download((n, m)), dn.download(m)
+
+
+# ---- solver ------------------------------------------------------------------------------
+
+
+def make_opts(stop_mode=STOP_REL_PER_COLUMN, eps=0.0, tol=1e-8, maxiter=100, order="fast",
+              print_period=0) -> Opts:
+    o = Opts()
+    lib().cacg_default_opts(ctypes.byref(o))
+    o.stop_mode = stop_mode
+    o.eps = eps
+    o.tol = tol
+    o.maxiter = maxiter
+    o.order = MODE[order] if isinstance(order, str) else int(order)
+    o.print_period = print_period
+    return o
+
+
+def _report_dict(rep: Report, hist: np.ndarray, m: int) -> dict:
+    k = int(rep.iters)
+    return {
+        "status": STATUS.get(rep.status, str(rep.status)),
+        "iters": k,
+        "bounds": np.array(rep.bounds[:]),
+        "res_hist": hist[:min(k, hist.shape[0])].copy(),
+        "seconds": rep.seconds,
+        "phase_ms": dict(zip(PHASES, rep.phase_ms[:])),
+        "phase_calls": dict(zip(PHASES, rep.phase_calls[:])),
+    }
+
+
+def bcbcg(A, B, X0, s, *, stop_mode=STOP_REL_PER_COLUMN, eps=0.0, tol=1e-8, maxiter=100,
+          order="fast", print_period=0) -> dict:
+    """Host-buffer solve through cacg_bcbcg().  Returns dict with X and the report."""
+    B = np.ascontiguousarray(B, dtype=np.float64).reshape(A.n, -1)
+    m = B.shape[1]
+    X = np.ascontiguousarray(X0, dtype=np.float64).reshape(A.n, m).copy()
+    rp = np.ascontiguousarray(A.rowptr, dtype=np.int64)
+    ci = np.ascontiguousarray(A.col, dtype=np.int64)
+    va = np.ascontiguousarray(A.val, dtype=np.float64)
+    o = make_opts(stop_mode, eps, tol, maxiter, order, print_period)
+    hist = np.zeros((maxiter, m))
+    rep = Report()
+    rep.res_hist = hist.ctypes.data_as(_dp)
+    rep.res_hist_cap = maxiter
+    rc = lib().cacg_bcbcg(A.n, rp.ctypes.data_as(_lp), ci.ctypes.data_as(_lp), va.ctypes.data_as(_dp),
+                          m, s, B.ctypes.data_as(_dp), X.ctypes.data_as(_dp), ctypes.byref(o),
+                          ctypes.byref(rep))
+    if rc not in (0, 1, 2):
+        raise RuntimeError(f"cacg_bcbcg failed: {STATUS.get(rc, rc)}")
+    out = _report_dict(rep, hist, m)
+    out["X"] = X
+    return out
+
+
+class Solver:
+    """Device-resident solver: matrix uploaded once, work blocks allocated once."""
+
+    def __init__(self, A, m, s, **opt_kw):
+        self.n, self.m, self.s = A.n, m, s
+        self.maxiter = opt_kw.get("maxiter", 100)
+        self._keep = (np.ascontiguousarray(A.rowptr, dtype=np.int64),
+                      np.ascontiguousarray(A.col, dtype=np.int64),
+                      np.ascontiguousarray(A.val, dtype=np.float64))
+        o = make_opts(**opt_kw)
+        h = _vp()
+        rc = lib().cacg_solver_create(ctypes.byref(h), A.n, self._keep[0].ctypes.data_as(_lp),
+                                      self._keep[1].ctypes.data_as(_lp),
+                                      self._keep[2].ctypes.data_as(_dp), m, s, ctypes.byref(o))
+        _check(rc, "cacg_solver_create")
+        self.h = h.value
+        self.hist = np.zeros((self.maxiter, m))
+
+    def footprint(self) -> int:
+        return lib().cacg_solver_footprint(self.h)
+
+    def solve(self, dB: DeviceBuffer, dX: DeviceBuffer) -> dict:
+        rep = Report()
+        rep.res_hist = self.hist.ctypes.data_as(_dp)
+        rep.res_hist_cap = self.maxiter
+        rc = lib().cacg_solver_solve_device(self.h, dB.ptr, dX.ptr, ctypes.byref(rep))
+        if rc not in (0, 1, 2):
+            raise RuntimeError(f"solve failed: {STATUS.get(rc, rc)}")
+        return _report_dict(rep, self.hist, self.m)
+
+    def close(self):
+        if self.h:
+            lib().cacg_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

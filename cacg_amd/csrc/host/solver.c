@@ -1,0 +1,596 @@
+/*
+ * Host driver of the s-step Chebyshev-basis (block) CG solver on one gfx950
+ * GPU.  C11 over the HIP runtime's C API and the kernel launchers of
+ * include/cacg_kernels.h.
+ *
+ * One outer iteration (the reference's src/bcbcgKernel.c:172-526):
+ *   S = [S_0 | ... | S_{s-1}], Chebyshev basis of R     s - 1 SpMMs, fused recurrence
+ *   k == 0: Q = S (buffer swap)                           src/bcbcgKernel.c:199-203
+ *   k >= 1: QtAS = AQ^T S; beta = QtAQ^{-1} QtAS;
+ *           Q = S - Q beta                                :214, :229, :249-262
+ *   AQ = A Q                                              :284
+ *   [QtAQ | QtR] = Q^T [AQ | R]                           :307, :330 (one pass over Q)
+ *   alpha = QtAQ^{-1} QtR                                 :344
+ *   X += Q alpha; R -= AQ alpha; ||R(:,c)||^2             :376-447
+ * The m column norms come back to the host once per iteration (one small
+ * device-to-host copy, the only synchronisation in the loop) for the stop test.
+ *
+ * Device layout (HBM): A as CSR with int32 row pointers / column ids and
+ * FP64 values; R n x m; S, Q, AQ n x sm row-major (sm = s*m), column block j of
+ * S at offset j*m of every row, exactly the reference's basis layout
+ * (src/MatrixOperations.c:70-92 output).  B and X are the caller's n x m
+ * row-major blocks.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <hip/hip_runtime_api.h>
+
+#include "cacg_kernels.h"
+#include "cacg.h"
+
+#define NPHASE 8
+#define MAXEV 16
+
+struct cacg_solver {
+    long n, nnz, m, s, sm;
+    cacg_opts opts;
+    hipStream_t st;
+    cacg_dcsr A;
+    int *d_rowptr, *d_col;
+    double *d_val;
+    double *d_R, *d_S, *d_Q, *d_AQ;
+    double *d_QtAQ, *d_QtAS, *d_QtR, *d_alpha, *d_beta;
+    double *d_norm2, *d_bounds, *d_gwork, *d_gram_work, *d_xr_work, *d_lu_work;
+    int *d_flag;
+    double *h_pin;   /* pinned: [0, m) column norms^2, [m, 2m) ||B(:,c)||^2, [2m, 2m+2) bounds, then the flag */
+    hipEvent_t ev[MAXEV];
+    int ev_phase[MAXEV];
+    int nev;
+    long footprint;
+};
+
+static _Thread_local int g_last_status = CACG_OK;
+static _Thread_local long g_last_iters = 0;
+
+int cacg_last_status(void) { return g_last_status; }
+long cacg_last_iterations(void) { return g_last_iters; }
+
+const char *cacg_status_string(int st)
+{
+    switch (st) {
+    case CACG_OK: return "converged";
+    case CACG_MAXITER: return "iteration cap reached";
+    case CACG_BREAKDOWN: return "breakdown (zero/non-finite pivot or non-finite residual)";
+    case CACG_EINVAL: return "invalid argument";
+    case CACG_ENOMEM: return "device allocation failed";
+    case CACG_EDEVICE: return "HIP runtime error or no gfx950 device";
+    default: return "unknown status";
+    }
+}
+
+void cacg_default_opts(cacg_opts *o)
+{
+    memset(o, 0, sizeof(*o));
+    o->stop_mode = CACG_STOP_REL_PER_COLUMN;
+    o->eps = 0.0;
+    o->tol = 1e-8;
+    o->maxiter = 100;
+    o->order = CACG_ORDER_FAST;
+    o->precision = CACG_PREC_FP64;
+    o->print_period = 0;
+}
+
+#define HIPCHK(call)                                                                             \
+    do {                                                                                         \
+        hipError_t e_ = (call);                                                                  \
+        if (e_ != hipSuccess) {                                                                  \
+            fprintf(stderr, "cacg: %s failed: %s (%s:%d)\n", #call, hipGetErrorString(e_),       \
+                    __FILE__, __LINE__);                                                         \
+            return CACG_EDEVICE;                                                                 \
+        }                                                                                        \
+    } while (0)
+
+#define KCHK(call)                                                                               \
+    do {                                                                                         \
+        int e_ = (call);                                                                         \
+        if (e_ != 0) {                                                                           \
+            fprintf(stderr, "cacg: kernel launch %s failed: %s (%s:%d)\n", #call,                \
+                    hipGetErrorString((hipError_t)e_), __FILE__, __LINE__);                      \
+            return CACG_EDEVICE;                                                                 \
+        }                                                                                        \
+    } while (0)
+
+/* ---- device helpers ---------------------------------------------------------- */
+
+static int check_device(void)
+{
+    static int checked = 0, ok = 0;
+    if (checked)
+        return ok ? CACG_OK : CACG_EDEVICE;
+    int dev = 0, cnt = 0;
+    if (hipGetDeviceCount(&cnt) != hipSuccess || cnt <= 0) {
+        fprintf(stderr, "cacg: no HIP device visible\n");
+        checked = 1;
+        return CACG_EDEVICE;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+        fprintf(stderr, "cacg: cannot query the HIP device\n");
+        checked = 1;
+        return CACG_EDEVICE;
+    }
+    ok = strncmp(prop.gcnArchName, "gfx950", 6) == 0;
+    if (!ok)
+        fprintf(stderr, "cacg: device %d is %s; this library is built for gfx950 only\n", dev,
+                prop.gcnArchName);
+    checked = 1;
+    return ok ? CACG_OK : CACG_EDEVICE;
+}
+
+int cacg_device_count(void)
+{
+    int cnt = 0;
+    if (hipGetDeviceCount(&cnt) != hipSuccess)
+        return 0;
+    return cnt;
+}
+
+int cacg_dev_alloc(void **d_ptr, long bytes)
+{
+    if (!d_ptr || bytes < 0)
+        return CACG_EINVAL;
+    *d_ptr = NULL;
+    if (bytes == 0)
+        return CACG_OK;
+    return hipMalloc(d_ptr, (size_t)bytes) == hipSuccess ? CACG_OK : CACG_ENOMEM;
+}
+
+int cacg_dev_free(void *d_ptr)
+{
+    if (!d_ptr)
+        return CACG_OK;
+    return hipFree(d_ptr) == hipSuccess ? CACG_OK : CACG_EDEVICE;
+}
+
+int cacg_h2d(void *d_dst, const void *src, long bytes)
+{
+    if (bytes <= 0)
+        return CACG_OK;
+    return hipMemcpy(d_dst, src, (size_t)bytes, hipMemcpyHostToDevice) == hipSuccess ? CACG_OK
+                                                                                     : CACG_EDEVICE;
+}
+
+int cacg_d2h(void *dst, const void *d_src, long bytes)
+{
+    if (bytes <= 0)
+        return CACG_OK;
+    return hipMemcpy(dst, d_src, (size_t)bytes, hipMemcpyDeviceToHost) == hipSuccess ? CACG_OK
+                                                                                     : CACG_EDEVICE;
+}
+
+int cacg_dev_sync(void) { return hipDeviceSynchronize() == hipSuccess ? CACG_OK : CACG_EDEVICE; }
+
+/* ---- solver lifetime ----------------------------------------------------------- */
+
+static int dalloc(cacg_solver *sv, void **p, long bytes)
+{
+    if (bytes <= 0)
+        bytes = 8;
+    if (hipMalloc(p, (size_t)bytes) != hipSuccess) {
+        *p = NULL;
+        return CACG_ENOMEM;
+    }
+    sv->footprint += bytes;
+    return CACG_OK;
+}
+
+int cacg_solver_destroy(cacg_solver *sv)
+{
+    if (!sv)
+        return CACG_OK;
+    void *ptrs[] = {sv->d_rowptr, sv->d_col,   sv->d_val,   sv->d_R,       sv->d_S,
+                    sv->d_Q,      sv->d_AQ,    sv->d_QtAQ,  sv->d_QtAS,    sv->d_QtR,
+                    sv->d_alpha,  sv->d_beta,  sv->d_norm2, sv->d_bounds,  sv->d_gwork,
+                    sv->d_gram_work, sv->d_xr_work, sv->d_lu_work, sv->d_flag};
+    for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++)
+        if (ptrs[i])
+            hipFree(ptrs[i]);
+    for (int i = 0; i < MAXEV; i++)
+        if (sv->ev[i])
+            hipEventDestroy(sv->ev[i]);
+    if (sv->st)
+        hipStreamDestroy(sv->st);
+    if (sv->h_pin)
+        hipHostFree(sv->h_pin);
+    free(sv);
+    return CACG_OK;
+}
+
+long cacg_solver_footprint(const cacg_solver *sv) { return sv ? sv->footprint : 0; }
+
+static int validate_csr(long n, const long *rp, const long *ci)
+{
+    if (n <= 0 || n >= (1L << 31) || !rp || rp[0] != 0)
+        return CACG_EINVAL;
+    const long nnz = rp[n];
+    if (nnz < 0 || nnz >= (1L << 31) || (nnz > 0 && !ci))
+        return CACG_EINVAL;
+    for (long i = 0; i < n; i++)
+        if (rp[i + 1] < rp[i])
+            return CACG_EINVAL;
+    for (long p = 0; p < nnz; p++)
+        if (ci[p] < 0 || ci[p] >= n)
+            return CACG_EINVAL;
+    return CACG_OK;
+}
+
+int cacg_solver_create(cacg_solver **out, long n, const long *row_start, const long *col_idx,
+                       const double *val, long m, long s, const cacg_opts *opts)
+{
+    if (!out)
+        return CACG_EINVAL;
+    *out = NULL;
+    if (m <= 0 || s <= 0 || (long)s * m > 4096 || !val)
+        return CACG_EINVAL;
+    if (validate_csr(n, row_start, col_idx) != CACG_OK)
+        return CACG_EINVAL;
+    int rc = check_device();
+    if (rc != CACG_OK)
+        return rc;
+    cacg_solver *sv = (cacg_solver *)calloc(1, sizeof(*sv));
+    if (!sv)
+        return CACG_ENOMEM;
+    sv->n = n;
+    sv->nnz = row_start[n];
+    sv->m = m;
+    sv->s = s;
+    sv->sm = s * m;
+    if (opts)
+        sv->opts = *opts;
+    else
+        cacg_default_opts(&sv->opts);
+    if (sv->opts.maxiter <= 0) {
+        free(sv);
+        return CACG_EINVAL;
+    }
+    if (hipHostMalloc((void **)&sv->h_pin, sizeof(double) * (size_t)(2 * m + 4), 0) != hipSuccess) {
+        sv->h_pin = NULL;
+        cacg_solver_destroy(sv);
+        return CACG_ENOMEM;
+    }
+    if (hipStreamCreate(&sv->st) != hipSuccess) {
+        cacg_solver_destroy(sv);
+        return CACG_EDEVICE;
+    }
+    for (int i = 0; i < MAXEV; i++)
+        if (hipEventCreate(&sv->ev[i]) != hipSuccess) {
+            cacg_solver_destroy(sv);
+            return CACG_EDEVICE;
+        }
+    const long nm = n * m, nsm = n * sv->sm, sm = sv->sm;
+    long gram_ws = cacg_gram_workspace((int)sm, (int)(sm + m));
+    rc = CACG_OK;
+    rc |= dalloc(sv, (void **)&sv->d_rowptr, (n + 1) * (long)sizeof(int));
+    rc |= dalloc(sv, (void **)&sv->d_col, sv->nnz * (long)sizeof(int));
+    rc |= dalloc(sv, (void **)&sv->d_val, sv->nnz * (long)sizeof(double));
+    rc |= dalloc(sv, (void **)&sv->d_R, nm * (long)sizeof(double));
+    rc |= dalloc(sv, (void **)&sv->d_S, nsm * (long)sizeof(double));
+    rc |= dalloc(sv, (void **)&sv->d_Q, nsm * (long)sizeof(double));
+    rc |= dalloc(sv, (void **)&sv->d_AQ, nsm * (long)sizeof(double));
+    rc |= dalloc(sv, (void **)&sv->d_QtAQ, sm * sm * (long)sizeof(double));
+    rc |= dalloc(sv, (void **)&sv->d_QtAS, sm * sm * (long)sizeof(double));
+    rc |= dalloc(sv, (void **)&sv->d_QtR, sm * m * (long)sizeof(double));
+    rc |= dalloc(sv, (void **)&sv->d_alpha, sm * m * (long)sizeof(double));
+    rc |= dalloc(sv, (void **)&sv->d_beta, sm * sm * (long)sizeof(double));
+    rc |= dalloc(sv, (void **)&sv->d_norm2, m * (long)sizeof(double));
+    rc |= dalloc(sv, (void **)&sv->d_bounds, 2 * (long)sizeof(double));
+    rc |= dalloc(sv, (void **)&sv->d_gwork, 2L * CACG_GERSH_BLOCKS * (long)sizeof(double));
+    rc |= dalloc(sv, (void **)&sv->d_gram_work, gram_ws * (long)sizeof(double));
+    rc |= dalloc(sv, (void **)&sv->d_xr_work, cacg_xrupdate_workspace((int)m) * (long)sizeof(double));
+    rc |= dalloc(sv, (void **)&sv->d_lu_work, sm * sm * (long)sizeof(double));
+    rc |= dalloc(sv, (void **)&sv->d_flag, (long)sizeof(int));
+    if (rc != CACG_OK) {
+        cacg_solver_destroy(sv);
+        return CACG_ENOMEM;
+    }
+    /* int32 CSR upload */
+    int *tmp = (int *)malloc(sizeof(int) * (size_t)((n + 1) > sv->nnz ? (n + 1) : sv->nnz));
+    if (!tmp) {
+        cacg_solver_destroy(sv);
+        return CACG_ENOMEM;
+    }
+    for (long i = 0; i <= n; i++)
+        tmp[i] = (int)row_start[i];
+    hipError_t e = hipMemcpy(sv->d_rowptr, tmp, sizeof(int) * (size_t)(n + 1), hipMemcpyHostToDevice);
+    for (long p = 0; p < sv->nnz && e == hipSuccess; p++)
+        tmp[p] = (int)col_idx[p];
+    if (e == hipSuccess && sv->nnz > 0)
+        e = hipMemcpy(sv->d_col, tmp, sizeof(int) * (size_t)sv->nnz, hipMemcpyHostToDevice);
+    if (e == hipSuccess && sv->nnz > 0)
+        e = hipMemcpy(sv->d_val, val, sizeof(double) * (size_t)sv->nnz, hipMemcpyHostToDevice);
+    free(tmp);
+    if (e != hipSuccess) {
+        fprintf(stderr, "cacg: matrix upload failed: %s\n", hipGetErrorString(e));
+        cacg_solver_destroy(sv);
+        return CACG_EDEVICE;
+    }
+    sv->A.n = (int)n;
+    sv->A.rowptr = sv->d_rowptr;
+    sv->A.col = sv->d_col;
+    sv->A.val = sv->d_val;
+    *out = sv;
+    return CACG_OK;
+}
+
+/* ---- phase timing: events recorded between phases on the solver's stream ---- */
+
+static void mark(cacg_solver *sv, int phase_done)
+{
+    if (sv->nev >= MAXEV)
+        return;
+    hipEventRecord(sv->ev[sv->nev], sv->st);
+    sv->ev_phase[sv->nev] = phase_done;
+    sv->nev++;
+}
+
+static void harvest(cacg_solver *sv, cacg_report *rep)
+{
+    for (int i = 1; i < sv->nev; i++) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, sv->ev[i - 1], sv->ev[i]) == hipSuccess && rep) {
+            const int ph = sv->ev_phase[i];
+            rep->phase_ms[ph] += ms;
+            rep->phase_calls[ph] += 1;
+            rep->seconds += ms * 1e-3;
+        }
+    }
+    if (sv->nev > 0) {
+        hipEvent_t last = sv->ev[sv->nev - 1];
+        sv->ev[sv->nev - 1] = sv->ev[0];
+        sv->ev[0] = last;
+        sv->ev_phase[0] = sv->ev_phase[sv->nev - 1];
+        sv->nev = 1;
+    }
+}
+
+/* ---- the solve ------------------------------------------------------------------ */
+
+static int gram(cacg_solver *sv, int mode, const double *L, const double *R1, long ld1, int kr1,
+                const double *R2, long ld2, int kr2, double *G1, double *G2)
+{
+    const int n = (int)sv->n, sm = (int)sv->sm;
+    if (mode == CACG_MODE_REFERENCE)
+        return cacg_k_gram_ref(n, L, sm, sm, R1, ld1, kr1, R2, ld2, kr2, G1, G2, sv->st);
+    return cacg_k_gram_fast(n, L, sm, sm, R1, ld1, kr1, R2, ld2, kr2, G1, G2, sv->d_gram_work,
+                            sv->st);
+}
+
+#define SCHK(call)                                                                               \
+    do {                                                                                         \
+        int e_ = (int)(call);                                                                    \
+        if (e_ != 0) {                                                                           \
+            fprintf(stderr, "cacg: %s failed: %s (%s:%d)\n", #call,                              \
+                    hipGetErrorString((hipError_t)e_), __FILE__, __LINE__);                      \
+            status = CACG_EDEVICE;                                                               \
+            goto out;                                                                            \
+        }                                                                                        \
+    } while (0)
+
+int cacg_solver_solve_device(cacg_solver *sv, const double *d_B, double *d_X, cacg_report *rep)
+{
+    if (!sv || !d_B || !d_X)
+        return CACG_EINVAL;
+    const cacg_opts *o = &sv->opts;
+    const int mode = o->order == CACG_ORDER_REFERENCE ? CACG_MODE_REFERENCE : CACG_MODE_FAST;
+    const int n = (int)sv->n, m = (int)sv->m, s = (int)sv->s, sm = (int)sv->sm;
+    hipStream_t st = sv->st;
+    double *h_norm2 = sv->h_pin, *h_bnorm2 = sv->h_pin + m, *h_bounds = sv->h_pin + 2 * m;
+    int *h_flag = (int *)(sv->h_pin + 2 * m + 2);
+    double *S = sv->d_S, *Q = sv->d_Q;
+    int status = CACG_MAXITER;
+    long it = 0;
+    if (rep) {
+        double *keep = rep->res_hist;
+        long cap = rep->res_hist_cap;
+        memset(rep, 0, sizeof(*rep));
+        rep->res_hist = keep;
+        rep->res_hist_cap = cap;
+    }
+
+    sv->nev = 0;
+    mark(sv, CACG_PHASE_OTHER);
+    /* setup: bounds, R = B - A X (src/bcbcgKernel.c:106-154) */
+    SCHK(cacg_k_fill_i32(sv->d_flag, 1, 0, st));
+    SCHK(cacg_k_gerschgorin(&sv->A, sv->d_gwork, sv->d_bounds, st));
+    SCHK(cacg_k_spmm(&sv->A, m, d_X, m, sv->d_R, m, CACG_EPI_RESID, 0.0, 0.0, d_B, m, NULL, 0,
+                     NULL, 0, st));
+    if (o->stop_mode == CACG_STOP_REL_PER_COLUMN) {
+        if (mode == CACG_MODE_REFERENCE) {
+            SCHK(cacg_k_colnorm2_ref(n, m, d_B, m, sv->d_norm2, st));
+            SCHK(hipMemcpyAsync(h_bnorm2, sv->d_norm2, sizeof(double) * m, hipMemcpyDeviceToHost, st));
+        } else {
+            /* B^T B, diagonal taken as one strided copy */
+            SCHK(cacg_k_gram_fast(n, d_B, m, m, d_B, m, m, NULL, 0, 0, sv->d_QtAS, NULL,
+                                  sv->d_gram_work, st));
+            SCHK(hipMemcpy2DAsync(h_bnorm2, sizeof(double), sv->d_QtAS, sizeof(double) * (m + 1),
+                                  sizeof(double), m, hipMemcpyDeviceToHost, st));
+        }
+    }
+    mark(sv, CACG_PHASE_SETUP);
+    SCHK(hipMemcpyAsync(h_bounds, sv->d_bounds, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    SCHK(hipStreamSynchronize(st));
+    harvest(sv, rep);
+    if (rep) {
+        rep->bounds[0] = h_bounds[0];
+        rep->bounds[1] = h_bounds[1];
+    }
+    /* src/bcbcgKernel.c:113-115 */
+    const double sa = 2.0 / (h_bounds[1] - h_bounds[0]);
+    const double sb = -(h_bounds[1] + h_bounds[0]) / (h_bounds[1] - h_bounds[0]);
+    const double sa2 = 2.0 * sa, sb2 = 2.0 * sb;
+
+    for (it = 0; it < o->maxiter; it++) {
+        mark(sv, CACG_PHASE_OTHER);
+        /* Chebyshev basis (src/common.c:182-256) */
+        if (s == 1) {
+            SCHK(hipMemcpyAsync(S, sv->d_R, sizeof(double) * (size_t)n * m, hipMemcpyDeviceToDevice, st));
+        } else {
+            SCHK(cacg_k_spmm(&sv->A, m, sv->d_R, m, S + m, sm, CACG_EPI_CHEB1, sa, sb, NULL, 0, NULL,
+                             0, S, sm, st));
+            for (int j = 2; j < s; j++)
+                SCHK(cacg_k_spmm(&sv->A, m, S + (long)(j - 1) * m, sm, S + (long)j * m, sm,
+                                 CACG_EPI_CHEB, sa2, sb2, S + (long)(j - 1) * m, sm,
+                                 S + (long)(j - 2) * m, sm, NULL, 0, st));
+        }
+        mark(sv, CACG_PHASE_BASIS);
+        if (it == 0) {
+            double *t = Q;
+            Q = S;
+            S = t;
+        } else {
+            SCHK(gram(sv, mode, sv->d_AQ, S, sm, sm, NULL, 0, 0, sv->d_QtAS, NULL));
+            mark(sv, CACG_PHASE_GRAM);
+            SCHK(cacg_k_lu_solve(sm, sv->d_QtAQ, sm, sv->d_QtAS, sv->d_beta, sv->d_lu_work,
+                                 sv->d_flag, mode, st));
+            mark(sv, CACG_PHASE_SOLVE);
+            SCHK(cacg_k_qupdate(n, sm, S, Q, sv->d_beta, mode, st));
+            mark(sv, CACG_PHASE_QUPDATE);
+        }
+        SCHK(cacg_k_spmm(&sv->A, sm, Q, sm, sv->d_AQ, sm, CACG_EPI_NONE, 0.0, 0.0, NULL, 0, NULL, 0,
+                         NULL, 0, st));
+        mark(sv, CACG_PHASE_AQ);
+        SCHK(gram(sv, mode, Q, sv->d_AQ, sm, sm, sv->d_R, m, m, sv->d_QtAQ, sv->d_QtR));
+        mark(sv, CACG_PHASE_GRAM);
+        SCHK(cacg_k_lu_solve(sm, sv->d_QtAQ, m, sv->d_QtR, sv->d_alpha, sv->d_lu_work, sv->d_flag,
+                             mode, st));
+        mark(sv, CACG_PHASE_SOLVE);
+        SCHK(cacg_k_xrupdate(n, sm, m, Q, sv->d_AQ, sv->d_alpha, d_X, sv->d_R, sv->d_norm2,
+                             sv->d_xr_work, mode, st));
+        mark(sv, CACG_PHASE_XRUPDATE);
+        SCHK(hipMemcpyAsync(h_norm2, sv->d_norm2, sizeof(double) * m, hipMemcpyDeviceToHost, st));
+        SCHK(hipMemcpyAsync(h_flag, sv->d_flag, sizeof(int), hipMemcpyDeviceToHost, st));
+        SCHK(hipStreamSynchronize(st));
+        harvest(sv, rep);
+
+        int done = 1, finite = 1;
+        for (int c = 0; c < m; c++) {
+            const double rn = sqrt(h_norm2[c]);
+            if (rep && rep->res_hist && it < rep->res_hist_cap)
+                rep->res_hist[it * m + c] = rn;
+            if (!isfinite(rn))
+                finite = 0;
+            if (o->stop_mode == CACG_STOP_REL_PER_COLUMN && !(rn < o->tol * sqrt(h_bnorm2[c])))
+                done = 0;
+        }
+        const double r0 = sqrt(h_norm2[0]);
+        if (o->stop_mode == CACG_STOP_REF_COL0_ABS)
+            done = r0 < o->eps; /* src/bcbcgKernel.c:445-447, 524-526 */
+        if (o->print_period > 0 &&
+            (it % o->print_period == 0 || done || it == o->maxiter - 1)) /* :449-452 */
+            printf("Loop: %ld, R[%d]_norm=%30.30f\n", it, 1, r0);
+        if (done) {
+            status = CACG_OK;
+            it++;
+            break;
+        }
+        if (!finite || *h_flag) {
+            status = CACG_BREAKDOWN;
+            it++;
+            break;
+        }
+    }
+out:
+    /* leave the two basis-sized buffers in allocation roles for the next solve */
+    sv->d_S = S;
+    sv->d_Q = Q;
+    if (rep) {
+        rep->status = status;
+        rep->iters = it;
+    }
+    return status;
+}
+
+/* ---- host-buffer API --------------------------------------------------------------- */
+
+int cacg_bcbcg(long n, const long *row_start, const long *col_idx, const double *val, long m,
+               long s, const double *B, double *X, const cacg_opts *opts, cacg_report *report)
+{
+    if (!B || !X)
+        return CACG_EINVAL;
+    cacg_solver *sv = NULL;
+    int rc = cacg_solver_create(&sv, n, row_start, col_idx, val, m, s, opts);
+    if (rc != CACG_OK)
+        return rc;
+    double *dB = NULL, *dX = NULL;
+    const size_t bytes = sizeof(double) * (size_t)n * (size_t)m;
+    if (hipMalloc((void **)&dB, bytes) != hipSuccess || hipMalloc((void **)&dX, bytes) != hipSuccess) {
+        hipFree(dB);
+        cacg_solver_destroy(sv);
+        return CACG_ENOMEM;
+    }
+    rc = CACG_EDEVICE;
+    if (hipMemcpy(dB, B, bytes, hipMemcpyHostToDevice) == hipSuccess &&
+        hipMemcpy(dX, X, bytes, hipMemcpyHostToDevice) == hipSuccess) {
+        rc = cacg_solver_solve_device(sv, dB, dX, report);
+        if (rc != CACG_EDEVICE && rc != CACG_EINVAL && rc != CACG_ENOMEM &&
+            hipMemcpy(X, dX, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = CACG_EDEVICE;
+    }
+    hipFree(dB);
+    hipFree(dX);
+    cacg_solver_destroy(sv);
+    return rc;
+}
+
+/* ---- drop-in entry points (src/cbcgKernel.h:21, src/bcbcgKernel.h:22) ------------ */
+
+static void dropin(csrType_local mat, denseType B, denseType X, long s, double epsilon, int myid,
+                   int numprocs, long maxiter)
+{
+    g_last_iters = 0;
+    if (numprocs != 1 || myid != 0 || mat.start != 0 || mat.num_rows != B.local_num_row ||
+        B.local_num_row != B.global_num_row || X.local_num_row != B.local_num_row ||
+        X.local_num_col != B.local_num_col || mat.num_cols != mat.num_rows || !B.data || !X.data) {
+        fprintf(stderr, "cacg: %s: the whole system must be local to one process (numprocs == 1)\n",
+                maxiter == 1000 ? "cbcg_v1" : "bcbcg_v1");
+        g_last_status = CACG_EINVAL;
+        return;
+    }
+    cacg_opts o;
+    cacg_default_opts(&o);
+    o.stop_mode = CACG_STOP_REF_COL0_ABS;
+    o.eps = epsilon;
+    o.maxiter = maxiter;
+    const char *ord = getenv("CACG_ORDER");
+    o.order = (ord && strcmp(ord, "fast") == 0) ? CACG_ORDER_FAST : CACG_ORDER_REFERENCE;
+    const char *quiet = getenv("CACG_QUIET");
+    o.print_period = (quiet && quiet[0] == '1') ? 0 : 10;
+    cacg_report rep;
+    memset(&rep, 0, sizeof(rep));
+    g_last_status = cacg_bcbcg(mat.num_rows, mat.row_start, mat.col_idx, mat.csrdata,
+                               B.local_num_col, s, B.data, X.data, &o, &rep);
+    g_last_iters = rep.iters;
+    fflush(stdout);
+}
+
+void cbcg_v1(csrType_local mat, denseType B, denseType X, long s, double epsilon, int myid,
+             int numprocs)
+{
+    /* one right-hand side, 1000-iteration cap (src/cbcgKernel.c:2) */
+    if (B.local_num_col != 1) {
+        fprintf(stderr, "cacg: cbcg_v1 takes a single right-hand side\n");
+        g_last_status = CACG_EINVAL;
+        g_last_iters = 0;
+        return;
+    }
+    dropin(mat, B, X, s, epsilon, myid, numprocs, 1000);
+}
+
+void bcbcg_v1(csrType_local mat, denseType B, denseType X, long s, double epsilon, int myid,
+              int numprocs)
+{
+    dropin(mat, B, X, s, epsilon, myid, numprocs, 100); /* src/bcbcgKernel.c:3 */
+}

@@ -1,0 +1,492 @@
+// Tall-skinny block updates of the outer iteration (src/bcbcgKernel.c:249-262
+// and 376-437): Q = S - Q beta, and X += Q alpha, R -= AQ alpha with the
+// column norms of the new R.  gfx950, wave64.
+//
+// Reference order (bitwise): one lane per output entry, the k-sum sequential
+// from 0 in the reference's order with separately rounded products and sums
+// (src/BLAS3.c:811-819), the rows staged through LDS so the in-place update
+// of Q reads old values only; norms as one sequential chain per column.
+//
+// Fast order: FP64 MFMA v_mfma_f64_16x16x4f64 per 16-row tile.  The
+// contraction index is permuted across lane groups: lane group g (lanes
+// 16g..16g+15) supplies k = g*K/4 + t at MFMA step t, so each lane's A operand
+// is K/4 contiguous doubles of one row (vector loads, no LDS staging); the B
+// operand (beta or alpha) is permuted the same way.  Any consistent k order
+// gives the same product; only rounding differs.
+#include "kcommon.hpp"
+
+namespace {
+
+__device__ __forceinline__ int xcd_remap(int g, int G)
+{
+    if ((G & 7) != 0)
+        return g;
+    return (g & 7) * (G >> 3) + (g >> 3);
+}
+
+// ---- reference order ----------------------------------------------------------
+
+template <bool BETA_LDS>
+__global__ __launch_bounds__(256) void qupdate_ref_kernel(int n, int sm, const double *__restrict__ S,
+                                                          double *Q, const double *__restrict__ beta,
+                                                          int rb)
+{
+    extern __shared__ double lds[];
+    double *bl = lds;
+    double *qt = lds + (BETA_LDS ? (long)sm * sm : 0);
+    const double *B = BETA_LDS ? bl : beta;
+    const int t = threadIdx.x;
+    if (BETA_LDS) {
+        for (long e = t; e < (long)sm * sm; e += 256)
+            bl[e] = beta[e];
+    }
+    __syncthreads();
+    const int ntile = (n + rb - 1) / rb;
+    for (int tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+        const long i0 = (long)tile * rb;
+        const int rows = (n - i0) < rb ? (int)(n - i0) : rb;
+        for (int e = t; e < rows * sm; e += 256)
+            qt[e] = Q[i0 * sm + e];
+        __syncthreads();
+        for (int e = t; e < rows * sm; e += 256) {
+            const int r = e / sm, c = e % sm;
+            const double *qr = qt + (long)r * sm;
+            double acc = 0.0;
+            for (int q = 0; q < sm; q++)
+                acc = add_rn(acc, mul_rn(B[(long)q * sm + c], qr[q]));
+            Q[i0 * sm + e] = sub_rn(S[i0 * sm + e], acc);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void xrupdate_ref_kernel(int n, int sm, int m,
+                                                           const double *__restrict__ Q,
+                                                           const double *__restrict__ AQ,
+                                                           const double *__restrict__ alpha,
+                                                           double *__restrict__ X,
+                                                           double *__restrict__ R, int rb)
+{
+    extern __shared__ double lds[];
+    double *al = lds;
+    double *qt = al + (long)sm * m;
+    double *aqt = qt + (long)rb * sm;
+    const int t = threadIdx.x;
+    for (long e = t; e < (long)sm * m; e += 256)
+        al[e] = alpha[e];
+    __syncthreads();
+    const int ntile = (n + rb - 1) / rb;
+    for (int tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+        const long i0 = (long)tile * rb;
+        const int rows = (n - i0) < rb ? (int)(n - i0) : rb;
+        for (int e = t; e < rows * sm; e += 256) {
+            qt[e] = Q[i0 * sm + e];
+            aqt[e] = AQ[i0 * sm + e];
+        }
+        __syncthreads();
+        for (int e = t; e < rows * m; e += 256) {
+            const int r = e / m, c = e % m;
+            const double *qr = qt + (long)r * sm, *ar = aqt + (long)r * sm;
+            double accq = 0.0, acca = 0.0;
+            for (int q = 0; q < sm; q++) {
+                const double w = al[(long)q * m + c];
+                accq = add_rn(accq, mul_rn(w, qr[q]));
+                acca = add_rn(acca, mul_rn(w, ar[q]));
+            }
+            X[i0 * m + e] = add_rn(X[i0 * m + e], accq);
+            R[i0 * m + e] = sub_rn(R[i0 * m + e], acca);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(64) void colnorm2_ref_kernel(int n, int m, const double *__restrict__ R,
+                                                         long ldr, double *__restrict__ out)
+{
+    const int c = blockIdx.x * 64 + threadIdx.x;
+    if (c >= m)
+        return;
+    const double *p = R + c;
+    double acc = 0.0;
+    int i = 0;
+    for (; i + 4 <= n; i += 4) {
+        const double v0 = p[(long)i * ldr], v1 = p[(long)(i + 1) * ldr];
+        const double v2 = p[(long)(i + 2) * ldr], v3 = p[(long)(i + 3) * ldr];
+        acc = add_rn(acc, mul_rn(v0, v0));
+        acc = add_rn(acc, mul_rn(v1, v1));
+        acc = add_rn(acc, mul_rn(v2, v2));
+        acc = add_rn(acc, mul_rn(v3, v3));
+    }
+    for (; i < n; i++) {
+        const double v = p[(long)i * ldr];
+        acc = add_rn(acc, mul_rn(v, v));
+    }
+    out[c] = acc;
+}
+
+// ---- fast order: MFMA -----------------------------------------------------------
+
+// rows per 16-row tile handled by one wave; K = SM, KQ = SM / 4 per lane group
+template <int SM>
+__global__ __launch_bounds__(512) void qupdate_mfma_kernel(int n, const double *__restrict__ S,
+                                                           double *Q, const double *__restrict__ beta)
+{
+    constexpr int KQ = SM / 4, NTL = SM / 16;
+    extern __shared__ double bl[];  // SM x SM
+    for (int e = threadIdx.x; e < SM * SM; e += 512)
+        bl[e] = beta[e];
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int lg = lane >> 4, lc = lane & 15;
+    const int nw = gridDim.x * 8;
+    const int wid = xcd_remap(blockIdx.x, gridDim.x) * 8 + wave;
+    const int ntiles = (n + 15) / 16;
+    // contiguous tile ranges per wave
+    const int per = (ntiles + nw - 1) / nw;
+    const int tb = wid * per;
+    int te = tb + per;
+    if (te > ntiles)
+        te = ntiles;
+    for (int tile = tb; tile < te; tile++) {
+        const long i0 = (long)tile * 16;
+        const long arow = i0 + lc;
+        double av[KQ];
+        if (arow < n) {
+            const double2 *src = reinterpret_cast<const double2 *>(Q + arow * SM + lg * KQ);
+#pragma unroll
+            for (int u = 0; u < KQ / 2; u++) {
+                const double2 v = src[u];
+                av[2 * u] = v.x;
+                av[2 * u + 1] = v.y;
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < KQ; u++)
+                av[u] = 0.0;
+        }
+#pragma unroll 1
+        for (int j = 0; j < NTL; j++) {
+            double sv[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const long i = i0 + lg * 4 + r;
+                sv[r] = i < n ? S[i * SM + j * 16 + lc] : 0.0;
+            }
+            dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+            const double *bcol = bl + (lg * KQ) * SM + j * 16 + lc;
+#pragma unroll
+            for (int tt = 0; tt < KQ; tt++)
+                acc = mfma16x16x4(av[tt], bcol[tt * SM], acc);
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const long i = i0 + lg * 4 + r;
+                if (i < n)
+                    Q[i * SM + j * 16 + lc] = sv[r] - acc[r];
+            }
+        }
+    }
+}
+
+// NTM = output column tiles (m <= 16 * NTM); norms: per-block partials
+template <int SM, int NTM>
+__global__ __launch_bounds__(256) void xrupdate_mfma_kernel(int n, int m, const double *__restrict__ Q,
+                                                            const double *__restrict__ AQ,
+                                                            const double *__restrict__ alpha,
+                                                            double *__restrict__ X,
+                                                            double *__restrict__ R,
+                                                            double *__restrict__ part)
+{
+    constexpr int KQ = SM / 4;
+    __shared__ double red[4][64][NTM];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int lg = lane >> 4, lc = lane & 15;
+    double alf[NTM][KQ];
+#pragma unroll
+    for (int j = 0; j < NTM; j++) {
+        const int c = j * 16 + lc;
+#pragma unroll
+        for (int tt = 0; tt < KQ; tt++)
+            alf[j][tt] = c < m ? alpha[(long)(lg * KQ + tt) * m + c] : 0.0;
+    }
+    double nrm[NTM];
+#pragma unroll
+    for (int j = 0; j < NTM; j++)
+        nrm[j] = 0.0;
+    const int nw = gridDim.x * 4;
+    const int wid = xcd_remap(blockIdx.x, gridDim.x) * 4 + wave;
+    const int ntiles = (n + 15) / 16;
+    const int per = (ntiles + nw - 1) / nw;
+    const int tb = wid * per;
+    int te = tb + per;
+    if (te > ntiles)
+        te = ntiles;
+    for (int tile = tb; tile < te; tile++) {
+        const long i0 = (long)tile * 16;
+        const long arow = i0 + lc;
+        double av[KQ], aq[KQ];
+        if (arow < n) {
+            const double2 *s1 = reinterpret_cast<const double2 *>(Q + arow * SM + lg * KQ);
+            const double2 *s2 = reinterpret_cast<const double2 *>(AQ + arow * SM + lg * KQ);
+#pragma unroll
+            for (int u = 0; u < KQ / 2; u++) {
+                const double2 v = s1[u], w = s2[u];
+                av[2 * u] = v.x;
+                av[2 * u + 1] = v.y;
+                aq[2 * u] = w.x;
+                aq[2 * u + 1] = w.y;
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < KQ; u++) {
+                av[u] = 0.0;
+                aq[u] = 0.0;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NTM; j++) {
+            dbl4 ax = {0.0, 0.0, 0.0, 0.0}, ar = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int tt = 0; tt < KQ; tt++) {
+                ax = mfma16x16x4(av[tt], alf[j][tt], ax);
+                ar = mfma16x16x4(aq[tt], alf[j][tt], ar);
+            }
+            const int c = j * 16 + lc;
+            if (c < m) {
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const long i = i0 + lg * 4 + r;
+                    if (i < n) {
+                        X[i * m + c] += ax[r];
+                        const double rn = R[i * m + c] - ar[r];
+                        R[i * m + c] = rn;
+                        nrm[j] += rn * rn;
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NTM; j++)
+        red[wave][lane][j] = nrm[j];
+    __syncthreads();
+    if (threadIdx.x < NTM * 16) {
+        const int j = threadIdx.x / 16, l = threadIdx.x % 16;
+        const int c = j * 16 + l;
+        double acc = 0.0;
+        for (int w = 0; w < 4; w++)
+            for (int g = 0; g < 4; g++)
+                acc += red[w][g * 16 + l][j];
+        if (c < m)
+            part[(long)blockIdx.x * m + c] = acc;
+    }
+}
+
+// per-block partial column norms of a row-major n x m block (fast order)
+__global__ __launch_bounds__(256) void colnorm2_part_kernel(int n, int m, const double *__restrict__ R,
+                                                           double *__restrict__ part)
+{
+    __shared__ double red[256];
+    const int t = threadIdx.x;
+    // lanes cover (row offset, column) pairs: 256 / m rows per step when m <= 256
+    const int cw = m < 256 ? m : 256;
+    const int rstep = 256 / cw;
+    const int c0 = t % cw, rr = t / cw;
+    const long rows = ((long)n + gridDim.x - 1) / gridDim.x;
+    const long r0 = (long)blockIdx.x * rows;
+    long r1 = r0 + rows;
+    if (r1 > n)
+        r1 = n;
+    for (int cb = 0; cb < m; cb += cw) {
+        double acc = 0.0;
+        const int c = cb + c0;
+        if (rr < rstep && c < m)
+            for (long i = r0 + rr; i < r1; i += rstep) {
+                const double v = R[i * m + c];
+                acc += v * v;
+            }
+        red[t] = acc;
+        __syncthreads();
+        if (t < cw && cb + t < m) {
+            double s = 0.0;
+            for (int q = 0; q < rstep; q++)
+                s += red[q * cw + t];
+            part[(long)blockIdx.x * m + cb + t] = s;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(64) void colnorm2_final_kernel(const double *__restrict__ part, int nb,
+                                                           int m, double *__restrict__ out)
+{
+    const int c = blockIdx.x * 64 + threadIdx.x;
+    if (c >= m)
+        return;
+    double acc = 0.0;
+    for (int b = 0; b < nb; b++)
+        acc += part[(long)b * m + c];
+    out[c] = acc;
+}
+
+constexpr int XR_BLOCKS = 1024;  // fixed: the norm partials depend on it
+
+template <int SM>
+int launch_qupdate_mfma(int n, const double *S, double *Q, const double *beta, hipStream_t st)
+{
+    const int ntiles = (n + 15) / 16;
+    int G = (ntiles + 7) / 8;
+    if (G > 256)
+        G = 256;
+    if (G >= 8)
+        G &= ~7;
+    cacg_allow_lds(qupdate_mfma_kernel<SM>, (size_t)SM * SM * sizeof(double));
+    hipLaunchKernelGGL(qupdate_mfma_kernel<SM>, dim3(G), dim3(512), (size_t)SM * SM * sizeof(double),
+                       st, n, S, Q, beta);
+    CACG_LAUNCH_CHECK();
+    return 0;
+}
+
+template <int SM, int NTM>
+int launch_xrupdate_mfma(int n, int m, const double *Q, const double *AQ, const double *alpha,
+                         double *X, double *R, double *d_norm2, double *work, hipStream_t st)
+{
+    hipLaunchKernelGGL((xrupdate_mfma_kernel<SM, NTM>), dim3(XR_BLOCKS), dim3(256), 0, st, n, m, Q,
+                       AQ, alpha, X, R, work);
+    CACG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(colnorm2_final_kernel, dim3((m + 63) / 64), dim3(64), 0, st, work, XR_BLOCKS,
+                       m, d_norm2);
+    CACG_LAUNCH_CHECK();
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int cacg_k_colnorm2_ref(int n, int m, const double *R, long ldr, double *d_norm2,
+                                   hipStream_t st)
+{
+    if (n < 0 || m <= 0)
+        return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(colnorm2_ref_kernel, dim3((m + 63) / 64), dim3(64), 0, st, n, m, R, ldr,
+                       d_norm2);
+    CACG_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int cacg_k_qupdate(int n, int sm, const double *S, double *Q, const double *beta,
+                              int mode, hipStream_t st)
+{
+    if (n < 0 || sm <= 0)
+        return (int)hipErrorInvalidValue;
+    if (n == 0)
+        return 0;
+    if (mode == CACG_MODE_FAST) {
+        switch (sm) {
+        case 128: return launch_qupdate_mfma<128>(n, S, Q, beta, st);
+        case 112: return launch_qupdate_mfma<112>(n, S, Q, beta, st);
+        case 96: return launch_qupdate_mfma<96>(n, S, Q, beta, st);
+        case 80: return launch_qupdate_mfma<80>(n, S, Q, beta, st);
+        case 64: return launch_qupdate_mfma<64>(n, S, Q, beta, st);
+        case 48: return launch_qupdate_mfma<48>(n, S, Q, beta, st);
+        case 32: return launch_qupdate_mfma<32>(n, S, Q, beta, st);
+        case 16: return launch_qupdate_mfma<16>(n, S, Q, beta, st);
+        default: break;  // other widths: the reference-order kernel
+        }
+    }
+    const int rb = sm >= 256 ? 1 : 256 / sm;
+    const bool beta_lds = (long)sm * sm + (long)rb * sm <= 20480 - 256;
+    const size_t shm = ((beta_lds ? (size_t)sm * sm : 0) + (size_t)rb * sm) * sizeof(double);
+    const int ntile = (n + rb - 1) / rb;
+    const int G = ntile < 1024 ? ntile : 1024;
+    cacg_allow_lds(beta_lds ? qupdate_ref_kernel<true> : qupdate_ref_kernel<false>, shm);
+    if (beta_lds)
+        hipLaunchKernelGGL(qupdate_ref_kernel<true>, dim3(G), dim3(256), shm, st, n, sm, S, Q, beta, rb);
+    else
+        hipLaunchKernelGGL(qupdate_ref_kernel<false>, dim3(G), dim3(256), shm, st, n, sm, S, Q, beta, rb);
+    CACG_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" long cacg_xrupdate_workspace(int m)
+{
+    return (long)XR_BLOCKS * m;
+}
+
+extern "C" int cacg_k_xrupdate(int n, int sm, int m, const double *Q, const double *AQ,
+                               const double *alpha, double *X, double *R, double *d_norm2,
+                               double *d_work, int mode, hipStream_t st)
+{
+    if (n < 0 || sm <= 0 || m <= 0)
+        return (int)hipErrorInvalidValue;
+    if (mode == CACG_MODE_FAST && m <= 32) {
+#define CACG_XR(SMV)                                                                              \
+    case SMV:                                                                                     \
+        return m <= 16 ? launch_xrupdate_mfma<SMV, 1>(n, m, Q, AQ, alpha, X, R, d_norm2, d_work, st) \
+                       : launch_xrupdate_mfma<SMV, 2>(n, m, Q, AQ, alpha, X, R, d_norm2, d_work, st);
+        switch (sm) {
+            CACG_XR(128)
+            CACG_XR(112)
+            CACG_XR(96)
+            CACG_XR(80)
+            CACG_XR(64)
+            CACG_XR(48)
+            CACG_XR(32)
+            CACG_XR(16)
+        default: break;
+        }
+#undef CACG_XR
+    }
+    if (n > 0) {
+        int rb = m >= 256 ? 1 : 256 / m;
+        if ((long)rb * sm > 4096)
+            rb = 4096 / sm > 0 ? 4096 / sm : 1;
+        const size_t shm = ((size_t)sm * m + 2 * (size_t)rb * sm) * sizeof(double);
+        if (shm > 160 * 1024)
+            return (int)hipErrorInvalidValue;
+        const int ntile = (n + rb - 1) / rb;
+        const int G = ntile < 1024 ? ntile : 1024;
+        cacg_allow_lds(xrupdate_ref_kernel, shm);
+        hipLaunchKernelGGL(xrupdate_ref_kernel, dim3(G), dim3(256), shm, st, n, sm, m, Q, AQ, alpha, X,
+                           R, rb);
+        CACG_LAUNCH_CHECK();
+    }
+    if (mode == CACG_MODE_REFERENCE)
+        return cacg_k_colnorm2_ref(n, m, R, m, d_norm2, st);
+    const int nb = XR_BLOCKS;
+    hipLaunchKernelGGL(colnorm2_part_kernel, dim3(nb), dim3(256), 0, st, n, m, R, d_work);
+    CACG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(colnorm2_final_kernel, dim3((m + 63) / 64), dim3(64), 0, st, d_work, nb, m,
+                       d_norm2);
+    CACG_LAUNCH_CHECK();
+    return 0;
+}
+
+// ---- fills ---------------------------------------------------------------------
+
+namespace {
+template <typename T>
+__global__ __launch_bounds__(256) void fill_kernel(T *d, long count, T v)
+{
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < count; e += (long)gridDim.x * 256)
+        d[e] = v;
+}
+}  // namespace
+
+extern "C" int cacg_k_fill_i32(int *d, long count, int v, hipStream_t st)
+{
+    if (count <= 0)
+        return 0;
+    hipLaunchKernelGGL(fill_kernel<int>, dim3(cacg_grid(count, 256, 1024)), dim3(256), 0, st, d,
+                       count, v);
+    CACG_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int cacg_k_fill_f64(double *d, long count, double v, hipStream_t st)
+{
+    if (count <= 0)
+        return 0;
+    hipLaunchKernelGGL(fill_kernel<double>, dim3(cacg_grid(count, 256, 1024)), dim3(256), 0, st, d,
+                       count, v);
+    CACG_LAUNCH_CHECK();
+    return 0;
+}

@@ -1,0 +1,335 @@
+// Gram blocks G = L^T [R1 | R2] over tall-skinny row-major blocks
+// (Q^T A S, Q^T A Q, Q^T R of src/BLAS3.c:699-758).  gfx950, wave64.
+//
+// Reference order: one lane per entry G[a][b], one sequential chain over all
+// rows in row order, product and sum rounded separately: the reference's
+// summation at one rank, bit for bit.  The chain makes this a verification
+// kernel (its time grows with n on sm*sm lanes only).
+//
+// Fast order: split-K.  Workgroup x owns a contiguous chunk of rows and a
+// group of 16x16 output tiles per wave; FP64 MFMA v_mfma_f64_16x16x4f64 with
+// the contraction index = row index, so both operands load straight from
+// row-major memory: the 16 lanes of a lane group read 16 consecutive doubles
+// (128 B) of one row and the 4 lane groups read 4 consecutive rows.  Loads for
+// the next 8 rows are issued before the MFMAs of the current 8 (register
+// double buffer).  Each chunk writes a kl_pad x kr_pad partial; a second
+// kernel sums the partials in chunk order, so the result is deterministic and
+// independent of scheduling.
+#include "kcommon.hpp"
+
+namespace {
+
+constexpr int GRAM_UNR = 4;   // rows per lane per chain step batch (reference kernel)
+
+__global__ __launch_bounds__(256) void gram_ref_kernel(int n, const double *__restrict__ L, long ldl,
+                                                       int kl, const double *__restrict__ R1,
+                                                       long ld1, int kr1,
+                                                       const double *__restrict__ R2, long ld2,
+                                                       int kr2, double *__restrict__ G1,
+                                                       double *__restrict__ G2)
+{
+    const int kr = kr1 + kr2;
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (long)kl * kr)
+        return;
+    const int a = (int)(e / kr), b = (int)(e % kr);
+    const double *rp = b < kr1 ? R1 + b : R2 + (b - kr1);
+    const long ldr = b < kr1 ? ld1 : ld2;
+    const double *lp = L + a;
+    double acc = 0.0;
+    int i = 0;
+    for (; i + GRAM_UNR <= n; i += GRAM_UNR) {
+        double lv[GRAM_UNR], rv[GRAM_UNR];
+#pragma unroll
+        for (int u = 0; u < GRAM_UNR; u++) {
+            lv[u] = lp[(long)(i + u) * ldl];
+            rv[u] = rp[(long)(i + u) * ldr];
+        }
+#pragma unroll
+        for (int u = 0; u < GRAM_UNR; u++)
+            acc = add_rn(acc, mul_rn(rv[u], lv[u]));
+    }
+    for (; i < n; i++)
+        acc = add_rn(acc, mul_rn(rp[(long)i * ldr], lp[(long)i * ldl]));
+    if (b < kr1)
+        G1[(long)a * kr1 + b] = acc;
+    else
+        G2[(long)a * kr2 + (b - kr1)] = acc;
+}
+
+// ---- fast: split-K FP64 MFMA ------------------------------------------------
+
+struct GramPlan {
+    int TM, TN;      // 16x16 tiles in M (kl) and N (kr)
+    int MT, NT;      // tiles per wave
+    int GN, NG;      // tile groups along N, total groups
+    int WGG, KS;     // groups per workgroup, waves sharing one group (split K inside the workgroup)
+    int gy;          // workgroups along the tile-group dimension
+    int gx;          // row chunks (split K across workgroups)
+    int chunk;       // rows per chunk
+};
+
+GramPlan plan_gram(int n, int kl, int kr)
+{
+    GramPlan p;
+    p.TM = (kl + 15) / 16;
+    p.TN = (kr + 15) / 16;
+    p.NT = p.TN < 9 ? p.TN : 9;
+    p.GN = (p.TN + p.NT - 1) / p.NT;
+    p.MT = p.TM >= 8 ? 2 : 1;
+    const int GM = (p.TM + p.MT - 1) / p.MT;
+    p.NG = GM * p.GN;
+    if (p.NG >= 4) {
+        p.WGG = 4;
+        p.KS = 1;
+    } else {
+        p.WGG = p.NG;
+        p.KS = 4 / p.NG;
+    }
+    p.gy = (p.NG + p.WGG - 1) / p.WGG;
+    // about 2 workgroups per CU over 256 CUs, at least 64 rows per chunk; a
+    // function of n only, so the summation order does not depend on the device
+    int gx = 512 / p.gy;
+    if (gx < 1)
+        gx = 1;
+    const int step = 8 * p.KS;
+    long maxc = ((long)n + 63) / 64;
+    if (gx > maxc)
+        gx = (int)(maxc > 0 ? maxc : 1);
+    long chunk = ((long)n + gx - 1) / gx;
+    chunk = (chunk + step - 1) / step * step;
+    if (chunk < step)
+        chunk = step;
+    p.chunk = (int)chunk;
+    p.gx = (int)(((long)n + chunk - 1) / chunk);
+    if (p.gx < 1)
+        p.gx = 1;
+    return p;
+}
+
+struct Src {
+    const double *L;
+    long ldl;
+    int kl;
+    const double *R1;
+    long ld1;
+    int kr1;
+    const double *R2;
+    long ld2;
+    int kr2;
+};
+
+template <int MT, int NT>
+__global__ __launch_bounds__(256) void gram_mfma_kernel(int n, Src s, double *__restrict__ part,
+                                                        int kl_pad, int kr_pad, int chunk, int GN,
+                                                        int NG, int WGG, int KS)
+{
+    extern __shared__ double red[];  // (KS - 1) * WGG * MT * NT * 4 * 64 doubles when KS > 1
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int lr = lane >> 4, lc = lane & 15;
+    const int gb = wave % WGG, ks = wave / WGG;
+    const int g = blockIdx.y * WGG + gb;
+    const bool active = g < NG && ks < KS;
+    const int gm = active ? g / GN : 0, gn = active ? g % GN : 0;
+    const int kr = s.kr1 + s.kr2;
+
+    // per-lane operand columns: A tile t reads column ca0 + 16 t of L; B tile t
+    // reads column cb0 + 16 t of [R1 | R2]
+    const int ca0 = gm * MT * 16 + lc;
+    const int cb0 = gn * NT * 16 + lc;
+
+    dbl4 acc[MT][NT];
+#pragma unroll
+    for (int i = 0; i < MT; i++)
+#pragma unroll
+        for (int j = 0; j < NT; j++)
+            acc[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
+
+    const int r0 = blockIdx.x * chunk;
+    int r1 = r0 + chunk;
+    if (r1 > n)
+        r1 = n;
+
+    // k-steps of 4 rows; wave ks takes steps ks, ks + KS, ...; two steps per
+    // batch, batches double-buffered in registers
+    double av[2][2][MT], bv[2][2][NT];
+    auto load = [&](int buf, int rb) {
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const int row = rb + u * 4 * KS + lr;
+            const bool rok = row < r1;
+            const double *pl = s.L + (long)row * s.ldl;
+            const double *p1 = s.R1 + (long)row * s.ld1;
+            const double *p2 = s.R2 + (long)row * s.ld2 - s.kr1;
+#pragma unroll
+            for (int t = 0; t < MT; t++) {
+                const int a = ca0 + 16 * t;
+                av[buf][u][t] = (rok && a < s.kl) ? pl[a] : 0.0;
+            }
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                const int b = cb0 + 16 * t;
+                bv[buf][u][t] = (rok && b < kr) ? (b < s.kr1 ? p1[b] : p2[b]) : 0.0;
+            }
+        }
+    };
+    auto compute = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+#pragma unroll
+            for (int i = 0; i < MT; i++)
+#pragma unroll
+                for (int j = 0; j < NT; j++)
+                    acc[i][j] = mfma16x16x4(av[buf][u][i], bv[buf][u][j], acc[i][j]);
+    };
+    if (active) {
+        const int stride = 8 * KS;
+        int rb = r0 + ks * 4;
+        if (rb < r1)
+            load(0, rb);
+        for (; rb < r1; rb += 2 * stride) {
+            if (rb + stride < r1)
+                load(1, rb + stride);
+            compute(0);
+            if (rb + stride >= r1)
+                break;
+            if (rb + 2 * stride < r1)
+                load(0, rb + 2 * stride);
+            compute(1);
+        }
+    }
+
+    // combine the KS waves of one tile group in wave order
+    if (KS > 1) {
+        const int per = MT * NT * 4;
+        if (active && ks > 0) {
+            double *dst = red + ((long)((ks - 1) * WGG + gb) * per) * 64;
+#pragma unroll
+            for (int i = 0; i < MT; i++)
+#pragma unroll
+                for (int j = 0; j < NT; j++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++)
+                        dst[((i * NT + j) * 4 + r) * 64 + lane] = acc[i][j][r];
+        }
+        __syncthreads();
+        if (active && ks == 0) {
+            for (int q = 1; q < KS; q++) {
+                const double *src = red + ((long)((q - 1) * WGG + gb) * per) * 64;
+#pragma unroll
+                for (int i = 0; i < MT; i++)
+#pragma unroll
+                    for (int j = 0; j < NT; j++)
+#pragma unroll
+                        for (int r = 0; r < 4; r++)
+                            acc[i][j][r] += src[((i * NT + j) * 4 + r) * 64 + lane];
+            }
+        }
+    }
+    if (!active || ks != 0)
+        return;
+    double *out = part + (long)blockIdx.x * kl_pad * kr_pad;
+#pragma unroll
+    for (int i = 0; i < MT; i++) {
+        const int tm = gm * MT + i;
+        if (tm * 16 >= kl_pad)
+            continue;
+#pragma unroll
+        for (int j = 0; j < NT; j++) {
+            const int tn = gn * NT + j;
+            if (tn * 16 >= kr_pad)
+                continue;
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                out[(long)(tm * 16 + lr * 4 + r) * kr_pad + tn * 16 + lc] = acc[i][j][r];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void gram_reduce_kernel(const double *__restrict__ part, int gx,
+                                                          int kl, int kr1, int kr2, int kl_pad,
+                                                          int kr_pad, double *__restrict__ G1,
+                                                          double *__restrict__ G2)
+{
+    const int kr = kr1 + kr2;
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (long)kl * kr)
+        return;
+    const int a = (int)(e / kr), b = (int)(e % kr);
+    const long stride = (long)kl_pad * kr_pad;
+    const double *p = part + (long)a * kr_pad + b;
+    double acc = p[0];
+    for (int x = 1; x < gx; x++)
+        acc += p[x * stride];
+    if (b < kr1)
+        G1[(long)a * kr1 + b] = acc;
+    else
+        G2[(long)a * kr2 + (b - kr1)] = acc;
+}
+
+template <int MT, int NT>
+int launch_gram_mfma(int n, const Src &s, const GramPlan &p, double *G1, double *G2, double *work,
+                     hipStream_t st)
+{
+    const int kl_pad = p.TM * 16, kr_pad = p.TN * 16;
+    const size_t shm = p.KS > 1 ? (size_t)(p.KS - 1) * p.WGG * MT * NT * 4 * 64 * sizeof(double) : 0;
+    cacg_allow_lds(gram_mfma_kernel<MT, NT>, shm);
+    hipLaunchKernelGGL((gram_mfma_kernel<MT, NT>), dim3(p.gx, p.gy), dim3(256), shm, st, n, s, work,
+                       kl_pad, kr_pad, p.chunk, p.GN, p.NG, p.WGG, p.KS);
+    CACG_LAUNCH_CHECK();
+    const long outs = (long)s.kl * (s.kr1 + s.kr2);
+    hipLaunchKernelGGL(gram_reduce_kernel, dim3(cacg_grid(outs, 256, 0)), dim3(256), 0, st, work,
+                       p.gx, s.kl, s.kr1, s.kr2, kl_pad, kr_pad, G1, G2);
+    CACG_LAUNCH_CHECK();
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int cacg_k_gram_ref(int n, const double *L, long ldl, int kl, const double *R1, long ld1,
+                               int kr1, const double *R2, long ld2, int kr2, double *G1,
+                               double *G2, hipStream_t st)
+{
+    if (n < 0 || kl <= 0 || kr1 < 0 || kr2 < 0 || kr1 + kr2 <= 0)
+        return (int)hipErrorInvalidValue;
+    const long outs = (long)kl * (kr1 + kr2);
+    hipLaunchKernelGGL(gram_ref_kernel, dim3(cacg_grid(outs, 256, 0)), dim3(256), 0, st, n, L, ldl,
+                       kl, R1, ld1, kr1, R2, ld2, kr2, G1, G2);
+    CACG_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" long cacg_gram_workspace(int kl, int kr)
+{
+    // upper bound over n: gx <= 512
+    return 512L * ((kl + 15) / 16 * 16) * ((kr + 15) / 16 * 16);
+}
+
+extern "C" int cacg_k_gram_fast(int n, const double *L, long ldl, int kl, const double *R1,
+                                long ld1, int kr1, const double *R2, long ld2, int kr2, double *G1,
+                                double *G2, double *d_work, hipStream_t st)
+{
+    if (n < 0 || kl <= 0 || kr1 < 0 || kr2 < 0 || kr1 + kr2 <= 0)
+        return (int)hipErrorInvalidValue;
+    const GramPlan p = plan_gram(n, kl, kr1 + kr2);
+    const Src s = {L, ldl, kl, R1, ld1, kr1, R2, ld2, kr2};
+    // MT in {1, 2}, NT in 1..9
+#define CACG_GRAM_NT(MTV)                                                                         \
+    switch (p.NT) {                                                                               \
+    case 1: return launch_gram_mfma<MTV, 1>(n, s, p, G1, G2, d_work, st);                              \
+    case 2: return launch_gram_mfma<MTV, 2>(n, s, p, G1, G2, d_work, st);                              \
+    case 3: return launch_gram_mfma<MTV, 3>(n, s, p, G1, G2, d_work, st);                              \
+    case 4: return launch_gram_mfma<MTV, 4>(n, s, p, G1, G2, d_work, st);                              \
+    case 5: return launch_gram_mfma<MTV, 5>(n, s, p, G1, G2, d_work, st);                              \
+    case 6: return launch_gram_mfma<MTV, 6>(n, s, p, G1, G2, d_work, st);                              \
+    case 7: return launch_gram_mfma<MTV, 7>(n, s, p, G1, G2, d_work, st);                              \
+    case 8: return launch_gram_mfma<MTV, 8>(n, s, p, G1, G2, d_work, st);                              \
+    default: return launch_gram_mfma<MTV, 9>(n, s, p, G1, G2, d_work, st);                             \
+    }
+    if (p.MT == 2) {
+        CACG_GRAM_NT(2)
+    } else {
+        CACG_GRAM_NT(1)
+    }
+#undef CACG_GRAM_NT
+}

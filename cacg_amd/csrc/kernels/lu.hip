@@ -1,0 +1,211 @@
+// Small dense solve x = lhs^{-1} rhs on one workgroup (1024 lanes), the
+// reference's LUsolver_v1_KeeplhsMat (src/DLSsolvers.c:106-208):
+// Crout factorisation with unit-diagonal U and no pivoting, forward then
+// backward substitution.  gfx950.
+//
+// Factorisation: right-looking, one barrier-separated step per pivot; every
+// element sees the same operations in the same order as in the reference's
+// loops, so the factor is bit-identical.  It lives in LDS when N*N doubles fit
+// (N <= 128: 128 KiB of the 160 KiB), in a global scratch block otherwise.
+//
+// Forward substitution runs column-oriented (for q: finalise row q, broadcast
+// it through LDS, update the rows below).  Each y[i][c] still receives its
+// subtractions in ascending q and the reciprocal scaling last, as in the
+// reference's row-oriented loop, so it is bit-identical too.  The right-hand
+// side is held in registers, JMAX rows per lane.
+//
+// Backward substitution: the reference subtracts x[q] u_iq in ascending q,
+// which makes each row wait for all later rows; mode REFERENCE keeps that
+// order (one sequential chain per column, x in LDS, U from the scratch copy).
+// Mode FAST runs it column-oriented (descending q, barrier per step).
+#include <math.h>
+
+#include "kcommon.hpp"
+
+namespace {
+
+constexpr int LU_T = 1024;
+constexpr int JMAX = 16;
+
+template <bool IN_LDS>
+__global__ __launch_bounds__(LU_T) void lu_solve_kernel(int N, const double *__restrict__ lhs, int k,
+                                                        const double *__restrict__ rhs,
+                                                        double *__restrict__ x,
+                                                        double *__restrict__ gw, int *flag,
+                                                        int mode, int kc)
+{
+    extern __shared__ double lds[];
+    double *a = IN_LDS ? lds : gw;
+    double *rowbuf = IN_LDS ? lds + (long)N * N : lds;
+    double *xs = IN_LDS ? lds : lds + 2 * kc;
+    const int t = threadIdx.x;
+
+    // ---- factorisation
+    for (long e = t; e < (long)N * N; e += LU_T)
+        a[e] = lhs[e];
+    __syncthreads();
+    int bad = 0;
+    for (int p = 0; p < N; p++) {
+        const double piv = a[(long)p * N + p];
+        if (!(piv != 0.0 && isfinite(piv)))
+            bad = 1;
+        const double inv = 1.0 / piv;
+        for (int c = p + 1 + t; c < N; c += LU_T)
+            a[(long)p * N + c] = mul_rn(a[(long)p * N + c], inv);
+        __syncthreads();
+        const int w = N - p - 1;
+        for (int e = t; e < w * w; e += LU_T) {
+            const int r = p + 1 + e / w, c = p + 1 + e % w;
+            a[(long)r * N + c] = sub_rn(a[(long)r * N + c], mul_rn(a[(long)r * N + p], a[(long)p * N + c]));
+        }
+        __syncthreads();
+    }
+    if (t == 0 && bad)
+        *flag = 1;
+    if (IN_LDS) {
+        for (long e = t; e < (long)N * N; e += LU_T)
+            gw[e] = a[e];
+        __syncthreads();
+    }
+
+    // ---- substitutions, kc columns at a time
+    for (int c0 = 0; c0 < k; c0 += kc) {
+        const int kw = (k - c0) < kc ? (k - c0) : kc;
+        const int G = LU_T / kw;
+        const int cl = t % kw, g = t / kw;
+        const bool own = g < G;
+        if (IN_LDS && mode == CACG_MODE_REFERENCE && c0 > 0) {
+            // the previous chunk's backward pass reused the factor's LDS
+            for (long e = t; e < (long)N * N; e += LU_T)
+                a[e] = gw[e];
+            __syncthreads();
+        }
+        double y[JMAX];
+#pragma unroll
+        for (int j = 0; j < JMAX; j++) {
+            const int i = g + G * j;
+            y[j] = (own && i < N) ? rhs[(long)i * k + c0 + cl] : 0.0;
+        }
+        // forward: L y = rhs
+        for (int q = 0; q < N; q++) {
+            double *rb = rowbuf + (q & 1) * kc;
+            if (own && q % G == g) {
+                const int jq = q / G;
+                const double inv = 1.0 / a[(long)q * N + q];
+#pragma unroll
+                for (int j = 0; j < JMAX; j++)
+                    if (j == jq) {
+                        y[j] = mul_rn(y[j], inv);
+                        rb[cl] = y[j];
+                    }
+            }
+            __syncthreads();
+            if (own) {
+                const double yq = rb[cl];
+#pragma unroll
+                for (int j = 0; j < JMAX; j++) {
+                    const int i = g + G * j;
+                    if (i > q && i < N)
+                        y[j] = sub_rn(y[j], mul_rn(yq, a[(long)i * N + q]));
+                }
+            }
+        }
+        if (mode == CACG_MODE_REFERENCE) {
+            __syncthreads();  // all reads of the factor's LDS done before xs overwrites it
+#pragma unroll
+            for (int j = 0; j < JMAX; j++) {
+                const int i = g + G * j;
+                if (own && i < N)
+                    xs[(long)i * kw + cl] = y[j];
+            }
+            __syncthreads();
+            if (t < kw) {
+                for (int i = N - 2; i >= 0; i--) {
+                    double xi = xs[(long)i * kw + t];
+                    const double *u = gw + (long)i * N;
+                    int q = i + 1;
+                    for (; q + 4 <= N; q += 4) {
+                        const double u0 = u[q], u1 = u[q + 1], u2 = u[q + 2], u3 = u[q + 3];
+                        xi = sub_rn(xi, mul_rn(xs[(long)q * kw + t], u0));
+                        xi = sub_rn(xi, mul_rn(xs[(long)(q + 1) * kw + t], u1));
+                        xi = sub_rn(xi, mul_rn(xs[(long)(q + 2) * kw + t], u2));
+                        xi = sub_rn(xi, mul_rn(xs[(long)(q + 3) * kw + t], u3));
+                    }
+                    for (; q < N; q++)
+                        xi = sub_rn(xi, mul_rn(xs[(long)q * kw + t], u[q]));
+                    xs[(long)i * kw + t] = xi;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < JMAX; j++) {
+                const int i = g + G * j;
+                if (own && i < N)
+                    x[(long)i * k + c0 + cl] = xs[(long)i * kw + cl];
+            }
+            __syncthreads();
+        } else {
+            __syncthreads();  // the forward pass's last broadcast row is read before reuse
+            for (int q = N - 1; q >= 0; q--) {
+                double *rb = rowbuf + (q & 1) * kc;
+                if (own && q % G == g) {
+                    const int jq = q / G;
+#pragma unroll
+                    for (int j = 0; j < JMAX; j++)
+                        if (j == jq)
+                            rb[cl] = y[j];
+                }
+                __syncthreads();
+                if (own) {
+                    const double xq = rb[cl];
+#pragma unroll
+                    for (int j = 0; j < JMAX; j++) {
+                        const int i = g + G * j;
+                        if (i < q)
+                            y[j] = sub_rn(y[j], mul_rn(xq, a[(long)i * N + q]));
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < JMAX; j++) {
+                const int i = g + G * j;
+                if (own && i < N)
+                    x[(long)i * k + c0 + cl] = y[j];
+            }
+            __syncthreads();
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" int cacg_k_lu_solve(int N, const double *lhs, int k, const double *rhs, double *x,
+                               double *d_work, int *d_flag, int mode, hipStream_t st)
+{
+    if (N <= 0 || k <= 0)
+        return (int)hipErrorInvalidValue;
+    const bool in_lds = (long)N * N <= 128L * 128L;
+    // columns per pass: the register block (JMAX rows per lane) must cover N,
+    // and without the LDS factor the reference pass keeps N x kc in LDS
+    int kc = k;
+    if ((long)N * kc > (long)LU_T * JMAX)
+        kc = (int)((long)LU_T * JMAX / N);
+    if (!in_lds && (long)N * kc > 8192)
+        kc = 8192 / N;
+    if (kc < 1 || (LU_T / kc) * JMAX < N)
+        return (int)hipErrorInvalidValue;  // N > 16384 is out of range
+    size_t shm;
+    if (in_lds)
+        shm = ((size_t)N * N + 2 * (size_t)kc) * sizeof(double);
+    else
+        shm = ((size_t)N * kc + 2 * (size_t)kc) * sizeof(double);
+    cacg_allow_lds(in_lds ? lu_solve_kernel<true> : lu_solve_kernel<false>, shm);
+    if (in_lds)
+        hipLaunchKernelGGL(lu_solve_kernel<true>, dim3(1), dim3(LU_T), shm, st, N, lhs, k, rhs, x,
+                           d_work, d_flag, mode, kc);
+    else
+        hipLaunchKernelGGL(lu_solve_kernel<false>, dim3(1), dim3(LU_T), shm, st, N, lhs, k, rhs, x,
+                           d_work, d_flag, mode, kc);
+    CACG_LAUNCH_CHECK();
+    return 0;
+}

@@ -1,0 +1,158 @@
+/*
+ * cacg.h — C-ABI of the MI355X (gfx950) s-step Chebyshev-basis CG library.
+ *
+ * Drop-in boundary: the two solver entry points of sclc/cacg, with the
+ * reference's own argument types and meanings, plus an extended API with
+ * status codes, options and a per-iteration report.  All pointers are plain
+ * host pointers unless a name says `d_` (device memory, HBM).  All dense
+ * blocks are row-major: B, X and R are n x m, element (i, c) at i*m + c
+ * (src/GetRHS.c:9 of the reference).
+ */
+#ifndef CACG_H
+#define CACG_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- reference types, byte-identical to src/DataTypes.h:14-61 ----------
+ * Guarded by the reference header's own include guard, so a translation
+ * unit that includes both sees one definition. */
+#ifndef DATATYPES_H
+#define DATATYPES_H
+typedef struct {
+    long num_rows;
+    long num_cols;
+    long nnz;
+} matInfo;
+
+typedef struct {
+    long num_rows, num_cols, nnz; /* local rows, global columns, local nonzeros */
+    long start;                   /* global id of the first local row */
+    long *row_start;              /* num_rows + 1 offsets into col_idx / csrdata */
+    long *col_idx;                /* global column ids, in the caller's order */
+    double *csrdata;
+} csrType_local;
+
+typedef struct {
+    long local_num_row;
+    long local_num_col;
+    long global_num_row;
+    long global_num_col;
+    long start_idx;
+    double *data;
+} denseType;
+#endif /* DATATYPES_H */
+
+/* ---- drop-in entry points ------------------------------------------------
+ * Replace src/cbcgKernel.h:21 and src/bcbcgKernel.h:22.  Same semantics as
+ * the reference at one rank: X (n x m, row-major) is read as the initial
+ * guess and overwritten with the solution; B and mat are not modified; the
+ * solve stops when ||R(:,0)||_2 < epsilon (absolute, column 0 of the
+ * recursively updated residual, src/bcbcgKernel.c:445-447, 524-526) or after
+ * 100 (bcbcg_v1) / 1000 (cbcg_v1) outer iterations.  The Chebyshev interval
+ * comes from Gerschgorin discs (src/common.c:53-106).
+ *
+ * The whole matrix must be local (myid == 0, numprocs == 1, mat.start == 0,
+ * mat.num_rows == B.global_num_row): the multi-rank row-block distribution of
+ * the reference becomes one matrix resident in one GPU's HBM.  Other values
+ * are rejected with CACG_EINVAL.  The functions return nothing, as in the
+ * reference; the outcome of the last call on the calling thread is
+ * available from cacg_last_status() / cacg_last_iterations().  They never
+ * call exit().  Arithmetic is FP64 in the reference's summation order
+ * (CACG_ORDER_REFERENCE) unless the environment variable CACG_ORDER=fast
+ * selects the split-K MFMA order. */
+void cbcg_v1(csrType_local mat, denseType B, denseType X, long s, double epsilon, int myid,
+             int numprocs);
+void bcbcg_v1(csrType_local mat, denseType B, denseType X, long s, double epsilon, int myid,
+              int numprocs);
+
+/* ---- status codes ------------------------------------------------------- */
+enum {
+    CACG_OK = 0,          /* stop criterion met */
+    CACG_MAXITER = 1,     /* iteration cap reached (the reference returns silently, src/bcbcgKernel.c:172) */
+    CACG_BREAKDOWN = 2,   /* non-finite residual or zero / non-finite pivot in the coefficient solve */
+    CACG_EINVAL = 3,      /* bad argument */
+    CACG_ENOMEM = 4,      /* device allocation failed */
+    CACG_EDEVICE = 5      /* HIP runtime error or no usable gfx950 device */
+};
+
+int cacg_last_status(void);
+long cacg_last_iterations(void);
+const char *cacg_status_string(int status);
+
+/* ---- extended API ------------------------------------------------------- */
+enum { CACG_STOP_REF_COL0_ABS = 0, CACG_STOP_REL_PER_COLUMN = 1 };
+/* CACG_ORDER_REFERENCE: every sum in the reference's order, products and sums
+ * rounded separately (no FMA); results equal the reference at one rank bit
+ * for bit.  The Gram sums then run as one sequential chain per entry, so this
+ * order is for verification sizes.  CACG_ORDER_FAST: split-K FP64 MFMA Gram
+ * and block products with a fixed (deterministic) reduction order. */
+enum { CACG_ORDER_REFERENCE = 0, CACG_ORDER_FAST = 1 };
+enum { CACG_PREC_FP64 = 0 };
+
+typedef struct {
+    int stop_mode;       /* CACG_STOP_* */
+    double eps;          /* absolute threshold on ||R(:,0)||_2, CACG_STOP_REF_COL0_ABS */
+    double tol;          /* every column ||R(:,j)||_2 < tol * ||B(:,j)||_2, CACG_STOP_REL_PER_COLUMN */
+    long maxiter;        /* outer-iteration cap, > 0 */
+    int order;           /* CACG_ORDER_* */
+    int precision;       /* CACG_PREC_* */
+    int print_period;    /* > 0: print the reference's "Loop: %d, R[1]_norm=%30.30f" line every
+                            print_period iterations, on stop and on the last iteration */
+} cacg_opts;
+
+typedef struct {
+    int status;          /* CACG_* */
+    long iters;          /* outer iterations performed */
+    double bounds[2];    /* Gerschgorin lower, upper */
+    double *res_hist;    /* optional, caller-owned [res_hist_cap][m]: ||R(:,j)||_2 after each iteration */
+    long res_hist_cap;
+    double seconds;      /* device time of the solve (setup + iterations) */
+    double phase_ms[8];  /* device time per phase, see CACG_PHASE_* */
+    long phase_calls[8];
+} cacg_report;
+
+enum {
+    CACG_PHASE_SETUP = 0,    /* Gerschgorin + R = B - A X */
+    CACG_PHASE_BASIS = 1,    /* Chebyshev basis SpMMs (s - 1 per iteration) */
+    CACG_PHASE_GRAM = 2,     /* Q^T A S, Q^T A Q, Q^T R */
+    CACG_PHASE_SOLVE = 3,    /* small LU solves */
+    CACG_PHASE_QUPDATE = 4,  /* Q = S - Q beta */
+    CACG_PHASE_AQ = 5,       /* A Q SpMM */
+    CACG_PHASE_XRUPDATE = 6, /* X += Q alpha, R -= A Q alpha, column norms */
+    CACG_PHASE_OTHER = 7
+};
+
+void cacg_default_opts(cacg_opts *o);
+
+/* Host-buffer solve: A from a csrType_local-like triple (row_start/col_idx of
+ * n + 1 / nnz longs, values), B and X n x m row-major; X in/out.  s >= 1. */
+int cacg_bcbcg(long n, const long *row_start, const long *col_idx, const double *val,
+               long m, long s, const double *B, double *X, const cacg_opts *opts,
+               cacg_report *report);
+
+/* Device-resident solver: the matrix is uploaded once (int32 indices when
+ * nnz < 2^31) and the work blocks are allocated once for (m, s).  solve_device
+ * takes device pointers to B and X (n x m row-major, X in/out) and runs on the
+ * solver's own HIP stream. */
+typedef struct cacg_solver cacg_solver;
+int cacg_solver_create(cacg_solver **out, long n, const long *row_start, const long *col_idx,
+                       const double *val, long m, long s, const cacg_opts *opts);
+int cacg_solver_solve_device(cacg_solver *sv, const double *d_B, double *d_X, cacg_report *report);
+int cacg_solver_destroy(cacg_solver *sv);
+/* bytes of HBM the solver holds (matrix + work blocks) */
+long cacg_solver_footprint(const cacg_solver *sv);
+
+/* ---- device memory helpers (for callers without their own HIP plumbing) -- */
+int cacg_device_count(void);
+int cacg_dev_alloc(void **d_ptr, long bytes);
+int cacg_dev_free(void *d_ptr);
+int cacg_h2d(void *d_dst, const void *src, long bytes);
+int cacg_d2h(void *dst, const void *d_src, long bytes);
+int cacg_dev_sync(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CACG_H */

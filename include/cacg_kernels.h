@@ -1,0 +1,93 @@
+/*
+ * cacg_kernels.h — kernel-level C-ABI: the launchers the host driver calls, one per
+ * hot-path kernel, exported for tests and benchmarks.
+ *
+ * Every function enqueues work on `st` and returns 0 or a hipError_t value.
+ * Pointers named d_* and every matrix/block argument are device pointers.
+ * Dense blocks are row-major with an explicit row stride (ld, in elements),
+ * so a column block of a wider array (S_j inside S = [S_0 | ... | S_{s-1}])
+ * is addressed as (base + j*m, ld = s*m).
+ */
+#ifndef CACG_KERNELS_H
+#define CACG_KERNELS_H
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* device CSR, int32 row pointers and column ids (nnz < 2^31) */
+typedef struct {
+    int n;
+    const int *rowptr;
+    const int *col;
+    const double *val;
+} cacg_dcsr;
+
+enum { CACG_MODE_REFERENCE = 0, CACG_MODE_FAST = 1 };
+
+/* Gerschgorin interval (src/common.c:53-106): d_out[0] = min_i (a_ii - sum|a_ij|),
+ * d_out[1] = max_i (a_ii + sum|a_ij|), the max seeded with DBL_MIN.
+ * d_work holds 2 * CACG_GERSH_BLOCKS doubles. */
+#define CACG_GERSH_BLOCKS 1024
+int cacg_k_gerschgorin(const cacg_dcsr *A, double *d_work, double *d_out, hipStream_t st);
+
+/* y(:, 0:k) = epilogue(A x(:, 0:k)), every sum in CSR order with separately
+ * rounded products and sums (src/BLAS3.c:78-112):
+ *   CACG_EPI_NONE : y = A x
+ *   CACG_EPI_RESID: y = p1 - A x                        (R = B - A X)
+ *   CACG_EPI_CHEB1: y = a (A x) + b x, and copy0 = x    (S_1 and S_0 = R)
+ *   CACG_EPI_CHEB : y = (a (A x) + b p1) - p2            (S_j, p1 = S_{j-1}, p2 = S_{j-2}) */
+enum { CACG_EPI_NONE = 0, CACG_EPI_RESID = 1, CACG_EPI_CHEB1 = 2, CACG_EPI_CHEB = 3 };
+int cacg_k_spmm(const cacg_dcsr *A, int k, const double *x, long ldx, double *y, long ldy, int epi,
+                double a, double b, const double *p1, long ld1, const double *p2, long ld2,
+                double *copy0, long ldc, hipStream_t st);
+
+/* [G1 | G2] = L^T [R1 | R2] over rows 0..n-1: G1 is kl x kr1 and G2 is kl x kr2,
+ * both row-major and contiguous (G2 may be NULL when kr2 == 0).
+ * Reference order: each entry is one sequential chain over the rows, products
+ * and sums rounded separately (src/BLAS3.c:709-717, 741-749).
+ * Fast order: split-K over row chunks with FP64 MFMA, then a fixed-order sum of
+ * the chunk partials; d_work holds cacg_gram_workspace(kl, kr1 + kr2) doubles. */
+int cacg_k_gram_ref(int n, const double *L, long ldl, int kl, const double *R1, long ld1, int kr1,
+                    const double *R2, long ld2, int kr2, double *G1, double *G2, hipStream_t st);
+long cacg_gram_workspace(int kl, int kr);
+int cacg_k_gram_fast(int n, const double *L, long ldl, int kl, const double *R1, long ld1, int kr1,
+                     const double *R2, long ld2, int kr2, double *G1, double *G2, double *d_work,
+                     hipStream_t st);
+
+/* x (N x k) = lhs^{-1} rhs with the reference's Crout factorisation (unit-diagonal
+ * U, no pivoting) and substitutions (src/DLSsolvers.c:106-208).  lhs is not
+ * modified.  mode REFERENCE keeps the reference's ascending backward-substitution
+ * order (bitwise equal); mode FAST runs the backward substitution column-oriented.
+ * d_work holds N*N doubles; *d_flag is set to 1 on a zero or non-finite pivot. */
+int cacg_k_lu_solve(int N, const double *lhs, int k, const double *rhs, double *x, double *d_work,
+                    int *d_flag, int mode, hipStream_t st);
+
+/* Q = S - Q beta, in place on Q (n x sm, row-major, ld = sm), beta sm x sm
+ * (src/BLAS3.c:794-825 then 579-612). */
+int cacg_k_qupdate(int n, int sm, const double *S, double *Q, const double *beta, int mode,
+                   hipStream_t st);
+
+/* X += Q alpha; R -= AQ alpha (Q, AQ n x sm; alpha sm x m; X, R n x m) and
+ * d_norm2[c] = sum_i R[i][c]^2 of the updated R.  Reference mode keeps the
+ * reference's orders (the norms as one sequential chain per column,
+ * src/MatrixOperations.c:110-123); fast mode uses MFMA products and a
+ * fixed-order two-level norm reduction.  d_work: cacg_xrupdate_workspace(m). */
+long cacg_xrupdate_workspace(int m);
+int cacg_k_xrupdate(int n, int sm, int m, const double *Q, const double *AQ, const double *alpha,
+                    double *X, double *R, double *d_norm2, double *d_work, int mode,
+                    hipStream_t st);
+
+/* d_norm2[c] = sum_i R[i][c]^2, one sequential chain per column (reference order) */
+int cacg_k_colnorm2_ref(int n, int m, const double *R, long ldr, double *d_norm2, hipStream_t st);
+
+/* d[0:count] = v, as a kernel (safe inside captured graphs, unlike a memset node) */
+int cacg_k_fill_i32(int *d, long count, int v, hipStream_t st);
+int cacg_k_fill_f64(double *d, long count, double v, hipStream_t st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

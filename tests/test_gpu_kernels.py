@@ -1,0 +1,175 @@
+"""Kernel-level parity of the gfx950 HIP kernels with the CPU oracle.
+
+Reference-order kernels must equal the oracle bit for bit.  Fast-order (MFMA)
+kernels are checked exactly on integer-valued data, where every partial sum
+is an integer below 2^53 so any summation order gives the exact result (a
+wrong lane layout, index or transposed operand cannot pass), and within a
+rounding bound on random data:
+    |G - L^T R| <= 2 n eps (|L|^T |R|)   (eps = 2^-53)."""
+
+import numpy as np
+import pytest
+
+from cacg_amd import problems as P
+
+pytestmark = pytest.mark.gpu
+EPS = 2.0 ** -53
+
+
+def _arrow(L=10, w=-0.1):
+    """SPD matrix with one full row/column (ragged rows: 1 row of n entries)."""
+    A = P.laplacian_2d(L)
+    D = A.to_dense()
+    D[0, 1:] += w
+    D[1:, 0] += w
+    np.fill_diagonal(D, 0.0)
+    np.fill_diagonal(D, 1.01 * np.abs(D).sum(axis=1))
+    rows, cols = np.nonzero(D)
+    rp = np.zeros(A.n + 1, dtype=np.int64)
+    np.add.at(rp, rows + 1, 1)
+    return P.CSR(A.n, np.cumsum(rp), cols.astype(np.int64), D[rows, cols])
+
+
+MATRICES = {
+    "lap2d_16": lambda: P.laplacian_2d(16),
+    "lap3d_7": lambda: P.laplacian_3d(7),
+    "spd_13": lambda: P.random_spd_like(P.laplacian_2d(13), seed=2),
+    "arrow_10": _arrow,
+}
+
+
+@pytest.mark.parametrize("mat", sorted(MATRICES))
+def test_gerschgorin_bitwise(gpu_lib, oracle_lib, mat):
+    A = MATRICES[mat]()
+    assert np.array_equal(gpu_lib.k_gerschgorin(A), oracle_lib.gerschgorin(A))
+
+
+@pytest.mark.parametrize("mat", sorted(MATRICES))
+@pytest.mark.parametrize("k", [1, 2, 3, 5, 8, 16, 17, 33, 64, 128, 144])
+def test_spmm_bitwise(gpu_lib, oracle_lib, mat, k):
+    A = MATRICES[mat]()
+    X = P.rhs_normal(A.n, k, 11)
+    assert np.array_equal(gpu_lib.k_spmm(A, X), oracle_lib.spmm(A, X))
+
+
+@pytest.mark.parametrize("m", [1, 4, 16])
+def test_spmm_epilogues_bitwise(gpu_lib, oracle_lib, m):
+    A = P.random_spd_like(P.laplacian_2d(12), seed=3)
+    R = P.rhs_uniform(A.n, m, 12)
+    lo, hi = oracle_lib.gerschgorin(A)
+    sa, sb = 2.0 / (hi - lo), -(hi + lo) / (hi - lo)
+    S = oracle_lib.cheb_basis(A, R, 3, sa, sb)
+    S1, S0 = gpu_lib.k_spmm(A, R, "cheb1", sa, sb, want_copy0=True)
+    assert np.array_equal(S0, S[:, :m])
+    assert np.array_equal(S1, S[:, m:2 * m])
+    S2 = gpu_lib.k_spmm(A, S1, "cheb", 2 * sa, 2 * sb, p1=S1, p2=S0)
+    assert np.array_equal(S2, S[:, 2 * m:3 * m])
+    X = P.rhs_normal(A.n, m, 13)
+    resid = gpu_lib.k_spmm(A, X, "resid", p1=R)
+    assert np.array_equal(resid, R - oracle_lib.spmm(A, X))
+
+
+GRAM_SHAPES = [(1000, 4, 4, 1), (1000, 9, 9, 3), (3000, 16, 16, 4), (2048, 64, 64, 16),
+               (5000, 128, 128, 16), (777, 128, 128, 0), (1500, 256, 256, 32), (0, 16, 16, 4)]
+
+
+@pytest.mark.parametrize("n,kl,kr1,kr2", GRAM_SHAPES)
+def test_gram_reference_bitwise(gpu_lib, oracle_lib, n, kl, kr1, kr2):
+    rng = np.random.default_rng(n + kl)
+    L, R1, R2 = rng.standard_normal((n, kl)), rng.standard_normal((n, kr1)), rng.standard_normal((n, kr2))
+    out = gpu_lib.k_gram(L, R1, R2 if kr2 else None, mode="reference")
+    G1, G2 = out if kr2 else (out, None)
+    assert np.array_equal(G1, oracle_lib.gram(L, R1))
+    if kr2:
+        assert np.array_equal(G2, oracle_lib.gram(L, R2))
+
+
+@pytest.mark.parametrize("n,kl,kr1,kr2", GRAM_SHAPES)
+def test_gram_fast_exact_on_integers(gpu_lib, n, kl, kr1, kr2):
+    rng = np.random.default_rng(n + kl + 1)
+    L = rng.integers(-8, 9, size=(n, kl)).astype(np.float64)
+    R1 = rng.integers(-8, 9, size=(n, kr1)).astype(np.float64)
+    R2 = rng.integers(-8, 9, size=(n, kr2)).astype(np.float64)
+    out = gpu_lib.k_gram(L, R1, R2 if kr2 else None, mode="fast")
+    G1, G2 = out if kr2 else (out, None)
+    Li = L.astype(np.int64)
+    assert np.array_equal(G1, (Li.T @ R1.astype(np.int64)).astype(np.float64))
+    if kr2:
+        assert np.array_equal(G2, (Li.T @ R2.astype(np.int64)).astype(np.float64))
+
+
+@pytest.mark.parametrize("n,kl,kr1,kr2", GRAM_SHAPES[:6])
+def test_gram_fast_rounding_bound(gpu_lib, n, kl, kr1, kr2):
+    rng = np.random.default_rng(n + kl + 2)
+    L, R1 = rng.standard_normal((n, kl)), rng.standard_normal((n, kr1))
+    G = gpu_lib.k_gram(L, R1, None, mode="fast")
+    exact = (L.astype(np.longdouble).T @ R1.astype(np.longdouble)).astype(np.float64)
+    bound = 2 * max(n, 1) * EPS * (np.abs(L).T @ np.abs(R1)) + 1e-300
+    assert np.all(np.abs(G - exact) <= bound)
+
+
+def _spd(N, seed):
+    rng = np.random.default_rng(seed)
+    M = rng.standard_normal((N, N))
+    return M @ M.T + N * np.eye(N)
+
+
+@pytest.mark.parametrize("N,k", [(1, 1), (4, 1), (4, 4), (9, 3), (16, 16), (64, 16), (128, 16),
+                                 (128, 128), (160, 32), (256, 32)])
+def test_lu_solve(gpu_lib, oracle_lib, N, k):
+    A = _spd(N, N + k)
+    rhs = np.random.default_rng(k).standard_normal((N, k))
+    ref = oracle_lib.lu_solve(A, rhs)
+    x, flag = gpu_lib.k_lu_solve(A, rhs, mode="reference")
+    assert flag == 0
+    assert np.array_equal(x, ref)
+    xf, flag = gpu_lib.k_lu_solve(A, rhs, mode="fast")
+    assert flag == 0
+    assert np.allclose(xf, ref, rtol=1e-12, atol=1e-14)
+
+
+def test_lu_zero_pivot_sets_flag(gpu_lib):
+    A = _spd(8, 1)
+    A[0, 0] = 0.0
+    _, flag = gpu_lib.k_lu_solve(A, np.ones((8, 2)), mode="reference")
+    assert flag == 1
+
+
+@pytest.mark.parametrize("n,sm", [(100, 4), (333, 9), (1000, 16), (999, 48), (2000, 128), (50, 256)])
+def test_qupdate(gpu_lib, oracle_lib, n, sm):
+    rng = np.random.default_rng(sm)
+    S, Q, beta = rng.standard_normal((n, sm)), rng.standard_normal((n, sm)), rng.standard_normal((sm, sm))
+    ref = S - oracle_lib.gemm_ts(Q, beta)
+    assert np.array_equal(gpu_lib.k_qupdate(S, Q, beta, mode="reference"), ref)
+    assert np.allclose(gpu_lib.k_qupdate(S, Q, beta, mode="fast"), ref, rtol=0, atol=1e-12 * sm)
+    Si = rng.integers(-8, 9, size=(n, sm)).astype(np.float64)
+    Qi = rng.integers(-8, 9, size=(n, sm)).astype(np.float64)
+    bi = rng.integers(-8, 9, size=(sm, sm)).astype(np.float64)
+    exact = (Si.astype(np.int64) - Qi.astype(np.int64) @ bi.astype(np.int64)).astype(np.float64)
+    assert np.array_equal(gpu_lib.k_qupdate(Si, Qi, bi, mode="fast"), exact)
+
+
+@pytest.mark.parametrize("n,sm,m", [(100, 4, 1), (333, 9, 3), (1000, 16, 4), (1001, 64, 16),
+                                    (2003, 128, 16), (700, 128, 32), (300, 96, 24)])
+def test_xrupdate(gpu_lib, oracle_lib, n, sm, m):
+    rng = np.random.default_rng(sm + m)
+    Q, AQ = rng.standard_normal((n, sm)), rng.standard_normal((n, sm))
+    alpha, X, R = rng.standard_normal((sm, m)), rng.standard_normal((n, m)), rng.standard_normal((n, m))
+    Xr = X + oracle_lib.gemm_ts(Q, alpha)
+    Rr = R - oracle_lib.gemm_ts(AQ, alpha)
+    nr = np.array([oracle_lib.col_norm2sq(Rr, c) for c in range(m)])
+    Xg, Rg, ng = gpu_lib.k_xrupdate(Q, AQ, alpha, X, R, mode="reference")
+    assert np.array_equal(Xg, Xr) and np.array_equal(Rg, Rr) and np.array_equal(ng, nr)
+    Xf, Rf, nf = gpu_lib.k_xrupdate(Q, AQ, alpha, X, R, mode="fast")
+    assert np.allclose(Xf, Xr, rtol=0, atol=1e-12 * sm)
+    assert np.allclose(Rf, Rr, rtol=0, atol=1e-12 * sm)
+    assert np.allclose(nf, nr, rtol=1e-12)
+    # integers: exact in any order
+    Qi, AQi = (rng.integers(-4, 5, size=(n, sm)).astype(np.float64) for _ in range(2))
+    ai = rng.integers(-4, 5, size=(sm, m)).astype(np.float64)
+    Xi, Ri = (rng.integers(-4, 5, size=(n, m)).astype(np.float64) for _ in range(2))
+    Xe = Xi + (Qi.astype(np.int64) @ ai.astype(np.int64))
+    Re = Ri - (AQi.astype(np.int64) @ ai.astype(np.int64))
+    Xf, Rf, nf = gpu_lib.k_xrupdate(Qi, AQi, ai, Xi, Ri, mode="fast")
+    assert np.array_equal(Xf, Xe) and np.array_equal(Rf, Re)
+    assert np.array_equal(nf, (Re.astype(np.int64) ** 2).sum(axis=0).astype(np.float64))
