@@ -169,7 +169,7 @@ __global__ __launch_bounds__(512) void qupdate_mfma_kernel(int n, const double *
             double sv[4];
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const long i = i0 + lg * 4 + r;
+                const long i = i0 + lg + 4 * r;
                 sv[r] = i < n ? S[i * SM + j * 16 + lc] : 0.0;
             }
             dbl4 acc = {0.0, 0.0, 0.0, 0.0};
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(512) void qupdate_mfma_kernel(int n, const double *
                 acc = mfma16x16x4(av[tt], bcol[tt * SM], acc);
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const long i = i0 + lg * 4 + r;
+                const long i = i0 + lg + 4 * r;
                 if (i < n)
                     Q[i * SM + j * 16 + lc] = sv[r] - acc[r];
             }
@@ -254,7 +254,7 @@ __global__ __launch_bounds__(256) void xrupdate_mfma_kernel(int n, int m, const 
             if (c < m) {
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    const long i = i0 + lg * 4 + r;
+                    const long i = i0 + lg + 4 * r;
                     if (i < n) {
                         X[i * m + c] += ax[r];
                         const double rn = R[i * m + c] - ar[r];

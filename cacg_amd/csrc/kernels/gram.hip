@@ -241,7 +241,7 @@ __global__ __launch_bounds__(256) void gram_mfma_kernel(int n, Src s, double *__
                 continue;
 #pragma unroll
             for (int r = 0; r < 4; r++)
-                out[(long)(tm * 16 + lr * 4 + r) * kr_pad + tn * 16 + lc] = acc[i][j][r];
+                out[(long)(tm * 16 + lr + 4 * r) * kr_pad + tn * 16 + lc] = acc[i][j][r];
         }
     }
 }

@@ -9,7 +9,9 @@
 #define CACG_WAVE 64
 
 // Fragment type of the FP64 MFMA accumulator (v_mfma_f64_16x16x4f64):
-// lane l holds D[4*(l/16) + r][l%16], r = 0..3.
+// lane l holds D[l/16 + 4*r][l%16], r = 0..3 (measured on gfx950: the four
+// lane groups interleave rows).  Operands: lane l supplies A[l%16][l/16] and
+// B[l/16][l%16].
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 // Products and sums that must round separately (the reference is compiled for

@@ -187,7 +187,7 @@ extern "C" int cacg_k_lu_solve(int N, const double *lhs, int k, const double *rh
     const bool in_lds = (long)N * N <= 128L * 128L;
     // columns per pass: the register block (JMAX rows per lane) must cover N,
     // and without the LDS factor the reference pass keeps N x kc in LDS
-    int kc = k;
+    int kc = k < N ? k : N;  // the reference backward pass reuses the N x N factor LDS for N x kc
     if ((long)N * kc > (long)LU_T * JMAX)
         kc = (int)((long)LU_T * JMAX / N);
     if (!in_lds && (long)N * kc > 8192)

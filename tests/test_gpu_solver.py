@@ -169,11 +169,13 @@ def test_breakdown_on_duplicate_columns(gpu_lib, order):
     assert out["status"] == "breakdown"
 
 
-def test_one_by_one_system(gpu_lib):
+@pytest.mark.parametrize("order", ["reference", "fast"])
+def test_one_by_one_system(gpu_lib, order):
     A = P.laplacian_2d(1)           # [4]
-    B = np.array([[2.0, -1.0]])
+    B = np.array([[2.0]])
     out = gpu_lib.bcbcg(A, B, np.zeros_like(B), 1, stop_mode=0, eps=1e-300, maxiter=3,
-                        order="reference")
+                        order=order)
+    assert out["status"] == "ok" and out["iters"] == 1
     assert np.array_equal(out["X"], B / 4.0)
 
 
