@@ -47,7 +47,25 @@ def parse():
     ap.add_argument("--order", default="fast", choices=["fast", "reference"])
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cpu count)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run 3 device-to-device copies of 2 GiB (HBM counter calibration)")
     return ap.parse_args()
+
+
+def copy_probe():
+    """Three plain device-to-device copies of 2 GiB: each reads and writes
+    2147483648 bytes, the yardstick for FETCH_SIZE / WRITE_SIZE readings."""
+    import cacg_amd
+    L = cacg_amd.lib()
+    nbytes = 2 << 30
+    src, dst = cacg_amd.DeviceBuffer(nbytes), cacg_amd.DeviceBuffer(nbytes)
+    L.cacg_k_fill_f64(src.ptr, nbytes // 8, 1.0, None)
+    L.cacg_dev_sync()
+    for _ in range(3):
+        if L.cacg_d2d(dst.ptr, src.ptr, nbytes) != 0:
+            raise RuntimeError("cacg_d2d failed")
+    L.cacg_dev_sync()
+    print(json.dumps({"copy_probe_bytes": nbytes, "copies": 3}))
 
 
 def cpu_baseline(A, B, s, iters_gpu, threads):
@@ -70,6 +88,9 @@ def cpu_baseline(A, B, s, iters_gpu, threads):
 
 def main():
     args = parse()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               f"(multi-GPU sharding is a design, not a measured path)", file=sys.stderr)

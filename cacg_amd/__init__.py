@@ -80,6 +80,7 @@ def lib() -> ctypes.CDLL:
         "cacg_dev_free": ([_vp], c_int),
         "cacg_h2d": ([_vp, _vp, c_long], c_int),
         "cacg_d2h": ([_vp, _vp, c_long], c_int),
+        "cacg_d2d": ([_vp, _vp, c_long], c_int),
         "cacg_dev_sync": ([], c_int),
         "cacg_k_gerschgorin": ([ctypes.POINTER(DCsr), _vp, _vp, _vp], c_int),
         "cacg_k_spmm": ([ctypes.POINTER(DCsr), c_int, _vp, c_long, _vp, c_long, c_int, c_double,
@@ -89,7 +90,7 @@ def lib() -> ctypes.CDLL:
         "cacg_gram_workspace": ([c_int, c_int], c_long),
         "cacg_k_gram_fast": ([c_int, _vp, c_long, c_int, _vp, c_long, c_int, _vp, c_long, c_int,
                               _vp, _vp, _vp, _vp], c_int),
-        "cacg_k_lu_solve": ([c_int, _vp, c_int, _vp, _vp, _vp, _vp, c_int, _vp], c_int),
+        "cacg_k_lu_solve": ([c_int, _vp, c_int, _vp, _vp, _vp, _vp, c_int, c_int, _vp], c_int),
         "cacg_k_qupdate": ([c_int, c_int, _vp, _vp, _vp, c_int, _vp], c_int),
         "cacg_xrupdate_workspace": ([c_int], c_long),
         "cacg_k_xrupdate": ([c_int, c_int, c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, c_int, _vp],
@@ -214,16 +215,26 @@ def k_gram(L, R1, R2=None, mode="reference"):
     return G1, g2.download((kl, kr2))
 
 
-def k_lu_solve(lhs, rhs, mode="reference"):
+def k_lu_solve(lhs, rhs, mode="reference", rhs2=None):
+    """x = lhs^{-1} rhs; with rhs2, a second call reuses the factor left by the
+    first and returns (x, x2, flag)."""
     lhs = np.ascontiguousarray(lhs, dtype=np.float64)
     rhs = np.ascontiguousarray(rhs, dtype=np.float64)
     N, k = rhs.shape
     dl, dr, dx = DeviceBuffer.from_array(lhs), DeviceBuffer.from_array(rhs), DeviceBuffer(rhs.nbytes)
     work = DeviceBuffer(8 * N * N)
     flag = DeviceBuffer.from_array(np.zeros(1, dtype=np.int32))
-    _check(lib().cacg_k_lu_solve(N, dl.ptr, k, dr.ptr, dx.ptr, work.ptr, flag.ptr, MODE[mode], None),
-           "lu_solve")
-    return dx.download((N, k)), int(flag.download(1, np.int32)[0])
+    _check(lib().cacg_k_lu_solve(N, dl.ptr, k, dr.ptr, dx.ptr, work.ptr, flag.ptr, MODE[mode], 0,
+                                 None), "lu_solve")
+    x = dx.download((N, k))
+    if rhs2 is None:
+        return x, int(flag.download(1, np.int32)[0])
+    rhs2 = np.ascontiguousarray(rhs2, dtype=np.float64)
+    k2 = rhs2.shape[1]
+    dr2, dx2 = DeviceBuffer.from_array(rhs2), DeviceBuffer(rhs2.nbytes)
+    _check(lib().cacg_k_lu_solve(N, dl.ptr, k2, dr2.ptr, dx2.ptr, work.ptr, flag.ptr, MODE[mode], 1,
+                                 None), "lu_solve")
+    return x, dx2.download((N, k2)), int(flag.download(1, np.int32)[0])
 
 
 def k_qupdate(S, Q, beta, mode="reference"):

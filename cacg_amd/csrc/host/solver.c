@@ -171,6 +171,15 @@ int cacg_d2h(void *dst, const void *d_src, long bytes)
                                                                                      : CACG_EDEVICE;
 }
 
+int cacg_d2d(void *d_dst, const void *d_src, long bytes)
+{
+    if (bytes <= 0)
+        return CACG_OK;
+    return hipMemcpy(d_dst, d_src, (size_t)bytes, hipMemcpyDeviceToDevice) == hipSuccess
+               ? CACG_OK
+               : CACG_EDEVICE;
+}
+
 int cacg_dev_sync(void) { return hipDeviceSynchronize() == hipSuccess ? CACG_OK : CACG_EDEVICE; }
 
 /* ---- solver lifetime ----------------------------------------------------------- */
@@ -453,8 +462,9 @@ int cacg_solver_solve_device(cacg_solver *sv, const double *d_B, double *d_X, ca
         } else {
             SCHK(gram(sv, mode, sv->d_AQ, S, sm, sm, NULL, 0, 0, sv->d_QtAS, NULL));
             mark(sv, CACG_PHASE_GRAM);
+            /* Q^T A Q of the previous iteration: its factor is still in d_lu_work */
             SCHK(cacg_k_lu_solve(sm, sv->d_QtAQ, sm, sv->d_QtAS, sv->d_beta, sv->d_lu_work,
-                                 sv->d_flag, mode, st));
+                                 sv->d_flag, mode, 1, st));
             mark(sv, CACG_PHASE_SOLVE);
             SCHK(cacg_k_qupdate(n, sm, S, Q, sv->d_beta, mode, st));
             mark(sv, CACG_PHASE_QUPDATE);
@@ -465,7 +475,7 @@ int cacg_solver_solve_device(cacg_solver *sv, const double *d_B, double *d_X, ca
         SCHK(gram(sv, mode, Q, sv->d_AQ, sm, sm, sv->d_R, m, m, sv->d_QtAQ, sv->d_QtR));
         mark(sv, CACG_PHASE_GRAM);
         SCHK(cacg_k_lu_solve(sm, sv->d_QtAQ, m, sv->d_QtR, sv->d_alpha, sv->d_lu_work, sv->d_flag,
-                             mode, st));
+                             mode, 0, st));
         mark(sv, CACG_PHASE_SOLVE);
         SCHK(cacg_k_xrupdate(n, sm, m, Q, sv->d_AQ, sv->d_alpha, d_X, sv->d_R, sv->d_norm2,
                              sv->d_xr_work, mode, st));

@@ -316,16 +316,24 @@ __global__ __launch_bounds__(256) void colnorm2_part_kernel(int n, int m, const 
     }
 }
 
-__global__ __launch_bounds__(64) void colnorm2_final_kernel(const double *__restrict__ part, int nb,
-                                                           int m, double *__restrict__ out)
+// one block per column: strided partial sums, then a fixed LDS tree
+__global__ __launch_bounds__(256) void colnorm2_final_kernel(const double *__restrict__ part, int nb,
+                                                            int m, double *__restrict__ out)
 {
-    const int c = blockIdx.x * 64 + threadIdx.x;
-    if (c >= m)
-        return;
+    __shared__ double red[256];
+    const int c = blockIdx.x, t = threadIdx.x;
     double acc = 0.0;
-    for (int b = 0; b < nb; b++)
+    for (int b = t; b < nb; b += 256)
         acc += part[(long)b * m + c];
-    out[c] = acc;
+    red[t] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w)
+            red[t] += red[t + w];
+        __syncthreads();
+    }
+    if (t == 0)
+        out[c] = red[0];
 }
 
 constexpr int XR_BLOCKS = 1024;  // fixed: the norm partials depend on it
@@ -353,7 +361,7 @@ int launch_xrupdate_mfma(int n, int m, const double *Q, const double *AQ, const 
     hipLaunchKernelGGL((xrupdate_mfma_kernel<SM, NTM>), dim3(XR_BLOCKS), dim3(256), 0, st, n, m, Q,
                        AQ, alpha, X, R, work);
     CACG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(colnorm2_final_kernel, dim3((m + 63) / 64), dim3(64), 0, st, work, XR_BLOCKS,
+    hipLaunchKernelGGL(colnorm2_final_kernel, dim3(m), dim3(256), 0, st, work, XR_BLOCKS,
                        m, d_norm2);
     CACG_LAUNCH_CHECK();
     return 0;
@@ -454,7 +462,7 @@ extern "C" int cacg_k_xrupdate(int n, int sm, int m, const double *Q, const doub
     const int nb = XR_BLOCKS;
     hipLaunchKernelGGL(colnorm2_part_kernel, dim3(nb), dim3(256), 0, st, n, m, R, d_work);
     CACG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(colnorm2_final_kernel, dim3((m + 63) / 64), dim3(64), 0, st, d_work, nb, m,
+    hipLaunchKernelGGL(colnorm2_final_kernel, dim3(m), dim3(256), 0, st, d_work, nb, m,
                        d_norm2);
     CACG_LAUNCH_CHECK();
     return 0;

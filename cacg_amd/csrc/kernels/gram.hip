@@ -246,25 +246,204 @@ __global__ __launch_bounds__(256) void gram_mfma_kernel(int n, Src s, double *__
     }
 }
 
+// 64 consecutive outputs per block; wave w sums chunk partials x in
+// [w*gx/4, (w+1)*gx/4) in order, then the four wave sums are added in wave order
 __global__ __launch_bounds__(256) void gram_reduce_kernel(const double *__restrict__ part, int gx,
                                                           int kl, int kr1, int kr2, int kl_pad,
                                                           int kr_pad, double *__restrict__ G1,
                                                           double *__restrict__ G2)
 {
+    __shared__ double red[4][64];
     const int kr = kr1 + kr2;
-    const long e = (long)blockIdx.x * 256 + threadIdx.x;
-    if (e >= (long)kl * kr)
-        return;
-    const int a = (int)(e / kr), b = (int)(e % kr);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const long e = (long)blockIdx.x * 64 + lane;
+    const bool ok = e < (long)kl * kr;
+    const int a = ok ? (int)(e / kr) : 0, b = ok ? (int)(e % kr) : 0;
     const long stride = (long)kl_pad * kr_pad;
     const double *p = part + (long)a * kr_pad + b;
-    double acc = p[0];
-    for (int x = 1; x < gx; x++)
-        acc += p[x * stride];
-    if (b < kr1)
-        G1[(long)a * kr1 + b] = acc;
-    else
-        G2[(long)a * kr2 + (b - kr1)] = acc;
+    const int q = (gx + 3) / 4;
+    const int x0 = wave * q, x1 = (x0 + q) < gx ? (x0 + q) : gx;
+    double acc = 0.0;
+    if (ok) {
+        int x = x0;
+        for (; x + 4 <= x1; x += 4) {
+            const double v0 = p[x * stride], v1 = p[(x + 1) * stride];
+            const double v2 = p[(x + 2) * stride], v3 = p[(x + 3) * stride];
+            acc += v0;
+            acc += v1;
+            acc += v2;
+            acc += v3;
+        }
+        for (; x < x1; x++)
+            acc += p[x * stride];
+    }
+    red[wave][lane] = acc;
+    __syncthreads();
+    if (wave == 0 && ok) {
+        const double sum = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+        if (b < kr1)
+            G1[(long)a * kr1 + b] = sum;
+        else
+            G2[(long)a * kr2 + (b - kr1)] = sum;
+    }
+}
+
+// LDS-staged variant for the large case (four tile groups per workgroup,
+// even column counts).  A stage is SR = 8 rows of L and of [R1 | R2]; the 256
+// lanes load it cooperatively with 16-byte loads, LDG stages ahead, holding
+// the data in registers; the stage is then written to one of two LDS buffers
+// and every wave reads its MFMA operands from there.  Each row is read from
+// HBM once per workgroup, and LDG * 17 KiB stay in flight per workgroup.
+constexpr int SR = 8;
+constexpr int LDG = 3;
+constexpr int VPT = 5;   // 16-byte vectors per lane per stage: SR * (kl + kr) / 2 <= 256 * VPT
+
+template <int MT, int NT>
+__global__ __launch_bounds__(256) void gram_lds_kernel(int n, Src s, double *__restrict__ part,
+                                                       int kl_pad, int kr_pad, int chunk, int GN,
+                                                       int NG)
+{
+    extern __shared__ double lds[];
+    const int kr = s.kr1 + s.kr2;
+    const int LDP = s.kl + 2, RDP = kr + 2;          // padded LDS row strides
+    const int BUF = SR * (LDP + RDP);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int lr = lane >> 4, lc = lane & 15;
+    const int g = blockIdx.y * 4 + wave;
+    const bool active = g < NG;
+    const int gm = active ? g / GN : 0, gn = active ? g % GN : 0;
+    const int ca0 = gm * MT * 16 + lc, cb0 = gn * NT * 16 + lc;
+
+    const int r0 = blockIdx.x * chunk;
+    int r1 = r0 + chunk;
+    if (r1 > n)
+        r1 = n;
+    const int nstage = (r1 - r0 + SR - 1) / SR;
+
+    // this lane's share of a stage: vector u -> (source row-0 pointer, ld, stage row, LDS offset)
+    const int vl = SR * s.kl / 2, v1 = SR * s.kr1 / 2, v2 = SR * s.kr2 / 2;
+    const double *src[VPT];
+    long ld[VPT];
+    int srow[VPT], loff[VPT];
+#pragma unroll
+    for (int u = 0; u < VPT; u++) {
+        int v = threadIdx.x + 256 * u;
+        srow[u] = -1;
+        src[u] = s.L;
+        ld[u] = 0;
+        loff[u] = 0;
+        if (v < vl) {
+            const int row = v / (s.kl / 2), c = 2 * (v % (s.kl / 2));
+            src[u] = s.L + c;
+            ld[u] = s.ldl;
+            srow[u] = row;
+            loff[u] = row * LDP + c;
+        } else if ((v -= vl) < v1) {
+            const int row = v / (s.kr1 / 2), c = 2 * (v % (s.kr1 / 2));
+            src[u] = s.R1 + c;
+            ld[u] = s.ld1;
+            srow[u] = row;
+            loff[u] = SR * LDP + row * RDP + c;
+        } else if ((v -= v1) < v2) {
+            const int row = v / (s.kr2 / 2), c = 2 * (v % (s.kr2 / 2));
+            src[u] = s.R2 + c;
+            ld[u] = s.ld2;
+            srow[u] = row;
+            loff[u] = SR * LDP + row * RDP + s.kr1 + c;
+        }
+    }
+    double2 stg[LDG][VPT];
+    auto gload = [&](int slot, int st) {
+        const int rb = r0 + st * SR;
+#pragma unroll
+        for (int u = 0; u < VPT; u++) {
+            const int row = rb + srow[u];
+            const bool ok = srow[u] >= 0 && st < nstage && row < r1;
+            stg[slot][u] = ok ? *reinterpret_cast<const double2 *>(src[u] + (long)row * ld[u])
+                              : double2{0.0, 0.0};
+        }
+    };
+    auto lstore = [&](int slot, int buf) {
+        double *b = lds + buf * BUF;
+#pragma unroll
+        for (int u = 0; u < VPT; u++)
+            if (srow[u] >= 0)
+                *reinterpret_cast<double2 *>(b + loff[u]) = stg[slot][u];
+    };
+
+    dbl4 acc[MT][NT];
+#pragma unroll
+    for (int i = 0; i < MT; i++)
+#pragma unroll
+        for (int j = 0; j < NT; j++)
+            acc[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
+    auto compute = [&](int buf) {
+        const double *bl = lds + buf * BUF;
+        const double *br = bl + SR * LDP;
+#pragma unroll
+        for (int ks = 0; ks < SR / 4; ks++) {
+            const int row = ks * 4 + lr;
+            double av[MT], bv[NT];
+#pragma unroll
+            for (int t = 0; t < MT; t++) {
+                const int a = ca0 + 16 * t;
+                av[t] = a < s.kl ? bl[row * LDP + a] : 0.0;
+            }
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                const int b = cb0 + 16 * t;
+                bv[t] = b < kr ? br[row * RDP + b] : 0.0;
+            }
+#pragma unroll
+            for (int i = 0; i < MT; i++)
+#pragma unroll
+                for (int j = 0; j < NT; j++)
+                    acc[i][j] = mfma16x16x4(av[i], bv[j], acc[i][j]);
+        }
+    };
+
+    // prologue: stages 0 .. LDG-1 in flight, stage 0 in LDS buffer 0
+#pragma unroll
+    for (int q = 0; q < LDG; q++)
+        gload(q, q);
+    lstore(0, 0);
+    gload(0, LDG);
+    __syncthreads();
+    // stage st: computed from buffer st & 1; its successor st + 1 sits in
+    // register slot (st + 1) % LDG and is stored to the other buffer
+    for (int st0 = 0; st0 < nstage; st0 += LDG) {
+#pragma unroll
+        for (int q = 0; q < LDG; q++) {
+            const int st = st0 + q;
+            if (st < nstage) {
+                if (active)
+                    compute(st & 1);
+                const int nslot = (q + 1) % LDG;
+                lstore(nslot, (st + 1) & 1);
+                gload(nslot, st + 1 + LDG);
+            }
+            __syncthreads();
+        }
+    }
+
+    if (!active)
+        return;
+    double *out = part + (long)blockIdx.x * kl_pad * kr_pad;
+#pragma unroll
+    for (int i = 0; i < MT; i++) {
+        const int tm = gm * MT + i;
+        if (tm * 16 >= kl_pad)
+            continue;
+#pragma unroll
+        for (int j = 0; j < NT; j++) {
+            const int tn = gn * NT + j;
+            if (tn * 16 >= kr_pad)
+                continue;
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                out[(long)(tm * 16 + lr + 4 * r) * kr_pad + tn * 16 + lc] = acc[i][j][r];
+        }
+    }
 }
 
 template <int MT, int NT>
@@ -272,13 +451,27 @@ int launch_gram_mfma(int n, const Src &s, const GramPlan &p, double *G1, double 
                      hipStream_t st)
 {
     const int kl_pad = p.TM * 16, kr_pad = p.TN * 16;
-    const size_t shm = p.KS > 1 ? (size_t)(p.KS - 1) * p.WGG * MT * NT * 4 * 64 * sizeof(double) : 0;
-    cacg_allow_lds(gram_mfma_kernel<MT, NT>, shm);
-    hipLaunchKernelGGL((gram_mfma_kernel<MT, NT>), dim3(p.gx, p.gy), dim3(256), shm, st, n, s, work,
-                       kl_pad, kr_pad, p.chunk, p.GN, p.NG, p.WGG, p.KS);
+    const int kr = s.kr1 + s.kr2;
+    auto al16 = [](const void *q) { return ((uintptr_t)q & 15) == 0; };
+    const bool staged = p.KS == 1 && s.kl % 2 == 0 && s.kr1 % 2 == 0 && s.kr2 % 2 == 0 &&
+                        s.ldl % 2 == 0 && s.ld1 % 2 == 0 && (s.kr2 == 0 || s.ld2 % 2 == 0) &&
+                        al16(s.L) && al16(s.R1) && (s.kr2 == 0 || al16(s.R2)) &&
+                        SR * (s.kl + kr) / 2 <= 256 * VPT && p.chunk % SR == 0;
+    if (staged) {
+        const size_t shm = 2 * (size_t)SR * (s.kl + 2 + kr + 2) * sizeof(double);
+        cacg_allow_lds(gram_lds_kernel<MT, NT>, shm);
+        hipLaunchKernelGGL((gram_lds_kernel<MT, NT>), dim3(p.gx, p.gy), dim3(256), shm, st, n, s,
+                           work, kl_pad, kr_pad, p.chunk, p.GN, p.NG);
+    } else {
+        const size_t shm =
+            p.KS > 1 ? (size_t)(p.KS - 1) * p.WGG * MT * NT * 4 * 64 * sizeof(double) : 0;
+        cacg_allow_lds(gram_mfma_kernel<MT, NT>, shm);
+        hipLaunchKernelGGL((gram_mfma_kernel<MT, NT>), dim3(p.gx, p.gy), dim3(256), shm, st, n, s,
+                           work, kl_pad, kr_pad, p.chunk, p.GN, p.NG, p.WGG, p.KS);
+    }
     CACG_LAUNCH_CHECK();
     const long outs = (long)s.kl * (s.kr1 + s.kr2);
-    hipLaunchKernelGGL(gram_reduce_kernel, dim3(cacg_grid(outs, 256, 0)), dim3(256), 0, st, work,
+    hipLaunchKernelGGL(gram_reduce_kernel, dim3(cacg_grid(outs, 64, 0)), dim3(256), 0, st, work,
                        p.gx, s.kl, s.kr1, s.kr2, kl_pad, kr_pad, G1, G2);
     CACG_LAUNCH_CHECK();
     return 0;

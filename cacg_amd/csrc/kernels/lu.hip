@@ -5,8 +5,10 @@
 //
 // Factorisation: right-looking, one barrier-separated step per pivot; every
 // element sees the same operations in the same order as in the reference's
-// loops, so the factor is bit-identical.  It lives in LDS when N*N doubles fit
-// (N <= 128: 128 KiB of the 160 KiB), in a global scratch block otherwise.
+// loops, so the factor is bit-identical.  For N <= 128 the matrix is held in
+// registers (4 x 4 elements per lane of a 32 x 32 lane grid) and the factor is
+// then kept in LDS (128 KiB of the 160 KiB); larger N work in a global
+// scratch block.
 //
 // Forward substitution runs column-oriented (for q: finalise row q, broadcast
 // it through LDS, update the rows below).  Each y[i][c] still receives its
@@ -32,7 +34,7 @@ __global__ __launch_bounds__(LU_T) void lu_solve_kernel(int N, const double *__r
                                                         const double *__restrict__ rhs,
                                                         double *__restrict__ x,
                                                         double *__restrict__ gw, int *flag,
-                                                        int mode, int kc)
+                                                        int mode, int kc, int reuse)
 {
     extern __shared__ double lds[];
     double *a = IN_LDS ? lds : gw;
@@ -40,33 +42,128 @@ __global__ __launch_bounds__(LU_T) void lu_solve_kernel(int N, const double *__r
     double *xs = IN_LDS ? lds : lds + 2 * kc;
     const int t = threadIdx.x;
 
-    // ---- factorisation
-    for (long e = t; e < (long)N * N; e += LU_T)
-        a[e] = lhs[e];
-    __syncthreads();
+    // ---- factorisation (or the factor left in gw by the previous call)
     int bad = 0;
-    for (int p = 0; p < N; p++) {
-        const double piv = a[(long)p * N + p];
-        if (!(piv != 0.0 && isfinite(piv)))
-            bad = 1;
-        const double inv = 1.0 / piv;
-        for (int c = p + 1 + t; c < N; c += LU_T)
-            a[(long)p * N + c] = mul_rn(a[(long)p * N + c], inv);
-        __syncthreads();
-        const int w = N - p - 1;
-        for (int e = t; e < w * w; e += LU_T) {
-            const int r = p + 1 + e / w, c = p + 1 + e % w;
-            a[(long)r * N + c] = sub_rn(a[(long)r * N + c], mul_rn(a[(long)r * N + p], a[(long)p * N + c]));
+    if (reuse) {
+        if (IN_LDS) {
+            for (long e = t; e < (long)N * N; e += LU_T)
+                a[e] = gw[e];
         }
         __syncthreads();
+    } else if (IN_LDS) {
+        // N <= 128: a 32 x 32 lane grid, lane (tr, tc) owns a[tr + 32 i][tc + 32 j]
+        // in registers.  Step p: the owners of row p and of column p publish them
+        // (double-buffered, one barrier per step); every lane scales the row by
+        // the reciprocal pivot itself (same bits as the row owner) and updates
+        // its elements.  Per element: the reference's operations, in its order.
+        const int tr = t >> 5, tc = t & 31;
+        double v[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int r = tr + 32 * i, c = tc + 32 * j;
+                v[i][j] = (r < N && c < N) ? lhs[(long)r * N + c] : 0.0;
+            }
+        double *ub = lds + (long)N * N + 2 * kc;  // 2 x 128: unscaled row p
+        double *lb = ub + 256;                     // 2 x 128: column p
+        double *ib = lb + 256;                     // 2: reciprocal pivot
+        // pivot p = 32 ip + pr: ip is a compile-time index into the register block
+#pragma unroll
+        for (int ip = 0; ip < 4; ip++) {
+            for (int pr = 0; pr < 32; pr++) {
+                const int p = ip * 32 + pr;
+                if (p >= N)
+                    break;
+                const int off = (p & 1) * 128;
+                if (tr == pr && tc == pr)
+                    ib[p & 1] = 1.0 / v[ip][ip];
+                if (tr == pr) {
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const int c = tc + 32 * j;
+                        if (c >= p && c < N)
+                            ub[off + c] = v[ip][j];
+                    }
+                }
+                if (tc == pr) {
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const int r = tr + 32 * i;
+                        if (r > p && r < N)
+                            lb[off + r] = v[i][ip];
+                    }
+                }
+                __syncthreads();
+                const double piv = ub[off + p];
+                if (!(piv != 0.0 && isfinite(piv)))
+                    bad = 1;
+                const double inv = ib[p & 1];
+                double u[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int c = tc + 32 * j;
+                    u[j] = (c > p && c < N) ? mul_rn(ub[off + c], inv) : 0.0;
+                }
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int r = tr + 32 * i;
+                    if (i == ip && tr == pr) {
+#pragma unroll
+                        for (int j = 0; j < 4; j++)
+                            if (tc + 32 * j > p && tc + 32 * j < N)
+                                v[i][j] = u[j];
+                    } else if (r > p && r < N) {
+                        const double l = lb[off + r];
+#pragma unroll
+                        for (int j = 0; j < 4; j++)
+                            if (tc + 32 * j > p && tc + 32 * j < N)
+                                v[i][j] = sub_rn(v[i][j], mul_rn(l, u[j]));
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int r = tr + 32 * i, c = tc + 32 * j;
+                if (r < N && c < N) {
+                    a[(long)r * N + c] = v[i][j];
+                    gw[(long)r * N + c] = v[i][j];
+                }
+            }
+        __syncthreads();
+    } else {
+        for (long e = t; e < (long)N * N; e += LU_T)
+            a[e] = lhs[e];
+        __syncthreads();
+        for (int p = 0; p < N; p++) {
+            const double piv = a[(long)p * N + p];
+            if (!(piv != 0.0 && isfinite(piv)))
+                bad = 1;
+            const double inv = 1.0 / piv;
+            for (int c = p + 1 + t; c < N; c += LU_T)
+                a[(long)p * N + c] = mul_rn(a[(long)p * N + c], inv);
+            __syncthreads();
+            const int w = N - p - 1;
+            // rows p+1.., one row per lane group of 64, columns strided by 64
+            for (int r = p + 1 + (t >> 6); r < N; r += LU_T / 64) {
+                const double l = a[(long)r * N + p];
+                for (int c = p + 1 + (t & 63); c < N; c += 64)
+                    a[(long)r * N + c] = sub_rn(a[(long)r * N + c], mul_rn(l, a[(long)p * N + c]));
+            }
+            (void)w;
+            __syncthreads();
+        }
     }
     if (t == 0 && bad)
         *flag = 1;
-    if (IN_LDS) {
-        for (long e = t; e < (long)N * N; e += LU_T)
-            gw[e] = a[e];
-        __syncthreads();
-    }
+    // reciprocal diagonal of L for the forward pass (the reference's 1.0 / L_ii)
+    double *dinv = IN_LDS ? lds + (long)N * N + 2 * kc + 512 + 2 : rowbuf + 2 * kc + (long)N * kc;
+    for (int q = t; q < N; q += LU_T)
+        dinv[q] = 1.0 / a[(long)q * N + q];
+    __syncthreads();
 
     // ---- substitutions, kc columns at a time
     for (int c0 = 0; c0 < k; c0 += kc) {
@@ -86,27 +183,28 @@ __global__ __launch_bounds__(LU_T) void lu_solve_kernel(int N, const double *__r
             const int i = g + G * j;
             y[j] = (own && i < N) ? rhs[(long)i * k + c0 + cl] : 0.0;
         }
-        // forward: L y = rhs
-        for (int q = 0; q < N; q++) {
-            double *rb = rowbuf + (q & 1) * kc;
-            if (own && q % G == g) {
-                const int jq = q / G;
-                const double inv = 1.0 / a[(long)q * N + q];
+        // forward: L y = rhs.  Row q = gq + G jq is owned by lane group gq as its
+        // register y[jq]; jq is a compile-time index.
 #pragma unroll
-                for (int j = 0; j < JMAX; j++)
-                    if (j == jq) {
-                        y[j] = mul_rn(y[j], inv);
-                        rb[cl] = y[j];
+        for (int jq = 0; jq < JMAX; jq++) {
+            for (int gq = 0; gq < G; gq++) {
+                const int q = gq + G * jq;
+                if (q >= N)
+                    break;
+                double *rb = rowbuf + (q & 1) * kc;
+                if (own && g == gq) {
+                    y[jq] = mul_rn(y[jq], dinv[q]);
+                    rb[cl] = y[jq];
+                }
+                __syncthreads();
+                if (own) {
+                    const double yq = rb[cl];
+#pragma unroll
+                    for (int j = jq; j < JMAX; j++) {
+                        const int i = g + G * j;
+                        if ((j > jq || g > gq) && i < N)
+                            y[j] = sub_rn(y[j], mul_rn(yq, a[(long)i * N + q]));
                     }
-            }
-            __syncthreads();
-            if (own) {
-                const double yq = rb[cl];
-#pragma unroll
-                for (int j = 0; j < JMAX; j++) {
-                    const int i = g + G * j;
-                    if (i > q && i < N)
-                        y[j] = sub_rn(y[j], mul_rn(yq, a[(long)i * N + q]));
                 }
             }
         }
@@ -146,23 +244,24 @@ __global__ __launch_bounds__(LU_T) void lu_solve_kernel(int N, const double *__r
             __syncthreads();
         } else {
             __syncthreads();  // the forward pass's last broadcast row is read before reuse
-            for (int q = N - 1; q >= 0; q--) {
-                double *rb = rowbuf + (q & 1) * kc;
-                if (own && q % G == g) {
-                    const int jq = q / G;
 #pragma unroll
-                    for (int j = 0; j < JMAX; j++)
-                        if (j == jq)
-                            rb[cl] = y[j];
-                }
-                __syncthreads();
-                if (own) {
-                    const double xq = rb[cl];
+            for (int jq = JMAX - 1; jq >= 0; jq--) {
+                for (int gq = G - 1; gq >= 0; gq--) {
+                    const int q = gq + G * jq;
+                    if (q >= N)
+                        continue;
+                    double *rb = rowbuf + (q & 1) * kc;
+                    if (own && g == gq)
+                        rb[cl] = y[jq];
+                    __syncthreads();
+                    if (own) {
+                        const double xq = rb[cl];
 #pragma unroll
-                    for (int j = 0; j < JMAX; j++) {
-                        const int i = g + G * j;
-                        if (i < q)
-                            y[j] = sub_rn(y[j], mul_rn(xq, a[(long)i * N + q]));
+                        for (int j = 0; j <= jq; j++) {
+                            const int i = g + G * j;
+                            if (j < jq || g < gq)
+                                y[j] = sub_rn(y[j], mul_rn(xq, a[(long)i * N + q]));
+                        }
                     }
                 }
             }
@@ -180,7 +279,8 @@ __global__ __launch_bounds__(LU_T) void lu_solve_kernel(int N, const double *__r
 }  // namespace
 
 extern "C" int cacg_k_lu_solve(int N, const double *lhs, int k, const double *rhs, double *x,
-                               double *d_work, int *d_flag, int mode, hipStream_t st)
+                               double *d_work, int *d_flag, int mode, int reuse_factor,
+                               hipStream_t st)
 {
     if (N <= 0 || k <= 0)
         return (int)hipErrorInvalidValue;
@@ -195,17 +295,18 @@ extern "C" int cacg_k_lu_solve(int N, const double *lhs, int k, const double *rh
     if (kc < 1 || (LU_T / kc) * JMAX < N)
         return (int)hipErrorInvalidValue;  // N > 16384 is out of range
     size_t shm;
-    if (in_lds)
-        shm = ((size_t)N * N + 2 * (size_t)kc) * sizeof(double);
+    if (in_lds)  // factor, broadcast rows (2 x kc), pivot row/column (2 x 2 x 128), 2 reciprocal
+                 // pivots, N reciprocal diagonal entries
+        shm = ((size_t)N * N + 2 * (size_t)kc + 512 + 2 + N) * sizeof(double);
     else
-        shm = ((size_t)N * kc + 2 * (size_t)kc) * sizeof(double);
+        shm = ((size_t)N * kc + 2 * (size_t)kc + N) * sizeof(double);
     cacg_allow_lds(in_lds ? lu_solve_kernel<true> : lu_solve_kernel<false>, shm);
     if (in_lds)
         hipLaunchKernelGGL(lu_solve_kernel<true>, dim3(1), dim3(LU_T), shm, st, N, lhs, k, rhs, x,
-                           d_work, d_flag, mode, kc);
+                           d_work, d_flag, mode, kc, reuse_factor);
     else
         hipLaunchKernelGGL(lu_solve_kernel<false>, dim3(1), dim3(LU_T), shm, st, N, lhs, k, rhs, x,
-                           d_work, d_flag, mode, kc);
+                           d_work, d_flag, mode, kc, reuse_factor);
     CACG_LAUNCH_CHECK();
     return 0;
 }

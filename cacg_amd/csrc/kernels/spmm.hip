@@ -4,122 +4,158 @@
 // Scheduling: vector-per-row.  A row is served by LPR consecutive lanes, one
 // output column per lane (and CPL columns per lane when k > 64), so the
 // gather of x(col, :) is one contiguous k*8-byte segment per nonzero and the
-// lanes of a row read the same (col, val) pair (one broadcast request).  Each
+// lanes of a row read the same (col, val) pair (an LDS broadcast).  Each
 // output is summed in CSR order from 0, product and sum rounded separately,
 // which is exactly the reference's arithmetic (src/BLAS3.c:78-112, 950-965):
 // the basis and A*Q blocks are bit-identical to the reference's.  The nonzero
-// loop is unrolled by UNR with all (col, val) and x loads issued before the
-// in-order accumulation, for memory-level parallelism.
+// loop is unrolled by 8 with all x loads issued before the in-order
+// accumulation, for memory-level parallelism.
 //
-// Row ranges are contiguous per workgroup and contiguous per XCD (hardware
-// workgroup ids are dealt round-robin over the 8 XCDs; xcd_remap undoes that)
-// so the rows that share gathered x lines run under one L2.
+// Tile order: hardware workgroup ids are dealt round-robin over the 8 XCDs,
+// so workgroup h runs on XCD h % 8.  Each XCD owns a contiguous band of row
+// tiles, and its workgroups take the band's tiles interleaved (workgroup j of
+// the XCD: tiles j, j + Gx, j + 2 Gx, ...), so the XCD sweeps its band as one
+// compact front: the rows that gather the same x rows (neighbours i +- 1,
+// i +- a grid line, i +- a grid plane for a stencil matrix) run close together
+// in time under the same L2.  Operands used once (the matrix, S_{j-2}, the
+// outputs) are loaded and stored non-temporally so they do not evict the
+// gathered rows from L2.
 #include <float.h>
 
 #include "kcommon.hpp"
 
 namespace {
 
-__device__ __forceinline__ int xcd_remap(int g, int G)
-{
-    if ((G & 7) != 0)
-        return g;
-    return (g & 7) * (G >> 3) + (g >> 3);
-}
 
-template <int LPR, int CPL, int UNR, int EPI>
+// One workgroup = 256 lanes = NG lane groups of LPR lanes; it walks a
+// contiguous range of row tiles (TR = NG * RPG rows each).  Per tile the
+// row pointers and, when they fit (CAP), the tile's (col, val) pairs are
+// staged in LDS by coalesced cooperative loads, so each row's gathers wait on
+// one global latency instead of three chained ones (row pointer -> column id
+// -> x).  Lane group g takes rows g, g + NG, ... of the tile; the epilogue
+// operands of a row are loaded before its gathers.
+template <int LPR, int CPL, int RPG, int EPI>
 __global__ __launch_bounds__(256) void spmm_kernel(
     int n, const int *__restrict__ rowptr, const int *__restrict__ colidx,
     const double *__restrict__ val, int k, const double *__restrict__ x, long ldx,
     double *__restrict__ y, long ldy, double a, double b, const double *__restrict__ p1, long ld1,
-    const double *__restrict__ p2, long ld2, double *__restrict__ copy0, long ldc, int chunk)
+    const double *__restrict__ p2, long ld2, double *__restrict__ copy0, long ldc, int tpb)
 {
-    constexpr int RPB = 256 / LPR;
-    const int lb = xcd_remap(blockIdx.x, gridDim.x);
-    const int lane = threadIdx.x % LPR;
-    const int r_begin = lb * chunk;
-    int r_end = r_begin + chunk;
-    if (r_end > n)
-        r_end = n;
-    for (int i = r_begin + threadIdx.x / LPR; i < r_end; i += RPB) {
-        const int pb = rowptr[i], pe = rowptr[i + 1];
-        for (int cb = 0; cb < k; cb += LPR * CPL) {
-            double acc[CPL];
-            int c[CPL];
-#pragma unroll
-            for (int t = 0; t < CPL; t++) {
-                acc[t] = 0.0;
-                c[t] = cb + lane + t * LPR;
+    constexpr int NG = 256 / LPR, TR = NG * RPG;
+    constexpr int CAP = TR * 8 > 256 ? TR * 8 : 256;
+    constexpr int UNR = 8;
+    __shared__ int s_rp[TR + 1];
+    __shared__ int s_col[CAP];
+    __shared__ double s_val[CAP];
+    const int t = threadIdx.x, g = t / LPR, lane = t % LPR;
+    const int ntiles = (n + TR - 1) / TR;
+    // band of this workgroup's XCD and the workgroup's slot in it
+    const int nx = gridDim.x >= 8 ? 8 : 1;
+    const int xcd = blockIdx.x % nx, gx = gridDim.x / nx, j = blockIdx.x / nx;
+    const int band = (ntiles + nx - 1) / nx;
+    const int tb = xcd * band;
+    const int te = (tb + band) < ntiles ? (tb + band) : ntiles;
+    (void)tpb;
+    for (int tile = tb + j; tile < te; tile += gx) {
+        const int r0 = tile * TR;
+        const int rows = (n - r0) < TR ? (n - r0) : TR;
+        for (int e = t; e <= rows; e += 256)
+            s_rp[e] = rowptr[r0 + e];
+        __syncthreads();
+        const int pbase = s_rp[0];
+        const int tnnz = s_rp[rows] - pbase;
+        const bool staged = tnnz <= CAP;
+        if (staged)
+            for (int e = t; e < tnnz; e += 256) {
+                s_col[e] = __builtin_nontemporal_load(colidx + pbase + e);
+                s_val[e] = __builtin_nontemporal_load(val + pbase + e);
             }
-            for (int p = pb; p < pe; p += UNR) {
-                int cc[UNR];
-                double vv[UNR];
-                double xv[UNR][CPL];
+        __syncthreads();
+        for (int rr = 0; rr < RPG; rr++) {
+            const int li = rr * NG + g;
+            if (li >= rows)
+                continue;
+            const long i = r0 + li;
+            const int pb = s_rp[li] - pbase, pe = s_rp[li + 1] - pbase;
+            for (int cb = 0; cb < k; cb += LPR * CPL) {
+                int c[CPL];
+                double acc[CPL], e1[CPL], e2[CPL];
 #pragma unroll
-                for (int u = 0; u < UNR; u++) {
-                    const bool in = p + u < pe;
-                    cc[u] = in ? colidx[p + u] : 0;
-                    vv[u] = in ? val[p + u] : 0.0;
+                for (int tt = 0; tt < CPL; tt++) {
+                    c[tt] = cb + lane + tt * LPR;
+                    acc[tt] = 0.0;
+                    const bool ok = c[tt] < k;
+                    if (EPI == CACG_EPI_RESID) {
+                        e1[tt] = ok ? __builtin_nontemporal_load(p1 + i * ld1 + c[tt]) : 0.0;
+                    } else if (EPI == CACG_EPI_CHEB1) {
+                        e1[tt] = ok ? x[i * ldx + c[tt]] : 0.0;
+                    } else if (EPI == CACG_EPI_CHEB) {
+                        e1[tt] = ok ? p1[i * ld1 + c[tt]] : 0.0;
+                        e2[tt] = ok ? __builtin_nontemporal_load(p2 + i * ld2 + c[tt]) : 0.0;
+                    }
+                }
+                for (int p = pb; p < pe; p += UNR) {
+                    int cc[UNR];
+                    double vv[UNR];
+#pragma unroll
+                    for (int u = 0; u < UNR; u++) {
+                        const bool in = p + u < pe;
+                        cc[u] = in ? (staged ? s_col[p + u] : colidx[pbase + p + u]) : 0;
+                        vv[u] = in ? (staged ? s_val[p + u] : val[pbase + p + u]) : 0.0;
+                    }
+                    double xv[UNR][CPL];
+#pragma unroll
+                    for (int u = 0; u < UNR; u++)
+#pragma unroll
+                        for (int tt = 0; tt < CPL; tt++)
+                            xv[u][tt] = (p + u < pe && c[tt] < k) ? x[(long)cc[u] * ldx + c[tt]] : 0.0;
+#pragma unroll
+                    for (int u = 0; u < UNR; u++)
+                        if (p + u < pe)
+#pragma unroll
+                            for (int tt = 0; tt < CPL; tt++)
+                                acc[tt] = add_rn(acc[tt], mul_rn(vv[u], xv[u][tt]));
                 }
 #pragma unroll
-                for (int u = 0; u < UNR; u++)
-#pragma unroll
-                    for (int t = 0; t < CPL; t++)
-                        xv[u][t] = (p + u < pe && c[t] < k) ? x[(long)cc[u] * ldx + c[t]] : 0.0;
-#pragma unroll
-                for (int u = 0; u < UNR; u++)
-                    if (p + u < pe)
-#pragma unroll
-                        for (int t = 0; t < CPL; t++)
-                            acc[t] = add_rn(acc[t], mul_rn(vv[u], xv[u][t]));
-            }
-#pragma unroll
-            for (int t = 0; t < CPL; t++) {
-                if (c[t] >= k)
-                    continue;
-                double out;
-                if (EPI == CACG_EPI_NONE) {
-                    out = acc[t];
-                } else if (EPI == CACG_EPI_RESID) {
-                    out = sub_rn(p1[(long)i * ld1 + c[t]], acc[t]);
-                } else if (EPI == CACG_EPI_CHEB1) {
-                    const double xi = x[(long)i * ldx + c[t]];
-                    out = add_rn(mul_rn(a, acc[t]), mul_rn(b, xi));
-                    copy0[(long)i * ldc + c[t]] = xi;
-                } else {
-                    out = sub_rn(add_rn(mul_rn(a, acc[t]), mul_rn(b, p1[(long)i * ld1 + c[t]])),
-                                 p2[(long)i * ld2 + c[t]]);
+                for (int tt = 0; tt < CPL; tt++) {
+                    if (c[tt] >= k)
+                        continue;
+                    double out;
+                    if (EPI == CACG_EPI_NONE) {
+                        out = acc[tt];
+                    } else if (EPI == CACG_EPI_RESID) {
+                        out = sub_rn(e1[tt], acc[tt]);
+                    } else if (EPI == CACG_EPI_CHEB1) {
+                        out = add_rn(mul_rn(a, acc[tt]), mul_rn(b, e1[tt]));
+                        __builtin_nontemporal_store(e1[tt], copy0 + i * ldc + c[tt]);
+                    } else {
+                        out = sub_rn(add_rn(mul_rn(a, acc[tt]), mul_rn(b, e1[tt])), e2[tt]);
+                    }
+                    __builtin_nontemporal_store(out, y + i * ldy + c[tt]);
                 }
-                y[(long)i * ldy + c[t]] = out;
             }
         }
+        __syncthreads();
     }
 }
 
-template <int LPR, int CPL, int UNR>
+template <int LPR, int CPL, int RPG>
 int launch_spmm_lpr(const cacg_dcsr *A, int k, const double *x, long ldx, double *y, long ldy,
                     int epi, double a, double b, const double *p1, long ld1, const double *p2,
                     long ld2, double *copy0, long ldc, hipStream_t st)
 {
-    constexpr int RPB = 256 / LPR;
-    // ~8 workgroups per CU over 256 CUs; a multiple of 8 for the XCD remap
-    long blocks = ((long)A->n + RPB - 1) / RPB;
-    int G = blocks > 2048 ? 2048 : (int)blocks;
+    constexpr int TR = (256 / LPR) * RPG;
+    const int ntiles = (A->n + TR - 1) / TR;
+    // ~8 workgroups per CU over 256 CUs; a multiple of 8 so every XCD gets the
+    // same number of workgroups for its band (see the kernel)
+    int G = ntiles < 2048 ? ntiles : 2048;
     if (G >= 8)
         G &= ~7;
-    if (G < 1)
-        G = 1;
-    long chunk = ((long)A->n + G - 1) / G;
-    chunk = (chunk + RPB - 1) / RPB * RPB;
-    G = (int)(((long)A->n + chunk - 1) / chunk);
-    if (G >= 8 && (G & 7) != 0) {
-        // keep the XCD remap a bijection: pad the grid, extra blocks get no rows
-        G = (G + 7) & ~7;
-    }
+    const int tpb = 0;
 #define CACG_SPMM_CASE(E)                                                                        \
-    hipLaunchKernelGGL((spmm_kernel<LPR, CPL, UNR, E>), dim3(G), dim3(256), 0, st, A->n,          \
+    hipLaunchKernelGGL((spmm_kernel<LPR, CPL, RPG, E>), dim3(G), dim3(256), 0, st, A->n,          \
                        A->rowptr, A->col, A->val, k, x, ldx, y, ldy, a, b, p1, ld1, p2, ld2, copy0, \
-                       ldc, (int)chunk)
+                       ldc, tpb)
     switch (epi) {
     case CACG_EPI_NONE:
         CACG_SPMM_CASE(CACG_EPI_NONE);
@@ -220,21 +256,21 @@ extern "C" int cacg_k_spmm(const cacg_dcsr *A, int k, const double *x, long ldx,
         return launch_spmm_lpr<64, 2, 4>(A, k, x, ldx, y, ldy, epi, a, b, p1, ld1, p2, ld2, copy0,
                                          ldc, st);
     if (k > 16)
-        return launch_spmm_lpr<32, 1, 8>(A, k, x, ldx, y, ldy, epi, a, b, p1, ld1, p2, ld2, copy0,
+        return launch_spmm_lpr<32, 1, 4>(A, k, x, ldx, y, ldy, epi, a, b, p1, ld1, p2, ld2, copy0,
                                          ldc, st);
     if (k > 8)
-        return launch_spmm_lpr<16, 1, 8>(A, k, x, ldx, y, ldy, epi, a, b, p1, ld1, p2, ld2, copy0,
+        return launch_spmm_lpr<16, 1, 4>(A, k, x, ldx, y, ldy, epi, a, b, p1, ld1, p2, ld2, copy0,
                                          ldc, st);
     if (k > 4)
-        return launch_spmm_lpr<8, 1, 8>(A, k, x, ldx, y, ldy, epi, a, b, p1, ld1, p2, ld2, copy0,
+        return launch_spmm_lpr<8, 1, 4>(A, k, x, ldx, y, ldy, epi, a, b, p1, ld1, p2, ld2, copy0,
                                         ldc, st);
     if (k > 2)
-        return launch_spmm_lpr<4, 1, 8>(A, k, x, ldx, y, ldy, epi, a, b, p1, ld1, p2, ld2, copy0,
+        return launch_spmm_lpr<4, 1, 4>(A, k, x, ldx, y, ldy, epi, a, b, p1, ld1, p2, ld2, copy0,
                                         ldc, st);
     if (k > 1)
-        return launch_spmm_lpr<2, 1, 8>(A, k, x, ldx, y, ldy, epi, a, b, p1, ld1, p2, ld2, copy0,
+        return launch_spmm_lpr<2, 1, 2>(A, k, x, ldx, y, ldy, epi, a, b, p1, ld1, p2, ld2, copy0,
                                         ldc, st);
-    return launch_spmm_lpr<1, 1, 8>(A, k, x, ldx, y, ldy, epi, a, b, p1, ld1, p2, ld2, copy0, ldc,
+    return launch_spmm_lpr<1, 1, 1>(A, k, x, ldx, y, ldy, epi, a, b, p1, ld1, p2, ld2, copy0, ldc,
                                     st);
 }
 

@@ -150,6 +150,7 @@ int cacg_dev_alloc(void **d_ptr, long bytes);
 int cacg_dev_free(void *d_ptr);
 int cacg_h2d(void *d_dst, const void *src, long bytes);
 int cacg_d2h(void *dst, const void *d_src, long bytes);
+int cacg_d2d(void *d_dst, const void *d_src, long bytes);
 int cacg_dev_sync(void);
 
 #ifdef __cplusplus

@@ -61,9 +61,12 @@ int cacg_k_gram_fast(int n, const double *L, long ldl, int kl, const double *R1,
  * U, no pivoting) and substitutions (src/DLSsolvers.c:106-208).  lhs is not
  * modified.  mode REFERENCE keeps the reference's ascending backward-substitution
  * order (bitwise equal); mode FAST runs the backward substitution column-oriented.
- * d_work holds N*N doubles; *d_flag is set to 1 on a zero or non-finite pivot. */
+ * d_work (N*N doubles) receives the factor; with reuse_factor != 0 the factor
+ * already in d_work (from a previous call on the same lhs) is used instead of
+ * refactoring lhs, which gives the same bits.  *d_flag is set to 1 on a zero or
+ * non-finite pivot. */
 int cacg_k_lu_solve(int N, const double *lhs, int k, const double *rhs, double *x, double *d_work,
-                    int *d_flag, int mode, hipStream_t st);
+                    int *d_flag, int mode, int reuse_factor, hipStream_t st);
 
 /* Q = S - Q beta, in place on Q (n x sm, row-major, ld = sm), beta sm x sm
  * (src/BLAS3.c:794-825 then 579-612). */

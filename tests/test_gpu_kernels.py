@@ -126,6 +126,11 @@ def test_lu_solve(gpu_lib, oracle_lib, N, k):
     xf, flag = gpu_lib.k_lu_solve(A, rhs, mode="fast")
     assert flag == 0
     assert np.allclose(xf, ref, rtol=1e-12, atol=1e-14)
+    # a second solve reusing the stored factor gives the same bits as a fresh one
+    rhs2 = np.random.default_rng(k + 1).standard_normal((N, max(1, k // 2)))
+    x1, x2, flag = gpu_lib.k_lu_solve(A, rhs, mode="reference", rhs2=rhs2)
+    assert flag == 0 and np.array_equal(x1, ref)
+    assert np.array_equal(x2, oracle_lib.lu_solve(A, rhs2))
 
 
 def test_lu_zero_pivot_sets_flag(gpu_lib):
