@@ -146,29 +146,39 @@ def main():
     calls = {k: sum(r["phase_calls"][k] for r in reps) for k in reps[0]["phase_calls"]}
     iters_total = sum(r["iters"] for r in reps)
 
-    # algorithmic work per phase over the timed region
-    basis_b = M.basis_bytes(n, nnz, m, s) * iters_total
-    aq_b = M.aq_bytes(n, nnz, sm) * iters_total
-    gram_f = (M.gram_flops(n, sm, sm + m) * iters_total
-              + M.gram_flops(n, sm, sm) * (iters_total - len(reps)))
-    qup_f = M.qupdate_flops(n, sm) * (iters_total - len(reps))
-    xr_b = M.xrupdate_bytes(n, sm, m) * iters_total
-    kernels = {
-        "basis_spmm": ("hbm", basis_b / (ph["basis"] * 1e-3) / 1e9, calls["basis"] * max(s - 1, 1)),
-        "aq_spmm": ("hbm", aq_b / (ph["aq"] * 1e-3) / 1e9, calls["aq"]),
-        "gram_mfma": ("mfma", gram_f / (ph["gram"] * 1e-3) / 1e12, calls["gram"]),
-        "qupdate_mfma": ("mfma", qup_f / max(ph["qupdate"] * 1e-3, 1e-12) / 1e12, calls["qupdate"]),
-        "xrupdate": ("hbm", xr_b / (ph["xrupdate"] * 1e-3) / 1e9, calls["xrupdate"]),
+    # one entry per kernel type: (phase, bound, algorithmic work per launch, launches per phase call)
+    nsolve = len(reps)
+    it_rest = iters_total - nsolve                       # iterations with Q^T A S and Q beta
+    cheb_b = nnz * 12 + 4 * (n + 1) + 3 * n * m * 8      # reads S_{j-1} (gathered) and S_{j-2}, writes S_j
+    cheb1_b = nnz * 12 + 4 * (n + 1) + 3 * n * m * 8     # reads R, writes S_0 and S_1
+    table = {
+        "basis_spmm_cheb_step": ("basis", "hbm", cheb_b, max(s - 2, 0)),
+        "basis_spmm_step1": ("basis1", "hbm", cheb1_b, 1),
+        "aq_spmm": ("aq", "hbm", M.aq_bytes(n, nnz, sm), 1),
+        "gram_QtAQ_QtR": ("gram", "mfma", M.gram_flops(n, sm, sm + m), 1),
+        "gram_QtAS": ("gram_as", "mfma", M.gram_flops(n, sm, sm), 1),
+        "qupdate": ("qupdate", "mfma", M.qupdate_flops(n, sm), 1),
+        "xrupdate": ("xrupdate", "hbm", M.xrupdate_bytes(n, sm, m), 1),
     }
-    phase_of = {"basis_spmm": "basis", "aq_spmm": "aq", "gram_mfma": "gram",
-                "qupdate_mfma": "qupdate", "xrupdate": "xrupdate"}
-    dom = max(kernels, key=lambda k: ph[phase_of[k]])
-    bound, ach, launches = kernels[dom]
-    peak = M.HBM_PEAK_GBPS if bound == "hbm" else M.FP64_PEAK_TFLOPS
-    roofline = {"kernel": dom, "bound": bound, "achieved": ach, "peak": peak,
-                "unit": "GB/s" if bound == "hbm" else "TFLOP/s", "frac": ach / peak,
-                "traffic": None,
-                "avg_launch_ms": ph[phase_of[dom]] / max(launches, 1)}
+    kernel_rates = {}
+    for name, (phase, bound, work, per_call) in table.items():
+        launches = calls[phase] * per_call
+        if launches == 0 or ph[phase] <= 0:
+            continue
+        avg_ms = ph[phase] / launches
+        ach = work / (avg_ms * 1e-3) / (1e9 if bound == "hbm" else 1e12)
+        peak = M.HBM_PEAK_GBPS if bound == "hbm" else M.FP64_PEAK_TFLOPS
+        kernel_rates[name] = {"bound": bound, "achieved": ach, "peak": peak,
+                              "unit": "GB/s" if bound == "hbm" else "TFLOP/s", "frac": ach / peak,
+                              "avg_launch_ms": avg_ms, "launches": launches,
+                              "total_ms": ph[phase]}
+    dom = max(kernel_rates, key=lambda k: kernel_rates[k]["total_ms"])
+    roofline = {k: kernel_rates[dom][k] for k in ("bound", "achieved", "peak", "unit", "frac")}
+    roofline["kernel"] = dom
+    roofline["avg_launch_ms"] = kernel_rates[dom]["avg_launch_ms"]
+    roofline["traffic"] = measured_traffic(dom)
+    basis_ms = ph["basis"] + ph["basis1"]
+    basis_gbps = M.basis_bytes(n, nnz, m, s) * iters_total / (basis_ms * 1e-3) / 1e9
 
     tr = np.linalg.norm(B - (A_dot(A, X)), axis=0) / np.linalg.norm(B, axis=0)
     flops = M.solve_flops(n, nnz, m, s, iters)
@@ -196,12 +206,9 @@ def main():
         "iters": iters,
         "status": reps[-1]["status"],
         "true_relres_max": float(tr.max()),
-        "basis_spmm_gbps": kernels["basis_spmm"][1],
+        "basis_spmm_gbps": basis_gbps,
         "gflops": flops / (wall / args.steps) / 1e9,
-        "kernel_rates": {k: {"bound": v[0], "achieved": v[1], "unit": "GB/s" if v[0] == "hbm" else "TFLOP/s",
-                             "frac": v[1] / (M.HBM_PEAK_GBPS if v[0] == "hbm" else M.FP64_PEAK_TFLOPS),
-                             "avg_launch_ms": ph[phase_of[k]] / max(v[2], 1)}
-                         for k, v in kernels.items()},
+        "kernel_rates": kernel_rates,
         "phase_ms_per_solve": {k: v / len(reps) for k, v in ph.items()},
         "roofline": roofline,
         "hbm_footprint_gb": sv.footprint() / 1e9,
@@ -214,6 +221,26 @@ def main():
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (profiles/<kernel>_c2_traffic.txt: FETCH_SIZE and WRITE_SIZE in KiB per
+    dispatch, and the copy-kernel scale factors), or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_c2_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    kv = {}
+    for line in open(path):
+        if ":" in line and not line.startswith("#"):
+            key, _, val = line.partition(":")
+            kv[key.strip()] = val.strip().split()[0]
+    try:
+        fetch = float(kv["FETCH_SIZE_KiB"]) * float(kv["fetch_scale"])
+        write = float(kv["WRITE_SIZE_KiB"]) * float(kv["write_scale"])
+    except (KeyError, ValueError):
+        return None
+    return (fetch + write) * 1024.0
 
 
 def A_dot(A, X):

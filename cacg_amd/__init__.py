@@ -27,7 +27,8 @@ STOP_REF_COL0_ABS, STOP_REL_PER_COLUMN = 0, 1
 ORDER_REFERENCE, ORDER_FAST = 0, 1
 MODE = {"reference": 0, "fast": 1}
 EPI = {"none": 0, "resid": 1, "cheb1": 2, "cheb": 3}
-PHASES = ["setup", "basis", "gram", "solve", "qupdate", "aq", "xrupdate", "other"]
+PHASES = ["setup", "basis", "gram", "solve", "qupdate", "aq", "xrupdate", "other", "basis1",
+          "gram_as", "unused10", "unused11"]
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _lp = ctypes.POINTER(ctypes.c_long)
@@ -43,8 +44,8 @@ class Opts(ctypes.Structure):
 class Report(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int), ("iters", ctypes.c_long),
                 ("bounds", ctypes.c_double * 2), ("res_hist", _dp), ("res_hist_cap", ctypes.c_long),
-                ("seconds", ctypes.c_double), ("phase_ms", ctypes.c_double * 8),
-                ("phase_calls", ctypes.c_long * 8)]
+                ("seconds", ctypes.c_double), ("phase_ms", ctypes.c_double * 12),
+                ("phase_calls", ctypes.c_long * 12)]
 
 
 class DCsr(ctypes.Structure):

@@ -31,8 +31,8 @@
 #include "cacg_kernels.h"
 #include "cacg.h"
 
-#define NPHASE 8
-#define MAXEV 16
+#define NPHASE 12
+#define MAXEV 20
 
 struct cacg_solver {
     long n, nnz, m, s, sm;
@@ -449,19 +449,21 @@ int cacg_solver_solve_device(cacg_solver *sv, const double *d_B, double *d_X, ca
         } else {
             SCHK(cacg_k_spmm(&sv->A, m, sv->d_R, m, S + m, sm, CACG_EPI_CHEB1, sa, sb, NULL, 0, NULL,
                              0, S, sm, st));
+            mark(sv, CACG_PHASE_BASIS1);
             for (int j = 2; j < s; j++)
                 SCHK(cacg_k_spmm(&sv->A, m, S + (long)(j - 1) * m, sm, S + (long)j * m, sm,
                                  CACG_EPI_CHEB, sa2, sb2, S + (long)(j - 1) * m, sm,
                                  S + (long)(j - 2) * m, sm, NULL, 0, st));
         }
-        mark(sv, CACG_PHASE_BASIS);
+        if (s != 2)
+            mark(sv, s == 1 ? CACG_PHASE_BASIS1 : CACG_PHASE_BASIS);
         if (it == 0) {
             double *t = Q;
             Q = S;
             S = t;
         } else {
             SCHK(gram(sv, mode, sv->d_AQ, S, sm, sm, NULL, 0, 0, sv->d_QtAS, NULL));
-            mark(sv, CACG_PHASE_GRAM);
+            mark(sv, CACG_PHASE_GRAM_AS);
             /* Q^T A Q of the previous iteration: its factor is still in d_lu_work */
             SCHK(cacg_k_lu_solve(sm, sv->d_QtAQ, sm, sv->d_QtAS, sv->d_beta, sv->d_lu_work,
                                  sv->d_flag, mode, 1, st));

@@ -109,19 +109,23 @@ typedef struct {
     double *res_hist;    /* optional, caller-owned [res_hist_cap][m]: ||R(:,j)||_2 after each iteration */
     long res_hist_cap;
     double seconds;      /* device time of the solve (setup + iterations) */
-    double phase_ms[8];  /* device time per phase, see CACG_PHASE_* */
-    long phase_calls[8];
+    double phase_ms[12]; /* device time per phase, see CACG_PHASE_* */
+    long phase_calls[12];
 } cacg_report;
 
+/* Each phase is one kernel type (plus its small reduction kernel), so its time
+ * divided by its launch count is that kernel's average launch duration. */
 enum {
     CACG_PHASE_SETUP = 0,    /* Gerschgorin + R = B - A X */
-    CACG_PHASE_BASIS = 1,    /* Chebyshev basis SpMMs (s - 1 per iteration) */
-    CACG_PHASE_GRAM = 2,     /* Q^T A S, Q^T A Q, Q^T R */
+    CACG_PHASE_BASIS = 1,    /* Chebyshev basis steps j >= 2 (s - 2 launches per iteration) */
+    CACG_PHASE_GRAM = 2,     /* [Q^T A Q | Q^T R] */
     CACG_PHASE_SOLVE = 3,    /* small LU solves */
     CACG_PHASE_QUPDATE = 4,  /* Q = S - Q beta */
     CACG_PHASE_AQ = 5,       /* A Q SpMM */
     CACG_PHASE_XRUPDATE = 6, /* X += Q alpha, R -= A Q alpha, column norms */
-    CACG_PHASE_OTHER = 7
+    CACG_PHASE_OTHER = 7,    /* host gaps between iterations */
+    CACG_PHASE_BASIS1 = 8,   /* Chebyshev basis step 1 (S_0 = R, S_1) */
+    CACG_PHASE_GRAM_AS = 9   /* Q^T A S */
 };
 
 void cacg_default_opts(cacg_opts *o);
