@@ -1,15 +1,16 @@
 // Multi-RHS CSR SpMM with fused Chebyshev-recurrence epilogues, and the
 // Gerschgorin interval.  gfx950 (CDNA4), wave64.
 //
-// Scheduling: vector-per-row.  A row is served by LPR consecutive lanes, one
-// output column per lane (and CPL columns per lane when k > 64), so the
-// gather of x(col, :) is one contiguous k*8-byte segment per nonzero and the
-// lanes of a row read the same (col, val) pair (an LDS broadcast).  Each
-// output is summed in CSR order from 0, product and sum rounded separately,
-// which is exactly the reference's arithmetic (src/BLAS3.c:78-112, 950-965):
-// the basis and A*Q blocks are bit-identical to the reference's.  The nonzero
-// loop is unrolled by 8 with all x loads issued before the in-order
-// accumulation, for memory-level parallelism.
+// Scheduling: vector-per-row.  A row is served by LPR consecutive lanes, each
+// lane owning VW contiguous output columns (VW = 4 when k is a multiple of 4
+// and x is 16-byte aligned with an even leading dimension, else 1), so the
+// gather of x(col, :) is one contiguous k*8-byte segment per nonzero read
+// with 16-byte loads, and the lanes of a row read the same (col, val) pair
+// (an LDS broadcast).  Each output is summed in CSR order from 0, product and
+// sum rounded separately, which is exactly the reference's arithmetic
+// (src/BLAS3.c:78-112, 950-965): the basis and A*Q blocks are bit-identical
+// to the reference's.  The nonzero loop is unrolled by 8 with all x loads
+// issued before the in-order accumulation, for memory-level parallelism.
 //
 // Tile order: hardware workgroup ids are dealt round-robin over the 8 XCDs,
 // so workgroup h runs on XCD h % 8.  Each XCD owns a contiguous band of row
@@ -21,25 +22,36 @@
 // outputs) are loaded and stored non-temporally so they do not evict the
 // gathered rows from L2.
 #include <float.h>
+#include <stdint.h>
 
 #include "kcommon.hpp"
 
 namespace {
 
 
-// One workgroup = 256 lanes = NG lane groups of LPR lanes; it walks a
-// contiguous range of row tiles (TR = NG * RPG rows each).  Per tile the
-// row pointers and, when they fit (CAP), the tile's (col, val) pairs are
-// staged in LDS by coalesced cooperative loads, so each row's gathers wait on
-// one global latency instead of three chained ones (row pointer -> column id
-// -> x).  Lane group g takes rows g, g + NG, ... of the tile; the epilogue
-// operands of a row are loaded before its gathers.
-template <int LPR, int CPL, int RPG, int EPI>
+template <int VW>
+struct dvec {
+    double2 h[VW / 2];
+};
+template <>
+struct dvec<1> {
+    double h;
+};
+
+// One workgroup = 256 lanes = NG lane groups of LPR lanes; it walks a range
+// of row tiles (TR = NG * RPG rows each).  Per tile the row pointers and,
+// when they fit (CAP), the tile's (col, val) pairs are staged in LDS by
+// coalesced cooperative loads, so each row's gathers wait on one global
+// latency instead of three chained ones (row pointer -> column id -> x).
+// Lane group g takes rows g, g + NG, ... of the tile; the epilogue operands of
+// a row are loaded before its gathers.  Requires k % VW == 0 and, for VW > 1,
+// x and ldx 16-byte aligned.
+template <int LPR, int VW, int RPG, int EPI>
 __global__ __launch_bounds__(256) void spmm_kernel(
     int n, const int *__restrict__ rowptr, const int *__restrict__ colidx,
     const double *__restrict__ val, int k, const double *__restrict__ x, long ldx,
     double *__restrict__ y, long ldy, double a, double b, const double *__restrict__ p1, long ld1,
-    const double *__restrict__ p2, long ld2, double *__restrict__ copy0, long ldc, int tpb)
+    const double *__restrict__ p2, long ld2, double *__restrict__ copy0, long ldc)
 {
     constexpr int NG = 256 / LPR, TR = NG * RPG;
     constexpr int CAP = TR * 8 > 256 ? TR * 8 : 256;
@@ -55,7 +67,6 @@ __global__ __launch_bounds__(256) void spmm_kernel(
     const int band = (ntiles + nx - 1) / nx;
     const int tb = xcd * band;
     const int te = (tb + band) < ntiles ? (tb + band) : ntiles;
-    (void)tpb;
     for (int tile = tb + j; tile < te; tile += gx) {
         const int r0 = tile * TR;
         const int rows = (n - r0) < TR ? (n - r0) : TR;
@@ -71,27 +82,27 @@ __global__ __launch_bounds__(256) void spmm_kernel(
                 s_val[e] = __builtin_nontemporal_load(val + pbase + e);
             }
         __syncthreads();
-        for (int rr = 0; rr < RPG; rr++) {
-            const int li = rr * NG + g;
-            if (li >= rows)
+        for (int cb = 0; cb < k; cb += LPR * VW) {
+            const int c0 = cb + lane * VW;
+            if (c0 >= k)
                 continue;
-            const long i = r0 + li;
-            const int pb = s_rp[li] - pbase, pe = s_rp[li + 1] - pbase;
-            for (int cb = 0; cb < k; cb += LPR * CPL) {
-                int c[CPL];
-                double acc[CPL], e1[CPL], e2[CPL];
+            for (int rr = 0; rr < RPG; rr++) {
+                const int li = rr * NG + g;
+                if (li >= rows)
+                    continue;
+                const long i = r0 + li;
+                const int pb = s_rp[li] - pbase, pe = s_rp[li + 1] - pbase;
+                double acc[VW], e1[VW], e2[VW];
 #pragma unroll
-                for (int tt = 0; tt < CPL; tt++) {
-                    c[tt] = cb + lane + tt * LPR;
-                    acc[tt] = 0.0;
-                    const bool ok = c[tt] < k;
+                for (int v = 0; v < VW; v++) {
+                    acc[v] = 0.0;
                     if (EPI == CACG_EPI_RESID) {
-                        e1[tt] = ok ? __builtin_nontemporal_load(p1 + i * ld1 + c[tt]) : 0.0;
+                        e1[v] = __builtin_nontemporal_load(p1 + i * ld1 + c0 + v);
                     } else if (EPI == CACG_EPI_CHEB1) {
-                        e1[tt] = ok ? x[i * ldx + c[tt]] : 0.0;
+                        e1[v] = x[i * ldx + c0 + v];
                     } else if (EPI == CACG_EPI_CHEB) {
-                        e1[tt] = ok ? p1[i * ld1 + c[tt]] : 0.0;
-                        e2[tt] = ok ? __builtin_nontemporal_load(p2 + i * ld2 + c[tt]) : 0.0;
+                        e1[v] = p1[i * ld1 + c0 + v];
+                        e2[v] = __builtin_nontemporal_load(p2 + i * ld2 + c0 + v);
                     }
                 }
                 for (int p = pb; p < pe; p += UNR) {
@@ -103,35 +114,34 @@ __global__ __launch_bounds__(256) void spmm_kernel(
                         cc[u] = in ? (staged ? s_col[p + u] : colidx[pbase + p + u]) : 0;
                         vv[u] = in ? (staged ? s_val[p + u] : val[pbase + p + u]) : 0.0;
                     }
-                    double xv[UNR][CPL];
-#pragma unroll
-                    for (int u = 0; u < UNR; u++)
-#pragma unroll
-                        for (int tt = 0; tt < CPL; tt++)
-                            xv[u][tt] = (p + u < pe && c[tt] < k) ? x[(long)cc[u] * ldx + c[tt]] : 0.0;
+                    dvec<VW> xv[UNR];
 #pragma unroll
                     for (int u = 0; u < UNR; u++)
                         if (p + u < pe)
+                            xv[u] = *(const dvec<VW> *)(x + (long)cc[u] * ldx + c0);
 #pragma unroll
-                            for (int tt = 0; tt < CPL; tt++)
-                                acc[tt] = add_rn(acc[tt], mul_rn(vv[u], xv[u][tt]));
+                    for (int u = 0; u < UNR; u++)
+                        if (p + u < pe) {
+                            const double *xs = (const double *)&xv[u];
+#pragma unroll
+                            for (int v = 0; v < VW; v++)
+                                acc[v] = add_rn(acc[v], mul_rn(vv[u], xs[v]));
+                        }
                 }
 #pragma unroll
-                for (int tt = 0; tt < CPL; tt++) {
-                    if (c[tt] >= k)
-                        continue;
+                for (int v = 0; v < VW; v++) {
                     double out;
                     if (EPI == CACG_EPI_NONE) {
-                        out = acc[tt];
+                        out = acc[v];
                     } else if (EPI == CACG_EPI_RESID) {
-                        out = sub_rn(e1[tt], acc[tt]);
+                        out = sub_rn(e1[v], acc[v]);
                     } else if (EPI == CACG_EPI_CHEB1) {
-                        out = add_rn(mul_rn(a, acc[tt]), mul_rn(b, e1[tt]));
-                        __builtin_nontemporal_store(e1[tt], copy0 + i * ldc + c[tt]);
+                        out = add_rn(mul_rn(a, acc[v]), mul_rn(b, e1[v]));
+                        __builtin_nontemporal_store(e1[v], copy0 + i * ldc + c0 + v);
                     } else {
-                        out = sub_rn(add_rn(mul_rn(a, acc[tt]), mul_rn(b, e1[tt])), e2[tt]);
+                        out = sub_rn(add_rn(mul_rn(a, acc[v]), mul_rn(b, e1[v])), e2[v]);
                     }
-                    __builtin_nontemporal_store(out, y + i * ldy + c[tt]);
+                    __builtin_nontemporal_store(out, y + i * ldy + c0 + v);
                 }
             }
         }
@@ -139,23 +149,26 @@ __global__ __launch_bounds__(256) void spmm_kernel(
     }
 }
 
-template <int LPR, int CPL, int RPG>
-int launch_spmm_lpr(const cacg_dcsr *A, int k, const double *x, long ldx, double *y, long ldy,
-                    int epi, double a, double b, const double *p1, long ld1, const double *p2,
-                    long ld2, double *copy0, long ldc, hipStream_t st)
+// Launch shape (measured on the 128^3 Laplacian, k = 16 and k = 128): narrow
+// rows (LPR <= 8) one row per lane group and 32 workgroups per CU; wide rows
+// two rows per lane group and 16 workgroups per CU.
+template <int LPR, int VW>
+int launch_spmm(const cacg_dcsr *A, int k, const double *x, long ldx, double *y, long ldy, int epi,
+                double a, double b, const double *p1, long ld1, const double *p2, long ld2,
+                double *copy0, long ldc, hipStream_t st)
 {
+    constexpr int RPG = LPR >= 16 ? 2 : 1;
     constexpr int TR = (256 / LPR) * RPG;
+    const int gmax = LPR >= 16 ? 4096 : 8192;
     const int ntiles = (A->n + TR - 1) / TR;
-    // ~8 workgroups per CU over 256 CUs; a multiple of 8 so every XCD gets the
-    // same number of workgroups for its band (see the kernel)
-    int G = ntiles < 2048 ? ntiles : 2048;
+    // a multiple of 8 so every XCD gets the same number of workgroups for its band
+    int G = ntiles < gmax ? ntiles : gmax;
     if (G >= 8)
         G &= ~7;
-    const int tpb = 0;
 #define CACG_SPMM_CASE(E)                                                                        \
-    hipLaunchKernelGGL((spmm_kernel<LPR, CPL, RPG, E>), dim3(G), dim3(256), 0, st, A->n,          \
+    hipLaunchKernelGGL((spmm_kernel<LPR, VW, RPG, E>), dim3(G), dim3(256), 0, st, A->n,           \
                        A->rowptr, A->col, A->val, k, x, ldx, y, ldy, a, b, p1, ld1, p2, ld2, copy0, \
-                       ldc, tpb)
+                       ldc)
     switch (epi) {
     case CACG_EPI_NONE:
         CACG_SPMM_CASE(CACG_EPI_NONE);
@@ -252,26 +265,37 @@ extern "C" int cacg_k_spmm(const cacg_dcsr *A, int k, const double *x, long ldx,
         return (int)hipErrorInvalidValue;
     if (A->n == 0)
         return 0;
-    if (k >= 64)
-        return launch_spmm_lpr<64, 2, 4>(A, k, x, ldx, y, ldy, epi, a, b, p1, ld1, p2, ld2, copy0,
-                                         ldc, st);
+#define CACG_SPMM_ARGS A, k, x, ldx, y, ldy, epi, a, b, p1, ld1, p2, ld2, copy0, ldc, st
+    const bool vec4 = k % 4 == 0 && ((uintptr_t)x & 15) == 0 && ldx % 2 == 0;
+    if (vec4) {
+        // LPR = k / 4 rounded up to a power of two, at most 32 (wider k loops)
+        const int q = k / 4;
+        if (q > 16)
+            return launch_spmm<32, 4>(CACG_SPMM_ARGS);
+        if (q > 8)
+            return launch_spmm<16, 4>(CACG_SPMM_ARGS);
+        if (q > 4)
+            return launch_spmm<8, 4>(CACG_SPMM_ARGS);
+        if (q > 2)
+            return launch_spmm<4, 4>(CACG_SPMM_ARGS);
+        if (q > 1)
+            return launch_spmm<2, 4>(CACG_SPMM_ARGS);
+        return launch_spmm<1, 4>(CACG_SPMM_ARGS);
+    }
+    if (k > 32)
+        return launch_spmm<64, 1>(CACG_SPMM_ARGS);
     if (k > 16)
-        return launch_spmm_lpr<32, 1, 4>(A, k, x, ldx, y, ldy, epi, a, b, p1, ld1, p2, ld2, copy0,
-                                         ldc, st);
+        return launch_spmm<32, 1>(CACG_SPMM_ARGS);
     if (k > 8)
-        return launch_spmm_lpr<16, 1, 4>(A, k, x, ldx, y, ldy, epi, a, b, p1, ld1, p2, ld2, copy0,
-                                         ldc, st);
+        return launch_spmm<16, 1>(CACG_SPMM_ARGS);
     if (k > 4)
-        return launch_spmm_lpr<8, 1, 4>(A, k, x, ldx, y, ldy, epi, a, b, p1, ld1, p2, ld2, copy0,
-                                        ldc, st);
+        return launch_spmm<8, 1>(CACG_SPMM_ARGS);
     if (k > 2)
-        return launch_spmm_lpr<4, 1, 4>(A, k, x, ldx, y, ldy, epi, a, b, p1, ld1, p2, ld2, copy0,
-                                        ldc, st);
+        return launch_spmm<4, 1>(CACG_SPMM_ARGS);
     if (k > 1)
-        return launch_spmm_lpr<2, 1, 2>(A, k, x, ldx, y, ldy, epi, a, b, p1, ld1, p2, ld2, copy0,
-                                        ldc, st);
-    return launch_spmm_lpr<1, 1, 1>(A, k, x, ldx, y, ldy, epi, a, b, p1, ld1, p2, ld2, copy0, ldc,
-                                    st);
+        return launch_spmm<2, 1>(CACG_SPMM_ARGS);
+    return launch_spmm<1, 1>(CACG_SPMM_ARGS);
+#undef CACG_SPMM_ARGS
 }
 
 extern "C" int cacg_k_gerschgorin(const cacg_dcsr *A, double *d_work, double *d_out,

@@ -7,6 +7,8 @@ wrong lane layout, index or transposed operand cannot pass), and within a
 rounding bound on random data:
     |G - L^T R| <= 2 n eps (|L|^T |R|)   (eps = 2^-53)."""
 
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -35,6 +37,9 @@ MATRICES = {
     "lap3d_7": lambda: P.laplacian_3d(7),
     "spd_13": lambda: P.random_spd_like(P.laplacian_2d(13), seed=2),
     "arrow_10": _arrow,
+    # a 1600-entry row: the first row tile holds more nonzeros than the LDS
+    # stage of any launch shape, so its rows read (col, val) from global memory
+    "arrow_40": lambda: _arrow(40),
 }
 
 
@@ -50,6 +55,35 @@ def test_spmm_bitwise(gpu_lib, oracle_lib, mat, k):
     A = MATRICES[mat]()
     X = P.rhs_normal(A.n, k, 11)
     assert np.array_equal(gpu_lib.k_spmm(A, X), oracle_lib.spmm(A, X))
+
+
+@pytest.mark.parametrize("k,off,ld", [(16, 16, 128), (16, 3, 40), (12, 4, 20), (20, 0, 20),
+                                      (256, 0, 256), (4, 2, 8), (128, 0, 144), (6, 1, 9)])
+@pytest.mark.parametrize("epi", ["none", "cheb"])
+def test_spmm_strided_bitwise(gpu_lib, oracle_lib, k, off, ld, epi):
+    """x, p1, p2 and y as column blocks (columns off .. off+k) of wider
+    row-major n x ld arrays, the layout of the basis blocks; off and ld select
+    the 16-byte gather path (4 | k, 16-byte aligned x, even ld) or the scalar
+    one."""
+    A = P.laplacian_3d(9)
+    n = A.n
+    Xw, P1w, P2w = (P.rhs_normal(n, ld, sd) for sd in (21, 22, 23))
+    a, b = 0.37, -0.21
+    dA = gpu_lib.DeviceCSR(A)
+    dX, d1, d2 = (gpu_lib.DeviceBuffer.from_array(M) for M in (Xw, P1w, P2w))
+    Yw = np.full((n, ld), 7.0)
+    dY = gpu_lib.DeviceBuffer.from_array(Yw)
+    epi_id = gpu_lib.EPI[epi]
+    rc = gpu_lib.lib().cacg_k_spmm(ctypes.byref(dA.s), k, dX.ptr + 8 * off, ld, dY.ptr + 8 * off,
+                                   ld, epi_id, a, b, d1.ptr + 8 * off, ld, d2.ptr + 8 * off, ld,
+                                   None, ld, None)
+    assert rc == 0
+    Y = dY.download((n, ld))
+    AX = oracle_lib.spmm(A, np.ascontiguousarray(Xw[:, off:off + k]))
+    want = AX if epi == "none" else (a * AX + b * P1w[:, off:off + k]) - P2w[:, off:off + k]
+    assert np.array_equal(Y[:, off:off + k], want)
+    # columns outside the block are untouched
+    assert np.all(Y[:, :off] == 7.0) and np.all(Y[:, off + k:] == 7.0)
 
 
 @pytest.mark.parametrize("m", [1, 4, 16])
