@@ -288,28 +288,32 @@ __global__ __launch_bounds__(256) void gram_reduce_kernel(const double *__restri
     }
 }
 
-// LDS-staged variant for the large case (four tile groups per workgroup,
-// even column counts).  A stage is SR = 8 rows of L and of [R1 | R2]; the 256
-// lanes load it cooperatively with 16-byte loads, LDG stages ahead, holding
-// the data in registers; the stage is then written to one of two LDS buffers
-// and every wave reads its MFMA operands from there.  Each row is read from
-// HBM once per workgroup, and LDG * 17 KiB stay in flight per workgroup.
-constexpr int SR = 8;
-constexpr int LDG = 3;
-constexpr int VPT = 5;   // 16-byte vectors per lane per stage: SR * (kl + kr) / 2 <= 256 * VPT
-
-template <int MT, int NT>
-__global__ __launch_bounds__(256) void gram_lds_kernel(int n, Src s, double *__restrict__ part,
-                                                       int kl_pad, int kr_pad, int chunk, int GN,
-                                                       int NG)
+// LDS-staged variant for the large case (even column counts, 16-byte aligned
+// rows).  A stage is SR rows of L and of [R1 | R2]; the NW * 64 lanes load it
+// cooperatively with 16-byte loads, LDG stages ahead, holding the data in
+// registers; the stage is then written to one of two LDS buffers and every
+// wave reads its MFMA operands from there.  Each row is read from HBM once per
+// workgroup.  Every wave owns one tile group (MT x NT tiles), so all tiles of
+// G are covered by one workgroup when NG <= NW.  Each tile accumulates its
+// k-steps of 4 rows in row order whatever SR, LDG, NW, MT and NT are, so the
+// partials depend on the row chunks only.
+//   <4, MT, NT, 8, 3, 5>: general large case (four tile groups per workgroup);
+//   <12, 2, 3, 16, 2, 3>, <16, 2, 2, 16, 2, 2>: a 128-row G against 144 and
+//   128 columns, as 12 and 16 waves of 2 x 3 and 2 x 2 tiles (measured on
+//   gfx950: 58 and 56 TFLOP/s against 42 and 50 for the general shape).
+template <int NW, int MT, int NT, int SR, int LDG, int VPT>
+__global__ __launch_bounds__(NW * 64) void gram_lds_kernel(int n, Src s, double *__restrict__ part,
+                                                           int kl_pad, int kr_pad, int chunk,
+                                                           int GN, int NG)
 {
+    constexpr int NTH = NW * 64;
     extern __shared__ double lds[];
     const int kr = s.kr1 + s.kr2;
     const int LDP = s.kl + 2, RDP = kr + 2;          // padded LDS row strides
     const int BUF = SR * (LDP + RDP);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int lr = lane >> 4, lc = lane & 15;
-    const int g = blockIdx.y * 4 + wave;
+    const int g = blockIdx.y * NW + wave;
     const bool active = g < NG;
     const int gm = active ? g / GN : 0, gn = active ? g % GN : 0;
     const int ca0 = gm * MT * 16 + lc, cb0 = gn * NT * 16 + lc;
@@ -327,7 +331,7 @@ __global__ __launch_bounds__(256) void gram_lds_kernel(int n, Src s, double *__r
     int srow[VPT], loff[VPT];
 #pragma unroll
     for (int u = 0; u < VPT; u++) {
-        int v = threadIdx.x + 256 * u;
+        int v = threadIdx.x + NTH * u;
         srow[u] = -1;
         src[u] = s.L;
         ld[u] = 0;
@@ -446,22 +450,51 @@ __global__ __launch_bounds__(256) void gram_lds_kernel(int n, Src s, double *__r
     }
 }
 
+// the 128-row shapes of the wide kernels: returns 1 when launched
+int launch_gram_wide(int n, const Src &s, const GramPlan &p, double *work, bool staged_ok,
+                     hipStream_t st)
+{
+    constexpr int SRW = 16;
+    const int kr = s.kr1 + s.kr2;
+    const int kl_pad = p.TM * 16, kr_pad = p.TN * 16;
+    if (!staged_ok || p.TM != 8 || p.chunk % SRW != 0 || (p.TN != 8 && p.TN != 9))
+        return 0;
+    const size_t shm = 2 * (size_t)SRW * (s.kl + 2 + kr + 2) * sizeof(double);
+    if (p.TN == 9) {
+        if (SRW * (s.kl + kr) / 2 > 12 * 64 * 3)
+            return 0;
+        cacg_allow_lds(gram_lds_kernel<12, 2, 3, SRW, 2, 3>, shm);
+        hipLaunchKernelGGL((gram_lds_kernel<12, 2, 3, SRW, 2, 3>), dim3(p.gx, 1), dim3(12 * 64),
+                           shm, st, n, s, work, kl_pad, kr_pad, p.chunk, 3, 12);
+    } else {
+        if (SRW * (s.kl + kr) / 2 > 16 * 64 * 2)
+            return 0;
+        cacg_allow_lds(gram_lds_kernel<16, 2, 2, SRW, 2, 2>, shm);
+        hipLaunchKernelGGL((gram_lds_kernel<16, 2, 2, SRW, 2, 2>), dim3(p.gx, 1), dim3(16 * 64),
+                           shm, st, n, s, work, kl_pad, kr_pad, p.chunk, 4, 16);
+    }
+    return 1;
+}
+
 template <int MT, int NT>
 int launch_gram_mfma(int n, const Src &s, const GramPlan &p, double *G1, double *G2, double *work,
                      hipStream_t st)
 {
+    constexpr int SR = 8, LDG = 3, VPT = 5;
     const int kl_pad = p.TM * 16, kr_pad = p.TN * 16;
     const int kr = s.kr1 + s.kr2;
     auto al16 = [](const void *q) { return ((uintptr_t)q & 15) == 0; };
-    const bool staged = p.KS == 1 && s.kl % 2 == 0 && s.kr1 % 2 == 0 && s.kr2 % 2 == 0 &&
-                        s.ldl % 2 == 0 && s.ld1 % 2 == 0 && (s.kr2 == 0 || s.ld2 % 2 == 0) &&
-                        al16(s.L) && al16(s.R1) && (s.kr2 == 0 || al16(s.R2)) &&
-                        SR * (s.kl + kr) / 2 <= 256 * VPT && p.chunk % SR == 0;
-    if (staged) {
+    const bool aligned = p.KS == 1 && s.kl % 2 == 0 && s.kr1 % 2 == 0 && s.kr2 % 2 == 0 &&
+                         s.ldl % 2 == 0 && s.ld1 % 2 == 0 && (s.kr2 == 0 || s.ld2 % 2 == 0) &&
+                         al16(s.L) && al16(s.R1) && (s.kr2 == 0 || al16(s.R2));
+    const bool staged = aligned && SR * (s.kl + kr) / 2 <= 256 * VPT && p.chunk % SR == 0;
+    if (launch_gram_wide(n, s, p, work, aligned, st)) {
+        // launched
+    } else if (staged) {
         const size_t shm = 2 * (size_t)SR * (s.kl + 2 + kr + 2) * sizeof(double);
-        cacg_allow_lds(gram_lds_kernel<MT, NT>, shm);
-        hipLaunchKernelGGL((gram_lds_kernel<MT, NT>), dim3(p.gx, p.gy), dim3(256), shm, st, n, s,
-                           work, kl_pad, kr_pad, p.chunk, p.GN, p.NG);
+        cacg_allow_lds(gram_lds_kernel<4, MT, NT, SR, LDG, VPT>, shm);
+        hipLaunchKernelGGL((gram_lds_kernel<4, MT, NT, SR, LDG, VPT>), dim3(p.gx, p.gy),
+                           dim3(256), shm, st, n, s, work, kl_pad, kr_pad, p.chunk, p.GN, p.NG);
     } else {
         const size_t shm =
             p.KS > 1 ? (size_t)(p.KS - 1) * p.WGG * MT * NT * 4 * 64 * sizeof(double) : 0;

@@ -105,6 +105,11 @@ def test_spmm_epilogues_bitwise(gpu_lib, oracle_lib, m):
 
 GRAM_SHAPES = [(1000, 4, 4, 1), (1000, 9, 9, 3), (3000, 16, 16, 4), (2048, 64, 64, 16),
                (5000, 128, 128, 16), (777, 128, 128, 0), (1500, 256, 256, 32), (0, 16, 16, 4)]
+# fast-order only: 128-row shapes of the wide-workgroup kernels with many row
+# chunks, a 120-row G (padded tiles), and a chunk length that is not a
+# multiple of their 16-row stage (general kernel instead)
+GRAM_WIDE = [(70000, 128, 128, 16), (131072, 120, 112, 16), (131072, 128, 128, 0),
+             (100003, 128, 128, 16)]
 
 
 @pytest.mark.parametrize("n,kl,kr1,kr2", GRAM_SHAPES)
@@ -118,7 +123,7 @@ def test_gram_reference_bitwise(gpu_lib, oracle_lib, n, kl, kr1, kr2):
         assert np.array_equal(G2, oracle_lib.gram(L, R2))
 
 
-@pytest.mark.parametrize("n,kl,kr1,kr2", GRAM_SHAPES)
+@pytest.mark.parametrize("n,kl,kr1,kr2", GRAM_SHAPES + GRAM_WIDE)
 def test_gram_fast_exact_on_integers(gpu_lib, n, kl, kr1, kr2):
     rng = np.random.default_rng(n + kl + 1)
     L = rng.integers(-8, 9, size=(n, kl)).astype(np.float64)
@@ -126,10 +131,11 @@ def test_gram_fast_exact_on_integers(gpu_lib, n, kl, kr1, kr2):
     R2 = rng.integers(-8, 9, size=(n, kr2)).astype(np.float64)
     out = gpu_lib.k_gram(L, R1, R2 if kr2 else None, mode="fast")
     G1, G2 = out if kr2 else (out, None)
-    Li = L.astype(np.int64)
-    assert np.array_equal(G1, (Li.T @ R1.astype(np.int64)).astype(np.float64))
+    # every partial sum is an integer below 2^53, so the float64 product is exact
+    # in any summation order (BLAS included)
+    assert np.array_equal(G1, L.T @ R1)
     if kr2:
-        assert np.array_equal(G2, (Li.T @ R2.astype(np.int64)).astype(np.float64))
+        assert np.array_equal(G2, L.T @ R2)
 
 
 @pytest.mark.parametrize("n,kl,kr1,kr2", GRAM_SHAPES[:6])
