@@ -28,7 +28,7 @@ ORDER_REFERENCE, ORDER_FAST = 0, 1
 MODE = {"reference": 0, "fast": 1}
 EPI = {"none": 0, "resid": 1, "cheb1": 2, "cheb": 3}
 PHASES = ["setup", "basis", "gram", "solve", "qupdate", "aq", "xrupdate", "other", "basis1",
-          "gram_as", "unused10", "unused11"]
+          "gram_as", "refine", "unused11"]
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _lp = ctypes.POINTER(ctypes.c_long)
@@ -38,18 +38,24 @@ _vp = ctypes.c_void_p
 class Opts(ctypes.Structure):
     _fields_ = [("stop_mode", ctypes.c_int), ("eps", ctypes.c_double), ("tol", ctypes.c_double),
                 ("maxiter", ctypes.c_long), ("order", ctypes.c_int), ("precision", ctypes.c_int),
-                ("print_period", ctypes.c_int)]
+                ("print_period", ctypes.c_int), ("inner_tol", ctypes.c_double),
+                ("inner_maxiter", ctypes.c_long), ("refine_restart", ctypes.c_int)]
 
 
 class Report(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int), ("iters", ctypes.c_long),
                 ("bounds", ctypes.c_double * 2), ("res_hist", _dp), ("res_hist_cap", ctypes.c_long),
                 ("seconds", ctypes.c_double), ("phase_ms", ctypes.c_double * 12),
-                ("phase_calls", ctypes.c_long * 12)]
+                ("phase_calls", ctypes.c_long * 12), ("refinements", ctypes.c_long),
+                ("true_relres", ctypes.c_double)]
 
 
 class DCsr(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int), ("rowptr", _vp), ("col", _vp), ("val", _vp)]
+
+
+# cacg_scsr: the same layout with FP32 values
+SCsr = DCsr
 
 
 _lib = None
@@ -97,6 +103,16 @@ def lib() -> ctypes.CDLL:
         "cacg_k_xrupdate": ([c_int, c_int, c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, c_int, _vp],
                             c_int),
         "cacg_k_colnorm2_ref": ([c_int, c_int, _vp, c_long, _vp, _vp], c_int),
+        "cacg_k_colnorm2_fast": ([c_int, c_int, _vp, _vp, _vp, _vp], c_int),
+        "cacg_k_spmm_f32": ([ctypes.POINTER(DCsr), c_int, _vp, c_long, _vp, c_long, c_int,
+                             c_double, c_double, _vp, c_long, _vp, c_long, _vp], c_int),
+        "cacg_k_cheb1_f32": ([ctypes.POINTER(DCsr), c_int, _vp, c_long, _vp, c_long, _vp, c_long,
+                              c_double, c_double, _vp], c_int),
+        "cacg_k_cvt_f64_f32": ([c_long, _vp, _vp, _vp], c_int),
+        "cacg_k_qupdate_s32": ([c_int, c_int, _vp, _vp, _vp, _vp], c_int),
+        "cacg_k_cvt_f32_f64": ([c_long, _vp, _vp, _vp], c_int),
+        "cacg_k_gram_fast_f64f32": ([c_int, _vp, c_long, c_int, _vp, c_long, c_int, _vp, c_long,
+                                     c_int, _vp, _vp, _vp, _vp], c_int),
         "cacg_k_fill_i32": ([_vp, c_long, c_int, _vp], c_int),
         "cacg_k_fill_f64": ([_vp, c_long, c_double, _vp], c_int),
     }
@@ -161,6 +177,18 @@ class DeviceCSR:
         self.col = DeviceBuffer.from_array(np.asarray(A.col, dtype=np.int32))
         self.val = DeviceBuffer.from_array(np.asarray(A.val, dtype=np.float64))
         self.s = DCsr(A.n, self.rowptr.ptr, self.col.ptr, self.val.ptr)
+        self._s32 = None
+
+    @property
+    def s32(self):
+        """cacg_scsr view: the same indices with an FP32 copy of the values
+        (converted on the device, as the solver does)."""
+        if self._s32 is None:
+            nnz = int(self.val.nbytes // 8)
+            self.val32 = DeviceBuffer(4 * max(nnz, 1))
+            _check(lib().cacg_k_cvt_f64_f32(nnz, self.val.ptr, self.val32.ptr, None), "cvt")
+            self._s32 = SCsr(self.n, self.rowptr.ptr, self.col.ptr, self.val32.ptr)
+        return self._s32
 
 
 # ---- kernel-level wrappers (host arrays in, host arrays out; for tests) -------------
@@ -259,11 +287,83 @@ def k_xrupdate(Q, AQ, alpha, X, R, mode="reference"):
     return dX.download((n, m)), dR.download((n, m)), dn.download(m)
 
 
+def k_spmm_f32(A, X, epi="none", a=0.0, b=0.0, p1=None, p2=None):
+    """FP32 Y = epilogue(A32 X) for X n x k float32 ('none' or 'cheb')."""
+    dA = DeviceCSR(A)
+    X = np.ascontiguousarray(X, dtype=np.float32).reshape(A.n, -1)
+    k = X.shape[1]
+    dX = DeviceBuffer.from_array(X)
+    dY = DeviceBuffer(X.nbytes)
+    d1 = DeviceBuffer.from_array(np.ascontiguousarray(p1, dtype=np.float32)) if p1 is not None else None
+    d2 = DeviceBuffer.from_array(np.ascontiguousarray(p2, dtype=np.float32)) if p2 is not None else None
+    rc = lib().cacg_k_spmm_f32(ctypes.byref(dA.s32), k, dX.ptr, k, dY.ptr, k, EPI[epi], a, b,
+                               d1.ptr if d1 else None, k, d2.ptr if d2 else None, k, None)
+    _check(rc, "spmm_f32")
+    return dY.download((A.n, k), np.float32)
+
+
+def k_cheb1_f32(A, R, a, b):
+    """(S_1, S_0) of the FP32 basis from the FP64 residual R (n x k)."""
+    dA = DeviceCSR(A)
+    R = np.ascontiguousarray(R, dtype=np.float64).reshape(A.n, -1)
+    k = R.shape[1]
+    dR = DeviceBuffer.from_array(R)
+    d1, d0 = DeviceBuffer(4 * A.n * k), DeviceBuffer(4 * A.n * k)
+    _check(lib().cacg_k_cheb1_f32(ctypes.byref(dA.s32), k, dR.ptr, k, d1.ptr, k, d0.ptr, k, a, b,
+                                  None), "cheb1_f32")
+    return d1.download((A.n, k), np.float32), d0.download((A.n, k), np.float32)
+
+
+def k_gram_f64f32(L, R1):
+    """L^T R1 with FP64 L and FP32 R1 (fast order)."""
+    L = np.ascontiguousarray(L, dtype=np.float64)
+    R1 = np.ascontiguousarray(R1, dtype=np.float32)
+    n, kl = L.shape
+    kr1 = R1.shape[1]
+    dL, d1 = DeviceBuffer.from_array(L), DeviceBuffer.from_array(R1)
+    g1 = DeviceBuffer(8 * kl * kr1)
+    ws = DeviceBuffer(8 * lib().cacg_gram_workspace(kl, kr1))
+    _check(lib().cacg_k_gram_fast_f64f32(n, dL.ptr, kl, kl, d1.ptr, kr1, kr1, None, 1, 0, g1.ptr,
+                                         None, ws.ptr, None), "gram_f64f32")
+    return g1.download((kl, kr1))
+
+
+def k_qupdate_s32(S, Q, beta):
+    """Q (FP64) = S (FP32) - Q beta."""
+    S = np.ascontiguousarray(S, dtype=np.float32)
+    n, sm = S.shape
+    dS, dQ = DeviceBuffer.from_array(S), DeviceBuffer.from_array(np.asarray(Q, dtype=np.float64))
+    db = DeviceBuffer.from_array(np.ascontiguousarray(beta, dtype=np.float64))
+    _check(lib().cacg_k_qupdate_s32(n, sm, dS.ptr, dQ.ptr, db.ptr, None), "qupdate_s32")
+    return dQ.download((n, sm))
+
+
+def k_cvt_f32_f64(S):
+    S = np.ascontiguousarray(S, dtype=np.float32)
+    dS, dQ = DeviceBuffer.from_array(S), DeviceBuffer(8 * S.size)
+    _check(lib().cacg_k_cvt_f32_f64(S.size, dS.ptr, dQ.ptr, None), "cvt_f32_f64")
+    return dQ.download(S.shape)
+
+
+def k_colnorm2_fast(R):
+    R = np.ascontiguousarray(R, dtype=np.float64)
+    n, m = R.shape
+    dR, dn = DeviceBuffer.from_array(R), DeviceBuffer(8 * m)
+    ws = DeviceBuffer(8 * lib().cacg_xrupdate_workspace(m))
+    _check(lib().cacg_k_colnorm2_fast(n, m, dR.ptr, dn.ptr, ws.ptr, None), "colnorm2_fast")
+    return dn.download(m)
+
+
 # ---- solver ------------------------------------------------------------------------------
 
 
+PRECISION = {"fp64": 0, "fp32": 1}
+
+
 def make_opts(stop_mode=STOP_REL_PER_COLUMN, eps=0.0, tol=1e-8, maxiter=100, order="fast",
-              print_period=0) -> Opts:
+              print_period=0, precision="fp64", inner_tol=0.0, inner_maxiter=0,
+              refine_restart=0) -> Opts:
+    """precision 'fp32' is the FP32-basis form with FP64 refinement."""
     o = Opts()
     lib().cacg_default_opts(ctypes.byref(o))
     o.stop_mode = stop_mode
@@ -272,6 +372,10 @@ def make_opts(stop_mode=STOP_REL_PER_COLUMN, eps=0.0, tol=1e-8, maxiter=100, ord
     o.maxiter = maxiter
     o.order = MODE[order] if isinstance(order, str) else int(order)
     o.print_period = print_period
+    o.precision = PRECISION[precision] if isinstance(precision, str) else int(precision)
+    o.inner_tol = inner_tol
+    o.inner_maxiter = inner_maxiter
+    o.refine_restart = int(refine_restart)
     return o
 
 
@@ -285,11 +389,14 @@ def _report_dict(rep: Report, hist: np.ndarray, m: int) -> dict:
         "seconds": rep.seconds,
         "phase_ms": dict(zip(PHASES, rep.phase_ms[:])),
         "phase_calls": dict(zip(PHASES, rep.phase_calls[:])),
+        "refinements": int(rep.refinements),
+        "true_relres": float(rep.true_relres),
     }
 
 
 def bcbcg(A, B, X0, s, *, stop_mode=STOP_REL_PER_COLUMN, eps=0.0, tol=1e-8, maxiter=100,
-          order="fast", print_period=0) -> dict:
+          order="fast", print_period=0, precision="fp64", inner_tol=0.0, inner_maxiter=0,
+          refine_restart=0) -> dict:
     """Host-buffer solve through cacg_bcbcg().  Returns dict with X and the report."""
     B = np.ascontiguousarray(B, dtype=np.float64).reshape(A.n, -1)
     m = B.shape[1]
@@ -297,7 +404,8 @@ def bcbcg(A, B, X0, s, *, stop_mode=STOP_REL_PER_COLUMN, eps=0.0, tol=1e-8, maxi
     rp = np.ascontiguousarray(A.rowptr, dtype=np.int64)
     ci = np.ascontiguousarray(A.col, dtype=np.int64)
     va = np.ascontiguousarray(A.val, dtype=np.float64)
-    o = make_opts(stop_mode, eps, tol, maxiter, order, print_period)
+    o = make_opts(stop_mode, eps, tol, maxiter, order, print_period, precision, inner_tol,
+                  inner_maxiter, refine_restart)
     hist = np.zeros((maxiter, m))
     rep = Report()
     rep.res_hist = hist.ctypes.data_as(_dp)

@@ -24,11 +24,37 @@ __device__ __forceinline__ int xcd_remap(int g, int G)
     return (g & 7) * (G >> 3) + (g >> 3);
 }
 
+// K contiguous values of one row (16-byte aligned, K a multiple of 4) into
+// doubles: 16-byte vector loads of 2 doubles or 4 floats
+template <typename T, int K>
+__device__ __forceinline__ void load_row(const T *p, double *out)
+{
+    if constexpr (sizeof(T) == 8) {
+        const double2 *v = reinterpret_cast<const double2 *>(p);
+#pragma unroll
+        for (int u = 0; u < K / 2; u++) {
+            const double2 w = v[u];
+            out[2 * u] = w.x;
+            out[2 * u + 1] = w.y;
+        }
+    } else {
+        const float4 *v = reinterpret_cast<const float4 *>(p);
+#pragma unroll
+        for (int u = 0; u < K / 4; u++) {
+            const float4 w = v[u];
+            out[4 * u] = w.x;
+            out[4 * u + 1] = w.y;
+            out[4 * u + 2] = w.z;
+            out[4 * u + 3] = w.w;
+        }
+    }
+}
+
 // ---- reference order ----------------------------------------------------------
 
-template <bool BETA_LDS>
-__global__ __launch_bounds__(256) void qupdate_ref_kernel(int n, int sm, const double *__restrict__ S,
-                                                          double *Q, const double *__restrict__ beta,
+template <typename TS, typename T, bool BETA_LDS>
+__global__ __launch_bounds__(256) void qupdate_ref_kernel(int n, int sm, const TS *__restrict__ S,
+                                                          T *Q, const double *__restrict__ beta,
                                                           int rb)
 {
     extern __shared__ double lds[];
@@ -46,7 +72,7 @@ __global__ __launch_bounds__(256) void qupdate_ref_kernel(int n, int sm, const d
         const long i0 = (long)tile * rb;
         const int rows = (n - i0) < rb ? (int)(n - i0) : rb;
         for (int e = t; e < rows * sm; e += 256)
-            qt[e] = Q[i0 * sm + e];
+            qt[e] = (double)Q[i0 * sm + e];
         __syncthreads();
         for (int e = t; e < rows * sm; e += 256) {
             const int r = e / sm, c = e % sm;
@@ -54,15 +80,16 @@ __global__ __launch_bounds__(256) void qupdate_ref_kernel(int n, int sm, const d
             double acc = 0.0;
             for (int q = 0; q < sm; q++)
                 acc = add_rn(acc, mul_rn(B[(long)q * sm + c], qr[q]));
-            Q[i0 * sm + e] = sub_rn(S[i0 * sm + e], acc);
+            Q[i0 * sm + e] = (T)sub_rn((double)S[i0 * sm + e], acc);
         }
         __syncthreads();
     }
 }
 
+template <typename T>
 __global__ __launch_bounds__(256) void xrupdate_ref_kernel(int n, int sm, int m,
-                                                           const double *__restrict__ Q,
-                                                           const double *__restrict__ AQ,
+                                                           const T *__restrict__ Q,
+                                                           const T *__restrict__ AQ,
                                                            const double *__restrict__ alpha,
                                                            double *__restrict__ X,
                                                            double *__restrict__ R, int rb)
@@ -80,8 +107,8 @@ __global__ __launch_bounds__(256) void xrupdate_ref_kernel(int n, int sm, int m,
         const long i0 = (long)tile * rb;
         const int rows = (n - i0) < rb ? (int)(n - i0) : rb;
         for (int e = t; e < rows * sm; e += 256) {
-            qt[e] = Q[i0 * sm + e];
-            aqt[e] = AQ[i0 * sm + e];
+            qt[e] = (double)Q[i0 * sm + e];
+            aqt[e] = (double)AQ[i0 * sm + e];
         }
         __syncthreads();
         for (int e = t; e < rows * m; e += 256) {
@@ -127,9 +154,11 @@ __global__ __launch_bounds__(64) void colnorm2_ref_kernel(int n, int m, const do
 // ---- fast order: MFMA -----------------------------------------------------------
 
 // rows per 16-row tile handled by one wave; K = SM, KQ = SM / 4 per lane group
-template <int SM>
-__global__ __launch_bounds__(512) void qupdate_mfma_kernel(int n, const double *__restrict__ S,
-                                                           double *Q, const double *__restrict__ beta)
+// TS, T = storage types of S and Q (double, or float for an FP32 basis); the
+// products and sums are FP64 either way
+template <typename TS, typename T, int SM>
+__global__ __launch_bounds__(512) void qupdate_mfma_kernel(int n, const TS *__restrict__ S, T *Q,
+                                                           const double *__restrict__ beta)
 {
     constexpr int KQ = SM / 4, NTL = SM / 16;
     extern __shared__ double bl[];  // SM x SM
@@ -152,13 +181,7 @@ __global__ __launch_bounds__(512) void qupdate_mfma_kernel(int n, const double *
         const long arow = i0 + lc;
         double av[KQ];
         if (arow < n) {
-            const double2 *src = reinterpret_cast<const double2 *>(Q + arow * SM + lg * KQ);
-#pragma unroll
-            for (int u = 0; u < KQ / 2; u++) {
-                const double2 v = src[u];
-                av[2 * u] = v.x;
-                av[2 * u + 1] = v.y;
-            }
+            load_row<T, KQ>(Q + arow * SM + lg * KQ, av);
         } else {
 #pragma unroll
             for (int u = 0; u < KQ; u++)
@@ -170,7 +193,7 @@ __global__ __launch_bounds__(512) void qupdate_mfma_kernel(int n, const double *
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const long i = i0 + lg + 4 * r;
-                sv[r] = i < n ? S[i * SM + j * 16 + lc] : 0.0;
+                sv[r] = i < n ? (double)S[i * SM + j * 16 + lc] : 0.0;
             }
             dbl4 acc = {0.0, 0.0, 0.0, 0.0};
             const double *bcol = bl + (lg * KQ) * SM + j * 16 + lc;
@@ -181,16 +204,16 @@ __global__ __launch_bounds__(512) void qupdate_mfma_kernel(int n, const double *
             for (int r = 0; r < 4; r++) {
                 const long i = i0 + lg + 4 * r;
                 if (i < n)
-                    Q[i * SM + j * 16 + lc] = sv[r] - acc[r];
+                    Q[i * SM + j * 16 + lc] = (T)(sv[r] - acc[r]);
             }
         }
     }
 }
 
 // NTM = output column tiles (m <= 16 * NTM); norms: per-block partials
-template <int SM, int NTM>
-__global__ __launch_bounds__(256) void xrupdate_mfma_kernel(int n, int m, const double *__restrict__ Q,
-                                                            const double *__restrict__ AQ,
+template <typename T, int SM, int NTM>
+__global__ __launch_bounds__(256) void xrupdate_mfma_kernel(int n, int m, const T *__restrict__ Q,
+                                                            const T *__restrict__ AQ,
                                                             const double *__restrict__ alpha,
                                                             double *__restrict__ X,
                                                             double *__restrict__ R,
@@ -225,16 +248,8 @@ __global__ __launch_bounds__(256) void xrupdate_mfma_kernel(int n, int m, const 
         const long arow = i0 + lc;
         double av[KQ], aq[KQ];
         if (arow < n) {
-            const double2 *s1 = reinterpret_cast<const double2 *>(Q + arow * SM + lg * KQ);
-            const double2 *s2 = reinterpret_cast<const double2 *>(AQ + arow * SM + lg * KQ);
-#pragma unroll
-            for (int u = 0; u < KQ / 2; u++) {
-                const double2 v = s1[u], w = s2[u];
-                av[2 * u] = v.x;
-                av[2 * u + 1] = v.y;
-                aq[2 * u] = w.x;
-                aq[2 * u + 1] = w.y;
-            }
+            load_row<T, KQ>(Q + arow * SM + lg * KQ, av);
+            load_row<T, KQ>(AQ + arow * SM + lg * KQ, aq);
         } else {
 #pragma unroll
             for (int u = 0; u < KQ; u++) {
@@ -338,8 +353,8 @@ __global__ __launch_bounds__(256) void colnorm2_final_kernel(const double *__res
 
 constexpr int XR_BLOCKS = 1024;  // fixed: the norm partials depend on it
 
-template <int SM>
-int launch_qupdate_mfma(int n, const double *S, double *Q, const double *beta, hipStream_t st)
+template <typename TS, typename T, int SM>
+int launch_qupdate_mfma(int n, const TS *S, T *Q, const double *beta, hipStream_t st)
 {
     const int ntiles = (n + 15) / 16;
     int G = (ntiles + 7) / 8;
@@ -347,19 +362,19 @@ int launch_qupdate_mfma(int n, const double *S, double *Q, const double *beta, h
         G = 256;
     if (G >= 8)
         G &= ~7;
-    cacg_allow_lds(qupdate_mfma_kernel<SM>, (size_t)SM * SM * sizeof(double));
-    hipLaunchKernelGGL(qupdate_mfma_kernel<SM>, dim3(G), dim3(512), (size_t)SM * SM * sizeof(double),
-                       st, n, S, Q, beta);
+    cacg_allow_lds(qupdate_mfma_kernel<TS, T, SM>, (size_t)SM * SM * sizeof(double));
+    hipLaunchKernelGGL((qupdate_mfma_kernel<TS, T, SM>), dim3(G), dim3(512),
+                       (size_t)SM * SM * sizeof(double), st, n, S, Q, beta);
     CACG_LAUNCH_CHECK();
     return 0;
 }
 
-template <int SM, int NTM>
-int launch_xrupdate_mfma(int n, int m, const double *Q, const double *AQ, const double *alpha,
-                         double *X, double *R, double *d_norm2, double *work, hipStream_t st)
+template <typename T, int SM, int NTM>
+int launch_xrupdate_mfma(int n, int m, const T *Q, const T *AQ, const double *alpha, double *X,
+                         double *R, double *d_norm2, double *work, hipStream_t st)
 {
-    hipLaunchKernelGGL((xrupdate_mfma_kernel<SM, NTM>), dim3(XR_BLOCKS), dim3(256), 0, st, n, m, Q,
-                       AQ, alpha, X, R, work);
+    hipLaunchKernelGGL((xrupdate_mfma_kernel<T, SM, NTM>), dim3(XR_BLOCKS), dim3(256), 0, st, n, m,
+                       Q, AQ, alpha, X, R, work);
     CACG_LAUNCH_CHECK();
     hipLaunchKernelGGL(colnorm2_final_kernel, dim3(m), dim3(256), 0, st, work, XR_BLOCKS,
                        m, d_norm2);
@@ -380,8 +395,10 @@ extern "C" int cacg_k_colnorm2_ref(int n, int m, const double *R, long ldr, doub
     return 0;
 }
 
-extern "C" int cacg_k_qupdate(int n, int sm, const double *S, double *Q, const double *beta,
-                              int mode, hipStream_t st)
+namespace {
+
+template <typename TS, typename T>
+int qupdate(int n, int sm, const TS *S, T *Q, const double *beta, int mode, hipStream_t st)
 {
     if (n < 0 || sm <= 0)
         return (int)hipErrorInvalidValue;
@@ -389,14 +406,14 @@ extern "C" int cacg_k_qupdate(int n, int sm, const double *S, double *Q, const d
         return 0;
     if (mode == CACG_MODE_FAST) {
         switch (sm) {
-        case 128: return launch_qupdate_mfma<128>(n, S, Q, beta, st);
-        case 112: return launch_qupdate_mfma<112>(n, S, Q, beta, st);
-        case 96: return launch_qupdate_mfma<96>(n, S, Q, beta, st);
-        case 80: return launch_qupdate_mfma<80>(n, S, Q, beta, st);
-        case 64: return launch_qupdate_mfma<64>(n, S, Q, beta, st);
-        case 48: return launch_qupdate_mfma<48>(n, S, Q, beta, st);
-        case 32: return launch_qupdate_mfma<32>(n, S, Q, beta, st);
-        case 16: return launch_qupdate_mfma<16>(n, S, Q, beta, st);
+        case 128: return launch_qupdate_mfma<TS, T, 128>(n, S, Q, beta, st);
+        case 112: return launch_qupdate_mfma<TS, T, 112>(n, S, Q, beta, st);
+        case 96: return launch_qupdate_mfma<TS, T, 96>(n, S, Q, beta, st);
+        case 80: return launch_qupdate_mfma<TS, T, 80>(n, S, Q, beta, st);
+        case 64: return launch_qupdate_mfma<TS, T, 64>(n, S, Q, beta, st);
+        case 48: return launch_qupdate_mfma<TS, T, 48>(n, S, Q, beta, st);
+        case 32: return launch_qupdate_mfma<TS, T, 32>(n, S, Q, beta, st);
+        case 16: return launch_qupdate_mfma<TS, T, 16>(n, S, Q, beta, st);
         default: break;  // other widths: the reference-order kernel
         }
     }
@@ -405,11 +422,29 @@ extern "C" int cacg_k_qupdate(int n, int sm, const double *S, double *Q, const d
     const size_t shm = ((beta_lds ? (size_t)sm * sm : 0) + (size_t)rb * sm) * sizeof(double);
     const int ntile = (n + rb - 1) / rb;
     const int G = ntile < 1024 ? ntile : 1024;
-    cacg_allow_lds(beta_lds ? qupdate_ref_kernel<true> : qupdate_ref_kernel<false>, shm);
+    cacg_allow_lds(beta_lds ? qupdate_ref_kernel<TS, T, true> : qupdate_ref_kernel<TS, T, false>,
+                   shm);
     if (beta_lds)
-        hipLaunchKernelGGL(qupdate_ref_kernel<true>, dim3(G), dim3(256), shm, st, n, sm, S, Q, beta, rb);
+        hipLaunchKernelGGL((qupdate_ref_kernel<TS, T, true>), dim3(G), dim3(256), shm, st, n, sm, S,
+                           Q, beta, rb);
     else
-        hipLaunchKernelGGL(qupdate_ref_kernel<false>, dim3(G), dim3(256), shm, st, n, sm, S, Q, beta, rb);
+        hipLaunchKernelGGL((qupdate_ref_kernel<TS, T, false>), dim3(G), dim3(256), shm, st, n, sm,
+                           S, Q, beta, rb);
+    CACG_LAUNCH_CHECK();
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int cacg_k_colnorm2_fast(int n, int m, const double *R, double *d_norm2,
+                                    double *d_work, hipStream_t st)
+{
+    if (n < 0 || m <= 0)
+        return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(colnorm2_part_kernel, dim3(XR_BLOCKS), dim3(256), 0, st, n, m, R, d_work);
+    CACG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(colnorm2_final_kernel, dim3(m), dim3(256), 0, st, d_work, XR_BLOCKS, m,
+                       d_norm2);
     CACG_LAUNCH_CHECK();
     return 0;
 }
@@ -419,17 +454,19 @@ extern "C" long cacg_xrupdate_workspace(int m)
     return (long)XR_BLOCKS * m;
 }
 
-extern "C" int cacg_k_xrupdate(int n, int sm, int m, const double *Q, const double *AQ,
-                               const double *alpha, double *X, double *R, double *d_norm2,
-                               double *d_work, int mode, hipStream_t st)
+namespace {
+
+template <typename T>
+int xrupdate(int n, int sm, int m, const T *Q, const T *AQ, const double *alpha, double *X,
+             double *R, double *d_norm2, double *d_work, int mode, hipStream_t st)
 {
     if (n < 0 || sm <= 0 || m <= 0)
         return (int)hipErrorInvalidValue;
     if (mode == CACG_MODE_FAST && m <= 32) {
 #define CACG_XR(SMV)                                                                              \
     case SMV:                                                                                     \
-        return m <= 16 ? launch_xrupdate_mfma<SMV, 1>(n, m, Q, AQ, alpha, X, R, d_norm2, d_work, st) \
-                       : launch_xrupdate_mfma<SMV, 2>(n, m, Q, AQ, alpha, X, R, d_norm2, d_work, st);
+        return m <= 16 ? launch_xrupdate_mfma<T, SMV, 1>(n, m, Q, AQ, alpha, X, R, d_norm2, d_work, st) \
+                       : launch_xrupdate_mfma<T, SMV, 2>(n, m, Q, AQ, alpha, X, R, d_norm2, d_work, st);
         switch (sm) {
             CACG_XR(128)
             CACG_XR(112)
@@ -452,9 +489,9 @@ extern "C" int cacg_k_xrupdate(int n, int sm, int m, const double *Q, const doub
             return (int)hipErrorInvalidValue;
         const int ntile = (n + rb - 1) / rb;
         const int G = ntile < 1024 ? ntile : 1024;
-        cacg_allow_lds(xrupdate_ref_kernel, shm);
-        hipLaunchKernelGGL(xrupdate_ref_kernel, dim3(G), dim3(256), shm, st, n, sm, m, Q, AQ, alpha, X,
-                           R, rb);
+        cacg_allow_lds(xrupdate_ref_kernel<T>, shm);
+        hipLaunchKernelGGL(xrupdate_ref_kernel<T>, dim3(G), dim3(256), shm, st, n, sm, m, Q, AQ,
+                           alpha, X, R, rb);
         CACG_LAUNCH_CHECK();
     }
     if (mode == CACG_MODE_REFERENCE)
@@ -466,6 +503,27 @@ extern "C" int cacg_k_xrupdate(int n, int sm, int m, const double *Q, const doub
                        d_norm2);
     CACG_LAUNCH_CHECK();
     return 0;
+}
+
+}  // namespace
+
+extern "C" int cacg_k_qupdate(int n, int sm, const double *S, double *Q, const double *beta,
+                              int mode, hipStream_t st)
+{
+    return qupdate<double, double>(n, sm, S, Q, beta, mode, st);
+}
+
+extern "C" int cacg_k_qupdate_s32(int n, int sm, const float *S, double *Q, const double *beta,
+                                  hipStream_t st)
+{
+    return qupdate<float, double>(n, sm, S, Q, beta, CACG_MODE_FAST, st);
+}
+
+extern "C" int cacg_k_xrupdate(int n, int sm, int m, const double *Q, const double *AQ,
+                               const double *alpha, double *X, double *R, double *d_norm2,
+                               double *d_work, int mode, hipStream_t st)
+{
+    return xrupdate<double>(n, sm, m, Q, AQ, alpha, X, R, d_norm2, d_work, mode, st);
 }
 
 // ---- fills ---------------------------------------------------------------------

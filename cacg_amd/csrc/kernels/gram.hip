@@ -107,11 +107,15 @@ GramPlan plan_gram(int n, int kl, int kr)
     return p;
 }
 
-struct Src {
-    const double *L;
+// L and R1 are FP64 (the FP64 solver) or FP32 (the FP32 basis: Q, AQ, S);
+// R2 (the residual R) is always FP64.  Loaded values are widened to FP64 and
+// every product and sum is FP64.
+template <typename TL, typename T1>
+struct SrcT {
+    const TL *L;
     long ldl;
     int kl;
-    const double *R1;
+    const T1 *R1;
     long ld1;
     int kr1;
     const double *R2;
@@ -119,8 +123,9 @@ struct Src {
     int kr2;
 };
 
-template <int MT, int NT>
-__global__ __launch_bounds__(256) void gram_mfma_kernel(int n, Src s, double *__restrict__ part,
+template <typename TL, typename T1, int MT, int NT>
+__global__ __launch_bounds__(256) void gram_mfma_kernel(int n, SrcT<TL, T1> s,
+                                                        double *__restrict__ part,
                                                         int kl_pad, int kr_pad, int chunk, int GN,
                                                         int NG, int WGG, int KS)
 {
@@ -158,18 +163,18 @@ __global__ __launch_bounds__(256) void gram_mfma_kernel(int n, Src s, double *__
         for (int u = 0; u < 2; u++) {
             const int row = rb + u * 4 * KS + lr;
             const bool rok = row < r1;
-            const double *pl = s.L + (long)row * s.ldl;
-            const double *p1 = s.R1 + (long)row * s.ld1;
+            const TL *pl = s.L + (long)row * s.ldl;
+            const T1 *p1 = s.R1 + (long)row * s.ld1;
             const double *p2 = s.R2 + (long)row * s.ld2 - s.kr1;
 #pragma unroll
             for (int t = 0; t < MT; t++) {
                 const int a = ca0 + 16 * t;
-                av[buf][u][t] = (rok && a < s.kl) ? pl[a] : 0.0;
+                av[buf][u][t] = (rok && a < s.kl) ? (double)pl[a] : 0.0;
             }
 #pragma unroll
             for (int t = 0; t < NT; t++) {
                 const int b = cb0 + 16 * t;
-                bv[buf][u][t] = (rok && b < kr) ? (b < s.kr1 ? p1[b] : p2[b]) : 0.0;
+                bv[buf][u][t] = (rok && b < kr) ? (b < s.kr1 ? (double)p1[b] : p2[b]) : 0.0;
             }
         }
     };
@@ -301,10 +306,10 @@ __global__ __launch_bounds__(256) void gram_reduce_kernel(const double *__restri
 //   <12, 2, 3, 16, 2, 3>, <16, 2, 2, 16, 2, 2>: a 128-row G against 144 and
 //   128 columns, as 12 and 16 waves of 2 x 3 and 2 x 2 tiles (measured on
 //   gfx950: 58 and 56 TFLOP/s against 42 and 50 for the general shape).
-template <int NW, int MT, int NT, int SR, int LDG, int VPT>
-__global__ __launch_bounds__(NW * 64) void gram_lds_kernel(int n, Src s, double *__restrict__ part,
-                                                           int kl_pad, int kr_pad, int chunk,
-                                                           int GN, int NG)
+template <typename TL, typename T1, int NW, int MT, int NT, int SR, int LDG, int VPT>
+__global__ __launch_bounds__(NW * 64) void gram_lds_kernel(int n, SrcT<TL, T1> s,
+                                                           double *__restrict__ part, int kl_pad,
+                                                           int kr_pad, int chunk, int GN, int NG)
 {
     constexpr int NTH = NW * 64;
     extern __shared__ double lds[];
@@ -324,55 +329,74 @@ __global__ __launch_bounds__(NW * 64) void gram_lds_kernel(int n, Src s, double 
         r1 = n;
     const int nstage = (r1 - r0 + SR - 1) / SR;
 
-    // this lane's share of a stage: vector u -> (source row-0 pointer, ld, stage row, LDS offset)
-    const int vl = SR * s.kl / 2, v1 = SR * s.kr1 / 2, v2 = SR * s.kr2 / 2;
-    const double *src[VPT];
-    long ld[VPT];
+    // this lane's share of a stage: 16-byte vector u -> (source row-0 address,
+    // row stride in bytes, stage row, LDS offset, FP32 source); an FP32 vector
+    // carries 4 columns and is widened into 4 doubles of LDS
+    constexpr int EL = 16 / sizeof(TL), E1 = 16 / sizeof(T1);
+    const int vl = SR * s.kl / EL, v1 = SR * s.kr1 / E1, v2 = SR * s.kr2 / 2;
+    const char *src[VPT];
+    long ldb[VPT];
     int srow[VPT], loff[VPT];
+    bool isf[VPT];
 #pragma unroll
     for (int u = 0; u < VPT; u++) {
         int v = threadIdx.x + NTH * u;
         srow[u] = -1;
-        src[u] = s.L;
-        ld[u] = 0;
+        src[u] = reinterpret_cast<const char *>(s.L);
+        ldb[u] = 0;
         loff[u] = 0;
+        isf[u] = false;
         if (v < vl) {
-            const int row = v / (s.kl / 2), c = 2 * (v % (s.kl / 2));
-            src[u] = s.L + c;
-            ld[u] = s.ldl;
+            const int row = v / (s.kl / EL), c = EL * (v % (s.kl / EL));
+            src[u] = reinterpret_cast<const char *>(s.L + c);
+            ldb[u] = s.ldl * (long)sizeof(TL);
             srow[u] = row;
             loff[u] = row * LDP + c;
+            isf[u] = sizeof(TL) == 4;
         } else if ((v -= vl) < v1) {
-            const int row = v / (s.kr1 / 2), c = 2 * (v % (s.kr1 / 2));
-            src[u] = s.R1 + c;
-            ld[u] = s.ld1;
+            const int row = v / (s.kr1 / E1), c = E1 * (v % (s.kr1 / E1));
+            src[u] = reinterpret_cast<const char *>(s.R1 + c);
+            ldb[u] = s.ld1 * (long)sizeof(T1);
             srow[u] = row;
             loff[u] = SR * LDP + row * RDP + c;
+            isf[u] = sizeof(T1) == 4;
         } else if ((v -= v1) < v2) {
             const int row = v / (s.kr2 / 2), c = 2 * (v % (s.kr2 / 2));
-            src[u] = s.R2 + c;
-            ld[u] = s.ld2;
+            src[u] = reinterpret_cast<const char *>(s.R2 + c);
+            ldb[u] = s.ld2 * (long)sizeof(double);
             srow[u] = row;
             loff[u] = SR * LDP + row * RDP + s.kr1 + c;
         }
     }
-    double2 stg[LDG][VPT];
+    uint4 stg[LDG][VPT];
     auto gload = [&](int slot, int st) {
         const int rb = r0 + st * SR;
 #pragma unroll
         for (int u = 0; u < VPT; u++) {
             const int row = rb + srow[u];
             const bool ok = srow[u] >= 0 && st < nstage && row < r1;
-            stg[slot][u] = ok ? *reinterpret_cast<const double2 *>(src[u] + (long)row * ld[u])
-                              : double2{0.0, 0.0};
+            stg[slot][u] = ok ? *reinterpret_cast<const uint4 *>(src[u] + (long)row * ldb[u])
+                              : uint4{0u, 0u, 0u, 0u};
         }
     };
     auto lstore = [&](int slot, int buf) {
         double *b = lds + buf * BUF;
 #pragma unroll
-        for (int u = 0; u < VPT; u++)
-            if (srow[u] >= 0)
-                *reinterpret_cast<double2 *>(b + loff[u]) = stg[slot][u];
+        for (int u = 0; u < VPT; u++) {
+            if (srow[u] < 0)
+                continue;
+            const uint4 w = stg[slot][u];
+            if ((sizeof(TL) == 4 || sizeof(T1) == 4) && isf[u]) {
+                float4 f;
+                __builtin_memcpy(&f, &w, 16);
+                *reinterpret_cast<double2 *>(b + loff[u]) = double2{(double)f.x, (double)f.y};
+                *reinterpret_cast<double2 *>(b + loff[u] + 2) = double2{(double)f.z, (double)f.w};
+            } else {
+                double2 d;
+                __builtin_memcpy(&d, &w, 16);
+                *reinterpret_cast<double2 *>(b + loff[u]) = d;
+            }
+        }
     };
 
     dbl4 acc[MT][NT];
@@ -450,8 +474,16 @@ __global__ __launch_bounds__(NW * 64) void gram_lds_kernel(int n, Src s, double 
     }
 }
 
+// 16-byte vectors of one stage of SR rows of L, R1 and R2
+template <typename TL, typename T1>
+long stage_vectors(const SrcT<TL, T1> &s, int SR)
+{
+    return (long)SR * (s.kl / (16 / (int)sizeof(TL)) + s.kr1 / (16 / (int)sizeof(T1)) + s.kr2 / 2);
+}
+
 // the 128-row shapes of the wide kernels: returns 1 when launched
-int launch_gram_wide(int n, const Src &s, const GramPlan &p, double *work, bool staged_ok,
+template <typename TL, typename T1>
+int launch_gram_wide(int n, const SrcT<TL, T1> &s, const GramPlan &p, double *work, bool staged_ok,
                      hipStream_t st)
 {
     constexpr int SRW = 16;
@@ -461,46 +493,48 @@ int launch_gram_wide(int n, const Src &s, const GramPlan &p, double *work, bool 
         return 0;
     const size_t shm = 2 * (size_t)SRW * (s.kl + 2 + kr + 2) * sizeof(double);
     if (p.TN == 9) {
-        if (SRW * (s.kl + kr) / 2 > 12 * 64 * 3)
+        if (stage_vectors(s, SRW) > 12 * 64 * 3)
             return 0;
-        cacg_allow_lds(gram_lds_kernel<12, 2, 3, SRW, 2, 3>, shm);
-        hipLaunchKernelGGL((gram_lds_kernel<12, 2, 3, SRW, 2, 3>), dim3(p.gx, 1), dim3(12 * 64),
-                           shm, st, n, s, work, kl_pad, kr_pad, p.chunk, 3, 12);
+        cacg_allow_lds(gram_lds_kernel<TL, T1, 12, 2, 3, SRW, 2, 3>, shm);
+        hipLaunchKernelGGL((gram_lds_kernel<TL, T1, 12, 2, 3, SRW, 2, 3>), dim3(p.gx, 1),
+                           dim3(12 * 64), shm, st, n, s, work, kl_pad, kr_pad, p.chunk, 3, 12);
     } else {
-        if (SRW * (s.kl + kr) / 2 > 16 * 64 * 2)
+        if (stage_vectors(s, SRW) > 16 * 64 * 2)
             return 0;
-        cacg_allow_lds(gram_lds_kernel<16, 2, 2, SRW, 2, 2>, shm);
-        hipLaunchKernelGGL((gram_lds_kernel<16, 2, 2, SRW, 2, 2>), dim3(p.gx, 1), dim3(16 * 64),
-                           shm, st, n, s, work, kl_pad, kr_pad, p.chunk, 4, 16);
+        cacg_allow_lds(gram_lds_kernel<TL, T1, 16, 2, 2, SRW, 2, 2>, shm);
+        hipLaunchKernelGGL((gram_lds_kernel<TL, T1, 16, 2, 2, SRW, 2, 2>), dim3(p.gx, 1),
+                           dim3(16 * 64), shm, st, n, s, work, kl_pad, kr_pad, p.chunk, 4, 16);
     }
     return 1;
 }
 
-template <int MT, int NT>
-int launch_gram_mfma(int n, const Src &s, const GramPlan &p, double *G1, double *G2, double *work,
-                     hipStream_t st)
+template <typename TL, typename T1, int MT, int NT>
+int launch_gram_mfma(int n, const SrcT<TL, T1> &s, const GramPlan &p, double *G1, double *G2,
+                     double *work, hipStream_t st)
 {
     constexpr int SR = 8, LDG = 3, VPT = 5;
+    constexpr int EL = 16 / sizeof(TL), E1 = 16 / sizeof(T1);
     const int kl_pad = p.TM * 16, kr_pad = p.TN * 16;
     const int kr = s.kr1 + s.kr2;
     auto al16 = [](const void *q) { return ((uintptr_t)q & 15) == 0; };
-    const bool aligned = p.KS == 1 && s.kl % 2 == 0 && s.kr1 % 2 == 0 && s.kr2 % 2 == 0 &&
-                         s.ldl % 2 == 0 && s.ld1 % 2 == 0 && (s.kr2 == 0 || s.ld2 % 2 == 0) &&
+    // 16-byte vectors: column counts and row strides in whole vectors, aligned rows
+    const bool aligned = p.KS == 1 && s.kl % EL == 0 && s.kr1 % E1 == 0 && s.kr2 % 2 == 0 &&
+                         s.ldl % EL == 0 && s.ld1 % E1 == 0 && (s.kr2 == 0 || s.ld2 % 2 == 0) &&
                          al16(s.L) && al16(s.R1) && (s.kr2 == 0 || al16(s.R2));
-    const bool staged = aligned && SR * (s.kl + kr) / 2 <= 256 * VPT && p.chunk % SR == 0;
+    const bool staged = aligned && stage_vectors(s, SR) <= 256 * VPT && p.chunk % SR == 0;
     if (launch_gram_wide(n, s, p, work, aligned, st)) {
         // launched
     } else if (staged) {
         const size_t shm = 2 * (size_t)SR * (s.kl + 2 + kr + 2) * sizeof(double);
-        cacg_allow_lds(gram_lds_kernel<4, MT, NT, SR, LDG, VPT>, shm);
-        hipLaunchKernelGGL((gram_lds_kernel<4, MT, NT, SR, LDG, VPT>), dim3(p.gx, p.gy),
+        cacg_allow_lds(gram_lds_kernel<TL, T1, 4, MT, NT, SR, LDG, VPT>, shm);
+        hipLaunchKernelGGL((gram_lds_kernel<TL, T1, 4, MT, NT, SR, LDG, VPT>), dim3(p.gx, p.gy),
                            dim3(256), shm, st, n, s, work, kl_pad, kr_pad, p.chunk, p.GN, p.NG);
     } else {
         const size_t shm =
             p.KS > 1 ? (size_t)(p.KS - 1) * p.WGG * MT * NT * 4 * 64 * sizeof(double) : 0;
-        cacg_allow_lds(gram_mfma_kernel<MT, NT>, shm);
-        hipLaunchKernelGGL((gram_mfma_kernel<MT, NT>), dim3(p.gx, p.gy), dim3(256), shm, st, n, s,
-                           work, kl_pad, kr_pad, p.chunk, p.GN, p.NG, p.WGG, p.KS);
+        cacg_allow_lds(gram_mfma_kernel<TL, T1, MT, NT>, shm);
+        hipLaunchKernelGGL((gram_mfma_kernel<TL, T1, MT, NT>), dim3(p.gx, p.gy), dim3(256), shm,
+                           st, n, s, work, kl_pad, kr_pad, p.chunk, p.GN, p.NG, p.WGG, p.KS);
     }
     CACG_LAUNCH_CHECK();
     const long outs = (long)s.kl * (s.kr1 + s.kr2);
@@ -508,6 +542,36 @@ int launch_gram_mfma(int n, const Src &s, const GramPlan &p, double *G1, double 
                        p.gx, s.kl, s.kr1, s.kr2, kl_pad, kr_pad, G1, G2);
     CACG_LAUNCH_CHECK();
     return 0;
+}
+
+template <typename TL, typename T1>
+int gram_fast(int n, const TL *L, long ldl, int kl, const T1 *R1, long ld1, int kr1,
+              const double *R2, long ld2, int kr2, double *G1, double *G2, double *d_work,
+              hipStream_t st)
+{
+    if (n < 0 || kl <= 0 || kr1 < 0 || kr2 < 0 || kr1 + kr2 <= 0)
+        return (int)hipErrorInvalidValue;
+    const GramPlan p = plan_gram(n, kl, kr1 + kr2);
+    const SrcT<TL, T1> s = {L, ldl, kl, R1, ld1, kr1, R2, ld2, kr2};
+    // MT in {1, 2}, NT in 1..9
+#define CACG_GRAM_NT(MTV)                                                                         \
+    switch (p.NT) {                                                                               \
+    case 1: return launch_gram_mfma<TL, T1, MTV, 1>(n, s, p, G1, G2, d_work, st);                 \
+    case 2: return launch_gram_mfma<TL, T1, MTV, 2>(n, s, p, G1, G2, d_work, st);                 \
+    case 3: return launch_gram_mfma<TL, T1, MTV, 3>(n, s, p, G1, G2, d_work, st);                 \
+    case 4: return launch_gram_mfma<TL, T1, MTV, 4>(n, s, p, G1, G2, d_work, st);                 \
+    case 5: return launch_gram_mfma<TL, T1, MTV, 5>(n, s, p, G1, G2, d_work, st);                 \
+    case 6: return launch_gram_mfma<TL, T1, MTV, 6>(n, s, p, G1, G2, d_work, st);                 \
+    case 7: return launch_gram_mfma<TL, T1, MTV, 7>(n, s, p, G1, G2, d_work, st);                 \
+    case 8: return launch_gram_mfma<TL, T1, MTV, 8>(n, s, p, G1, G2, d_work, st);                 \
+    default: return launch_gram_mfma<TL, T1, MTV, 9>(n, s, p, G1, G2, d_work, st);                \
+    }
+    if (p.MT == 2) {
+        CACG_GRAM_NT(2)
+    } else {
+        CACG_GRAM_NT(1)
+    }
+#undef CACG_GRAM_NT
 }
 
 }  // namespace
@@ -535,27 +599,14 @@ extern "C" int cacg_k_gram_fast(int n, const double *L, long ldl, int kl, const 
                                 long ld1, int kr1, const double *R2, long ld2, int kr2, double *G1,
                                 double *G2, double *d_work, hipStream_t st)
 {
-    if (n < 0 || kl <= 0 || kr1 < 0 || kr2 < 0 || kr1 + kr2 <= 0)
-        return (int)hipErrorInvalidValue;
-    const GramPlan p = plan_gram(n, kl, kr1 + kr2);
-    const Src s = {L, ldl, kl, R1, ld1, kr1, R2, ld2, kr2};
-    // MT in {1, 2}, NT in 1..9
-#define CACG_GRAM_NT(MTV)                                                                         \
-    switch (p.NT) {                                                                               \
-    case 1: return launch_gram_mfma<MTV, 1>(n, s, p, G1, G2, d_work, st);                              \
-    case 2: return launch_gram_mfma<MTV, 2>(n, s, p, G1, G2, d_work, st);                              \
-    case 3: return launch_gram_mfma<MTV, 3>(n, s, p, G1, G2, d_work, st);                              \
-    case 4: return launch_gram_mfma<MTV, 4>(n, s, p, G1, G2, d_work, st);                              \
-    case 5: return launch_gram_mfma<MTV, 5>(n, s, p, G1, G2, d_work, st);                              \
-    case 6: return launch_gram_mfma<MTV, 6>(n, s, p, G1, G2, d_work, st);                              \
-    case 7: return launch_gram_mfma<MTV, 7>(n, s, p, G1, G2, d_work, st);                              \
-    case 8: return launch_gram_mfma<MTV, 8>(n, s, p, G1, G2, d_work, st);                              \
-    default: return launch_gram_mfma<MTV, 9>(n, s, p, G1, G2, d_work, st);                             \
-    }
-    if (p.MT == 2) {
-        CACG_GRAM_NT(2)
-    } else {
-        CACG_GRAM_NT(1)
-    }
-#undef CACG_GRAM_NT
+    return gram_fast<double, double>(n, L, ldl, kl, R1, ld1, kr1, R2, ld2, kr2, G1, G2, d_work,
+                                     st);
+}
+
+extern "C" int cacg_k_gram_fast_f64f32(int n, const double *L, long ldl, int kl, const float *R1,
+                                       long ld1, int kr1, const double *R2, long ld2, int kr2,
+                                       double *G1, double *G2, double *d_work, hipStream_t st)
+{
+    return gram_fast<double, float>(n, L, ldl, kl, R1, ld1, kr1, R2, ld2, kr2, G1, G2, d_work,
+                                    st);
 }

@@ -34,6 +34,24 @@ __device__ __forceinline__ double sub_rn(double a, double b)
     return a - b;
 }
 
+// FP32 versions for the FP32-basis kernels (no parity target; the same
+// separate rounding keeps their results independent of compiler contraction)
+__device__ __forceinline__ float mul_rn(float a, float b)
+{
+#pragma clang fp contract(off)
+    return a * b;
+}
+__device__ __forceinline__ float add_rn(float a, float b)
+{
+#pragma clang fp contract(off)
+    return a + b;
+}
+__device__ __forceinline__ float sub_rn(float a, float b)
+{
+#pragma clang fp contract(off)
+    return a - b;
+}
+
 __device__ __forceinline__ dbl4 mfma16x16x4(double a, double b, dbl4 c)
 {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);

@@ -29,13 +29,39 @@
 namespace {
 
 
-template <int VW>
-struct dvec {
-    double2 h[VW / 2];
+// VW consecutive elements of T from a 16-byte aligned address (VW = 4) or
+// one element (VW = 1)
+template <typename T, int VW>
+struct vecld;
+template <typename T>
+struct vecld<T, 1> {
+    T v[1];
+    __device__ __forceinline__ void load(const T *p) { v[0] = *p; }
 };
 template <>
-struct dvec<1> {
-    double h;
+struct vecld<double, 4> {
+    double v[4];
+    __device__ __forceinline__ void load(const double *p)
+    {
+        const double2 a = reinterpret_cast<const double2 *>(p)[0];
+        const double2 b = reinterpret_cast<const double2 *>(p)[1];
+        v[0] = a.x;
+        v[1] = a.y;
+        v[2] = b.x;
+        v[3] = b.y;
+    }
+};
+template <>
+struct vecld<float, 4> {
+    float v[4];
+    __device__ __forceinline__ void load(const float *p)
+    {
+        const float4 a = *reinterpret_cast<const float4 *>(p);
+        v[0] = a.x;
+        v[1] = a.y;
+        v[2] = a.z;
+        v[3] = a.w;
+    }
 };
 
 // One workgroup = 256 lanes = NG lane groups of LPR lanes; it walks a range
@@ -46,19 +72,25 @@ struct dvec<1> {
 // Lane group g takes rows g, g + NG, ... of the tile; the epilogue operands of
 // a row are loaded before its gathers.  Requires k % VW == 0 and, for VW > 1,
 // x and ldx 16-byte aligned.
-template <int LPR, int VW, int RPG, int EPI>
+// Types: TV = matrix values and arithmetic, TX = gathered x, TY = y, p1, p2
+// and copy0; x and the epilogue operands are converted to TV on load.  The
+// FP64 solver uses <double, double, double>; the FP32 basis <float, float,
+// float>, and <float, double, float> for its first step, which reads the FP64
+// residual and writes its FP32 copy S_0.
+template <typename TV, typename TX, typename TY, int LPR, int VW, int RPG, int EPI>
 __global__ __launch_bounds__(256) void spmm_kernel(
     int n, const int *__restrict__ rowptr, const int *__restrict__ colidx,
-    const double *__restrict__ val, int k, const double *__restrict__ x, long ldx,
-    double *__restrict__ y, long ldy, double a, double b, const double *__restrict__ p1, long ld1,
-    const double *__restrict__ p2, long ld2, double *__restrict__ copy0, long ldc)
+    const TV *__restrict__ val, int k, const TX *__restrict__ x, long ldx, TY *__restrict__ y,
+    long ldy, double ad, double bd, const TY *__restrict__ p1, long ld1, const TY *__restrict__ p2,
+    long ld2, TY *__restrict__ copy0, long ldc)
 {
     constexpr int NG = 256 / LPR, TR = NG * RPG;
     constexpr int CAP = TR * 8 > 256 ? TR * 8 : 256;
     constexpr int UNR = 8;
     __shared__ int s_rp[TR + 1];
     __shared__ int s_col[CAP];
-    __shared__ double s_val[CAP];
+    __shared__ TV s_val[CAP];
+    const TV a = (TV)ad, b = (TV)bd;
     const int t = threadIdx.x, g = t / LPR, lane = t % LPR;
     const int ntiles = (n + TR - 1) / TR;
     // band of this workgroup's XCD and the workgroup's slot in it
@@ -92,56 +124,55 @@ __global__ __launch_bounds__(256) void spmm_kernel(
                     continue;
                 const long i = r0 + li;
                 const int pb = s_rp[li] - pbase, pe = s_rp[li + 1] - pbase;
-                double acc[VW], e1[VW], e2[VW];
+                TV acc[VW], e1[VW], e2[VW];
 #pragma unroll
                 for (int v = 0; v < VW; v++) {
-                    acc[v] = 0.0;
+                    acc[v] = (TV)0;
                     if (EPI == CACG_EPI_RESID) {
-                        e1[v] = __builtin_nontemporal_load(p1 + i * ld1 + c0 + v);
+                        e1[v] = (TV)__builtin_nontemporal_load(p1 + i * ld1 + c0 + v);
                     } else if (EPI == CACG_EPI_CHEB1) {
-                        e1[v] = x[i * ldx + c0 + v];
+                        e1[v] = (TV)x[i * ldx + c0 + v];
                     } else if (EPI == CACG_EPI_CHEB) {
-                        e1[v] = p1[i * ld1 + c0 + v];
-                        e2[v] = __builtin_nontemporal_load(p2 + i * ld2 + c0 + v);
+                        e1[v] = (TV)p1[i * ld1 + c0 + v];
+                        e2[v] = (TV)__builtin_nontemporal_load(p2 + i * ld2 + c0 + v);
                     }
                 }
                 for (int p = pb; p < pe; p += UNR) {
                     int cc[UNR];
-                    double vv[UNR];
+                    TV vv[UNR];
 #pragma unroll
                     for (int u = 0; u < UNR; u++) {
                         const bool in = p + u < pe;
                         cc[u] = in ? (staged ? s_col[p + u] : colidx[pbase + p + u]) : 0;
-                        vv[u] = in ? (staged ? s_val[p + u] : val[pbase + p + u]) : 0.0;
+                        vv[u] = in ? (staged ? s_val[p + u] : val[pbase + p + u]) : (TV)0;
                     }
-                    dvec<VW> xv[UNR];
+                    vecld<TX, VW> xv[UNR];
 #pragma unroll
                     for (int u = 0; u < UNR; u++)
                         if (p + u < pe)
-                            xv[u] = *(const dvec<VW> *)(x + (long)cc[u] * ldx + c0);
+                            xv[u].load(x + (long)cc[u] * ldx + c0);
 #pragma unroll
                     for (int u = 0; u < UNR; u++)
                         if (p + u < pe) {
-                            const double *xs = (const double *)&xv[u];
 #pragma unroll
                             for (int v = 0; v < VW; v++)
-                                acc[v] = add_rn(acc[v], mul_rn(vv[u], xs[v]));
+                                acc[v] = add_rn(acc[v], mul_rn(vv[u], (TV)xv[u].v[v]));
                         }
                 }
 #pragma unroll
                 for (int v = 0; v < VW; v++) {
-                    double out;
+                    TV out;
                     if (EPI == CACG_EPI_NONE) {
                         out = acc[v];
                     } else if (EPI == CACG_EPI_RESID) {
                         out = sub_rn(e1[v], acc[v]);
                     } else if (EPI == CACG_EPI_CHEB1) {
                         out = add_rn(mul_rn(a, acc[v]), mul_rn(b, e1[v]));
-                        __builtin_nontemporal_store(e1[v], copy0 + i * ldc + c0 + v);
+                        __builtin_nontemporal_store((TY)e1[v], copy0 + i * ldc + c0 + v);
                     } else {
                         out = sub_rn(add_rn(mul_rn(a, acc[v]), mul_rn(b, e1[v])), e2[v]);
                     }
-                    __builtin_nontemporal_store(out, y + i * ldy + c0 + v);
+                    __builtin_nontemporal_store((TY)out, y + i * ldy + c0 + v);
                 }
             }
         }
@@ -152,42 +183,91 @@ __global__ __launch_bounds__(256) void spmm_kernel(
 // Launch shape (measured on the 128^3 Laplacian, k = 16 and k = 128): narrow
 // rows (LPR <= 8) one row per lane group and 32 workgroups per CU; wide rows
 // two rows per lane group and 16 workgroups per CU.
-template <int LPR, int VW>
-int launch_spmm(const cacg_dcsr *A, int k, const double *x, long ldx, double *y, long ldy, int epi,
-                double a, double b, const double *p1, long ld1, const double *p2, long ld2,
-                double *copy0, long ldc, hipStream_t st)
+template <typename TV, typename TX, typename TY, int LPR, int VW>
+int launch_spmm(int n, const int *rowptr, const int *col, const TV *val, int k, const TX *x,
+                long ldx, TY *y, long ldy, int epi, double a, double b, const TY *p1, long ld1,
+                const TY *p2, long ld2, TY *copy0, long ldc, hipStream_t st)
 {
     constexpr int RPG = LPR >= 16 ? 2 : 1;
     constexpr int TR = (256 / LPR) * RPG;
     const int gmax = LPR >= 16 ? 4096 : 8192;
-    const int ntiles = (A->n + TR - 1) / TR;
+    const int ntiles = (n + TR - 1) / TR;
     // a multiple of 8 so every XCD gets the same number of workgroups for its band
     int G = ntiles < gmax ? ntiles : gmax;
     if (G >= 8)
         G &= ~7;
 #define CACG_SPMM_CASE(E)                                                                        \
-    hipLaunchKernelGGL((spmm_kernel<LPR, VW, RPG, E>), dim3(G), dim3(256), 0, st, A->n,           \
-                       A->rowptr, A->col, A->val, k, x, ldx, y, ldy, a, b, p1, ld1, p2, ld2, copy0, \
-                       ldc)
-    switch (epi) {
-    case CACG_EPI_NONE:
-        CACG_SPMM_CASE(CACG_EPI_NONE);
-        break;
-    case CACG_EPI_RESID:
-        CACG_SPMM_CASE(CACG_EPI_RESID);
-        break;
-    case CACG_EPI_CHEB1:
-        CACG_SPMM_CASE(CACG_EPI_CHEB1);
-        break;
-    case CACG_EPI_CHEB:
-        CACG_SPMM_CASE(CACG_EPI_CHEB);
-        break;
-    default:
-        return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL((spmm_kernel<TV, TX, TY, LPR, VW, RPG, E>), dim3(G), dim3(256), 0, st, n,  \
+                       rowptr, col, val, k, x, ldx, y, ldy, a, b, p1, ld1, p2, ld2, copy0, ldc)
+    constexpr bool f64 = sizeof(TV) == 8 && sizeof(TX) == 8 && sizeof(TY) == 8;
+    constexpr bool f32 = sizeof(TV) == 4 && sizeof(TX) == 4 && sizeof(TY) == 4;
+    if constexpr (f64) {
+        if (epi == CACG_EPI_NONE)
+            CACG_SPMM_CASE(CACG_EPI_NONE);
+        else if (epi == CACG_EPI_RESID)
+            CACG_SPMM_CASE(CACG_EPI_RESID);
+        else if (epi == CACG_EPI_CHEB1)
+            CACG_SPMM_CASE(CACG_EPI_CHEB1);
+        else if (epi == CACG_EPI_CHEB)
+            CACG_SPMM_CASE(CACG_EPI_CHEB);
+        else
+            return (int)hipErrorInvalidValue;
+    } else if constexpr (f32) {
+        if (epi == CACG_EPI_NONE)
+            CACG_SPMM_CASE(CACG_EPI_NONE);
+        else if (epi == CACG_EPI_CHEB)
+            CACG_SPMM_CASE(CACG_EPI_CHEB);
+        else
+            return (int)hipErrorInvalidValue;
+    } else {
+        // FP32 arithmetic on an FP64 x: the first basis step only
+        if (epi == CACG_EPI_CHEB1)
+            CACG_SPMM_CASE(CACG_EPI_CHEB1);
+        else
+            return (int)hipErrorInvalidValue;
     }
 #undef CACG_SPMM_CASE
     CACG_LAUNCH_CHECK();
     return 0;
+}
+
+// lanes per row and columns per lane from k and the alignment of x
+template <typename TV, typename TX, typename TY>
+int dispatch_spmm(int n, const int *rowptr, const int *col, const TV *val, int k, const TX *x,
+                  long ldx, TY *y, long ldy, int epi, double a, double b, const TY *p1, long ld1,
+                  const TY *p2, long ld2, TY *copy0, long ldc, hipStream_t st)
+{
+#define CACG_SPMM_ARGS n, rowptr, col, val, k, x, ldx, y, ldy, epi, a, b, p1, ld1, p2, ld2, copy0, ldc, st
+    const bool vec4 = k % 4 == 0 && ((uintptr_t)x & 15) == 0 && (ldx * (long)sizeof(TX)) % 16 == 0;
+    if (vec4) {
+        // LPR = k / 4 rounded up to a power of two, at most 32 (wider k loops)
+        const int q = k / 4;
+        if (q > 16)
+            return launch_spmm<TV, TX, TY, 32, 4>(CACG_SPMM_ARGS);
+        if (q > 8)
+            return launch_spmm<TV, TX, TY, 16, 4>(CACG_SPMM_ARGS);
+        if (q > 4)
+            return launch_spmm<TV, TX, TY, 8, 4>(CACG_SPMM_ARGS);
+        if (q > 2)
+            return launch_spmm<TV, TX, TY, 4, 4>(CACG_SPMM_ARGS);
+        if (q > 1)
+            return launch_spmm<TV, TX, TY, 2, 4>(CACG_SPMM_ARGS);
+        return launch_spmm<TV, TX, TY, 1, 4>(CACG_SPMM_ARGS);
+    }
+    if (k > 32)
+        return launch_spmm<TV, TX, TY, 64, 1>(CACG_SPMM_ARGS);
+    if (k > 16)
+        return launch_spmm<TV, TX, TY, 32, 1>(CACG_SPMM_ARGS);
+    if (k > 8)
+        return launch_spmm<TV, TX, TY, 16, 1>(CACG_SPMM_ARGS);
+    if (k > 4)
+        return launch_spmm<TV, TX, TY, 8, 1>(CACG_SPMM_ARGS);
+    if (k > 2)
+        return launch_spmm<TV, TX, TY, 4, 1>(CACG_SPMM_ARGS);
+    if (k > 1)
+        return launch_spmm<TV, TX, TY, 2, 1>(CACG_SPMM_ARGS);
+    return launch_spmm<TV, TX, TY, 1, 1>(CACG_SPMM_ARGS);
+#undef CACG_SPMM_ARGS
 }
 
 // ---- Gerschgorin ------------------------------------------------------------
@@ -265,37 +345,74 @@ extern "C" int cacg_k_spmm(const cacg_dcsr *A, int k, const double *x, long ldx,
         return (int)hipErrorInvalidValue;
     if (A->n == 0)
         return 0;
-#define CACG_SPMM_ARGS A, k, x, ldx, y, ldy, epi, a, b, p1, ld1, p2, ld2, copy0, ldc, st
-    const bool vec4 = k % 4 == 0 && ((uintptr_t)x & 15) == 0 && ldx % 2 == 0;
-    if (vec4) {
-        // LPR = k / 4 rounded up to a power of two, at most 32 (wider k loops)
-        const int q = k / 4;
-        if (q > 16)
-            return launch_spmm<32, 4>(CACG_SPMM_ARGS);
-        if (q > 8)
-            return launch_spmm<16, 4>(CACG_SPMM_ARGS);
-        if (q > 4)
-            return launch_spmm<8, 4>(CACG_SPMM_ARGS);
-        if (q > 2)
-            return launch_spmm<4, 4>(CACG_SPMM_ARGS);
-        if (q > 1)
-            return launch_spmm<2, 4>(CACG_SPMM_ARGS);
-        return launch_spmm<1, 4>(CACG_SPMM_ARGS);
-    }
-    if (k > 32)
-        return launch_spmm<64, 1>(CACG_SPMM_ARGS);
-    if (k > 16)
-        return launch_spmm<32, 1>(CACG_SPMM_ARGS);
-    if (k > 8)
-        return launch_spmm<16, 1>(CACG_SPMM_ARGS);
-    if (k > 4)
-        return launch_spmm<8, 1>(CACG_SPMM_ARGS);
-    if (k > 2)
-        return launch_spmm<4, 1>(CACG_SPMM_ARGS);
-    if (k > 1)
-        return launch_spmm<2, 1>(CACG_SPMM_ARGS);
-    return launch_spmm<1, 1>(CACG_SPMM_ARGS);
-#undef CACG_SPMM_ARGS
+    return dispatch_spmm<double, double, double>(A->n, A->rowptr, A->col, A->val, k, x, ldx, y,
+                                                 ldy, epi, a, b, p1, ld1, p2, ld2, copy0, ldc, st);
+}
+
+extern "C" int cacg_k_spmm_f32(const cacg_scsr *A, int k, const float *x, long ldx, float *y,
+                               long ldy, int epi, double a, double b, const float *p1, long ld1,
+                               const float *p2, long ld2, hipStream_t st)
+{
+    if (!A || k <= 0 || A->n < 0 || (epi != CACG_EPI_NONE && epi != CACG_EPI_CHEB))
+        return (int)hipErrorInvalidValue;
+    if (A->n == 0)
+        return 0;
+    return dispatch_spmm<float, float, float>(A->n, A->rowptr, A->col, A->val, k, x, ldx, y, ldy,
+                                              epi, a, b, p1, ld1, p2, ld2, NULL, 0, st);
+}
+
+extern "C" int cacg_k_cheb1_f32(const cacg_scsr *A, int k, const double *r, long ldr, float *s1,
+                                long ld1, float *s0, long ld0, double a, double b, hipStream_t st)
+{
+    if (!A || k <= 0 || A->n < 0 || !s0)
+        return (int)hipErrorInvalidValue;
+    if (A->n == 0)
+        return 0;
+    return dispatch_spmm<float, double, float>(A->n, A->rowptr, A->col, A->val, k, r, ldr, s1,
+                                               ld1, CACG_EPI_CHEB1, a, b, NULL, 0, NULL, 0, s0,
+                                               ld0, st);
+}
+
+namespace {
+__global__ __launch_bounds__(256) void cvt_f64_f32_kernel(long count, const double *__restrict__ src,
+                                                          float *__restrict__ dst)
+{
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < count; e += (long)gridDim.x * 256)
+        dst[e] = (float)src[e];
+}
+}  // namespace
+
+namespace {
+__global__ __launch_bounds__(256) void cvt_f32_f64_kernel(long count, const float *__restrict__ src,
+                                                          double *__restrict__ dst)
+{
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < count; e += (long)gridDim.x * 256)
+        dst[e] = (double)src[e];
+}
+}  // namespace
+
+extern "C" int cacg_k_cvt_f32_f64(long count, const float *src, double *dst, hipStream_t st)
+{
+    if (count < 0)
+        return (int)hipErrorInvalidValue;
+    if (count == 0)
+        return 0;
+    hipLaunchKernelGGL(cvt_f32_f64_kernel, dim3(cacg_grid(count, 256, 4096)), dim3(256), 0, st,
+                       count, src, dst);
+    CACG_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int cacg_k_cvt_f64_f32(long count, const double *src, float *dst, hipStream_t st)
+{
+    if (count < 0)
+        return (int)hipErrorInvalidValue;
+    if (count == 0)
+        return 0;
+    hipLaunchKernelGGL(cvt_f64_f32_kernel, dim3(cacg_grid(count, 256, 4096)), dim3(256), 0, st,
+                       count, src, dst);
+    CACG_LAUNCH_CHECK();
+    return 0;
 }
 
 extern "C" int cacg_k_gerschgorin(const cacg_dcsr *A, double *d_work, double *d_out,

@@ -89,7 +89,14 @@ enum { CACG_STOP_REF_COL0_ABS = 0, CACG_STOP_REL_PER_COLUMN = 1 };
  * order is for verification sizes.  CACG_ORDER_FAST: split-K FP64 MFMA Gram
  * and block products with a fixed (deterministic) reduction order. */
 enum { CACG_ORDER_REFERENCE = 0, CACG_ORDER_FAST = 1 };
-enum { CACG_PREC_FP64 = 0 };
+/* CACG_PREC_FP64: the reference's arithmetic, everything FP64.
+ * CACG_PREC_FP32_REFINE: matrix values, basis S, Q and A Q stored in FP32 and
+ * the Chebyshev / A Q SpMMs run in FP32 (half the bytes); Gram products, the
+ * coefficient solves, X, R and the stop test stay FP64.  The true residual
+ * R = B - A X is recomputed in FP64 (FP64 matrix) at refinement points (see
+ * cacg_opts.inner_tol) and the stop test is on it.  No reference counterpart
+ * (the reference is FP64 only); requires CACG_ORDER_FAST. */
+enum { CACG_PREC_FP64 = 0, CACG_PREC_FP32_REFINE = 1 };
 
 typedef struct {
     int stop_mode;       /* CACG_STOP_* */
@@ -100,6 +107,18 @@ typedef struct {
     int precision;       /* CACG_PREC_* */
     int print_period;    /* > 0: print the reference's "Loop: %d, R[1]_norm=%30.30f" line every
                             print_period iterations, on stop and on the last iteration */
+    /* CACG_PREC_FP32_REFINE only.  The true residual B - A X is recomputed in
+     * FP64 when every column's recursive residual is below inner_tol times its
+     * true residual at the previous recompute (or below the final target), or
+     * after inner_maxiter iterations since it.  refine_restart = 0 (residual
+     * replacement): the recomputed residual replaces the recursive one and the
+     * recurrence continues with its Q, A Q and Gram factor; 1 (classic
+     * iterative refinement): the s-step recurrence restarts from it.
+     * 0 selects the defaults (inner_tol 1e-2 when replacing, 1e-5 when
+     * restarting; no cap besides maxiter). */
+    double inner_tol;
+    long inner_maxiter;
+    int refine_restart;
 } cacg_opts;
 
 typedef struct {
@@ -111,6 +130,9 @@ typedef struct {
     double seconds;      /* device time of the solve (setup + iterations) */
     double phase_ms[12]; /* device time per phase, see CACG_PHASE_* */
     long phase_calls[12];
+    long refinements;    /* CACG_PREC_FP32_REFINE: true-residual recomputes after the first */
+    double true_relres;  /* CACG_PREC_FP32_REFINE: max over columns of ||B - A X||/||B|| at exit
+                            (stop mode REL_PER_COLUMN); -1 when not computed */
 } cacg_report;
 
 /* Each phase is one kernel type (plus its small reduction kernel), so its time
@@ -125,7 +147,8 @@ enum {
     CACG_PHASE_XRUPDATE = 6, /* X += Q alpha, R -= A Q alpha, column norms */
     CACG_PHASE_OTHER = 7,    /* host gaps between iterations */
     CACG_PHASE_BASIS1 = 8,   /* Chebyshev basis step 1 (S_0 = R, S_1) */
-    CACG_PHASE_GRAM_AS = 9   /* Q^T A S */
+    CACG_PHASE_GRAM_AS = 9,  /* Q^T A S */
+    CACG_PHASE_REFINE = 10   /* true residual R = B - A X and its norms (CACG_PREC_FP32_REFINE) */
 };
 
 void cacg_default_opts(cacg_opts *o);

@@ -25,6 +25,14 @@ typedef struct {
     const double *val;
 } cacg_dcsr;
 
+/* the same pattern with FP32 values (the FP32-basis precision form) */
+typedef struct {
+    int n;
+    const int *rowptr;
+    const int *col;
+    const float *val;
+} cacg_scsr;
+
 enum { CACG_MODE_REFERENCE = 0, CACG_MODE_FAST = 1 };
 
 /* Gerschgorin interval (src/common.c:53-106): d_out[0] = min_i (a_ii - sum|a_ij|),
@@ -44,6 +52,20 @@ int cacg_k_spmm(const cacg_dcsr *A, int k, const double *x, long ldx, double *y,
                 double a, double b, const double *p1, long ld1, const double *p2, long ld2,
                 double *copy0, long ldc, hipStream_t st);
 
+/* FP32 forms for the FP32 basis, same CSR order with FP32 products and sums
+ * (no reference counterpart: the reference is FP64 only):
+ *   cacg_k_spmm_f32 : epi CACG_EPI_CHEB (S_j) or CACG_EPI_NONE (y = A x), all FP32;
+ *   cacg_k_cheb1_f32: s0 = fl32(r), s1 = a (A s0) + b s0, r the FP64 residual. */
+int cacg_k_spmm_f32(const cacg_scsr *A, int k, const float *x, long ldx, float *y, long ldy,
+                    int epi, double a, double b, const float *p1, long ld1, const float *p2,
+                    long ld2, hipStream_t st);
+int cacg_k_cheb1_f32(const cacg_scsr *A, int k, const double *r, long ldr, float *s1, long ld1,
+                     float *s0, long ld0, double a, double b, hipStream_t st);
+/* dst[e] = fl32(src[e]), e < count (the FP32 copy of the matrix values) */
+int cacg_k_cvt_f64_f32(long count, const double *src, float *dst, hipStream_t st);
+/* dst[e] = src[e] widened to FP64 (Q = S of a fresh FP32 basis) */
+int cacg_k_cvt_f32_f64(long count, const float *src, double *dst, hipStream_t st);
+
 /* [G1 | G2] = L^T [R1 | R2] over rows 0..n-1: G1 is kl x kr1 and G2 is kl x kr2,
  * both row-major and contiguous (G2 may be NULL when kr2 == 0).
  * Reference order: each entry is one sequential chain over the rows, products
@@ -56,6 +78,11 @@ long cacg_gram_workspace(int kl, int kr);
 int cacg_k_gram_fast(int n, const double *L, long ldl, int kl, const double *R1, long ld1, int kr1,
                      const double *R2, long ld2, int kr2, double *G1, double *G2, double *d_work,
                      hipStream_t st);
+/* The fast order with FP64 L and FP32 R1 (Q^T A S = AQ^T S against an FP32
+ * basis): R1 widened to FP64 on load, FP64 products and accumulation. */
+int cacg_k_gram_fast_f64f32(int n, const double *L, long ldl, int kl, const float *R1, long ld1,
+                            int kr1, const double *R2, long ld2, int kr2, double *G1, double *G2,
+                            double *d_work, hipStream_t st);
 
 /* x (N x k) = lhs^{-1} rhs with the reference's Crout factorisation (unit-diagonal
  * U, no pivoting) and substitutions (src/DLSsolvers.c:106-208).  lhs is not
@@ -72,6 +99,9 @@ int cacg_k_lu_solve(int N, const double *lhs, int k, const double *rhs, double *
  * (src/BLAS3.c:794-825 then 579-612). */
 int cacg_k_qupdate(int n, int sm, const double *S, double *Q, const double *beta, int mode,
                    hipStream_t st);
+/* FP32 basis S, FP64 Q (the precision form's update): Q = S - Q beta in FP64. */
+int cacg_k_qupdate_s32(int n, int sm, const float *S, double *Q, const double *beta,
+                       hipStream_t st);
 
 /* X += Q alpha; R -= AQ alpha (Q, AQ n x sm; alpha sm x m; X, R n x m) and
  * d_norm2[c] = sum_i R[i][c]^2 of the updated R.  Reference mode keeps the
@@ -85,6 +115,10 @@ int cacg_k_xrupdate(int n, int sm, int m, const double *Q, const double *AQ, con
 
 /* d_norm2[c] = sum_i R[i][c]^2, one sequential chain per column (reference order) */
 int cacg_k_colnorm2_ref(int n, int m, const double *R, long ldr, double *d_norm2, hipStream_t st);
+/* d_norm2[c] = sum_i R[i][c]^2 of a contiguous n x m block, fixed-order
+ * two-level reduction (fast order); d_work: cacg_xrupdate_workspace(m). */
+int cacg_k_colnorm2_fast(int n, int m, const double *R, double *d_norm2, double *d_work,
+                         hipStream_t st);
 
 /* d[0:count] = v, as a kernel (safe inside captured graphs, unlike a memset node) */
 int cacg_k_fill_i32(int *d, long count, int v, hipStream_t st);

@@ -84,6 +84,40 @@ def test_types_byte_identical_to_reference(tmp_path):
     assert ours == _layout(REF_TYPES, tmp_path, "ref")
 
 
+OPTS_C = r"""
+#include <stddef.h>
+#include <stdio.h>
+#include "cacg.h"
+int main(void) {
+  printf("opts %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(cacg_opts),
+         offsetof(cacg_opts, stop_mode), offsetof(cacg_opts, eps), offsetof(cacg_opts, tol),
+         offsetof(cacg_opts, maxiter), offsetof(cacg_opts, order), offsetof(cacg_opts, precision),
+         offsetof(cacg_opts, print_period), offsetof(cacg_opts, inner_tol),
+         offsetof(cacg_opts, inner_maxiter), offsetof(cacg_opts, refine_restart));
+  printf("report %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(cacg_report),
+         offsetof(cacg_report, status), offsetof(cacg_report, iters), offsetof(cacg_report, bounds),
+         offsetof(cacg_report, res_hist), offsetof(cacg_report, res_hist_cap),
+         offsetof(cacg_report, seconds), offsetof(cacg_report, phase_ms),
+         offsetof(cacg_report, phase_calls), offsetof(cacg_report, refinements),
+         offsetof(cacg_report, true_relres));
+  return 0;
+}
+"""
+
+
+def test_python_option_and_report_structs_match_header(tmp_path):
+    import cacg_amd
+    src, exe = tmp_path / "opts.c", tmp_path / "opts"
+    src.write_text(OPTS_C)
+    subprocess.run(["gcc", "-std=gnu11", "-I", INCLUDE, str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    O, R = cacg_amd.Opts, cacg_amd.Report
+    want_o = [ctypes.sizeof(O)] + [getattr(O, f).offset for f, _ in O._fields_]
+    want_r = [ctypes.sizeof(R)] + [getattr(R, f).offset for f, _ in R._fields_]
+    assert out[0].split()[1:] == [str(v) for v in want_o]
+    assert out[1].split()[1:] == [str(v) for v in want_r]
+
+
 class CsrType(ctypes.Structure):
     _fields_ = [("num_rows", ctypes.c_long), ("num_cols", ctypes.c_long), ("nnz", ctypes.c_long),
                 ("start", ctypes.c_long), ("row_start", ctypes.c_void_p),
@@ -132,4 +166,25 @@ def test_host_api_rejects_bad_arguments_without_device():
     assert rc == 3
     rc = L.cacg_bcbcg(A.n, rp.ctypes.data_as(lp), col.ctypes.data_as(lp), val.ctypes.data_as(dp),
                       1, 0, B.ctypes.data_as(dp), X.ctypes.data_as(dp), None, None)   # s = 0
+    assert rc == 3
+
+
+@pytest.mark.parametrize("bad", [dict(precision="fp32", order="reference"), dict(precision=7),
+                                 dict(precision="fp32", inner_tol=-1.0), dict(maxiter=0),
+                                 dict(precision="fp32", refine_restart=2)])
+def test_options_rejected_without_device(bad):
+    """Invalid options are refused (CACG_EINVAL) before any device is touched:
+    the FP32-basis form exists in the fast order only."""
+    import numpy as np
+    import cacg_amd
+    from cacg_amd import problems as P
+    A = P.laplacian_2d(4)
+    L = cacg_amd.lib()
+    lp = ctypes.POINTER(ctypes.c_long)
+    dp = ctypes.POINTER(ctypes.c_double)
+    rp, col, val = (np.ascontiguousarray(a) for a in (A.rowptr, A.col, A.val))
+    B, X = np.zeros((A.n, 1)), np.zeros((A.n, 1))
+    o = cacg_amd.make_opts(**bad)
+    rc = L.cacg_bcbcg(A.n, rp.ctypes.data_as(lp), col.ctypes.data_as(lp), val.ctypes.data_as(dp),
+                      1, 2, B.ctypes.data_as(dp), X.ctypes.data_as(dp), ctypes.byref(o), None)
     assert rc == 3
