@@ -1,0 +1,244 @@
+"""Python binding of the batching / serving engine (include/cacg_batch.h)
+and the request generators and drivers of the batching and serving forms.
+
+    b = Batch(s=4, precision="fp32")
+    ids = [b.submit(A, B, tol=1e-10) for A, B in systems]
+    results = b.run()                     # rounds until every request retired
+
+Continuous serving: `serve_stream` submits each request when its arrival
+time has passed (wall clock), runs one round, collects what retired and
+records submit-to-retire latency; `serve_sequential` is the comparison
+server that solves the same stream one request at a time (FIFO) with the
+single-system FP64 path.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import time
+
+import numpy as np
+
+from . import STATUS, lib as _lib
+from . import problems as P
+
+_vp = ctypes.c_void_p
+_dp = ctypes.POINTER(ctypes.c_double)
+_lp = ctypes.POINTER(ctypes.c_long)
+PRECISION = {"fp64": 0, "fp32": 1}
+
+
+class BatchOpts(ctypes.Structure):
+    _fields_ = [("hbm_bytes", ctypes.c_long), ("slots", ctypes.c_int), ("s", ctypes.c_int),
+                ("precision", ctypes.c_int), ("maxiter", ctypes.c_long)]
+
+
+class BatchResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("iters", ctypes.c_long),
+                ("rounds_queued", ctypes.c_long), ("true_relres", ctypes.c_double)]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("active", ctypes.c_int), ("queued", ctypes.c_int), ("finished", ctypes.c_int),
+                ("admitted", ctypes.c_int), ("retired", ctypes.c_int), ("rounds", ctypes.c_long),
+                ("arena_used", ctypes.c_long), ("arena_bytes", ctypes.c_long)]
+
+
+def lib():
+    L = _lib()
+    if not getattr(L, "_batch_sigs", False):
+        c_int, c_long, c_double = ctypes.c_int, ctypes.c_long, ctypes.c_double
+        sig = {
+            "cacg_batch_default_opts": ([ctypes.POINTER(BatchOpts)], None),
+            "cacg_batch_create": ([ctypes.POINTER(_vp), ctypes.POINTER(BatchOpts)], c_int),
+            "cacg_batch_destroy": ([_vp], c_int),
+            "cacg_batch_footprint": ([c_long, c_long, c_long, c_int, c_int], c_long),
+            "cacg_batch_submit": ([_vp, c_long, _lp, _lp, _dp, c_long, _dp, _dp, c_double], c_long),
+            "cacg_batch_step": ([_vp, ctypes.POINTER(BatchStats)], c_int),
+            "cacg_batch_collect": ([_vp, c_long, _dp, ctypes.POINTER(BatchResult)], c_int),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        L._batch_sigs = True
+    return L
+
+
+def footprint(n, nnz, m, s, precision="fp32") -> int:
+    return lib().cacg_batch_footprint(n, nnz, m, s, PRECISION[precision])
+
+
+class Batch:
+    """A batch of device-resident systems advanced together (cacg_batch_*)."""
+
+    def __init__(self, s=4, precision="fp32", slots=1024, hbm_bytes=0, maxiter=1000):
+        o = BatchOpts()
+        lib().cacg_batch_default_opts(ctypes.byref(o))
+        o.hbm_bytes, o.slots, o.s = int(hbm_bytes), int(slots), int(s)
+        o.precision, o.maxiter = PRECISION[precision], int(maxiter)
+        h = _vp()
+        rc = lib().cacg_batch_create(ctypes.byref(h), ctypes.byref(o))
+        if rc != 0:
+            raise RuntimeError(f"cacg_batch_create failed: {STATUS.get(rc, rc)}")
+        self.h = h.value
+        self.shapes = {}
+        self.stats = BatchStats()
+
+    def submit(self, A, B, X0=None, tol=1e-10) -> int:
+        B = np.ascontiguousarray(B, dtype=np.float64).reshape(A.n, -1)
+        m = B.shape[1]
+        rp = np.ascontiguousarray(A.rowptr, dtype=np.int64)
+        ci = np.ascontiguousarray(A.col, dtype=np.int64)
+        va = np.ascontiguousarray(A.val, dtype=np.float64)
+        x0 = None if X0 is None else np.ascontiguousarray(X0, dtype=np.float64).reshape(A.n, m)
+        rid = lib().cacg_batch_submit(self.h, A.n, rp.ctypes.data_as(_lp), ci.ctypes.data_as(_lp),
+                                      va.ctypes.data_as(_dp), m, B.ctypes.data_as(_dp),
+                                      x0.ctypes.data_as(_dp) if x0 is not None else None, tol)
+        if rid < 0:
+            raise ValueError(f"cacg_batch_submit refused the system: {STATUS.get(-rid, -rid)}")
+        self.shapes[rid] = (A.n, m)
+        return rid
+
+    def step(self) -> dict:
+        rc = lib().cacg_batch_step(self.h, ctypes.byref(self.stats))
+        if rc != 0:
+            raise RuntimeError(f"cacg_batch_step failed: {STATUS.get(rc, rc)}")
+        st = self.stats
+        return {"active": st.active, "queued": st.queued, "finished": st.finished,
+                "admitted": st.admitted, "retired": st.retired, "rounds": st.rounds,
+                "arena_used": st.arena_used, "arena_bytes": st.arena_bytes}
+
+    def collect(self, rid):
+        n, m = self.shapes[rid]
+        X = np.empty((n, m))
+        res = BatchResult()
+        rc = lib().cacg_batch_collect(self.h, rid, X.ctypes.data_as(_dp), ctypes.byref(res))
+        if rc < 0:
+            raise KeyError(rid)
+        if rc == 0:
+            return None
+        del self.shapes[rid]
+        return {"X": X, "status": STATUS.get(res.status, str(res.status)), "iters": res.iters,
+                "rounds_queued": res.rounds_queued, "true_relres": res.true_relres}
+
+    def run(self, max_rounds=100000) -> dict:
+        """Rounds until every submitted request retired; returns results by id."""
+        out = {}
+        for _ in range(max_rounds):
+            st = self.step()
+            for rid in list(self.shapes):
+                r = self.collect(rid)
+                if r is not None:
+                    out[rid] = r
+            if st["active"] == 0 and st["queued"] == 0 and not self.shapes:
+                return out
+        raise RuntimeError("batch did not drain")
+
+    def close(self):
+        if self.h:
+            lib().cacg_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---- workloads ---------------------------------------------------------------------------
+
+
+def batch_systems(count=256, seed=3, lmin=32, lmax=256, ms=(1, 2, 4, 8, 16, 32)):
+    """The batching form: shifted 2D Laplacians, L log-uniform in [lmin, lmax],
+    shift U(0, 0.1), m uniform over ms (problems.sample_batch_systems), B
+    entries U[0,1) drawn from the same stream after the 3*count parameters,
+    system after system, row by row."""
+    out = []
+    off = 3 * count
+    for L, shift, m in P.sample_batch_systems(count, seed, lmin, lmax, ms):
+        A = P.laplacian_2d(L, shift)
+        B = P.uniform(seed, A.n * m, off).reshape(A.n, m)
+        off += A.n * m
+        out.append((A, B))
+    return out
+
+
+def serving_stream(count, rate, seed=4, l2=(64, 512), e3=(32, 128), mrange=(1, 32)):
+    """The serving form: `count` requests with Poisson arrivals at `rate`
+    per second; each is a 2D Laplacian (edge uniform in l2) or a 3D 7-point
+    Laplacian (edge uniform in e3) with equal probability and m uniform in
+    mrange, B entries U[0,1).  Five stream outputs per request (gap, kind,
+    edge, m, spare), then the B entries request after request.
+    Returns [(t_arrival, A, B)]."""
+    u = P.uniform(seed, 5 * count).reshape(count, 5)
+    t = np.cumsum(-np.log1p(-u[:, 0]) / rate)
+    out = []
+    off = 5 * count
+    for i in range(count):
+        if u[i, 1] < 0.5:
+            L = int(l2[0] + np.floor(u[i, 2] * (l2[1] - l2[0] + 1)))
+            A = P.laplacian_2d(min(L, l2[1]))
+        else:
+            E = int(e3[0] + np.floor(u[i, 2] * (e3[1] - e3[0] + 1)))
+            A = P.laplacian_3d(min(E, e3[1]))
+        m = int(mrange[0] + np.floor(u[i, 3] * (mrange[1] - mrange[0] + 1)))
+        m = min(m, mrange[1])
+        B = P.uniform(seed, A.n * m, off).reshape(A.n, m)
+        off += A.n * m
+        out.append((float(t[i]), A, B))
+    return out
+
+
+# ---- drivers -----------------------------------------------------------------------------
+
+
+def serve_stream(stream, batch: Batch, tol=1e-10, timeout=600.0):
+    """Continuous batching: a request is submitted once its arrival time has
+    passed; every loop runs one round and collects what retired.  Returns
+    (results in stream order with 'latency', wall seconds from the first
+    arrival to the last retirement)."""
+    pending = list(range(len(stream)))
+    ids, t_sub, res = {}, {}, {}
+    t0 = time.perf_counter()
+    while len(res) < len(stream):
+        now = time.perf_counter() - t0
+        if now > timeout:
+            raise RuntimeError("serving stream timed out")
+        while pending and stream[pending[0]][0] <= now:
+            i = pending.pop(0)
+            _, A, B = stream[i]
+            ids[batch.submit(A, B, tol=tol)] = i
+            t_sub[i] = stream[i][0]
+        if not ids or all(k in res for k in ids.values()):
+            if pending:   # idle until the next arrival
+                time.sleep(max(0.0, stream[pending[0]][0] - (time.perf_counter() - t0)))
+                continue
+        batch.step()
+        t_now = time.perf_counter() - t0
+        for rid in [r for r, i in ids.items() if i not in res]:
+            r = batch.collect(rid)
+            if r is not None:
+                r["latency"] = t_now - t_sub[ids[rid]]
+                res[ids[rid]] = r
+    wall = (time.perf_counter() - t0) - stream[0][0]
+    return [res[i] for i in range(len(stream))], wall
+
+
+def serve_sequential(stream, tol=1e-10, s=4, maxiter=1000, precision="fp64"):
+    """One request at a time, FIFO, with the single-system path
+    (cacg_amd.bcbcg: create, upload, solve, download per request).  The
+    server waits for arrivals when idle.  Returns (results, wall seconds)."""
+    from . import bcbcg
+    res = []
+    t0 = time.perf_counter()
+    for t_arr, A, B in stream:
+        now = time.perf_counter() - t0
+        if now < t_arr:
+            time.sleep(t_arr - now)
+        out = bcbcg(A, B, np.zeros_like(B), s, tol=tol, maxiter=maxiter, precision=precision)
+        out["latency"] = (time.perf_counter() - t0) - t_arr
+        res.append(out)
+    wall = (time.perf_counter() - t0) - stream[0][0]
+    return res, wall

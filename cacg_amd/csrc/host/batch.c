@@ -1,0 +1,761 @@
+/*
+ * The batching / serving engine: many s-step block-CG systems resident on
+ * one GPU, advanced together one outer iteration per round.
+ *
+ * Memory: one device arena (cacg_batch_opts.hbm_bytes) split first-fit into
+ * one block per admitted system (its CSR, FP32 values, B, X, R, S, Q, A Q,
+ * small blocks, Gram and norm partials), so admission and retirement never
+ * call hipMalloc / hipFree (hipFree would synchronise the device).  Slot
+ * descriptors (cacg_bsys) live in device memory; the column norms and
+ * breakdown flags of all slots sit in two slot-indexed arrays that come back
+ * to the host in one copy per round.
+ *
+ * A round (per system, the iteration of src/bcbcgKernel.c:172-526):
+ *   basis S from R; first iteration Q = S, else QtAS = AQ^T S, beta,
+ *   Q = S - Q beta; AQ = A Q; [QtAQ | QtR], alpha; X += Q alpha;
+ *   then for every admitted system R = B - A X (FP64) and its column norms.
+ * The recomputed R replaces the recursive update R -= AQ alpha, which keeps
+ * the stop test on the true residual and needs no extra pass.  Newly
+ * admitted systems join at the residual step.  Work lists of {slot, tile}
+ * items are built on the host each round and uploaded in one copy.
+ */
+#include <float.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <hip/hip_runtime_api.h>
+
+#include "cacg.h"
+#include "cacg_batch.h"
+#include "cacg_kernels.h"
+
+enum { RQ_QUEUED = 0, RQ_ACTIVE = 1, RQ_DONE = 2 };
+
+typedef struct {
+    long id;
+    int state, status, slot;
+    int joined;          /* 0: admitted this round (residual only), 1: iterating */
+    int fresh;           /* the next iteration starts the recurrence (Q = S) */
+    long n, nnz, m, iters, round_submitted, rounds_queued;
+    double tol, sa, sb, true_relres;
+    int *rp, *ci;        /* host copies, int32 */
+    double *val, *B, *X; /* X: X0 in, the solution out */
+    double *bn2;         /* ||B(:,c)||^2 */
+    long off, bytes;     /* arena block */
+} breq;
+
+typedef struct {
+    long off, size;
+} fblk;
+
+struct cacg_batch {
+    cacg_batch_opts o;
+    hipStream_t st;
+    char *arena;
+    long arena_bytes, arena_used;
+    fblk *fl;            /* free list, sorted by offset */
+    int nfl, capfl;
+    int nslots;
+    breq **slot;
+    cacg_bsys *h_sys, *d_sys;
+    double *d_norms, *h_norms;   /* nslots x CACG_BATCH_MAX_M */
+    int *d_flags, *h_flags;      /* nslots */
+    int2 *h_items, *d_items;
+    long items_cap;
+    breq **req;          /* by id; NULL once collected */
+    long nreq, capreq, qhead;
+    long rounds;
+};
+
+#define HCHK(call)                                                                               \
+    do {                                                                                         \
+        hipError_t e_ = (call);                                                                  \
+        if (e_ != hipSuccess) {                                                                  \
+            fprintf(stderr, "cacg_batch: %s failed: %s (%s:%d)\n", #call, hipGetErrorString(e_), \
+                    __FILE__, __LINE__);                                                         \
+            return CACG_EDEVICE;                                                                 \
+        }                                                                                        \
+    } while (0)
+
+#define KCHK(call)                                                                               \
+    do {                                                                                         \
+        int e_ = (call);                                                                         \
+        if (e_ != 0) {                                                                           \
+            fprintf(stderr, "cacg_batch: %s failed: %s (%s:%d)\n", #call,                        \
+                    hipGetErrorString((hipError_t)e_), __FILE__, __LINE__);                      \
+            return CACG_EDEVICE;                                                                 \
+        }                                                                                        \
+    } while (0)
+
+static long al256(long x) { return (x + 255) & ~255L; }
+
+/* ---- per-system layout in its arena block ----------------------------------------- */
+
+typedef struct {
+    long rowptr, col, val, val32, B, X, R, S, Q, AQ, QtAQ, QtAS, beta, lu, QtR, alpha, gpart,
+        npart, total;
+} layout;
+
+static layout make_layout(long n, long nnz, long m, int s, int precision)
+{
+    layout L;
+    const long sm = s * m, ntile = (n + CACG_BT - 1) / CACG_BT, nch = (n + CACG_GR - 1) / CACG_GR;
+    const int f32 = precision == CACG_PREC_FP32_REFINE;
+    long o = 0;
+#define PUT(field, bytes)                                                                        \
+    do {                                                                                         \
+        L.field = o;                                                                             \
+        o += al256(bytes);                                                                       \
+    } while (0)
+    PUT(rowptr, (n + 1) * 4);
+    PUT(col, nnz * 4);
+    PUT(val, nnz * 8);
+    PUT(val32, f32 ? nnz * 4 : 0);
+    PUT(B, n * m * 8);
+    PUT(X, n * m * 8);
+    PUT(R, n * m * 8);
+    PUT(S, n * sm * (f32 ? 4 : 8));
+    PUT(Q, n * sm * 8);
+    PUT(AQ, n * sm * 8);
+    PUT(QtAQ, sm * sm * 8);
+    PUT(QtAS, sm * sm * 8);
+    PUT(beta, sm * sm * 8);
+    PUT(lu, sm * sm * 8);
+    PUT(QtR, sm * m * 8);
+    PUT(alpha, sm * m * 8);
+    PUT(gpart, nch * sm * (sm + m) * 8);
+    PUT(npart, ntile * m * 8);
+#undef PUT
+    L.total = o;
+    return L;
+}
+
+long cacg_batch_footprint(long n, long nnz, long m, int s, int precision)
+{
+    if (n <= 0 || nnz < 0 || m <= 0 || s <= 0)
+        return -1;
+    return make_layout(n, nnz, m, s, precision).total;
+}
+
+/* ---- arena: first fit over a sorted free list, neighbours merged on free ------------- */
+
+static long arena_alloc(cacg_batch *b, long bytes)
+{
+    bytes = al256(bytes);
+    for (int i = 0; i < b->nfl; i++)
+        if (b->fl[i].size >= bytes) {
+            const long off = b->fl[i].off;
+            b->fl[i].off += bytes;
+            b->fl[i].size -= bytes;
+            if (b->fl[i].size == 0) {
+                memmove(b->fl + i, b->fl + i + 1, sizeof(fblk) * (size_t)(b->nfl - i - 1));
+                b->nfl--;
+            }
+            b->arena_used += bytes;
+            return off;
+        }
+    return -1;
+}
+
+static int arena_free(cacg_batch *b, long off, long bytes)
+{
+    bytes = al256(bytes);
+    int i = 0;
+    while (i < b->nfl && b->fl[i].off < off)
+        i++;
+    if (b->nfl == b->capfl) {
+        fblk *nf = (fblk *)realloc(b->fl, sizeof(fblk) * (size_t)(2 * b->capfl));
+        if (!nf)
+            return CACG_ENOMEM;
+        b->fl = nf;
+        b->capfl *= 2;
+    }
+    memmove(b->fl + i + 1, b->fl + i, sizeof(fblk) * (size_t)(b->nfl - i));
+    b->fl[i].off = off;
+    b->fl[i].size = bytes;
+    b->nfl++;
+    if (i + 1 < b->nfl && b->fl[i].off + b->fl[i].size == b->fl[i + 1].off) {
+        b->fl[i].size += b->fl[i + 1].size;
+        memmove(b->fl + i + 1, b->fl + i + 2, sizeof(fblk) * (size_t)(b->nfl - i - 2));
+        b->nfl--;
+    }
+    if (i > 0 && b->fl[i - 1].off + b->fl[i - 1].size == b->fl[i].off) {
+        b->fl[i - 1].size += b->fl[i].size;
+        memmove(b->fl + i, b->fl + i + 1, sizeof(fblk) * (size_t)(b->nfl - i - 1));
+        b->nfl--;
+    }
+    b->arena_used -= bytes;
+    return CACG_OK;
+}
+
+/* ---- lifetime ------------------------------------------------------------------------ */
+
+void cacg_batch_default_opts(cacg_batch_opts *o)
+{
+    memset(o, 0, sizeof(*o));
+    o->hbm_bytes = 0;
+    o->slots = 1024;
+    o->s = 4;
+    o->precision = CACG_PREC_FP32_REFINE;
+    o->maxiter = 1000;
+}
+
+static void free_req(breq *r)
+{
+    if (!r)
+        return;
+    free(r->rp);
+    free(r->ci);
+    free(r->val);
+    free(r->B);
+    free(r->X);
+    free(r->bn2);
+    free(r);
+}
+
+int cacg_batch_destroy(cacg_batch *b)
+{
+    if (!b)
+        return CACG_OK;
+    if (b->st)
+        hipStreamSynchronize(b->st);
+    for (long i = 0; i < b->nreq; i++)
+        free_req(b->req[i]);
+    free(b->req);
+    free(b->slot);
+    free(b->fl);
+    if (b->arena)
+        hipFree(b->arena);
+    if (b->d_sys)
+        hipFree(b->d_sys);
+    if (b->d_norms)
+        hipFree(b->d_norms);
+    if (b->d_flags)
+        hipFree(b->d_flags);
+    if (b->d_items)
+        hipFree(b->d_items);
+    if (b->h_sys)
+        hipHostFree(b->h_sys);
+    if (b->h_norms)
+        hipHostFree(b->h_norms);
+    if (b->h_flags)
+        hipHostFree(b->h_flags);
+    if (b->h_items)
+        hipHostFree(b->h_items);
+    if (b->st)
+        hipStreamDestroy(b->st);
+    free(b);
+    return CACG_OK;
+}
+
+static int grow_items(cacg_batch *b, long need)
+{
+    if (need <= b->items_cap)
+        return CACG_OK;
+    long cap = b->items_cap > 0 ? b->items_cap : 65536;
+    while (cap < need)
+        cap *= 2;
+    if (b->h_items)
+        hipHostFree(b->h_items);
+    if (b->d_items)
+        hipFree(b->d_items);
+    b->h_items = NULL;
+    b->d_items = NULL;
+    b->items_cap = 0;
+    if (hipHostMalloc((void **)&b->h_items, sizeof(int2) * (size_t)cap, 0) != hipSuccess ||
+        hipMalloc((void **)&b->d_items, sizeof(int2) * (size_t)cap) != hipSuccess)
+        return CACG_ENOMEM;
+    b->items_cap = cap;
+    return CACG_OK;
+}
+
+int cacg_batch_create(cacg_batch **out, const cacg_batch_opts *opts)
+{
+    if (!out)
+        return CACG_EINVAL;
+    *out = NULL;
+    cacg_batch_opts o;
+    if (opts)
+        o = *opts;
+    else
+        cacg_batch_default_opts(&o);
+    if (o.slots == 0)
+        o.slots = 1024;
+    if (o.slots < 1 || o.slots > 4096 || o.s < 1 || o.maxiter <= 0 || o.hbm_bytes < 0 ||
+        (o.precision != CACG_PREC_FP64 && o.precision != CACG_PREC_FP32_REFINE))
+        return CACG_EINVAL;
+    int cnt = 0;
+    if (hipGetDeviceCount(&cnt) != hipSuccess || cnt <= 0)
+        return CACG_EDEVICE;
+    cacg_batch *b = (cacg_batch *)calloc(1, sizeof(*b));
+    if (!b)
+        return CACG_ENOMEM;
+    b->o = o;
+    b->nslots = o.slots;
+    if (o.hbm_bytes == 0) {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+            cacg_batch_destroy(b);
+            return CACG_EDEVICE;
+        }
+        b->o.hbm_bytes = (long)(fr / 4) & ~255L;
+    }
+    b->arena_bytes = al256(b->o.hbm_bytes);
+    b->capfl = 64;
+    b->fl = (fblk *)malloc(sizeof(fblk) * (size_t)b->capfl);
+    b->slot = (breq **)calloc((size_t)b->nslots, sizeof(breq *));
+    b->capreq = 1024;
+    b->req = (breq **)calloc((size_t)b->capreq, sizeof(breq *));
+    if (!b->fl || !b->slot || !b->req) {
+        cacg_batch_destroy(b);
+        return CACG_ENOMEM;
+    }
+    b->fl[0].off = 0;
+    b->fl[0].size = b->arena_bytes;
+    b->nfl = 1;
+    const size_t ns = (size_t)b->nslots;
+    if (hipStreamCreate(&b->st) != hipSuccess) {
+        cacg_batch_destroy(b);
+        return CACG_EDEVICE;
+    }
+    if (hipMalloc((void **)&b->arena, (size_t)b->arena_bytes) != hipSuccess ||
+        hipMalloc((void **)&b->d_sys, sizeof(cacg_bsys) * ns) != hipSuccess ||
+        hipMalloc((void **)&b->d_norms, sizeof(double) * ns * CACG_BATCH_MAX_M) != hipSuccess ||
+        hipMalloc((void **)&b->d_flags, sizeof(int) * ns) != hipSuccess ||
+        hipHostMalloc((void **)&b->h_sys, sizeof(cacg_bsys) * ns, 0) != hipSuccess ||
+        hipHostMalloc((void **)&b->h_norms, sizeof(double) * ns * CACG_BATCH_MAX_M, 0) != hipSuccess ||
+        hipHostMalloc((void **)&b->h_flags, sizeof(int) * ns, 0) != hipSuccess ||
+        grow_items(b, 65536) != CACG_OK) {
+        cacg_batch_destroy(b);
+        return CACG_ENOMEM;
+    }
+    memset(b->h_sys, 0, sizeof(cacg_bsys) * ns);
+    *out = b;
+    return CACG_OK;
+}
+
+/* ---- submission -------------------------------------------------------------------- */
+
+long cacg_batch_submit(cacg_batch *b, long n, const long *row_start, const long *col_idx,
+                       const double *val, long m, const double *B, const double *X0, double tol)
+{
+    if (!b || n <= 0 || n >= (1L << 31) || !row_start || !B || m < 1 || m > CACG_BATCH_MAX_M ||
+        (long)b->o.s * m > CACG_BATCH_MAX_SM || !(tol >= 0.0) || row_start[0] != 0)
+        return -CACG_EINVAL;
+    const long nnz = row_start[n];
+    if (nnz < 0 || nnz >= (1L << 31) || (nnz > 0 && (!col_idx || !val)))
+        return -CACG_EINVAL;
+    for (long i = 0; i < n; i++)
+        if (row_start[i + 1] < row_start[i])
+            return -CACG_EINVAL;
+    for (long p = 0; p < nnz; p++)
+        if (col_idx[p] < 0 || col_idx[p] >= n)
+            return -CACG_EINVAL;
+    const long bytes = cacg_batch_footprint(n, nnz, m, b->o.s, b->o.precision);
+    if (bytes > b->arena_bytes)
+        return -CACG_ENOMEM;
+    breq *r = (breq *)calloc(1, sizeof(*r));
+    if (!r)
+        return -CACG_ENOMEM;
+    r->n = n;
+    r->nnz = nnz;
+    r->m = m;
+    r->tol = tol;
+    r->bytes = bytes;
+    r->slot = -1;
+    r->rp = (int *)malloc(sizeof(int) * (size_t)(n + 1));
+    r->ci = (int *)malloc(sizeof(int) * (size_t)(nnz > 0 ? nnz : 1));
+    r->val = (double *)malloc(sizeof(double) * (size_t)(nnz > 0 ? nnz : 1));
+    r->B = (double *)malloc(sizeof(double) * (size_t)(n * m));
+    r->X = (double *)malloc(sizeof(double) * (size_t)(n * m));
+    r->bn2 = (double *)calloc((size_t)m, sizeof(double));
+    if (!r->rp || !r->ci || !r->val || !r->B || !r->X || !r->bn2) {
+        free_req(r);
+        return -CACG_ENOMEM;
+    }
+    for (long i = 0; i <= n; i++)
+        r->rp[i] = (int)row_start[i];
+    for (long p = 0; p < nnz; p++)
+        r->ci[p] = (int)col_idx[p];
+    if (nnz > 0)
+        memcpy(r->val, val, sizeof(double) * (size_t)nnz);
+    memcpy(r->B, B, sizeof(double) * (size_t)(n * m));
+    if (X0)
+        memcpy(r->X, X0, sizeof(double) * (size_t)(n * m));
+    else
+        memset(r->X, 0, sizeof(double) * (size_t)(n * m));
+    for (long i = 0; i < n; i++)
+        for (long c = 0; c < m; c++)
+            r->bn2[c] += B[i * m + c] * B[i * m + c];
+    /* Gerschgorin interval, the arithmetic of src/common.c:53-106 */
+    double lo = DBL_MAX, hi = DBL_MIN;
+    for (long i = 0; i < n; i++) {
+        double off = 0.0, d = 0.0;
+        for (long p = row_start[i]; p < row_start[i + 1]; p++) {
+            const double v = val[p];
+            if (col_idx[p] != i)
+                off += v > 0 ? v : -v;
+            else
+                d = v;
+        }
+        const double up = off + d, dn = d - off;
+        if (hi < up)
+            hi = up;
+        if (lo > dn)
+            lo = dn;
+    }
+    r->sa = 2.0 / (hi - lo);
+    r->sb = -(hi + lo) / (hi - lo);
+    if (b->nreq == b->capreq) {
+        breq **nr = (breq **)realloc(b->req, sizeof(breq *) * (size_t)(2 * b->capreq));
+        if (!nr) {
+            free_req(r);
+            return -CACG_ENOMEM;
+        }
+        memset(nr + b->capreq, 0, sizeof(breq *) * (size_t)b->capreq);
+        b->req = nr;
+        b->capreq *= 2;
+    }
+    r->id = b->nreq;
+    r->round_submitted = b->rounds;
+    r->state = RQ_QUEUED;
+    b->req[b->nreq++] = r;
+    return r->id;
+}
+
+/* ---- rounds ------------------------------------------------------------------------ */
+
+/* 1: admitted into slot; 0: the arena has no room now; < 0: -CACG_EDEVICE */
+#define ACHK(call)                                                                               \
+    do {                                                                                         \
+        if ((call) != hipSuccess) {                                                              \
+            fprintf(stderr, "cacg_batch: %s failed (%s:%d)\n", #call, __FILE__, __LINE__);       \
+            return -CACG_EDEVICE;                                                                \
+        }                                                                                        \
+    } while (0)
+#define AKCHK(call)                                                                              \
+    do {                                                                                         \
+        if ((call) != 0) {                                                                       \
+            fprintf(stderr, "cacg_batch: %s failed (%s:%d)\n", #call, __FILE__, __LINE__);       \
+            return -CACG_EDEVICE;                                                                \
+        }                                                                                        \
+    } while (0)
+
+static int admit(cacg_batch *b, breq *r, int slot)
+{
+    const long off = arena_alloc(b, r->bytes);
+    if (off < 0)
+        return 0;
+    const int s = b->o.s, f32 = b->o.precision == CACG_PREC_FP32_REFINE;
+    const long n = r->n, m = r->m, sm = s * m;
+    const layout L = make_layout(n, r->nnz, m, s, b->o.precision);
+    char *base = b->arena + off;
+    cacg_bsys *d = &b->h_sys[slot];
+    memset(d, 0, sizeof(*d));
+    d->n = (int)n;
+    d->m = (int)m;
+    d->sm = (int)sm;
+    d->ntile = (int)((n + CACG_BT - 1) / CACG_BT);
+    d->nchunk = (int)((n + CACG_GR - 1) / CACG_GR);
+    d->rowptr = (const int *)(base + L.rowptr);
+    d->col = (const int *)(base + L.col);
+    d->val = (const double *)(base + L.val);
+    d->val32 = f32 ? (const float *)(base + L.val32) : NULL;
+    d->B = (const double *)(base + L.B);
+    d->X = (double *)(base + L.X);
+    d->R = (double *)(base + L.R);
+    d->S = base + L.S;
+    d->Q = (double *)(base + L.Q);
+    d->AQ = (double *)(base + L.AQ);
+    d->QtAQ = (double *)(base + L.QtAQ);
+    d->QtAS = (double *)(base + L.QtAS);
+    d->beta = (double *)(base + L.beta);
+    d->lu = (double *)(base + L.lu);
+    d->QtR = (double *)(base + L.QtR);
+    d->alpha = (double *)(base + L.alpha);
+    d->gpart = (double *)(base + L.gpart);
+    d->npart = (double *)(base + L.npart);
+    d->norm2 = b->d_norms + (long)slot * CACG_BATCH_MAX_M;
+    d->flag = b->d_flags + slot;
+    d->sa = r->sa;
+    d->sb = r->sb;
+    hipStream_t st = b->st;
+    ACHK(hipMemcpyAsync((void *)d->rowptr, r->rp, sizeof(int) * (size_t)(n + 1),
+                        hipMemcpyHostToDevice, st));
+    if (r->nnz > 0) {
+        ACHK(hipMemcpyAsync((void *)d->col, r->ci, sizeof(int) * (size_t)r->nnz,
+                            hipMemcpyHostToDevice, st));
+        ACHK(hipMemcpyAsync((void *)d->val, r->val, sizeof(double) * (size_t)r->nnz,
+                            hipMemcpyHostToDevice, st));
+        if (f32)
+            AKCHK(cacg_k_cvt_f64_f32(r->nnz, d->val, (float *)d->val32, st));
+    }
+    ACHK(hipMemcpyAsync((void *)d->B, r->B, sizeof(double) * (size_t)(n * m),
+                        hipMemcpyHostToDevice, st));
+    ACHK(hipMemcpyAsync(d->X, r->X, sizeof(double) * (size_t)(n * m), hipMemcpyHostToDevice, st));
+    AKCHK(cacg_k_fill_i32(d->flag, 1, 0, st));
+    ACHK(hipMemcpyAsync(b->d_sys + slot, d, sizeof(*d), hipMemcpyHostToDevice, st));
+    r->off = off;
+    r->slot = slot;
+    r->state = RQ_ACTIVE;
+    r->joined = 0;
+    r->fresh = 1;
+    r->iters = 0;
+    r->rounds_queued = b->rounds - r->round_submitted;
+    b->slot[slot] = r;
+    return 1;
+}
+
+static int retire(cacg_batch *b, breq *r, int status, const double *norm2)
+{
+    double worst = 0.0;
+    for (long c = 0; c < r->m; c++) {
+        const double q = sqrt(norm2[c]) / sqrt(r->bn2[c]);
+        if (!(q <= worst))
+            worst = q;
+    }
+    HCHK(hipMemcpyAsync(r->X, b->h_sys[r->slot].X, sizeof(double) * (size_t)(r->n * r->m),
+                        hipMemcpyDeviceToHost, b->st));
+    HCHK(hipStreamSynchronize(b->st));
+    r->true_relres = worst;
+    r->status = status;
+    r->state = RQ_DONE;
+    b->slot[r->slot] = NULL;
+    r->slot = -1;
+    /* keep the matrix copies only as long as they can be needed */
+    free(r->rp);
+    free(r->ci);
+    free(r->val);
+    free(r->B);
+    r->rp = NULL;
+    r->ci = NULL;
+    r->val = NULL;
+    r->B = NULL;
+    return arena_free(b, r->off, r->bytes);
+}
+
+/* append {slot, 0 .. count - 1} */
+static long put_range(int2 *it, long k, int slot, int count)
+{
+    for (int i = 0; i < count; i++) {
+        it[k].x = slot;
+        it[k].y = i;
+        k++;
+    }
+    return k;
+}
+
+int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
+{
+    if (!b)
+        return CACG_EINVAL;
+    hipStream_t st = b->st;
+    const int s = b->o.s, f32 = b->o.precision == CACG_PREC_FP32_REFINE;
+    int admitted = 0, retired = 0;
+
+    /* admission, FIFO: the head waits for a slot and memory */
+    int free_slot = 0;
+    while (b->qhead < b->nreq) {
+        breq *r = b->req[b->qhead];
+        if (!r || r->state != RQ_QUEUED) {
+            b->qhead++;
+            continue;
+        }
+        while (free_slot < b->nslots && b->slot[free_slot])
+            free_slot++;
+        if (free_slot == b->nslots)
+            break;
+        const int ok = admit(b, r, free_slot);
+        if (ok < 0)
+            return -ok;
+        if (ok == 0)
+            break;
+        admitted++;
+        b->qhead++;
+    }
+
+    /* work lists */
+    long need = 0;
+    int nact = 0, nit = 0, max_sm = 1;
+    for (int i = 0; i < b->nslots; i++) {
+        const breq *r = b->slot[i];
+        if (!r)
+            continue;
+        const cacg_bsys *d = &b->h_sys[i];
+        nact++;
+        need += 1 + d->ntile;
+        if (r->joined) {
+            nit++;
+            need += 3L * d->ntile + 2L * d->nchunk + 2 +
+                    ((long)d->sm * (d->sm + d->m) + 255) / 256 + ((long)d->sm * d->sm + 255) / 256;
+            if (d->sm > max_sm)
+                max_sm = d->sm;
+        }
+    }
+    if (grow_items(b, need) != CACG_OK)
+        return CACG_ENOMEM;
+    int2 *it = b->h_items;
+    long k = 0;
+    /* order: iterating tiles | fresh tiles | update tiles | Gram0 chunks | Gram0 blocks |
+     *        LU0 | Gram1 chunks | Gram1 blocks | LU1 | all tiles | all systems */
+    long o_it = k;
+    for (int i = 0; i < b->nslots; i++)
+        if (b->slot[i] && b->slot[i]->joined)
+            k = put_range(it, k, i, b->h_sys[i].ntile);
+    const long n_it = k - o_it, o_fr = k;
+    for (int i = 0; i < b->nslots; i++)
+        if (b->slot[i] && b->slot[i]->joined && b->slot[i]->fresh)
+            k = put_range(it, k, i, b->h_sys[i].ntile);
+    const long n_fr = k - o_fr, o_up = k;
+    for (int i = 0; i < b->nslots; i++)
+        if (b->slot[i] && b->slot[i]->joined && !b->slot[i]->fresh)
+            k = put_range(it, k, i, b->h_sys[i].ntile);
+    const long n_up = k - o_up, o_g0 = k;
+    for (int i = 0; i < b->nslots; i++)
+        if (b->slot[i] && b->slot[i]->joined && !b->slot[i]->fresh)
+            k = put_range(it, k, i, b->h_sys[i].nchunk);
+    const long n_g0 = k - o_g0, o_b0 = k;
+    for (int i = 0; i < b->nslots; i++)
+        if (b->slot[i] && b->slot[i]->joined && !b->slot[i]->fresh) {
+            const cacg_bsys *d = &b->h_sys[i];
+            k = put_range(it, k, i, (int)(((long)d->sm * d->sm + 255) / 256));
+        }
+    const long n_b0 = k - o_b0, o_l0 = k;
+    for (int i = 0; i < b->nslots; i++)
+        if (b->slot[i] && b->slot[i]->joined && !b->slot[i]->fresh)
+            k = put_range(it, k, i, 1);
+    const long n_l0 = k - o_l0, o_g1 = k;
+    for (int i = 0; i < b->nslots; i++)
+        if (b->slot[i] && b->slot[i]->joined)
+            k = put_range(it, k, i, b->h_sys[i].nchunk);
+    const long n_g1 = k - o_g1, o_b1 = k;
+    for (int i = 0; i < b->nslots; i++)
+        if (b->slot[i] && b->slot[i]->joined) {
+            const cacg_bsys *d = &b->h_sys[i];
+            k = put_range(it, k, i, (int)(((long)d->sm * (d->sm + d->m) + 255) / 256));
+        }
+    const long n_b1 = k - o_b1, o_l1 = k;
+    for (int i = 0; i < b->nslots; i++)
+        if (b->slot[i] && b->slot[i]->joined)
+            k = put_range(it, k, i, 1);
+    const long n_l1 = k - o_l1, o_all = k;
+    for (int i = 0; i < b->nslots; i++)
+        if (b->slot[i])
+            k = put_range(it, k, i, b->h_sys[i].ntile);
+    const long n_all = k - o_all, o_sys = k;
+    for (int i = 0; i < b->nslots; i++)
+        if (b->slot[i])
+            k = put_range(it, k, i, 1);
+    const long n_sys = k - o_sys;
+    if (k > 0)
+        HCHK(hipMemcpyAsync(b->d_items, it, sizeof(int2) * (size_t)k, hipMemcpyHostToDevice, st));
+    const int2 *di = b->d_items;
+
+    /* one outer iteration of the joined systems */
+    if (n_it > 0) {
+        if (s == 1) {
+            KCHK(cacg_kb_basis(b->d_sys, di + o_it, (int)n_it, 0, f32, st));
+        } else {
+            KCHK(cacg_kb_basis(b->d_sys, di + o_it, (int)n_it, 1, f32, st));
+            for (int j = 2; j < s; j++)
+                KCHK(cacg_kb_basis(b->d_sys, di + o_it, (int)n_it, j, f32, st));
+        }
+        KCHK(cacg_kb_q_from_s(b->d_sys, di + o_fr, (int)n_fr, f32, st));
+        KCHK(cacg_kb_gram(b->d_sys, di + o_g0, (int)n_g0, di + o_b0, (int)n_b0, 0, f32, st));
+        KCHK(cacg_k_lu_batch(b->d_sys, di + o_l0, (int)n_l0, 0, max_sm, st));
+        KCHK(cacg_kb_qupdate(b->d_sys, di + o_up, (int)n_up, f32, st));
+        KCHK(cacg_kb_aq(b->d_sys, di + o_it, (int)n_it, st));
+        KCHK(cacg_kb_gram(b->d_sys, di + o_g1, (int)n_g1, di + o_b1, (int)n_b1, 1, f32, st));
+        KCHK(cacg_k_lu_batch(b->d_sys, di + o_l1, (int)n_l1, 1, max_sm, st));
+        KCHK(cacg_kb_xupdate(b->d_sys, di + o_it, (int)n_it, st));
+    }
+    /* the true residual of every admitted system */
+    KCHK(cacg_kb_resid(b->d_sys, di + o_all, (int)n_all, di + o_sys, (int)n_sys, st));
+    int hi_slot = 0;
+    for (int i = 0; i < b->nslots; i++)
+        if (b->slot[i])
+            hi_slot = i + 1;
+    if (hi_slot > 0) {
+        HCHK(hipMemcpyAsync(b->h_norms, b->d_norms,
+                            sizeof(double) * (size_t)hi_slot * CACG_BATCH_MAX_M,
+                            hipMemcpyDeviceToHost, st));
+        HCHK(hipMemcpyAsync(b->h_flags, b->d_flags, sizeof(int) * (size_t)hi_slot,
+                            hipMemcpyDeviceToHost, st));
+    }
+    HCHK(hipStreamSynchronize(st));
+    b->rounds++;
+
+    /* stop tests on the true residuals; retire */
+    for (int i = 0; i < hi_slot; i++) {
+        breq *r = b->slot[i];
+        if (!r)
+            continue;
+        const double *nr = b->h_norms + (long)i * CACG_BATCH_MAX_M;
+        if (r->joined) {
+            r->iters++;
+            r->fresh = 0;
+        }
+        r->joined = 1;
+        int done = 1, finite = 1;
+        for (long c = 0; c < r->m; c++) {
+            if (!isfinite(nr[c]))
+                finite = 0;
+            if (!(sqrt(nr[c]) < r->tol * sqrt(r->bn2[c])))
+                done = 0;
+        }
+        int status = -1;
+        if (done)
+            status = CACG_OK;
+        else if (!finite || b->h_flags[i])
+            status = CACG_BREAKDOWN;
+        else if (r->iters >= b->o.maxiter)
+            status = CACG_MAXITER;
+        if (status >= 0) {
+            const int rc = retire(b, r, status, nr);
+            if (rc != CACG_OK)
+                return rc;
+            retired++;
+            nact--;
+        }
+    }
+    if (stats) {
+        int queued = 0, finished = 0;
+        for (long i = b->qhead; i < b->nreq; i++)
+            if (b->req[i] && b->req[i]->state == RQ_QUEUED)
+                queued++;
+        for (long i = 0; i < b->nreq; i++)
+            if (b->req[i] && b->req[i]->state == RQ_DONE)
+                finished++;
+        stats->active = nact;
+        stats->queued = queued;
+        stats->finished = finished;
+        stats->admitted = admitted;
+        stats->retired = retired;
+        stats->rounds = b->rounds;
+        stats->arena_used = b->arena_used;
+        stats->arena_bytes = b->arena_bytes;
+    }
+    return CACG_OK;
+}
+
+int cacg_batch_collect(cacg_batch *b, long id, double *X, cacg_batch_result *res)
+{
+    if (!b || id < 0 || id >= b->nreq || !b->req[id])
+        return -CACG_EINVAL;
+    breq *r = b->req[id];
+    if (r->state != RQ_DONE)
+        return 0;
+    if (X)
+        memcpy(X, r->X, sizeof(double) * (size_t)(r->n * r->m));
+    if (res) {
+        res->status = r->status;
+        res->iters = r->iters;
+        res->rounds_queued = r->rounds_queued;
+        res->true_relres = r->true_relres;
+    }
+    free_req(r);
+    b->req[id] = NULL;
+    return 1;
+}

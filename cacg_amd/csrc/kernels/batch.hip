@@ -1,0 +1,386 @@
+// Batched kernels: one launch advances a phase of the s-step iteration for
+// every system in a work list (the batching / serving form).  gfx950, wave64.
+//
+// Systems of the batch differ in n, nnz, m (<= 32) and sm = s*m (<= 128), so
+// the row-parallel kernels map lanes to the elements (row, column) of a
+// 64-row tile: lane e of the tile handles row e / k, column e % k, which
+// keeps every lane busy for any k.  The Gram kernel gives each 1024-lane
+// workgroup a 2048-row chunk of one system, staged through LDS 16 rows at a
+// time; lane (ta, tb) of a 32 x 32 grid owns the entries (ta + 32u, tb + 32v)
+// of the sm x (sm + m) result, and a second kernel sums the chunk partials in
+// chunk order.  Every sum of a system runs in an order fixed by that system
+// alone (its rows, tiles and chunks), so a system's results are the same bits
+// whatever else shares the launch.
+#include "kcommon.hpp"
+
+namespace {
+
+constexpr int BT = CACG_BT;
+constexpr int GR = CACG_GR;
+constexpr int GS = 16;  // rows per LDS stage of the Gram
+constexpr int MS = CACG_BATCH_MAX_SM, MM = CACG_BATCH_MAX_M;
+
+__device__ __forceinline__ int tile_rows(int n, int tile)
+{
+    const int r = n - tile * BT;
+    return r < BT ? r : BT;
+}
+
+// basis step j for one tile (see cacg_kb_basis); TB = basis type
+template <typename TB>
+__global__ __launch_bounds__(256) void bbasis_kernel(const cacg_bsys *__restrict__ sys,
+                                                     const int2 *__restrict__ items, int j)
+{
+    const int2 it = items[blockIdx.x];
+    const cacg_bsys &d = sys[it.x];
+    const int n = d.n, m = d.m, sm = d.sm;
+    const long r0 = (long)it.y * BT;
+    const int rows = tile_rows(n, it.y);
+    TB *S = reinterpret_cast<TB *>(d.S);
+    const TB *val = sizeof(TB) == 4 ? reinterpret_cast<const TB *>(d.val32)
+                                    : reinterpret_cast<const TB *>(d.val);
+    const TB a1 = (TB)d.sa, b1 = (TB)d.sb, a2 = (TB)(2.0 * d.sa), b2 = (TB)(2.0 * d.sb);
+    for (int e = threadIdx.x; e < rows * m; e += 256) {
+        const int r = e / m, c = e - r * m;
+        const long i = r0 + r;
+        if (j == 0) {
+            S[i * sm + c] = (TB)d.R[i * m + c];
+            continue;
+        }
+        const int pb = d.rowptr[i], pe = d.rowptr[i + 1];
+        TB acc = (TB)0;
+        if (j == 1) {
+            for (int p = pb; p < pe; p++)
+                acc = add_rn(acc, mul_rn(val[p], (TB)d.R[(long)d.col[p] * m + c]));
+            const TB e1 = (TB)d.R[i * m + c];
+            S[i * sm + c] = e1;
+            S[i * sm + m + c] = add_rn(mul_rn(a1, acc), mul_rn(b1, e1));
+        } else {
+            const TB *xs = S + (long)(j - 1) * m;
+            for (int p = pb; p < pe; p++)
+                acc = add_rn(acc, mul_rn(val[p], xs[(long)d.col[p] * sm + c]));
+            const TB e1 = xs[i * sm + c], e2 = S[i * sm + (long)(j - 2) * m + c];
+            S[i * sm + (long)j * m + c] = sub_rn(add_rn(mul_rn(a2, acc), mul_rn(b2, e1)), e2);
+        }
+    }
+}
+
+template <typename TB>
+__global__ __launch_bounds__(256) void bq_from_s_kernel(const cacg_bsys *__restrict__ sys,
+                                                        const int2 *__restrict__ items)
+{
+    const int2 it = items[blockIdx.x];
+    const cacg_bsys &d = sys[it.x];
+    const long base = (long)it.y * BT * d.sm;
+    const int cnt = tile_rows(d.n, it.y) * d.sm;
+    const TB *S = reinterpret_cast<const TB *>(d.S);
+    for (int e = threadIdx.x; e < cnt; e += 256)
+        d.Q[base + e] = (double)S[base + e];
+}
+
+__global__ __launch_bounds__(256) void baq_kernel(const cacg_bsys *__restrict__ sys,
+                                                  const int2 *__restrict__ items)
+{
+    const int2 it = items[blockIdx.x];
+    const cacg_bsys &d = sys[it.x];
+    const int sm = d.sm;
+    const long r0 = (long)it.y * BT;
+    const int rows = tile_rows(d.n, it.y);
+    for (int e = threadIdx.x; e < rows * sm; e += 256) {
+        const int r = e / sm, c = e - r * sm;
+        const long i = r0 + r;
+        double acc = 0.0;
+        for (int p = d.rowptr[i]; p < d.rowptr[i + 1]; p++)
+            acc = add_rn(acc, mul_rn(d.val[p], d.Q[(long)d.col[p] * sm + c]));
+        d.AQ[i * sm + c] = acc;
+    }
+}
+
+// Gram partial of one 2048-row chunk: which 0: AQ^T S, which 1: Q^T [AQ | R]
+template <typename TB>
+__global__ __launch_bounds__(1024) void bgram_kernel(const cacg_bsys *__restrict__ sys,
+                                                     const int2 *__restrict__ items, int which)
+{
+    __shared__ double Ls[GS][MS];
+    __shared__ double Rs[GS][MS + MM];
+    const int2 it = items[blockIdx.x];
+    const cacg_bsys &d = sys[it.x];
+    const int n = d.n, m = d.m, sm = d.sm;
+    const int kl = sm, kr1 = sm, kr = sm + (which ? m : 0);
+    const double *L = which ? d.Q : d.AQ;
+    const TB *S = reinterpret_cast<const TB *>(d.S);
+    const int c0 = it.y * GR, c1 = (c0 + GR) < n ? (c0 + GR) : n;
+    const int t = threadIdx.x, ta = t >> 5, tb = t & 31;
+    double acc[4][5];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+            acc[u][v] = 0.0;
+    for (int rb = c0; rb < c1; rb += GS) {
+        const int rows = (c1 - rb) < GS ? (c1 - rb) : GS;
+        for (int e = t; e < rows * kl; e += 1024) {
+            const int r = e / kl, a = e - r * kl;
+            Ls[r][a] = L[(long)(rb + r) * sm + a];
+        }
+        for (int e = t; e < rows * kr; e += 1024) {
+            const int r = e / kr, b = e - r * kr;
+            const long i = rb + r;
+            double v;
+            if (b < kr1)
+                v = which ? d.AQ[i * sm + b] : (double)S[i * sm + b];
+            else
+                v = d.R[i * m + (b - kr1)];
+            Rs[r][b] = v;
+        }
+        __syncthreads();
+        for (int r = 0; r < rows; r++) {
+            double lv[4], rv[5];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int a = ta + 32 * u;
+                lv[u] = a < kl ? Ls[r][a] : 0.0;
+            }
+#pragma unroll
+            for (int v = 0; v < 5; v++) {
+                const int b = tb + 32 * v;
+                rv[v] = b < kr ? Rs[r][b] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int v = 0; v < 5; v++)
+                    acc[u][v] = fma(lv[u], rv[v], acc[u][v]);
+        }
+        __syncthreads();
+    }
+    double *gp = d.gpart + (long)it.y * kl * kr;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int a = ta + 32 * u;
+        if (a >= kl)
+            continue;
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+            const int b = tb + 32 * v;
+            if (b < kr)
+                gp[(long)a * kr + b] = acc[u][v];
+        }
+    }
+}
+
+// chunk-ordered sum of the partials, 256 result entries per workgroup
+__global__ __launch_bounds__(256) void bgram_reduce_kernel(const cacg_bsys *__restrict__ sys,
+                                                           const int2 *__restrict__ items,
+                                                           int which)
+{
+    const int2 it = items[blockIdx.x];
+    const cacg_bsys &d = sys[it.x];
+    const int sm = d.sm, m = d.m;
+    const int kr = sm + (which ? m : 0);
+    const int e = it.y * 256 + threadIdx.x;
+    if (e >= sm * kr)
+        return;
+    const long stride = (long)sm * kr;
+    double acc = 0.0;
+    for (int ch = 0; ch < d.nchunk; ch++)
+        acc += d.gpart[ch * stride + e];
+    const int a = e / kr, b = e - a * kr;
+    if (which == 0)
+        d.QtAS[(long)a * sm + b] = acc;
+    else if (b < sm)
+        d.QtAQ[(long)a * sm + b] = acc;
+    else
+        d.QtR[(long)a * m + (b - sm)] = acc;
+}
+
+// Q = S - Q beta: the tile's Q rows are staged in LDS first (in-place update)
+template <typename TB>
+__global__ __launch_bounds__(256) void bqupdate_kernel(const cacg_bsys *__restrict__ sys,
+                                                       const int2 *__restrict__ items)
+{
+    extern __shared__ double qt[];  // BT x sm
+    const int2 it = items[blockIdx.x];
+    const cacg_bsys &d = sys[it.x];
+    const int sm = d.sm;
+    const long base = (long)it.y * BT * sm;
+    const int cnt = tile_rows(d.n, it.y) * sm;
+    const TB *S = reinterpret_cast<const TB *>(d.S);
+    for (int e = threadIdx.x; e < cnt; e += 256)
+        qt[e] = d.Q[base + e];
+    __syncthreads();
+    for (int e = threadIdx.x; e < cnt; e += 256) {
+        const int r = e / sm, c = e - r * sm;
+        const double *qr = qt + (long)r * sm;
+        double acc = 0.0;
+        for (int q = 0; q < sm; q++)
+            acc = fma(qr[q], d.beta[(long)q * sm + c], acc);
+        d.Q[base + e] = (double)S[base + e] - acc;
+    }
+}
+
+// X += Q alpha
+__global__ __launch_bounds__(256) void bxupdate_kernel(const cacg_bsys *__restrict__ sys,
+                                                       const int2 *__restrict__ items)
+{
+    const int2 it = items[blockIdx.x];
+    const cacg_bsys &d = sys[it.x];
+    const int m = d.m, sm = d.sm;
+    const long r0 = (long)it.y * BT;
+    const int rows = tile_rows(d.n, it.y);
+    for (int e = threadIdx.x; e < rows * m; e += 256) {
+        const int r = e / m, c = e - r * m;
+        const long i = r0 + r;
+        const double *qr = d.Q + i * sm;
+        double acc = 0.0;
+        for (int q = 0; q < sm; q++)
+            acc = fma(qr[q], d.alpha[(long)q * m + c], acc);
+        d.X[i * m + c] += acc;
+    }
+}
+
+// R = B - A X in the reference's SpMM arithmetic, plus the tile's column sums
+// of R^2 (rows in order)
+__global__ __launch_bounds__(256) void bresid_kernel(const cacg_bsys *__restrict__ sys,
+                                                     const int2 *__restrict__ items)
+{
+    __shared__ double sq[BT * MM];
+    const int2 it = items[blockIdx.x];
+    const cacg_bsys &d = sys[it.x];
+    const int m = d.m;
+    const long r0 = (long)it.y * BT;
+    const int rows = tile_rows(d.n, it.y);
+    for (int e = threadIdx.x; e < rows * m; e += 256) {
+        const int r = e / m, c = e - r * m;
+        const long i = r0 + r;
+        double acc = 0.0;
+        for (int p = d.rowptr[i]; p < d.rowptr[i + 1]; p++)
+            acc = add_rn(acc, mul_rn(d.val[p], d.X[(long)d.col[p] * m + c]));
+        const double rv = sub_rn(d.B[i * m + c], acc);
+        d.R[i * m + c] = rv;
+        sq[e] = rv * rv;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < m) {
+        double s = 0.0;
+        for (int r = 0; r < rows; r++)
+            s += sq[r * m + threadIdx.x];
+        d.npart[(long)it.y * m + threadIdx.x] = s;
+    }
+}
+
+// norm2[c] = sum over the system's tiles, in tile order
+__global__ __launch_bounds__(64) void bnorm_kernel(const cacg_bsys *__restrict__ sys,
+                                                   const int2 *__restrict__ items)
+{
+    const cacg_bsys &d = sys[items[blockIdx.x].x];
+    const int c = threadIdx.x;
+    if (c >= d.m)
+        return;
+    double s = 0.0;
+    for (int t = 0; t < d.ntile; t++)
+        s += d.npart[(long)t * d.m + c];
+    d.norm2[c] = s;
+}
+
+}  // namespace
+
+extern "C" int cacg_kb_basis(const cacg_bsys *d_sys, const int2 *d_tiles, int count, int j, int f32,
+                             hipStream_t st)
+{
+    if (count <= 0)
+        return 0;
+    if (j < 0)
+        return (int)hipErrorInvalidValue;
+    if (f32)
+        hipLaunchKernelGGL(bbasis_kernel<float>, dim3(count), dim3(256), 0, st, d_sys, d_tiles, j);
+    else
+        hipLaunchKernelGGL(bbasis_kernel<double>, dim3(count), dim3(256), 0, st, d_sys, d_tiles, j);
+    CACG_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int cacg_kb_q_from_s(const cacg_bsys *d_sys, const int2 *d_tiles, int count, int f32,
+                                hipStream_t st)
+{
+    if (count <= 0)
+        return 0;
+    if (f32)
+        hipLaunchKernelGGL(bq_from_s_kernel<float>, dim3(count), dim3(256), 0, st, d_sys, d_tiles);
+    else
+        hipLaunchKernelGGL(bq_from_s_kernel<double>, dim3(count), dim3(256), 0, st, d_sys, d_tiles);
+    CACG_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int cacg_kb_aq(const cacg_bsys *d_sys, const int2 *d_tiles, int count, hipStream_t st)
+{
+    if (count <= 0)
+        return 0;
+    hipLaunchKernelGGL(baq_kernel, dim3(count), dim3(256), 0, st, d_sys, d_tiles);
+    CACG_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int cacg_kb_gram(const cacg_bsys *d_sys, const int2 *d_chunks, int nchunks,
+                            const int2 *d_blocks, int nblocks, int which, int f32, hipStream_t st)
+{
+    if (which != 0 && which != 1)
+        return (int)hipErrorInvalidValue;
+    if (nchunks > 0) {
+        if (f32)
+            hipLaunchKernelGGL(bgram_kernel<float>, dim3(nchunks), dim3(1024), 0, st, d_sys,
+                               d_chunks, which);
+        else
+            hipLaunchKernelGGL(bgram_kernel<double>, dim3(nchunks), dim3(1024), 0, st, d_sys,
+                               d_chunks, which);
+        CACG_LAUNCH_CHECK();
+    }
+    if (nblocks > 0) {
+        hipLaunchKernelGGL(bgram_reduce_kernel, dim3(nblocks), dim3(256), 0, st, d_sys, d_blocks,
+                           which);
+        CACG_LAUNCH_CHECK();
+    }
+    return 0;
+}
+
+extern "C" int cacg_kb_qupdate(const cacg_bsys *d_sys, const int2 *d_tiles, int count, int f32,
+                               hipStream_t st)
+{
+    if (count <= 0)
+        return 0;
+    const size_t shm = (size_t)BT * MS * sizeof(double);
+    if (f32) {
+        cacg_allow_lds(bqupdate_kernel<float>, shm);
+        hipLaunchKernelGGL(bqupdate_kernel<float>, dim3(count), dim3(256), shm, st, d_sys, d_tiles);
+    } else {
+        cacg_allow_lds(bqupdate_kernel<double>, shm);
+        hipLaunchKernelGGL(bqupdate_kernel<double>, dim3(count), dim3(256), shm, st, d_sys, d_tiles);
+    }
+    CACG_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int cacg_kb_xupdate(const cacg_bsys *d_sys, const int2 *d_tiles, int count,
+                               hipStream_t st)
+{
+    if (count <= 0)
+        return 0;
+    hipLaunchKernelGGL(bxupdate_kernel, dim3(count), dim3(256), 0, st, d_sys, d_tiles);
+    CACG_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int cacg_kb_resid(const cacg_bsys *d_sys, const int2 *d_tiles, int count,
+                             const int2 *d_sysitems, int nsys, hipStream_t st)
+{
+    if (count > 0) {
+        hipLaunchKernelGGL(bresid_kernel, dim3(count), dim3(256), 0, st, d_sys, d_tiles);
+        CACG_LAUNCH_CHECK();
+    }
+    if (nsys > 0) {
+        hipLaunchKernelGGL(bnorm_kernel, dim3(nsys), dim3(64), 0, st, d_sys, d_sysitems);
+        CACG_LAUNCH_CHECK();
+    }
+    return 0;
+}

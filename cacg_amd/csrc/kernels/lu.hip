@@ -29,14 +29,41 @@ namespace {
 constexpr int LU_T = 1024;
 constexpr int JMAX = 16;
 
-template <bool IN_LDS>
-__global__ __launch_bounds__(LU_T) void lu_solve_kernel(int N, const double *__restrict__ lhs, int k,
-                                                        const double *__restrict__ rhs,
-                                                        double *__restrict__ x,
-                                                        double *__restrict__ gw, int *flag,
-                                                        int mode, int kc, int reuse)
+// right-hand-side columns per substitution pass: the register block (JMAX
+// rows per lane) must cover N, and without the LDS factor the reference pass
+// keeps N x kc in LDS; 0 when N is out of range
+__host__ __device__ inline int lu_columns(int N, int k, bool in_lds)
 {
-    extern __shared__ double lds[];
+    int kc = k < N ? k : N;
+    if ((long)N * kc > (long)LU_T * JMAX)
+        kc = (int)((long)LU_T * JMAX / N);
+    if (!in_lds && (long)N * kc > 8192)
+        kc = 8192 / N;
+    if (kc < 1 || (LU_T / kc) * JMAX < N)
+        return 0;
+    return kc;
+}
+
+// dynamic LDS of a solve: factor, broadcast rows (2 x kc), pivot row/column
+// (2 x 2 x 128), 2 reciprocal pivots, N reciprocal diagonal entries; without
+// the LDS factor an N x kc block instead of the factor
+__host__ __device__ inline size_t lu_lds_bytes(int N, int k, bool in_lds)
+{
+    const int kc = lu_columns(N, k, in_lds);
+    if (in_lds)
+        return ((size_t)N * N + 2 * (size_t)kc + 512 + 2 + N) * sizeof(double);
+    return ((size_t)N * kc + 2 * (size_t)kc + N) * sizeof(double);
+}
+
+// The solve of one system by one workgroup of LU_T lanes; lds is the dynamic
+// LDS block (lu_lds_bytes).  Shared by the single-system kernel and the
+// batched one.
+template <bool IN_LDS>
+__device__ __forceinline__ void lu_solve_body(int N, const double *__restrict__ lhs, int k,
+                                              const double *__restrict__ rhs,
+                                              double *__restrict__ x, double *__restrict__ gw,
+                                              int *flag, int mode, int kc, int reuse, double *lds)
+{
     double *a = IN_LDS ? lds : gw;
     double *rowbuf = IN_LDS ? lds + (long)N * N : lds;
     double *xs = IN_LDS ? lds : lds + 2 * kc;
@@ -276,7 +303,46 @@ __global__ __launch_bounds__(LU_T) void lu_solve_kernel(int N, const double *__r
     }
 }
 
+template <bool IN_LDS>
+__global__ __launch_bounds__(LU_T) void lu_solve_kernel(int N, const double *__restrict__ lhs, int k,
+                                                        const double *__restrict__ rhs,
+                                                        double *__restrict__ x,
+                                                        double *__restrict__ gw, int *flag,
+                                                        int mode, int kc, int reuse)
+{
+    extern __shared__ double lds[];
+    lu_solve_body<IN_LDS>(N, lhs, k, rhs, x, gw, flag, mode, kc, reuse, lds);
+}
+
+// batched: workgroup b solves system items[b].x (N = sm <= 128, factor in LDS);
+// which 0: beta = QtAQ^{-1} QtAS with the factor kept from the last alpha solve,
+// which 1: alpha = QtAQ^{-1} QtR, factoring QtAQ
+__global__ __launch_bounds__(LU_T) void lu_batch_kernel(const cacg_bsys *__restrict__ sys,
+                                                        const int2 *__restrict__ items, int which)
+{
+    extern __shared__ double lds[];
+    const cacg_bsys &d = sys[items[blockIdx.x].x];
+    const int N = d.sm, k = which == 0 ? d.sm : d.m;
+    const int kc = lu_columns(N, k, true);
+    lu_solve_body<true>(N, d.QtAQ, k, which == 0 ? d.QtAS : d.QtR, which == 0 ? d.beta : d.alpha,
+                        d.lu, d.flag, CACG_MODE_FAST, kc, which == 0 ? 1 : 0, lds);
+}
+
 }  // namespace
+
+extern "C" int cacg_k_lu_batch(const cacg_bsys *d_sys, const int2 *d_items, int count, int which,
+                               int max_sm, hipStream_t st)
+{
+    if (count <= 0)
+        return 0;
+    if (max_sm <= 0 || max_sm > 128 || (which != 0 && which != 1))
+        return (int)hipErrorInvalidValue;
+    const size_t shm = lu_lds_bytes(max_sm, max_sm, true);
+    cacg_allow_lds(lu_batch_kernel, shm);
+    hipLaunchKernelGGL(lu_batch_kernel, dim3(count), dim3(LU_T), shm, st, d_sys, d_items, which);
+    CACG_LAUNCH_CHECK();
+    return 0;
+}
 
 extern "C" int cacg_k_lu_solve(int N, const double *lhs, int k, const double *rhs, double *x,
                                double *d_work, int *d_flag, int mode, int reuse_factor,
@@ -285,21 +351,10 @@ extern "C" int cacg_k_lu_solve(int N, const double *lhs, int k, const double *rh
     if (N <= 0 || k <= 0)
         return (int)hipErrorInvalidValue;
     const bool in_lds = (long)N * N <= 128L * 128L;
-    // columns per pass: the register block (JMAX rows per lane) must cover N,
-    // and without the LDS factor the reference pass keeps N x kc in LDS
-    int kc = k < N ? k : N;  // the reference backward pass reuses the N x N factor LDS for N x kc
-    if ((long)N * kc > (long)LU_T * JMAX)
-        kc = (int)((long)LU_T * JMAX / N);
-    if (!in_lds && (long)N * kc > 8192)
-        kc = 8192 / N;
-    if (kc < 1 || (LU_T / kc) * JMAX < N)
+    const int kc = lu_columns(N, k, in_lds);
+    if (kc < 1)
         return (int)hipErrorInvalidValue;  // N > 16384 is out of range
-    size_t shm;
-    if (in_lds)  // factor, broadcast rows (2 x kc), pivot row/column (2 x 2 x 128), 2 reciprocal
-                 // pivots, N reciprocal diagonal entries
-        shm = ((size_t)N * N + 2 * (size_t)kc + 512 + 2 + N) * sizeof(double);
-    else
-        shm = ((size_t)N * kc + 2 * (size_t)kc + N) * sizeof(double);
+    const size_t shm = lu_lds_bytes(N, k, in_lds);
     cacg_allow_lds(in_lds ? lu_solve_kernel<true> : lu_solve_kernel<false>, shm);
     if (in_lds)
         hipLaunchKernelGGL(lu_solve_kernel<true>, dim3(1), dim3(LU_T), shm, st, N, lhs, k, rhs, x,

@@ -1,0 +1,84 @@
+/*
+ * cacg_batch.h — C-ABI of the batching / serving form: many independent
+ * s-step Chebyshev block-CG solves resident on one GPU at once.
+ *
+ * No counterpart in the reference (it solves one system per MPI job,
+ * src/main.c:140-156); the per-system iteration is the one of
+ * src/bcbcgKernel.c:172-526 with the true residual recomputed in FP64 after
+ * every outer iteration (it drives the stop test and seeds the next basis).
+ *
+ * Requests are queued by cacg_batch_submit (the matrix, B and X0 are copied,
+ * so the caller's buffers can be reused at once).  Each cacg_batch_step is one
+ * round: queued requests are admitted in FIFO order while a slot and their HBM
+ * footprint are free (the head waits, later requests do not overtake it);
+ * then every admitted system advances one outer iteration, in launches shared
+ * by all of them; systems that converged (every column's true residual below
+ * tol * ||B(:,c)||), hit maxiter or broke down are retired and their memory is
+ * reused by the next admission.  Results are fetched with cacg_batch_collect.
+ *
+ * Every per-system sum runs in an order fixed by that system alone, so a
+ * system's result is the same bits whether it runs alone or in any mix.
+ * The calls are not thread-safe: one thread drives a batch.
+ */
+#ifndef CACG_BATCH_H
+#define CACG_BATCH_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cacg_batch cacg_batch;
+
+typedef struct {
+    long hbm_bytes;  /* device arena for the systems' data; 0: a quarter of the free HBM */
+    int slots;       /* systems in flight at most, 1..4096; 0: 1024 */
+    int s;           /* basis length of every system, >= 1 (s * m <= 128) */
+    int precision;   /* CACG_PREC_FP64, or CACG_PREC_FP32_REFINE: FP32 matrix copy, basis and
+                        Chebyshev SpMMs; Q, A Q, Gram, LU, X and R FP64 */
+    long maxiter;    /* outer-iteration cap per system */
+} cacg_batch_opts;
+
+typedef struct {
+    int status;          /* CACG_OK, CACG_MAXITER or CACG_BREAKDOWN */
+    long iters;          /* outer iterations */
+    long rounds_queued;  /* rounds spent waiting for admission */
+    double true_relres;  /* max over columns of ||B - A X||/||B|| at retirement */
+} cacg_batch_result;
+
+typedef struct {
+    int active;          /* systems admitted and iterating */
+    int queued;          /* submitted, waiting for a slot or memory */
+    int finished;        /* retired, not yet collected */
+    int admitted;        /* admitted in the last round */
+    int retired;         /* retired in the last round */
+    long rounds;         /* rounds so far */
+    long arena_used, arena_bytes;
+} cacg_batch_stats;
+
+void cacg_batch_default_opts(cacg_batch_opts *o);
+int cacg_batch_create(cacg_batch **out, const cacg_batch_opts *opts);
+int cacg_batch_destroy(cacg_batch *b);
+
+/* bytes of arena one system takes (its matrix, blocks and work space) */
+long cacg_batch_footprint(long n, long nnz, long m, int s, int precision);
+
+/* Queue one system: CSR of n rows (row_start n + 1, col_idx / val nnz), B and
+ * X0 (NULL: zero) n x m row-major, 1 <= m <= 32, s * m <= 128.  Returns the
+ * request id (>= 0), or -CACG_EINVAL for bad arguments and -CACG_ENOMEM when
+ * the system can never fit the arena. */
+long cacg_batch_submit(cacg_batch *b, long n, const long *row_start, const long *col_idx,
+                       const double *val, long m, const double *B, const double *X0, double tol);
+
+/* One round (admit, one outer iteration of every active system, retire).
+ * Returns CACG_OK or a CACG_E* error; stats may be NULL. */
+int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats);
+
+/* Fetch a retired request: copies its solution into X (n x m) and fills res;
+ * returns 1 and forgets the request, 0 while it is queued or active, or
+ * -CACG_EINVAL for an unknown id. */
+int cacg_batch_collect(cacg_batch *b, long id, double *X, cacg_batch_result *res);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CACG_BATCH_H */

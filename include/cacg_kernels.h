@@ -124,6 +124,63 @@ int cacg_k_colnorm2_fast(int n, int m, const double *R, double *d_norm2, double 
 int cacg_k_fill_i32(int *d, long count, int v, hipStream_t st);
 int cacg_k_fill_f64(double *d, long count, double v, hipStream_t st);
 
+/* ---- batched kernels (many independent systems per launch) ----------------
+ * One descriptor per system slot, in device memory.  Work items are int2
+ * {slot, index}: a 64-row tile (CACG_BT), a 2048-row Gram chunk (CACG_GR) or
+ * a 256-entry block of a Gram result, depending on the kernel.  Every
+ * per-system sum runs in an order fixed by the system alone, so a system's
+ * results do not depend on which other systems share the launch.
+ * No reference counterpart: the batched / serving form is new. */
+#define CACG_BT 64
+#define CACG_GR 2048
+#define CACG_BATCH_MAX_M 32
+#define CACG_BATCH_MAX_SM 128
+typedef struct {
+    int n, m, sm, ntile, nchunk, pad_;
+    const int *rowptr, *col;
+    const double *val;
+    const float *val32;   /* FP32 basis only */
+    const double *B;
+    double *X, *R;        /* n x m */
+    void *S;              /* n x sm basis: float (FP32 basis) or double */
+    double *Q, *AQ;       /* n x sm */
+    double *QtAQ, *QtAS, *beta, *lu;  /* sm x sm */
+    double *QtR, *alpha;  /* sm x m */
+    double *gpart;        /* nchunk x sm x (sm + m) Gram partials */
+    double *npart;        /* ntile x m norm partials */
+    double *norm2;        /* m: ||R(:,c)||^2 of the last true residual */
+    int *flag;            /* set on a zero / non-finite pivot */
+    double sa, sb;        /* Chebyshev interval map */
+} cacg_bsys;
+
+/* basis step j of the systems' tiles: j == 1 writes S_0 = R (rounded to the
+ * basis type) and S_1, j >= 2 writes S_j; f32 selects the FP32 basis (FP32
+ * values and arithmetic).  s == 1 systems take j == 0: S_0 = R only. */
+int cacg_kb_basis(const cacg_bsys *d_sys, const int2 *d_tiles, int count, int j, int f32,
+                  hipStream_t st);
+/* Q = S widened to FP64 (first iteration after admission) */
+int cacg_kb_q_from_s(const cacg_bsys *d_sys, const int2 *d_tiles, int count, int f32,
+                     hipStream_t st);
+/* AQ = A Q (FP64) */
+int cacg_kb_aq(const cacg_bsys *d_sys, const int2 *d_tiles, int count, hipStream_t st);
+/* Gram partials per 2048-row chunk, then the chunk-ordered sums:
+ * which 0: QtAS = AQ^T S; which 1: [QtAQ | QtR] = Q^T [AQ | R] */
+int cacg_kb_gram(const cacg_bsys *d_sys, const int2 *d_chunks, int nchunks, const int2 *d_blocks,
+                 int nblocks, int which, int f32, hipStream_t st);
+/* which 0: beta = QtAQ^{-1} QtAS (factor from the last which-1 call); which 1:
+ * alpha = QtAQ^{-1} QtR (factors QtAQ into lu); max_sm bounds every sm */
+int cacg_k_lu_batch(const cacg_bsys *d_sys, const int2 *d_items, int count, int which, int max_sm,
+                    hipStream_t st);
+/* Q = S - Q beta */
+int cacg_kb_qupdate(const cacg_bsys *d_sys, const int2 *d_tiles, int count, int f32,
+                    hipStream_t st);
+/* X += Q alpha */
+int cacg_kb_xupdate(const cacg_bsys *d_sys, const int2 *d_tiles, int count, hipStream_t st);
+/* R = B - A X (FP64) with per-tile column norm partials, then norm2 of every
+ * listed system (d_sysitems: {slot, 0}) */
+int cacg_kb_resid(const cacg_bsys *d_sys, const int2 *d_tiles, int count,
+                  const int2 *d_sysitems, int nsys, hipStream_t st);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,134 @@
+"""The batching / serving engine on the GPU.
+
+A system's result must be the same bits alone and in any mix (every
+per-system sum runs in an order fixed by the system), including when a small
+HBM arena makes requests wait for admission; every retired system must meet
+its stop test on the true residual; and the solutions must agree with the
+single-system solver within cond(A) * tol.  No reference counterpart exists
+for batching (the reference solves one system per MPI job)."""
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from cacg_amd import problems as P
+from cacg_amd import batch as BT
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+def _mix():
+    """Eight small systems of differing n, nnz, m and shift (2D and 3D)."""
+    specs = [(P.laplacian_2d(40, 0.05), 1), (P.laplacian_2d(33, 0.01), 4),
+             (P.laplacian_3d(9, 0.02), 16), (P.laplacian_2d(64, 0.08), 32),
+             (P.random_spd_like(P.laplacian_2d(20), seed=2), 8), (P.laplacian_2d(50, 0.0), 2),
+             (P.laplacian_3d(12), 3), (P.laplacian_2d(17, 0.1), 5)]
+    out = []
+    for k, (A, m) in enumerate(specs):
+        out.append((A, P.uniform(100 + k, A.n * m).reshape(A.n, m)))
+    return out
+
+
+def _relres(A, B, X):
+    As = sp.csr_matrix((A.val, A.col, A.rowptr), shape=(A.n, A.n))
+    return (np.linalg.norm(B - As @ X, axis=0) / np.linalg.norm(B, axis=0)).max()
+
+
+def _run(systems, **kw):
+    b = BT.Batch(**kw)
+    ids = [b.submit(A, B, tol=TOL) for A, B in systems]
+    res = b.run()
+    b.close()
+    return [res[i] for i in ids]
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_alone_equals_mix_bitwise(gpu_lib, precision):
+    systems = _mix()
+    mixed = _run(systems, s=4, precision=precision, hbm_bytes=1 << 30)
+    for (A, B), r in zip(systems, mixed):
+        assert r["status"] == "ok"
+        rr = _relres(A, B, r["X"])
+        assert rr < TOL and abs(rr - r["true_relres"]) <= 1e-3 * rr
+    for k, sysk in enumerate(systems):
+        alone = _run([sysk], s=4, precision=precision, hbm_bytes=1 << 28)[0]
+        assert alone["iters"] == mixed[k]["iters"]
+        assert np.array_equal(alone["X"], mixed[k]["X"]), k
+
+
+def test_admission_by_footprint_keeps_results(gpu_lib):
+    """An arena that holds only a few systems: the rest wait in FIFO order,
+    every request still completes, with the same bits as an unconstrained run."""
+    systems = _mix()
+    fps = [BT.footprint(A.n, A.nnz, B.shape[1], 4) for A, B in systems]
+    small = int(max(fps) * 1.05) + 4096   # the largest fits alone: the rest must wait
+    free = _run(systems, s=4, hbm_bytes=1 << 30)
+    b = BT.Batch(s=4, hbm_bytes=small)
+    ids = [b.submit(A, B, tol=TOL) for A, B in systems]
+    seen_queue, res = 0, {}
+    while len(res) < len(ids):
+        st = b.step()
+        seen_queue = max(seen_queue, st["queued"])
+        assert st["arena_used"] <= st["arena_bytes"]
+        for rid in ids:
+            if rid not in res:
+                r = b.collect(rid)
+                if r is not None:
+                    res[rid] = r
+    b.close()
+    assert seen_queue > 0
+    assert max(res[i]["rounds_queued"] for i in ids) > 0
+    for k, rid in enumerate(ids):
+        assert res[rid]["status"] == "ok"
+        assert np.array_equal(res[rid]["X"], free[k]["X"])
+
+
+def test_slots_limit_admission(gpu_lib):
+    systems = _mix()[:5]
+    a = _run(systems, s=4, slots=2, hbm_bytes=1 << 30)
+    c = _run(systems, s=4, slots=64, hbm_bytes=1 << 30)
+    for x, y in zip(a, c):
+        assert np.array_equal(x["X"], y["X"])
+
+
+def test_agrees_with_single_system_solver(gpu_lib):
+    for A, B in _mix()[:4]:
+        r = _run([(A, B)], s=4, precision="fp64", hbm_bytes=1 << 28)[0]
+        ref = gpu_lib.bcbcg(A, B, np.zeros_like(B), 4, tol=TOL, maxiter=2000)
+        As = sp.csr_matrix((A.val, A.col, A.rowptr), shape=(A.n, A.n)).toarray()
+        cond = np.linalg.cond(As)
+        err = np.linalg.norm(r["X"] - ref["X"]) / np.linalg.norm(ref["X"])
+        assert err <= 2 * cond * TOL
+
+
+def test_maxiter_and_x0(gpu_lib):
+    A, B = _mix()[3]
+    r = _run([(A, B)], s=4, maxiter=2, hbm_bytes=1 << 28)[0]
+    assert r["status"] == "maxiter" and r["iters"] == 2
+    # a converged X0 retires in the admission round without iterating
+    full = _run([(A, B)], s=4, hbm_bytes=1 << 28)[0]
+    b = BT.Batch(s=4, hbm_bytes=1 << 28)
+    rid = b.submit(A, B, X0=full["X"], tol=TOL)
+    out = b.run()[rid]
+    b.close()
+    assert out["status"] == "ok" and out["iters"] == 0
+
+
+def test_submit_rejects_bad_systems(gpu_lib):
+    b = BT.Batch(s=4, hbm_bytes=1 << 24)
+    A = P.laplacian_2d(8)
+    with pytest.raises(ValueError):
+        b.submit(A, np.ones((A.n, 33)))          # m > 32
+    with pytest.raises(ValueError):
+        b.submit(P.laplacian_2d(400), np.ones((160000, 16)))   # larger than the arena
+    b.close()
+
+
+def test_batching_form_sample(gpu_lib):
+    """A slice of the batching form (shifted 2D Laplacians, FP32 basis)."""
+    systems = BT.batch_systems(count=24, seed=3, lmax=96)
+    res = _run(systems, s=4, precision="fp32", hbm_bytes=4 << 30)
+    for (A, B), r in zip(systems, res):
+        assert r["status"] == "ok"
+        assert _relres(A, B, r["X"]) < TOL
