@@ -56,6 +56,7 @@ def lib():
             "cacg_batch_submit": ([_vp, c_long, _lp, _lp, _dp, c_long, _dp, _dp, c_double], c_long),
             "cacg_batch_step": ([_vp, ctypes.POINTER(BatchStats)], c_int),
             "cacg_batch_collect": ([_vp, c_long, _dp, ctypes.POINTER(BatchResult)], c_int),
+            "cacg_batch_retired": ([_vp, _lp, c_int], c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -109,6 +110,16 @@ class Batch:
                 "admitted": st.admitted, "retired": st.retired, "rounds": st.rounds,
                 "arena_used": st.arena_used, "arena_bytes": st.arena_bytes}
 
+    def retired(self) -> list:
+        """Ids retired since the last call."""
+        out = []
+        buf = np.empty(1024, dtype=np.int64)
+        while True:
+            k = lib().cacg_batch_retired(self.h, buf.ctypes.data_as(_lp), len(buf))
+            out.extend(int(v) for v in buf[:k])
+            if k < len(buf):
+                return out
+
     def collect(self, rid):
         n, m = self.shapes[rid]
         X = np.empty((n, m))
@@ -127,10 +138,8 @@ class Batch:
         out = {}
         for _ in range(max_rounds):
             st = self.step()
-            for rid in list(self.shapes):
-                r = self.collect(rid)
-                if r is not None:
-                    out[rid] = r
+            for rid in self.retired():
+                out[rid] = self.collect(rid)
             if st["active"] == 0 and st["queued"] == 0 and not self.shapes:
                 return out
         raise RuntimeError("batch did not drain")
@@ -217,11 +226,10 @@ def serve_stream(stream, batch: Batch, tol=1e-10, timeout=600.0):
                 continue
         batch.step()
         t_now = time.perf_counter() - t0
-        for rid in [r for r, i in ids.items() if i not in res]:
+        for rid in batch.retired():
             r = batch.collect(rid)
-            if r is not None:
-                r["latency"] = t_now - t_sub[ids[rid]]
-                res[ids[rid]] = r
+            r["latency"] = t_now - t_sub[ids[rid]]
+            res[ids[rid]] = r
     wall = (time.perf_counter() - t0) - stream[0][0]
     return [res[i] for i in range(len(stream))], wall
 

@@ -67,6 +67,8 @@ struct cacg_batch {
     breq **req;          /* by id; NULL once collected */
     long nreq, capreq, qhead;
     long rounds;
+    long *ret;           /* ids retired and not yet reported by cacg_batch_retired */
+    long nret, capret;
 };
 
 #define HCHK(call)                                                                               \
@@ -224,6 +226,7 @@ int cacg_batch_destroy(cacg_batch *b)
     for (long i = 0; i < b->nreq; i++)
         free_req(b->req[i]);
     free(b->req);
+    free(b->ret);
     free(b->slot);
     free(b->fl);
     if (b->arena)
@@ -519,6 +522,15 @@ static int retire(cacg_batch *b, breq *r, int status, const double *norm2)
     HCHK(hipMemcpyAsync(r->X, b->h_sys[r->slot].X, sizeof(double) * (size_t)(r->n * r->m),
                         hipMemcpyDeviceToHost, b->st));
     HCHK(hipStreamSynchronize(b->st));
+    if (b->nret == b->capret) {
+        const long cap = b->capret ? 2 * b->capret : 256;
+        long *nr = (long *)realloc(b->ret, sizeof(long) * (size_t)cap);
+        if (!nr)
+            return CACG_ENOMEM;
+        b->ret = nr;
+        b->capret = cap;
+    }
+    b->ret[b->nret++] = r->id;
     r->true_relres = worst;
     r->status = status;
     r->state = RQ_DONE;
@@ -536,6 +548,19 @@ static int retire(cacg_batch *b, breq *r, int status, const double *norm2)
     return arena_free(b, r->off, r->bytes);
 }
 
+/* MFMA tile class of a system's Gram (cacg_kb_gram_class): 1 and (tm, tn) when
+ * tm = ceil(sm / 16) is 1, 2, 4 or 8 and tn = ceil(kr / 16) is in tm .. tm + 2 */
+static int gram_class(const cacg_bsys *d, int which, int *tm, int *tn)
+{
+    const int kr = d->sm + (which ? d->m : 0);
+    const int a = (d->sm + 15) / 16, c = (kr + 15) / 16;
+    if ((a != 1 && a != 2 && a != 4 && a != 8) || c < a || c > a + 2)
+        return 0;
+    *tm = a;
+    *tn = c;
+    return 1;
+}
+
 /* append {slot, 0 .. count - 1} */
 static long put_range(int2 *it, long k, int slot, int count)
 {
@@ -545,6 +570,22 @@ static long put_range(int2 *it, long k, int slot, int count)
         k++;
     }
     return k;
+}
+
+/* the Gram partials of every listed system (VALU list, then one MFMA launch
+ * per tile class), then the chunk-ordered sums */
+static int gram_phase(cacg_batch *b, const int2 *di, const long *o, const long *cnt,
+                      const int2 *blocks, long nblocks, int which, int f32)
+{
+    hipStream_t st = b->st;
+    int rc = cacg_kb_gram(b->d_sys, di + o[0], (int)cnt[0], NULL, 0, which, f32, st);
+    for (int c = 1; c < 13 && rc == 0; c++) {
+        const int tm = 1 << ((c - 1) / 3), tn = tm + (c - 1) % 3;
+        rc = cacg_kb_gram_class(b->d_sys, di + o[c], (int)cnt[c], which, f32, tm, tn, st);
+    }
+    if (rc == 0)
+        rc = cacg_kb_gram_reduce(b->d_sys, blocks, (int)nblocks, which, st);
+    return rc;
 }
 
 int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
@@ -588,7 +629,7 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
         need += 1 + d->ntile;
         if (r->joined) {
             nit++;
-            need += 3L * d->ntile + 2L * d->nchunk + 2 +
+            need += 4L * d->ntile + 4L * d->nchunk + 2 +
                     ((long)d->sm * (d->sm + d->m) + 255) / 256 + ((long)d->sm * d->sm + 255) / 256;
             if (d->sm > max_sm)
                 max_sm = d->sm;
@@ -598,8 +639,8 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
         return CACG_ENOMEM;
     int2 *it = b->h_items;
     long k = 0;
-    /* order: iterating tiles | fresh tiles | update tiles | Gram0 chunks | Gram0 blocks |
-     *        LU0 | Gram1 chunks | Gram1 blocks | LU1 | all tiles | all systems */
+    /* order: iterating tiles | fresh tiles | update tiles | Gram chunks by class (which 0,
+     *        then 1) | Gram0 blocks | LU0 | Gram1 blocks | LU1 | all tiles | all systems */
     long o_it = k;
     for (int i = 0; i < b->nslots; i++)
         if (b->slot[i] && b->slot[i]->joined)
@@ -609,38 +650,77 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
         if (b->slot[i] && b->slot[i]->joined && b->slot[i]->fresh)
             k = put_range(it, k, i, b->h_sys[i].ntile);
     const long n_fr = k - o_fr, o_up = k;
+    /* Q updates: VALU tiles for widths that are not a multiple of 16, MFMA chunks per width */
     for (int i = 0; i < b->nslots; i++)
-        if (b->slot[i] && b->slot[i]->joined && !b->slot[i]->fresh)
+        if (b->slot[i] && b->slot[i]->joined && !b->slot[i]->fresh && b->h_sys[i].sm % 16 != 0)
             k = put_range(it, k, i, b->h_sys[i].ntile);
-    const long n_up = k - o_up, o_g0 = k;
+    const long n_up = k - o_up, o_xv = k;
     for (int i = 0; i < b->nslots; i++)
-        if (b->slot[i] && b->slot[i]->joined && !b->slot[i]->fresh)
-            k = put_range(it, k, i, b->h_sys[i].nchunk);
-    const long n_g0 = k - o_g0, o_b0 = k;
+        if (b->slot[i] && b->slot[i]->joined && b->h_sys[i].sm % 16 != 0)
+            k = put_range(it, k, i, b->h_sys[i].ntile);
+    const long n_xv = k - o_xv;
+    long o_xc[9][2], n_xc[9][2];
+    for (int c = 1; c <= 8; c++)
+        for (int h = 0; h < 2; h++) {
+            o_xc[c][h] = k;
+            for (int i = 0; i < b->nslots; i++)
+                if (b->slot[i] && b->slot[i]->joined && b->h_sys[i].sm == 16 * c &&
+                    (b->h_sys[i].m > 16) == h)
+                    k = put_range(it, k, i, b->h_sys[i].nchunk);
+            n_xc[c][h] = k - o_xc[c][h];
+        }
+    long o_qc[9], n_qc[9];
+    for (int c = 1; c <= 8; c++) {
+        o_qc[c] = k;
+        for (int i = 0; i < b->nslots; i++)
+            if (b->slot[i] && b->slot[i]->joined && !b->slot[i]->fresh &&
+                b->h_sys[i].sm == 16 * c)
+                k = put_range(it, k, i, b->h_sys[i].nchunk);
+        n_qc[c] = k - o_qc[c];
+    }
+    /* Gram chunks per tile class: [which][0] VALU, [which][1 + 3 * log2(tm) + tn - tm] MFMA */
+    long o_gc[2][13], n_gc[2][13];
+    for (int w = 0; w < 2; w++)
+        for (int c = 0; c < 13; c++) {
+            o_gc[w][c] = k;
+            for (int i = 0; i < b->nslots; i++) {
+                const breq *r = b->slot[i];
+                if (!r || !r->joined || (w == 0 && r->fresh))
+                    continue;
+                int tm = 0, tn = 0, cls = 0;
+                if (gram_class(&b->h_sys[i], w, &tm, &tn))
+                    cls = 1 + 3 * (tm == 1 ? 0 : tm == 2 ? 1 : tm == 4 ? 2 : 3) + (tn - tm);
+                if (cls == c)
+                    k = put_range(it, k, i, b->h_sys[i].nchunk);
+            }
+            n_gc[w][c] = k - o_gc[w][c];
+        }
+    const long o_b0 = k;
     for (int i = 0; i < b->nslots; i++)
         if (b->slot[i] && b->slot[i]->joined && !b->slot[i]->fresh) {
             const cacg_bsys *d = &b->h_sys[i];
             k = put_range(it, k, i, (int)(((long)d->sm * d->sm + 255) / 256));
         }
-    const long n_b0 = k - o_b0, o_l0 = k;
-    for (int i = 0; i < b->nslots; i++)
-        if (b->slot[i] && b->slot[i]->joined && !b->slot[i]->fresh)
-            k = put_range(it, k, i, 1);
-    const long n_l0 = k - o_l0, o_g1 = k;
-    for (int i = 0; i < b->nslots; i++)
-        if (b->slot[i] && b->slot[i]->joined)
-            k = put_range(it, k, i, b->h_sys[i].nchunk);
-    const long n_g1 = k - o_g1, o_b1 = k;
+    const long n_b0 = k - o_b0;
+    /* LU solves in two LDS classes: sm <= 64 (several workgroups per CU), larger */
+    long o_lu[2][2], n_lu[2][2];
+    for (int w = 0; w < 2; w++)
+        for (int c = 0; c < 2; c++) {
+            o_lu[w][c] = k;
+            for (int i = 0; i < b->nslots; i++) {
+                const breq *r = b->slot[i];
+                if (r && r->joined && (w == 1 || !r->fresh) && ((b->h_sys[i].sm > 64) == c))
+                    k = put_range(it, k, i, 1);
+            }
+            n_lu[w][c] = k - o_lu[w][c];
+        }
+    const long o_b1 = k;
     for (int i = 0; i < b->nslots; i++)
         if (b->slot[i] && b->slot[i]->joined) {
             const cacg_bsys *d = &b->h_sys[i];
             k = put_range(it, k, i, (int)(((long)d->sm * (d->sm + d->m) + 255) / 256));
         }
-    const long n_b1 = k - o_b1, o_l1 = k;
-    for (int i = 0; i < b->nslots; i++)
-        if (b->slot[i] && b->slot[i]->joined)
-            k = put_range(it, k, i, 1);
-    const long n_l1 = k - o_l1, o_all = k;
+    const long n_b1 = k - o_b1, o_all = k;
     for (int i = 0; i < b->nslots; i++)
         if (b->slot[i])
             k = put_range(it, k, i, b->h_sys[i].ntile);
@@ -663,13 +743,21 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
                 KCHK(cacg_kb_basis(b->d_sys, di + o_it, (int)n_it, j, f32, st));
         }
         KCHK(cacg_kb_q_from_s(b->d_sys, di + o_fr, (int)n_fr, f32, st));
-        KCHK(cacg_kb_gram(b->d_sys, di + o_g0, (int)n_g0, di + o_b0, (int)n_b0, 0, f32, st));
-        KCHK(cacg_k_lu_batch(b->d_sys, di + o_l0, (int)n_l0, 0, max_sm, st));
+        KCHK(gram_phase(b, di, o_gc[0], n_gc[0], di + o_b0, n_b0, 0, f32));
+        KCHK(cacg_k_lu_batch(b->d_sys, di + o_lu[0][0], (int)n_lu[0][0], 0, 64, st));
+        KCHK(cacg_k_lu_batch(b->d_sys, di + o_lu[0][1], (int)n_lu[0][1], 0, max_sm, st));
         KCHK(cacg_kb_qupdate(b->d_sys, di + o_up, (int)n_up, f32, st));
+        for (int c = 1; c <= 8; c++)
+            KCHK(cacg_kb_qupdate_class(b->d_sys, di + o_qc[c], (int)n_qc[c], 16 * c, f32, st));
         KCHK(cacg_kb_aq(b->d_sys, di + o_it, (int)n_it, st));
-        KCHK(cacg_kb_gram(b->d_sys, di + o_g1, (int)n_g1, di + o_b1, (int)n_b1, 1, f32, st));
-        KCHK(cacg_k_lu_batch(b->d_sys, di + o_l1, (int)n_l1, 1, max_sm, st));
-        KCHK(cacg_kb_xupdate(b->d_sys, di + o_it, (int)n_it, st));
+        KCHK(gram_phase(b, di, o_gc[1], n_gc[1], di + o_b1, n_b1, 1, f32));
+        KCHK(cacg_k_lu_batch(b->d_sys, di + o_lu[1][0], (int)n_lu[1][0], 1, 64, st));
+        KCHK(cacg_k_lu_batch(b->d_sys, di + o_lu[1][1], (int)n_lu[1][1], 1, max_sm, st));
+        KCHK(cacg_kb_xupdate(b->d_sys, di + o_xv, (int)n_xv, st));
+        for (int c = 1; c <= 8; c++)
+            for (int h = 0; h < 2; h++)
+                KCHK(cacg_kb_xupdate_class(b->d_sys, di + o_xc[c][h], (int)n_xc[c][h], 16 * c,
+                                           h ? 32 : 16, st));
     }
     /* the true residual of every admitted system */
     KCHK(cacg_kb_resid(b->d_sys, di + o_all, (int)n_all, di + o_sys, (int)n_sys, st));
@@ -738,6 +826,17 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
         stats->arena_bytes = b->arena_bytes;
     }
     return CACG_OK;
+}
+
+int cacg_batch_retired(cacg_batch *b, long *ids, int cap)
+{
+    if (!b || cap < 0 || (cap > 0 && !ids))
+        return -CACG_EINVAL;
+    const int k = b->nret < cap ? (int)b->nret : cap;
+    memcpy(ids, b->ret, sizeof(long) * (size_t)k);
+    memmove(b->ret, b->ret + k, sizeof(long) * (size_t)(b->nret - k));
+    b->nret -= k;
+    return k;
 }
 
 int cacg_batch_collect(cacg_batch *b, long id, double *X, cacg_batch_result *res)

@@ -169,6 +169,97 @@ __global__ __launch_bounds__(1024) void bgram_kernel(const cacg_bsys *__restrict
     }
 }
 
+// MFMA Gram partial of one 2048-row chunk for systems with TM = ceil(sm / 16)
+// row tiles and TN = ceil(kr / 16) column tiles (columns past sm and kr are
+// zeros in LDS and are not written).  NW = min(TM, 4)
+// waves each own MT = ceil(TM / NW) M tiles x TN N tiles; stages of 16 rows
+// of L and R are loaded by all 256 lanes into LDS (rows past the chunk as
+// zeros) and read as MFMA operands (contraction index = row).  Per tile the
+// k-steps run in row order, so the partial depends on the system alone.
+template <typename TB, int TM, int TN>
+__global__ __launch_bounds__(256) void bgram_mfma_kernel(const cacg_bsys *__restrict__ sys,
+                                                         const int2 *__restrict__ items, int which)
+{
+    constexpr int NW = TM < 4 ? TM : 4, MT = (TM + NW - 1) / NW;
+    constexpr int KL = TM * 16, LP = KL + 2, RP = TN * 16 + 2;
+    __shared__ double Ls[GS * LP];
+    __shared__ double Rs[GS * RP];
+    const int2 it = items[blockIdx.x];
+    const cacg_bsys &d = sys[it.x];
+    const int n = d.n, m = d.m, sm = d.sm;
+    const int kr1 = sm, kr = sm + (which ? m : 0);
+    const double *L = which ? d.Q : d.AQ;
+    const TB *S = reinterpret_cast<const TB *>(d.S);
+    const int c0 = it.y * GR, c1 = (c0 + GR) < n ? (c0 + GR) : n;
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63, lr = lane >> 4, lc = lane & 15;
+    dbl4 acc[MT][TN];
+#pragma unroll
+    for (int i = 0; i < MT; i++)
+#pragma unroll
+        for (int j = 0; j < TN; j++)
+            acc[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
+    for (int rb = c0; rb < c1; rb += GS) {
+        const int rows = (c1 - rb) < GS ? (c1 - rb) : GS;
+        for (int e = t; e < GS * KL; e += 256) {
+            const int r = e / KL, a = e - r * KL;
+            Ls[r * LP + a] = (r < rows && a < sm) ? L[(long)(rb + r) * sm + a] : 0.0;
+        }
+        for (int e = t; e < GS * kr; e += 256) {
+            const int r = e / kr, b = e - r * kr;
+            double v = 0.0;
+            if (r < rows) {
+                const long i = rb + r;
+                if (b < kr1)
+                    v = which ? d.AQ[i * sm + b] : (double)S[i * sm + b];
+                else
+                    v = d.R[i * m + (b - kr1)];
+            }
+            Rs[r * RP + b] = v;
+        }
+        __syncthreads();
+        if (wave < NW) {
+#pragma unroll
+            for (int ks = 0; ks < GS / 4; ks++) {
+                const int row = ks * 4 + lr;
+                double av[MT], bv[TN];
+#pragma unroll
+                for (int i = 0; i < MT; i++) {
+                    const int a = (wave * MT + i) * 16 + lc;
+                    av[i] = a < KL ? Ls[row * LP + a] : 0.0;
+                }
+#pragma unroll
+                for (int j = 0; j < TN; j++) {
+                    const int b = j * 16 + lc;
+                    bv[j] = b < kr ? Rs[row * RP + b] : 0.0;
+                }
+#pragma unroll
+                for (int i = 0; i < MT; i++)
+#pragma unroll
+                    for (int j = 0; j < TN; j++)
+                        acc[i][j] = mfma16x16x4(av[i], bv[j], acc[i][j]);
+            }
+        }
+        __syncthreads();
+    }
+    if (wave >= NW)
+        return;
+    double *gp = d.gpart + (long)it.y * sm * kr;
+#pragma unroll
+    for (int i = 0; i < MT; i++) {
+        const int tm = wave * MT + i;
+        if (tm >= TM)
+            continue;
+#pragma unroll
+        for (int j = 0; j < TN; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int a = tm * 16 + lr + 4 * r, b = j * 16 + lc;
+                if (a < sm && b < kr)
+                    gp[(long)a * kr + b] = acc[i][j][r];
+            }
+    }
+}
+
 // chunk-ordered sum of the partials, 256 result entries per workgroup
 __global__ __launch_bounds__(256) void bgram_reduce_kernel(const cacg_bsys *__restrict__ sys,
                                                            const int2 *__restrict__ items,
@@ -219,6 +310,68 @@ __global__ __launch_bounds__(256) void bqupdate_kernel(const cacg_bsys *__restri
     }
 }
 
+// Q = S - Q beta with FP64 MFMA for systems with sm = SM (a multiple of 16):
+// one 2048-row chunk per 512-lane workgroup, beta in LDS, each wave takes
+// 16-row tiles of the chunk; the contraction index is permuted across the
+// lane groups (group g supplies k = g SM/4 + t at step t), so each lane's A
+// operand is SM/4 contiguous values of one row, read before the tile is
+// written (in place).  Per output the k order is fixed by SM alone.
+template <typename TB, int SM>
+__global__ __launch_bounds__(512) void bqupdate_mfma_kernel(const cacg_bsys *__restrict__ sys,
+                                                            const int2 *__restrict__ items)
+{
+    constexpr int KQ = SM / 4, NTL = SM / 16;
+    extern __shared__ double bl[];  // SM x SM
+    const int2 it = items[blockIdx.x];
+    const cacg_bsys &d = sys[it.x];
+    const int n = d.n;
+    for (int e = threadIdx.x; e < SM * SM; e += 512)
+        bl[e] = d.beta[e];
+    __syncthreads();
+    const TB *S = reinterpret_cast<const TB *>(d.S);
+    double *Q = d.Q;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int lg = lane >> 4, lc = lane & 15;
+    const int c0 = it.y * GR, c1 = (c0 + GR) < n ? (c0 + GR) : n;
+    for (int i0 = c0 + wave * 16; i0 < c1; i0 += 8 * 16) {
+        const long arow = i0 + lc;
+        double av[KQ];
+        if (arow < c1) {
+            const double2 *src = reinterpret_cast<const double2 *>(Q + arow * SM + lg * KQ);
+#pragma unroll
+            for (int u = 0; u < KQ / 2; u++) {
+                const double2 v = src[u];
+                av[2 * u] = v.x;
+                av[2 * u + 1] = v.y;
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < KQ; u++)
+                av[u] = 0.0;
+        }
+#pragma unroll 1
+        for (int j = 0; j < NTL; j++) {
+            double sv[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const long i = i0 + lg + 4 * r;
+                sv[r] = i < c1 ? (double)S[i * SM + j * 16 + lc] : 0.0;
+            }
+            dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+            const double *bcol = bl + (lg * KQ) * SM + j * 16 + lc;
+#pragma unroll
+            for (int t = 0; t < KQ; t++)
+                acc = mfma16x16x4(av[t], bcol[t * SM], acc);
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const long i = i0 + lg + 4 * r;
+                if (i < c1)
+                    Q[i * SM + j * 16 + lc] = sv[r] - acc[r];
+            }
+        }
+    }
+}
+
 // X += Q alpha
 __global__ __launch_bounds__(256) void bxupdate_kernel(const cacg_bsys *__restrict__ sys,
                                                        const int2 *__restrict__ items)
@@ -236,6 +389,63 @@ __global__ __launch_bounds__(256) void bxupdate_kernel(const cacg_bsys *__restri
         for (int q = 0; q < sm; q++)
             acc = fma(qr[q], d.alpha[(long)q * m + c], acc);
         d.X[i * m + c] += acc;
+    }
+}
+
+// X += Q alpha with FP64 MFMA for systems with sm = SM (a multiple of 16):
+// one 2048-row chunk per 256-lane workgroup, alpha (SM x m, m <= 16 NTM) in
+// registers in the K-permuted order of the A operand, 16-row tiles per wave.
+template <int SM, int NTM>
+__global__ __launch_bounds__(256) void bxupdate_mfma_kernel(const cacg_bsys *__restrict__ sys,
+                                                            const int2 *__restrict__ items)
+{
+    constexpr int KQ = SM / 4;
+    const int2 it = items[blockIdx.x];
+    const cacg_bsys &d = sys[it.x];
+    const int n = d.n, m = d.m;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int lg = lane >> 4, lc = lane & 15;
+    double alf[NTM][KQ];
+#pragma unroll
+    for (int j = 0; j < NTM; j++) {
+        const int c = j * 16 + lc;
+#pragma unroll
+        for (int t = 0; t < KQ; t++)
+            alf[j][t] = c < m ? d.alpha[(long)(lg * KQ + t) * m + c] : 0.0;
+    }
+    const int c0 = it.y * GR, c1 = (c0 + GR) < n ? (c0 + GR) : n;
+    for (int i0 = c0 + wave * 16; i0 < c1; i0 += 4 * 16) {
+        const long arow = i0 + lc;
+        double av[KQ];
+        if (arow < c1) {
+            const double2 *src = reinterpret_cast<const double2 *>(d.Q + arow * SM + lg * KQ);
+#pragma unroll
+            for (int u = 0; u < KQ / 2; u++) {
+                const double2 v = src[u];
+                av[2 * u] = v.x;
+                av[2 * u + 1] = v.y;
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < KQ; u++)
+                av[u] = 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < NTM; j++) {
+            dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int t = 0; t < KQ; t++)
+                acc = mfma16x16x4(av[t], alf[j][t], acc);
+            const int c = j * 16 + lc;
+            if (c < m) {
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const long i = i0 + lg + 4 * r;
+                    if (i < c1)
+                        d.X[i * m + c] += acc[r];
+                }
+            }
+        }
     }
 }
 
@@ -344,6 +554,65 @@ extern "C" int cacg_kb_gram(const cacg_bsys *d_sys, const int2 *d_chunks, int nc
     return 0;
 }
 
+namespace {
+template <typename TB, int TM>
+int launch_gram_class(const cacg_bsys *d_sys, const int2 *d_chunks, int nchunks, int which, int tn,
+                      hipStream_t st)
+{
+    switch (tn - TM) {
+    case 0:
+        hipLaunchKernelGGL((bgram_mfma_kernel<TB, TM, TM>), dim3(nchunks), dim3(256), 0, st, d_sys,
+                           d_chunks, which);
+        break;
+    case 1:
+        hipLaunchKernelGGL((bgram_mfma_kernel<TB, TM, TM + 1>), dim3(nchunks), dim3(256), 0, st,
+                           d_sys, d_chunks, which);
+        break;
+    case 2:
+        hipLaunchKernelGGL((bgram_mfma_kernel<TB, TM, TM + 2>), dim3(nchunks), dim3(256), 0, st,
+                           d_sys, d_chunks, which);
+        break;
+    default:
+        return (int)hipErrorInvalidValue;
+    }
+    CACG_LAUNCH_CHECK();
+    return 0;
+}
+}  // namespace
+
+extern "C" int cacg_kb_gram_class(const cacg_bsys *d_sys, const int2 *d_chunks, int nchunks,
+                                  int which, int f32, int tm, int tn, hipStream_t st)
+{
+    if (nchunks <= 0)
+        return 0;
+    if (which != 0 && which != 1)
+        return (int)hipErrorInvalidValue;
+#define CACG_GCLASS(TMV)                                                                          \
+    case TMV:                                                                                     \
+        return f32 ? launch_gram_class<float, TMV>(d_sys, d_chunks, nchunks, which, tn, st)      \
+                   : launch_gram_class<double, TMV>(d_sys, d_chunks, nchunks, which, tn, st);
+    switch (tm) {
+        CACG_GCLASS(1)
+        CACG_GCLASS(2)
+        CACG_GCLASS(4)
+        CACG_GCLASS(8)
+    default:
+        return (int)hipErrorInvalidValue;
+    }
+#undef CACG_GCLASS
+}
+
+extern "C" int cacg_kb_gram_reduce(const cacg_bsys *d_sys, const int2 *d_blocks, int nblocks,
+                                   int which, hipStream_t st)
+{
+    if (nblocks <= 0)
+        return 0;
+    hipLaunchKernelGGL(bgram_reduce_kernel, dim3(nblocks), dim3(256), 0, st, d_sys, d_blocks,
+                       which);
+    CACG_LAUNCH_CHECK();
+    return 0;
+}
+
 extern "C" int cacg_kb_qupdate(const cacg_bsys *d_sys, const int2 *d_tiles, int count, int f32,
                                hipStream_t st)
 {
@@ -361,12 +630,80 @@ extern "C" int cacg_kb_qupdate(const cacg_bsys *d_sys, const int2 *d_tiles, int 
     return 0;
 }
 
+extern "C" int cacg_kb_qupdate_class(const cacg_bsys *d_sys, const int2 *d_chunks, int count,
+                                     int sm, int f32, hipStream_t st)
+{
+    if (count <= 0)
+        return 0;
+    const size_t shm = (size_t)sm * sm * sizeof(double);
+#define CACG_QCLASS(SMV)                                                                          \
+    case SMV:                                                                                     \
+        if (f32) {                                                                                \
+            cacg_allow_lds(bqupdate_mfma_kernel<float, SMV>, shm);                                \
+            hipLaunchKernelGGL((bqupdate_mfma_kernel<float, SMV>), dim3(count), dim3(512), shm,   \
+                               st, d_sys, d_chunks);                                              \
+        } else {                                                                                  \
+            cacg_allow_lds(bqupdate_mfma_kernel<double, SMV>, shm);                               \
+            hipLaunchKernelGGL((bqupdate_mfma_kernel<double, SMV>), dim3(count), dim3(512), shm,  \
+                               st, d_sys, d_chunks);                                              \
+        }                                                                                         \
+        break;
+    switch (sm) {
+        CACG_QCLASS(16)
+        CACG_QCLASS(32)
+        CACG_QCLASS(48)
+        CACG_QCLASS(64)
+        CACG_QCLASS(80)
+        CACG_QCLASS(96)
+        CACG_QCLASS(112)
+        CACG_QCLASS(128)
+    default:
+        return (int)hipErrorInvalidValue;
+    }
+#undef CACG_QCLASS
+    CACG_LAUNCH_CHECK();
+    return 0;
+}
+
 extern "C" int cacg_kb_xupdate(const cacg_bsys *d_sys, const int2 *d_tiles, int count,
                                hipStream_t st)
 {
     if (count <= 0)
         return 0;
     hipLaunchKernelGGL(bxupdate_kernel, dim3(count), dim3(256), 0, st, d_sys, d_tiles);
+    CACG_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int cacg_kb_xupdate_class(const cacg_bsys *d_sys, const int2 *d_chunks, int count,
+                                     int sm, int m, hipStream_t st)
+{
+    if (count <= 0)
+        return 0;
+    if (m < 1 || m > 32)
+        return (int)hipErrorInvalidValue;
+#define CACG_XCLASS(SMV)                                                                          \
+    case SMV:                                                                                     \
+        if (m <= 16)                                                                              \
+            hipLaunchKernelGGL((bxupdate_mfma_kernel<SMV, 1>), dim3(count), dim3(256), 0, st,     \
+                               d_sys, d_chunks);                                                  \
+        else                                                                                      \
+            hipLaunchKernelGGL((bxupdate_mfma_kernel<SMV, 2>), dim3(count), dim3(256), 0, st,     \
+                               d_sys, d_chunks);                                                  \
+        break;
+    switch (sm) {
+        CACG_XCLASS(16)
+        CACG_XCLASS(32)
+        CACG_XCLASS(48)
+        CACG_XCLASS(64)
+        CACG_XCLASS(80)
+        CACG_XCLASS(96)
+        CACG_XCLASS(112)
+        CACG_XCLASS(128)
+    default:
+        return (int)hipErrorInvalidValue;
+    }
+#undef CACG_XCLASS
     CACG_LAUNCH_CHECK();
     return 0;
 }

@@ -73,6 +73,10 @@ long cacg_batch_submit(cacg_batch *b, long n, const long *row_start, const long 
  * Returns CACG_OK or a CACG_E* error; stats may be NULL. */
 int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats);
 
+/* Ids of the requests retired since the last call (at most cap of them, the
+ * rest stay for the next call); returns how many were written. */
+int cacg_batch_retired(cacg_batch *b, long *ids, int cap);
+
 /* Fetch a retired request: copies its solution into X (n x m) and fills res;
  * returns 1 and forgets the request, 0 while it is queued or active, or
  * -CACG_EINVAL for an unknown id. */

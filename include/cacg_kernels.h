@@ -167,6 +167,14 @@ int cacg_kb_aq(const cacg_bsys *d_sys, const int2 *d_tiles, int count, hipStream
  * which 0: QtAS = AQ^T S; which 1: [QtAQ | QtR] = Q^T [AQ | R] */
 int cacg_kb_gram(const cacg_bsys *d_sys, const int2 *d_chunks, int nchunks, const int2 *d_blocks,
                  int nblocks, int which, int f32, hipStream_t st);
+/* MFMA form of the Gram partials for the systems of one tile class:
+ * tm = ceil(sm / 16) in 1, 2, 4, 8 and tn = ceil(kr / 16) in tm .. tm + 2, kr = sm
+ * (which 0) or sm + m (which 1); the chunk-ordered sums then by cacg_kb_gram_reduce.
+ * cacg_kb_gram (nblocks = 0) covers every other system with VALU FP64. */
+int cacg_kb_gram_class(const cacg_bsys *d_sys, const int2 *d_chunks, int nchunks, int which,
+                       int f32, int tm, int tn, hipStream_t st);
+int cacg_kb_gram_reduce(const cacg_bsys *d_sys, const int2 *d_blocks, int nblocks, int which,
+                        hipStream_t st);
 /* which 0: beta = QtAQ^{-1} QtAS (factor from the last which-1 call); which 1:
  * alpha = QtAQ^{-1} QtR (factors QtAQ into lu); max_sm bounds every sm */
 int cacg_k_lu_batch(const cacg_bsys *d_sys, const int2 *d_items, int count, int which, int max_sm,
@@ -174,8 +182,17 @@ int cacg_k_lu_batch(const cacg_bsys *d_sys, const int2 *d_items, int count, int 
 /* Q = S - Q beta */
 int cacg_kb_qupdate(const cacg_bsys *d_sys, const int2 *d_tiles, int count, int f32,
                     hipStream_t st);
+/* MFMA form of Q = S - Q beta over 2048-row chunks ({slot, chunk} items) of
+ * systems whose sm equals the given multiple of 16 (16 .. 128) */
+int cacg_kb_qupdate_class(const cacg_bsys *d_sys, const int2 *d_chunks, int count, int sm,
+                          int f32, hipStream_t st);
 /* X += Q alpha */
 int cacg_kb_xupdate(const cacg_bsys *d_sys, const int2 *d_tiles, int count, hipStream_t st);
+/* MFMA form of X += Q alpha over 2048-row chunks of systems with the given sm
+ * (a multiple of 16, <= 128) and m (<= 32); every listed system must have
+ * this m class: m <= 16 or 16 < m <= 32 use different instantiations */
+int cacg_kb_xupdate_class(const cacg_bsys *d_sys, const int2 *d_chunks, int count, int sm, int m,
+                          hipStream_t st);
 /* R = B - A X (FP64) with per-tile column norm partials, then norm2 of every
  * listed system (d_sysitems: {slot, 0}) */
 int cacg_kb_resid(const cacg_bsys *d_sys, const int2 *d_tiles, int count,

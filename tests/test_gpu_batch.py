@@ -71,11 +71,8 @@ def test_admission_by_footprint_keeps_results(gpu_lib):
         st = b.step()
         seen_queue = max(seen_queue, st["queued"])
         assert st["arena_used"] <= st["arena_bytes"]
-        for rid in ids:
-            if rid not in res:
-                r = b.collect(rid)
-                if r is not None:
-                    res[rid] = r
+        for rid in b.retired():
+            res[rid] = b.collect(rid)
     b.close()
     assert seen_queue > 0
     assert max(res[i]["rounds_queued"] for i in ids) > 0
