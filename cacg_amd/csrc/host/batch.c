@@ -549,12 +549,12 @@ static int retire(cacg_batch *b, breq *r, int status, const double *norm2)
 }
 
 /* MFMA tile class of a system's Gram (cacg_kb_gram_class): 1 and (tm, tn) when
- * tm = ceil(sm / 16) is 1, 2, 4 or 8 and tn = ceil(kr / 16) is in tm .. tm + 2 */
+ * tm = ceil(sm / 16) is 1 .. 8 and tn = ceil(kr / 16) is in tm .. tm + 2 */
 static int gram_class(const cacg_bsys *d, int which, int *tm, int *tn)
 {
     const int kr = d->sm + (which ? d->m : 0);
     const int a = (d->sm + 15) / 16, c = (kr + 15) / 16;
-    if ((a != 1 && a != 2 && a != 4 && a != 8) || c < a || c > a + 2)
+    if (a < 1 || a > 8 || c < a || c > a + 2)
         return 0;
     *tm = a;
     *tn = c;
@@ -572,15 +572,15 @@ static long put_range(int2 *it, long k, int slot, int count)
     return k;
 }
 
-/* the Gram partials of every listed system (VALU list, then one MFMA launch
- * per tile class), then the chunk-ordered sums */
+/* the Gram partials of every listed system (one MFMA launch per tile class),
+ * then the chunk-ordered sums */
 static int gram_phase(cacg_batch *b, const int2 *di, const long *o, const long *cnt,
                       const int2 *blocks, long nblocks, int which, int f32)
 {
     hipStream_t st = b->st;
-    int rc = cacg_kb_gram(b->d_sys, di + o[0], (int)cnt[0], NULL, 0, which, f32, st);
-    for (int c = 1; c < 13 && rc == 0; c++) {
-        const int tm = 1 << ((c - 1) / 3), tn = tm + (c - 1) % 3;
+    int rc = cnt[0] > 0 ? (int)hipErrorInvalidValue : 0;   /* no class: not admissible */
+    for (int c = 1; c < 25 && rc == 0; c++) {
+        const int tm = 1 + (c - 1) / 3, tn = tm + (c - 1) % 3;
         rc = cacg_kb_gram_class(b->d_sys, di + o[c], (int)cnt[c], which, f32, tm, tn, st);
     }
     if (rc == 0)
@@ -629,7 +629,7 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
         need += 1 + d->ntile;
         if (r->joined) {
             nit++;
-            need += 4L * d->ntile + 4L * d->nchunk + 2 +
+            need += 2L * d->ntile + 4L * d->nchunk + 2 +
                     ((long)d->sm * (d->sm + d->m) + 255) / 256 + ((long)d->sm * d->sm + 255) / 256;
             if (d->sm > max_sm)
                 max_sm = d->sm;
@@ -649,39 +649,37 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
     for (int i = 0; i < b->nslots; i++)
         if (b->slot[i] && b->slot[i]->joined && b->slot[i]->fresh)
             k = put_range(it, k, i, b->h_sys[i].ntile);
-    const long n_fr = k - o_fr, o_up = k;
-    /* Q updates: VALU tiles for widths that are not a multiple of 16, MFMA chunks per width */
-    for (int i = 0; i < b->nslots; i++)
-        if (b->slot[i] && b->slot[i]->joined && !b->slot[i]->fresh && b->h_sys[i].sm % 16 != 0)
-            k = put_range(it, k, i, b->h_sys[i].ntile);
-    const long n_up = k - o_up, o_xv = k;
-    for (int i = 0; i < b->nslots; i++)
-        if (b->slot[i] && b->slot[i]->joined && b->h_sys[i].sm % 16 != 0)
-            k = put_range(it, k, i, b->h_sys[i].ntile);
-    const long n_xv = k - o_xv;
-    long o_xc[9][2], n_xc[9][2];
+    const long n_fr = k - o_fr;
+    /* Q and X updates as MFMA per padded width smp = 16 ceil(sm / 16), split by
+     * sm == smp (16-byte loads) and, for X, by m <= 16 */
+    long o_qc[9][2], n_qc[9][2], o_xc[9][2][2], n_xc[9][2][2];
     for (int c = 1; c <= 8; c++)
-        for (int h = 0; h < 2; h++) {
-            o_xc[c][h] = k;
-            for (int i = 0; i < b->nslots; i++)
-                if (b->slot[i] && b->slot[i]->joined && b->h_sys[i].sm == 16 * c &&
-                    (b->h_sys[i].m > 16) == h)
+        for (int ex = 0; ex < 2; ex++) {
+            o_qc[c][ex] = k;
+            for (int i = 0; i < b->nslots; i++) {
+                const breq *r = b->slot[i];
+                const int sm = b->h_sys[i].sm;
+                if (r && r->joined && !r->fresh && (sm + 15) / 16 == c && (sm == 16 * c) == ex)
                     k = put_range(it, k, i, b->h_sys[i].nchunk);
-            n_xc[c][h] = k - o_xc[c][h];
+            }
+            n_qc[c][ex] = k - o_qc[c][ex];
+            for (int h = 0; h < 2; h++) {
+                o_xc[c][ex][h] = k;
+                for (int i = 0; i < b->nslots; i++) {
+                    const breq *r = b->slot[i];
+                    const int sm = b->h_sys[i].sm;
+                    if (r && r->joined && (sm + 15) / 16 == c && (sm == 16 * c) == ex &&
+                        (b->h_sys[i].m > 16) == h)
+                        k = put_range(it, k, i, b->h_sys[i].nchunk);
+                }
+                n_xc[c][ex][h] = k - o_xc[c][ex][h];
+            }
         }
-    long o_qc[9], n_qc[9];
-    for (int c = 1; c <= 8; c++) {
-        o_qc[c] = k;
-        for (int i = 0; i < b->nslots; i++)
-            if (b->slot[i] && b->slot[i]->joined && !b->slot[i]->fresh &&
-                b->h_sys[i].sm == 16 * c)
-                k = put_range(it, k, i, b->h_sys[i].nchunk);
-        n_qc[c] = k - o_qc[c];
-    }
-    /* Gram chunks per tile class: [which][0] VALU, [which][1 + 3 * log2(tm) + tn - tm] MFMA */
-    long o_gc[2][13], n_gc[2][13];
+    /* Gram chunks per tile class: [which][1 + 3 (tm - 1) + tn - tm]; [which][0] stays empty
+     * (every admissible system has a class) */
+    long o_gc[2][25], n_gc[2][25];
     for (int w = 0; w < 2; w++)
-        for (int c = 0; c < 13; c++) {
+        for (int c = 0; c < 25; c++) {
             o_gc[w][c] = k;
             for (int i = 0; i < b->nslots; i++) {
                 const breq *r = b->slot[i];
@@ -689,7 +687,7 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
                     continue;
                 int tm = 0, tn = 0, cls = 0;
                 if (gram_class(&b->h_sys[i], w, &tm, &tn))
-                    cls = 1 + 3 * (tm == 1 ? 0 : tm == 2 ? 1 : tm == 4 ? 2 : 3) + (tn - tm);
+                    cls = 1 + 3 * (tm - 1) + (tn - tm);
                 if (cls == c)
                     k = put_range(it, k, i, b->h_sys[i].nchunk);
             }
@@ -746,18 +744,19 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
         KCHK(gram_phase(b, di, o_gc[0], n_gc[0], di + o_b0, n_b0, 0, f32));
         KCHK(cacg_k_lu_batch(b->d_sys, di + o_lu[0][0], (int)n_lu[0][0], 0, 64, st));
         KCHK(cacg_k_lu_batch(b->d_sys, di + o_lu[0][1], (int)n_lu[0][1], 0, max_sm, st));
-        KCHK(cacg_kb_qupdate(b->d_sys, di + o_up, (int)n_up, f32, st));
         for (int c = 1; c <= 8; c++)
-            KCHK(cacg_kb_qupdate_class(b->d_sys, di + o_qc[c], (int)n_qc[c], 16 * c, f32, st));
+            for (int ex = 0; ex < 2; ex++)
+                KCHK(cacg_kb_qupdate_class(b->d_sys, di + o_qc[c][ex], (int)n_qc[c][ex], 16 * c, ex,
+                                           f32, st));
         KCHK(cacg_kb_aq(b->d_sys, di + o_it, (int)n_it, st));
         KCHK(gram_phase(b, di, o_gc[1], n_gc[1], di + o_b1, n_b1, 1, f32));
         KCHK(cacg_k_lu_batch(b->d_sys, di + o_lu[1][0], (int)n_lu[1][0], 1, 64, st));
         KCHK(cacg_k_lu_batch(b->d_sys, di + o_lu[1][1], (int)n_lu[1][1], 1, max_sm, st));
-        KCHK(cacg_kb_xupdate(b->d_sys, di + o_xv, (int)n_xv, st));
         for (int c = 1; c <= 8; c++)
-            for (int h = 0; h < 2; h++)
-                KCHK(cacg_kb_xupdate_class(b->d_sys, di + o_xc[c][h], (int)n_xc[c][h], 16 * c,
-                                           h ? 32 : 16, st));
+            for (int ex = 0; ex < 2; ex++)
+                for (int h = 0; h < 2; h++)
+                    KCHK(cacg_kb_xupdate_class(b->d_sys, di + o_xc[c][ex][h], (int)n_xc[c][ex][h],
+                                               16 * c, ex, h ? 32 : 16, st));
     }
     /* the true residual of every admitted system */
     KCHK(cacg_kb_resid(b->d_sys, di + o_all, (int)n_all, di + o_sys, (int)n_sys, st));

@@ -2,15 +2,15 @@
 // every system in a work list (the batching / serving form).  gfx950, wave64.
 //
 // Systems of the batch differ in n, nnz, m (<= 32) and sm = s*m (<= 128), so
-// the row-parallel kernels map lanes to the elements (row, column) of a
-// 64-row tile: lane e of the tile handles row e / k, column e % k, which
-// keeps every lane busy for any k.  The Gram kernel gives each 1024-lane
-// workgroup a 2048-row chunk of one system, staged through LDS 16 rows at a
-// time; lane (ta, tb) of a 32 x 32 grid owns the entries (ta + 32u, tb + 32v)
-// of the sm x (sm + m) result, and a second kernel sums the chunk partials in
-// chunk order.  Every sum of a system runs in an order fixed by that system
-// alone (its rows, tiles and chunks), so a system's results are the same bits
-// whatever else shares the launch.
+// the SpMM kernels map lanes to the elements (row, column) of a 64-row tile:
+// lane e of the tile handles row e / k, column e % k, which keeps every lane
+// busy for any k.  The dense products (Gram blocks, Q = S - Q beta,
+// X += Q alpha) run on FP64 MFMA over 2048-row chunks, one launch per tile
+// class (the system's sm and m rounded up to 16-column tiles, zero-padded);
+// a second kernel sums the Gram chunk partials in chunk order.  Every sum of
+// a system runs in an order fixed by that system alone (its rows, tiles,
+// chunks and class), so a system's results are the same bits whatever else
+// shares the launch.
 #include "kcommon.hpp"
 
 namespace {
@@ -24,6 +24,37 @@ __device__ __forceinline__ int tile_rows(int n, int tile)
 {
     const int r = n - tile * BT;
     return r < BT ? r : BT;
+}
+
+// sum over row i of val[p] * x[col[p] * ldx + c] in CSR order, products and sums
+// rounded separately (the reference's SpMM arithmetic); the gathers of 8
+// nonzeros are issued before they are accumulated, for memory-level parallelism
+template <typename TV, typename TX>
+__device__ __forceinline__ TV row_dot(const int *__restrict__ rowptr, const int *__restrict__ col,
+                                      const TV *__restrict__ val, const TX *__restrict__ x, long ldx,
+                                      int c, long i)
+{
+    const int pb = rowptr[i], pe = rowptr[i + 1];
+    TV acc = (TV)0;
+    for (int p = pb; p < pe; p += 8) {
+        int cc[8];
+        TV vv[8];
+        TX xv[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const bool in = p + u < pe;
+            cc[u] = in ? col[p + u] : 0;
+            vv[u] = in ? val[p + u] : (TV)0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            xv[u] = p + u < pe ? x[(long)cc[u] * ldx + c] : (TX)0;
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            if (p + u < pe)
+                acc = add_rn(acc, mul_rn(vv[u], (TV)xv[u]));
+    }
+    return acc;
 }
 
 // basis step j for one tile (see cacg_kb_basis); TB = basis type
@@ -47,18 +78,14 @@ __global__ __launch_bounds__(256) void bbasis_kernel(const cacg_bsys *__restrict
             S[i * sm + c] = (TB)d.R[i * m + c];
             continue;
         }
-        const int pb = d.rowptr[i], pe = d.rowptr[i + 1];
-        TB acc = (TB)0;
         if (j == 1) {
-            for (int p = pb; p < pe; p++)
-                acc = add_rn(acc, mul_rn(val[p], (TB)d.R[(long)d.col[p] * m + c]));
+            const TB acc = row_dot<TB, double>(d.rowptr, d.col, val, d.R, m, c, i);
             const TB e1 = (TB)d.R[i * m + c];
             S[i * sm + c] = e1;
             S[i * sm + m + c] = add_rn(mul_rn(a1, acc), mul_rn(b1, e1));
         } else {
             const TB *xs = S + (long)(j - 1) * m;
-            for (int p = pb; p < pe; p++)
-                acc = add_rn(acc, mul_rn(val[p], xs[(long)d.col[p] * sm + c]));
+            const TB acc = row_dot<TB, TB>(d.rowptr, d.col, val, xs, sm, c, i);
             const TB e1 = xs[i * sm + c], e2 = S[i * sm + (long)(j - 2) * m + c];
             S[i * sm + (long)j * m + c] = sub_rn(add_rn(mul_rn(a2, acc), mul_rn(b2, e1)), e2);
         }
@@ -89,83 +116,7 @@ __global__ __launch_bounds__(256) void baq_kernel(const cacg_bsys *__restrict__ 
     for (int e = threadIdx.x; e < rows * sm; e += 256) {
         const int r = e / sm, c = e - r * sm;
         const long i = r0 + r;
-        double acc = 0.0;
-        for (int p = d.rowptr[i]; p < d.rowptr[i + 1]; p++)
-            acc = add_rn(acc, mul_rn(d.val[p], d.Q[(long)d.col[p] * sm + c]));
-        d.AQ[i * sm + c] = acc;
-    }
-}
-
-// Gram partial of one 2048-row chunk: which 0: AQ^T S, which 1: Q^T [AQ | R]
-template <typename TB>
-__global__ __launch_bounds__(1024) void bgram_kernel(const cacg_bsys *__restrict__ sys,
-                                                     const int2 *__restrict__ items, int which)
-{
-    __shared__ double Ls[GS][MS];
-    __shared__ double Rs[GS][MS + MM];
-    const int2 it = items[blockIdx.x];
-    const cacg_bsys &d = sys[it.x];
-    const int n = d.n, m = d.m, sm = d.sm;
-    const int kl = sm, kr1 = sm, kr = sm + (which ? m : 0);
-    const double *L = which ? d.Q : d.AQ;
-    const TB *S = reinterpret_cast<const TB *>(d.S);
-    const int c0 = it.y * GR, c1 = (c0 + GR) < n ? (c0 + GR) : n;
-    const int t = threadIdx.x, ta = t >> 5, tb = t & 31;
-    double acc[4][5];
-#pragma unroll
-    for (int u = 0; u < 4; u++)
-#pragma unroll
-        for (int v = 0; v < 5; v++)
-            acc[u][v] = 0.0;
-    for (int rb = c0; rb < c1; rb += GS) {
-        const int rows = (c1 - rb) < GS ? (c1 - rb) : GS;
-        for (int e = t; e < rows * kl; e += 1024) {
-            const int r = e / kl, a = e - r * kl;
-            Ls[r][a] = L[(long)(rb + r) * sm + a];
-        }
-        for (int e = t; e < rows * kr; e += 1024) {
-            const int r = e / kr, b = e - r * kr;
-            const long i = rb + r;
-            double v;
-            if (b < kr1)
-                v = which ? d.AQ[i * sm + b] : (double)S[i * sm + b];
-            else
-                v = d.R[i * m + (b - kr1)];
-            Rs[r][b] = v;
-        }
-        __syncthreads();
-        for (int r = 0; r < rows; r++) {
-            double lv[4], rv[5];
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const int a = ta + 32 * u;
-                lv[u] = a < kl ? Ls[r][a] : 0.0;
-            }
-#pragma unroll
-            for (int v = 0; v < 5; v++) {
-                const int b = tb + 32 * v;
-                rv[v] = b < kr ? Rs[r][b] : 0.0;
-            }
-#pragma unroll
-            for (int u = 0; u < 4; u++)
-#pragma unroll
-                for (int v = 0; v < 5; v++)
-                    acc[u][v] = fma(lv[u], rv[v], acc[u][v]);
-        }
-        __syncthreads();
-    }
-    double *gp = d.gpart + (long)it.y * kl * kr;
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-        const int a = ta + 32 * u;
-        if (a >= kl)
-            continue;
-#pragma unroll
-        for (int v = 0; v < 5; v++) {
-            const int b = tb + 32 * v;
-            if (b < kr)
-                gp[(long)a * kr + b] = acc[u][v];
-        }
+        d.AQ[i * sm + c] = row_dot<double, double>(d.rowptr, d.col, d.val, d.Q, sm, c, i);
     }
 }
 
@@ -285,38 +236,15 @@ __global__ __launch_bounds__(256) void bgram_reduce_kernel(const cacg_bsys *__re
         d.QtR[(long)a * m + (b - sm)] = acc;
 }
 
-// Q = S - Q beta: the tile's Q rows are staged in LDS first (in-place update)
-template <typename TB>
-__global__ __launch_bounds__(256) void bqupdate_kernel(const cacg_bsys *__restrict__ sys,
-                                                       const int2 *__restrict__ items)
-{
-    extern __shared__ double qt[];  // BT x sm
-    const int2 it = items[blockIdx.x];
-    const cacg_bsys &d = sys[it.x];
-    const int sm = d.sm;
-    const long base = (long)it.y * BT * sm;
-    const int cnt = tile_rows(d.n, it.y) * sm;
-    const TB *S = reinterpret_cast<const TB *>(d.S);
-    for (int e = threadIdx.x; e < cnt; e += 256)
-        qt[e] = d.Q[base + e];
-    __syncthreads();
-    for (int e = threadIdx.x; e < cnt; e += 256) {
-        const int r = e / sm, c = e - r * sm;
-        const double *qr = qt + (long)r * sm;
-        double acc = 0.0;
-        for (int q = 0; q < sm; q++)
-            acc = fma(qr[q], d.beta[(long)q * sm + c], acc);
-        d.Q[base + e] = (double)S[base + e] - acc;
-    }
-}
-
-// Q = S - Q beta with FP64 MFMA for systems with sm = SM (a multiple of 16):
-// one 2048-row chunk per 512-lane workgroup, beta in LDS, each wave takes
-// 16-row tiles of the chunk; the contraction index is permuted across the
-// lane groups (group g supplies k = g SM/4 + t at step t), so each lane's A
-// operand is SM/4 contiguous values of one row, read before the tile is
-// written (in place).  Per output the k order is fixed by SM alone.
-template <typename TB, int SM>
+// Q = S - Q beta with FP64 MFMA for systems with 16 (SM / 16 - 1) < sm <= SM:
+// one 2048-row chunk per 512-lane workgroup, beta in LDS (zero-padded to
+// SM x SM), each wave takes 16-row tiles of the chunk; the contraction index
+// is permuted across the lane groups (group g supplies k = g SM/4 + t at step
+// t), so each lane's A operand is SM/4 contiguous values of one row (16-byte
+// loads when sm == SM, EXACT; guarded loads with zeros past sm otherwise),
+// read before the tile is written (in place).  Per output the k order is
+// fixed by the system alone.
+template <typename TB, int SM, bool EXACT>
 __global__ __launch_bounds__(512) void bqupdate_mfma_kernel(const cacg_bsys *__restrict__ sys,
                                                             const int2 *__restrict__ items)
 {
@@ -324,9 +252,11 @@ __global__ __launch_bounds__(512) void bqupdate_mfma_kernel(const cacg_bsys *__r
     extern __shared__ double bl[];  // SM x SM
     const int2 it = items[blockIdx.x];
     const cacg_bsys &d = sys[it.x];
-    const int n = d.n;
-    for (int e = threadIdx.x; e < SM * SM; e += 512)
-        bl[e] = d.beta[e];
+    const int n = d.n, sm = EXACT ? SM : d.sm;
+    for (int e = threadIdx.x; e < SM * SM; e += 512) {
+        const int q = e / SM, c = e - q * SM;
+        bl[e] = (q < sm && c < sm) ? d.beta[(long)q * sm + c] : 0.0;
+    }
     __syncthreads();
     const TB *S = reinterpret_cast<const TB *>(d.S);
     double *Q = d.Q;
@@ -336,7 +266,7 @@ __global__ __launch_bounds__(512) void bqupdate_mfma_kernel(const cacg_bsys *__r
     for (int i0 = c0 + wave * 16; i0 < c1; i0 += 8 * 16) {
         const long arow = i0 + lc;
         double av[KQ];
-        if (arow < c1) {
+        if (EXACT && arow < c1) {
             const double2 *src = reinterpret_cast<const double2 *>(Q + arow * SM + lg * KQ);
 #pragma unroll
             for (int u = 0; u < KQ / 2; u++) {
@@ -346,63 +276,47 @@ __global__ __launch_bounds__(512) void bqupdate_mfma_kernel(const cacg_bsys *__r
             }
         } else {
 #pragma unroll
-            for (int u = 0; u < KQ; u++)
-                av[u] = 0.0;
+            for (int u = 0; u < KQ; u++) {
+                const int k = lg * KQ + u;
+                av[u] = (arow < c1 && k < sm) ? Q[arow * sm + k] : 0.0;
+            }
         }
 #pragma unroll 1
         for (int j = 0; j < NTL; j++) {
+            const int c = j * 16 + lc;
             double sv[4];
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const long i = i0 + lg + 4 * r;
-                sv[r] = i < c1 ? (double)S[i * SM + j * 16 + lc] : 0.0;
+                sv[r] = (i < c1 && c < sm) ? (double)S[i * sm + c] : 0.0;
             }
             dbl4 acc = {0.0, 0.0, 0.0, 0.0};
-            const double *bcol = bl + (lg * KQ) * SM + j * 16 + lc;
+            const double *bcol = bl + (lg * KQ) * SM + c;
 #pragma unroll
             for (int t = 0; t < KQ; t++)
                 acc = mfma16x16x4(av[t], bcol[t * SM], acc);
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const long i = i0 + lg + 4 * r;
-                if (i < c1)
-                    Q[i * SM + j * 16 + lc] = sv[r] - acc[r];
+                if (i < c1 && c < sm)
+                    Q[i * sm + c] = sv[r] - acc[r];
             }
         }
     }
 }
 
-// X += Q alpha
-__global__ __launch_bounds__(256) void bxupdate_kernel(const cacg_bsys *__restrict__ sys,
-                                                       const int2 *__restrict__ items)
-{
-    const int2 it = items[blockIdx.x];
-    const cacg_bsys &d = sys[it.x];
-    const int m = d.m, sm = d.sm;
-    const long r0 = (long)it.y * BT;
-    const int rows = tile_rows(d.n, it.y);
-    for (int e = threadIdx.x; e < rows * m; e += 256) {
-        const int r = e / m, c = e - r * m;
-        const long i = r0 + r;
-        const double *qr = d.Q + i * sm;
-        double acc = 0.0;
-        for (int q = 0; q < sm; q++)
-            acc = fma(qr[q], d.alpha[(long)q * m + c], acc);
-        d.X[i * m + c] += acc;
-    }
-}
-
-// X += Q alpha with FP64 MFMA for systems with sm = SM (a multiple of 16):
-// one 2048-row chunk per 256-lane workgroup, alpha (SM x m, m <= 16 NTM) in
-// registers in the K-permuted order of the A operand, 16-row tiles per wave.
-template <int SM, int NTM>
+// X += Q alpha with FP64 MFMA for systems with 16 (SM / 16 - 1) < sm <= SM:
+// one 2048-row chunk per 256-lane workgroup, alpha (sm x m, m <= 16 NTM) in
+// registers in the K-permuted order of the A operand (zeros past sm), 16-row
+// tiles per wave; 16-byte Q loads when sm == SM (EXACT).
+template <int SM, int NTM, bool EXACT>
 __global__ __launch_bounds__(256) void bxupdate_mfma_kernel(const cacg_bsys *__restrict__ sys,
                                                             const int2 *__restrict__ items)
 {
     constexpr int KQ = SM / 4;
     const int2 it = items[blockIdx.x];
     const cacg_bsys &d = sys[it.x];
-    const int n = d.n, m = d.m;
+    const int n = d.n, m = d.m, sm = EXACT ? SM : d.sm;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int lg = lane >> 4, lc = lane & 15;
     double alf[NTM][KQ];
@@ -410,14 +324,16 @@ __global__ __launch_bounds__(256) void bxupdate_mfma_kernel(const cacg_bsys *__r
     for (int j = 0; j < NTM; j++) {
         const int c = j * 16 + lc;
 #pragma unroll
-        for (int t = 0; t < KQ; t++)
-            alf[j][t] = c < m ? d.alpha[(long)(lg * KQ + t) * m + c] : 0.0;
+        for (int t = 0; t < KQ; t++) {
+            const int k = lg * KQ + t;
+            alf[j][t] = (c < m && k < sm) ? d.alpha[(long)k * m + c] : 0.0;
+        }
     }
     const int c0 = it.y * GR, c1 = (c0 + GR) < n ? (c0 + GR) : n;
     for (int i0 = c0 + wave * 16; i0 < c1; i0 += 4 * 16) {
         const long arow = i0 + lc;
         double av[KQ];
-        if (arow < c1) {
+        if (EXACT && arow < c1) {
             const double2 *src = reinterpret_cast<const double2 *>(d.Q + arow * SM + lg * KQ);
 #pragma unroll
             for (int u = 0; u < KQ / 2; u++) {
@@ -427,8 +343,10 @@ __global__ __launch_bounds__(256) void bxupdate_mfma_kernel(const cacg_bsys *__r
             }
         } else {
 #pragma unroll
-            for (int u = 0; u < KQ; u++)
-                av[u] = 0.0;
+            for (int u = 0; u < KQ; u++) {
+                const int k = lg * KQ + u;
+                av[u] = (arow < c1 && k < sm) ? d.Q[arow * sm + k] : 0.0;
+            }
         }
 #pragma unroll
         for (int j = 0; j < NTM; j++) {
@@ -463,9 +381,7 @@ __global__ __launch_bounds__(256) void bresid_kernel(const cacg_bsys *__restrict
     for (int e = threadIdx.x; e < rows * m; e += 256) {
         const int r = e / m, c = e - r * m;
         const long i = r0 + r;
-        double acc = 0.0;
-        for (int p = d.rowptr[i]; p < d.rowptr[i + 1]; p++)
-            acc = add_rn(acc, mul_rn(d.val[p], d.X[(long)d.col[p] * m + c]));
+        const double acc = row_dot<double, double>(d.rowptr, d.col, d.val, d.X, m, c, i);
         const double rv = sub_rn(d.B[i * m + c], acc);
         d.R[i * m + c] = rv;
         sq[e] = rv * rv;
@@ -532,28 +448,6 @@ extern "C" int cacg_kb_aq(const cacg_bsys *d_sys, const int2 *d_tiles, int count
     return 0;
 }
 
-extern "C" int cacg_kb_gram(const cacg_bsys *d_sys, const int2 *d_chunks, int nchunks,
-                            const int2 *d_blocks, int nblocks, int which, int f32, hipStream_t st)
-{
-    if (which != 0 && which != 1)
-        return (int)hipErrorInvalidValue;
-    if (nchunks > 0) {
-        if (f32)
-            hipLaunchKernelGGL(bgram_kernel<float>, dim3(nchunks), dim3(1024), 0, st, d_sys,
-                               d_chunks, which);
-        else
-            hipLaunchKernelGGL(bgram_kernel<double>, dim3(nchunks), dim3(1024), 0, st, d_sys,
-                               d_chunks, which);
-        CACG_LAUNCH_CHECK();
-    }
-    if (nblocks > 0) {
-        hipLaunchKernelGGL(bgram_reduce_kernel, dim3(nblocks), dim3(256), 0, st, d_sys, d_blocks,
-                           which);
-        CACG_LAUNCH_CHECK();
-    }
-    return 0;
-}
-
 namespace {
 template <typename TB, int TM>
 int launch_gram_class(const cacg_bsys *d_sys, const int2 *d_chunks, int nchunks, int which, int tn,
@@ -594,7 +488,11 @@ extern "C" int cacg_kb_gram_class(const cacg_bsys *d_sys, const int2 *d_chunks, 
     switch (tm) {
         CACG_GCLASS(1)
         CACG_GCLASS(2)
+        CACG_GCLASS(3)
         CACG_GCLASS(4)
+        CACG_GCLASS(5)
+        CACG_GCLASS(6)
+        CACG_GCLASS(7)
         CACG_GCLASS(8)
     default:
         return (int)hipErrorInvalidValue;
@@ -613,42 +511,33 @@ extern "C" int cacg_kb_gram_reduce(const cacg_bsys *d_sys, const int2 *d_blocks,
     return 0;
 }
 
-extern "C" int cacg_kb_qupdate(const cacg_bsys *d_sys, const int2 *d_tiles, int count, int f32,
-                               hipStream_t st)
-{
-    if (count <= 0)
-        return 0;
-    const size_t shm = (size_t)BT * MS * sizeof(double);
-    if (f32) {
-        cacg_allow_lds(bqupdate_kernel<float>, shm);
-        hipLaunchKernelGGL(bqupdate_kernel<float>, dim3(count), dim3(256), shm, st, d_sys, d_tiles);
-    } else {
-        cacg_allow_lds(bqupdate_kernel<double>, shm);
-        hipLaunchKernelGGL(bqupdate_kernel<double>, dim3(count), dim3(256), shm, st, d_sys, d_tiles);
-    }
-    CACG_LAUNCH_CHECK();
-    return 0;
-}
-
 extern "C" int cacg_kb_qupdate_class(const cacg_bsys *d_sys, const int2 *d_chunks, int count,
-                                     int sm, int f32, hipStream_t st)
+                                     int smp, int exact, int f32, hipStream_t st)
 {
     if (count <= 0)
         return 0;
-    const size_t shm = (size_t)sm * sm * sizeof(double);
+    const size_t shm = (size_t)smp * smp * sizeof(double);
+#define CACG_QLAUNCH(TBV, SMV, EX)                                                                \
+    do {                                                                                          \
+        cacg_allow_lds(bqupdate_mfma_kernel<TBV, SMV, EX>, shm);                                  \
+        hipLaunchKernelGGL((bqupdate_mfma_kernel<TBV, SMV, EX>), dim3(count), dim3(512), shm, st, \
+                           d_sys, d_chunks);                                                      \
+    } while (0)
 #define CACG_QCLASS(SMV)                                                                          \
     case SMV:                                                                                     \
         if (f32) {                                                                                \
-            cacg_allow_lds(bqupdate_mfma_kernel<float, SMV>, shm);                                \
-            hipLaunchKernelGGL((bqupdate_mfma_kernel<float, SMV>), dim3(count), dim3(512), shm,   \
-                               st, d_sys, d_chunks);                                              \
+            if (exact)                                                                            \
+                CACG_QLAUNCH(float, SMV, true);                                                   \
+            else                                                                                  \
+                CACG_QLAUNCH(float, SMV, false);                                                  \
         } else {                                                                                  \
-            cacg_allow_lds(bqupdate_mfma_kernel<double, SMV>, shm);                               \
-            hipLaunchKernelGGL((bqupdate_mfma_kernel<double, SMV>), dim3(count), dim3(512), shm,  \
-                               st, d_sys, d_chunks);                                              \
+            if (exact)                                                                            \
+                CACG_QLAUNCH(double, SMV, true);                                                  \
+            else                                                                                  \
+                CACG_QLAUNCH(double, SMV, false);                                                 \
         }                                                                                         \
         break;
-    switch (sm) {
+    switch (smp) {
         CACG_QCLASS(16)
         CACG_QCLASS(32)
         CACG_QCLASS(48)
@@ -661,37 +550,36 @@ extern "C" int cacg_kb_qupdate_class(const cacg_bsys *d_sys, const int2 *d_chunk
         return (int)hipErrorInvalidValue;
     }
 #undef CACG_QCLASS
-    CACG_LAUNCH_CHECK();
-    return 0;
-}
-
-extern "C" int cacg_kb_xupdate(const cacg_bsys *d_sys, const int2 *d_tiles, int count,
-                               hipStream_t st)
-{
-    if (count <= 0)
-        return 0;
-    hipLaunchKernelGGL(bxupdate_kernel, dim3(count), dim3(256), 0, st, d_sys, d_tiles);
+#undef CACG_QLAUNCH
     CACG_LAUNCH_CHECK();
     return 0;
 }
 
 extern "C" int cacg_kb_xupdate_class(const cacg_bsys *d_sys, const int2 *d_chunks, int count,
-                                     int sm, int m, hipStream_t st)
+                                     int smp, int exact, int m, hipStream_t st)
 {
     if (count <= 0)
         return 0;
     if (m < 1 || m > 32)
         return (int)hipErrorInvalidValue;
+#define CACG_XLAUNCH(SMV, NT, EX)                                                                 \
+    hipLaunchKernelGGL((bxupdate_mfma_kernel<SMV, NT, EX>), dim3(count), dim3(256), 0, st, d_sys, \
+                       d_chunks)
 #define CACG_XCLASS(SMV)                                                                          \
     case SMV:                                                                                     \
-        if (m <= 16)                                                                              \
-            hipLaunchKernelGGL((bxupdate_mfma_kernel<SMV, 1>), dim3(count), dim3(256), 0, st,     \
-                               d_sys, d_chunks);                                                  \
-        else                                                                                      \
-            hipLaunchKernelGGL((bxupdate_mfma_kernel<SMV, 2>), dim3(count), dim3(256), 0, st,     \
-                               d_sys, d_chunks);                                                  \
+        if (m <= 16) {                                                                            \
+            if (exact)                                                                            \
+                CACG_XLAUNCH(SMV, 1, true);                                                       \
+            else                                                                                  \
+                CACG_XLAUNCH(SMV, 1, false);                                                      \
+        } else {                                                                                  \
+            if (exact)                                                                            \
+                CACG_XLAUNCH(SMV, 2, true);                                                       \
+            else                                                                                  \
+                CACG_XLAUNCH(SMV, 2, false);                                                      \
+        }                                                                                         \
         break;
-    switch (sm) {
+    switch (smp) {
         CACG_XCLASS(16)
         CACG_XCLASS(32)
         CACG_XCLASS(48)
@@ -704,6 +592,7 @@ extern "C" int cacg_kb_xupdate_class(const cacg_bsys *d_sys, const int2 *d_chunk
         return (int)hipErrorInvalidValue;
     }
 #undef CACG_XCLASS
+#undef CACG_XLAUNCH
     CACG_LAUNCH_CHECK();
     return 0;
 }

@@ -163,14 +163,12 @@ int cacg_kb_q_from_s(const cacg_bsys *d_sys, const int2 *d_tiles, int count, int
                      hipStream_t st);
 /* AQ = A Q (FP64) */
 int cacg_kb_aq(const cacg_bsys *d_sys, const int2 *d_tiles, int count, hipStream_t st);
-/* Gram partials per 2048-row chunk, then the chunk-ordered sums:
- * which 0: QtAS = AQ^T S; which 1: [QtAQ | QtR] = Q^T [AQ | R] */
-int cacg_kb_gram(const cacg_bsys *d_sys, const int2 *d_chunks, int nchunks, const int2 *d_blocks,
-                 int nblocks, int which, int f32, hipStream_t st);
-/* MFMA form of the Gram partials for the systems of one tile class:
- * tm = ceil(sm / 16) in 1, 2, 4, 8 and tn = ceil(kr / 16) in tm .. tm + 2, kr = sm
- * (which 0) or sm + m (which 1); the chunk-ordered sums then by cacg_kb_gram_reduce.
- * cacg_kb_gram (nblocks = 0) covers every other system with VALU FP64. */
+/* Gram partials per 2048-row chunk with FP64 MFMA for the systems of one tile
+ * class, tm = ceil(sm / 16) in 1 .. 8 and tn = ceil(kr / 16) in tm .. tm + 2,
+ * kr = sm (which 0: QtAS = AQ^T S) or sm + m (which 1: [QtAQ | QtR] =
+ * Q^T [AQ | R]); then the chunk-ordered sums by cacg_kb_gram_reduce ({slot,
+ * 256-entry block} items).  Every admissible system (m <= 32, sm <= 128) is
+ * in some class. */
 int cacg_kb_gram_class(const cacg_bsys *d_sys, const int2 *d_chunks, int nchunks, int which,
                        int f32, int tm, int tn, hipStream_t st);
 int cacg_kb_gram_reduce(const cacg_bsys *d_sys, const int2 *d_blocks, int nblocks, int which,
@@ -179,20 +177,17 @@ int cacg_kb_gram_reduce(const cacg_bsys *d_sys, const int2 *d_blocks, int nblock
  * alpha = QtAQ^{-1} QtR (factors QtAQ into lu); max_sm bounds every sm */
 int cacg_k_lu_batch(const cacg_bsys *d_sys, const int2 *d_items, int count, int which, int max_sm,
                     hipStream_t st);
-/* Q = S - Q beta */
-int cacg_kb_qupdate(const cacg_bsys *d_sys, const int2 *d_tiles, int count, int f32,
-                    hipStream_t st);
-/* MFMA form of Q = S - Q beta over 2048-row chunks ({slot, chunk} items) of
- * systems whose sm equals the given multiple of 16 (16 .. 128) */
-int cacg_kb_qupdate_class(const cacg_bsys *d_sys, const int2 *d_chunks, int count, int sm,
-                          int f32, hipStream_t st);
-/* X += Q alpha */
-int cacg_kb_xupdate(const cacg_bsys *d_sys, const int2 *d_tiles, int count, hipStream_t st);
-/* MFMA form of X += Q alpha over 2048-row chunks of systems with the given sm
- * (a multiple of 16, <= 128) and m (<= 32); every listed system must have
- * this m class: m <= 16 or 16 < m <= 32 use different instantiations */
-int cacg_kb_xupdate_class(const cacg_bsys *d_sys, const int2 *d_chunks, int count, int sm, int m,
-                          hipStream_t st);
+/* Q = S - Q beta with FP64 MFMA over 2048-row chunks ({slot, chunk} items) of
+ * systems with smp = 16 ceil(sm / 16) (16 .. 128); exact != 0 when every listed
+ * system has sm == smp (16-byte loads), else zero-padded guarded loads */
+int cacg_kb_qupdate_class(const cacg_bsys *d_sys, const int2 *d_chunks, int count, int smp,
+                          int exact, int f32, hipStream_t st);
+/* X += Q alpha with FP64 MFMA over 2048-row chunks of systems with
+ * smp = 16 ceil(sm / 16) and exact as for cacg_kb_qupdate_class; m selects the
+ * instantiation (m <= 16, or 16 < m <= 32) and every listed system must be in
+ * that m class */
+int cacg_kb_xupdate_class(const cacg_bsys *d_sys, const int2 *d_chunks, int count, int smp,
+                          int exact, int m, hipStream_t st);
 /* R = B - A X (FP64) with per-tile column norm partials, then norm2 of every
  * listed system (d_sysitems: {slot, 0}) */
 int cacg_kb_resid(const cacg_bsys *d_sys, const int2 *d_tiles, int count,
