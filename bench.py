@@ -18,6 +18,19 @@ algorithmic bytes / flops of cacg_amd.metrics.  The CPU baseline times the
 C restatement of the reference solver (oracle/) on this host: setup plus two
 outer iterations of the same workload at one row-block rank per thread,
 projected to the GPU run's iteration count.
+
+Beside the headline line (unless --no-forms), the same JSON object carries
+the two further forms, each measured in this run:
+  precision_form : the same workload with the FP32 basis and FP64 refinement,
+                   its solves/s against the FP64 path and its solution's
+                   error against the FP64 solution;
+  batching_form  : 256 shifted 2D Laplacians (L log-uniform in [32, 256],
+                   shift U(0, 0.1), m from {1, 2, 4, 8, 16, 32}, seed 3,
+                   s = 4, 1e-10) in one batch against the same systems solved
+                   one at a time through the single-system FP64 path;
+  serving_form   : a Poisson stream of 2D (L 64..512) and 3D (edge 32..128)
+                   Laplacians with 1..32 RHS (seed 4), continuous batching
+                   against one request at a time with the FP64 path.
 """
 
 import argparse
@@ -47,6 +60,10 @@ def parse():
     ap.add_argument("--order", default="fast", choices=["fast", "reference"])
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cpu count)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the precision / batching / serving measurements")
+    ap.add_argument("--serve-requests", type=int, default=40)
+    ap.add_argument("--serve-rate", type=float, default=400.0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run 3 device-to-device copies of 2 GiB (HBM counter calibration)")
     return ap.parse_args()
@@ -214,6 +231,10 @@ def main():
         "hbm_footprint_gb": sv.footprint() / 1e9,
     }
     sv.close()
+    if not args.no_forms:
+        out["precision_form"] = precision_form(A, B, X, dB, dX, m, s, args, out["ms_per_step"])
+        out["batching_form"] = batching_form()
+        out["serving_form"] = serving_form(args.serve_requests, args.serve_rate)
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         print(f"bench: CPU baseline, {threads} threads", file=sys.stderr, flush=True)
@@ -221,6 +242,91 @@ def main():
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))
+
+
+def precision_form(A, B, X64, dB, dX, m, s, args, ms64):
+    """The FP32-basis / FP64-refined solve of the same workload."""
+    import cacg_amd
+    L = cacg_amd.lib()
+    sv = cacg_amd.Solver(A, m, s, tol=args.tol, maxiter=3 * args.maxiter, order="fast",
+                         precision="fp32")
+    lat, reps = [], []
+    for k in range(1 + 3):
+        L.cacg_k_fill_f64(dX.ptr, A.n * m, 0.0, None)
+        L.cacg_dev_sync()
+        t = time.perf_counter()
+        rep = sv.solve(dB, dX)
+        L.cacg_dev_sync()
+        if k > 0:
+            lat.append(time.perf_counter() - t)
+            reps.append(rep)
+    X = dX.download(B.shape)
+    sv.close()
+    err = np.linalg.norm(X - X64, axis=0) / np.linalg.norm(X64, axis=0)
+    ms = float(np.mean(lat) * 1e3)
+    print(f"bench: precision form {ms:.1f} ms per solve, {reps[-1]['iters']} iterations",
+          file=sys.stderr, flush=True)
+    return {"workload": "same system, FP32 matrix copy / basis / Chebyshev SpMM, FP64 Q, A Q, Gram, "
+                        "LU, X, R; true residual recomputed in FP64 (residual replacement)",
+            "solves_per_s": 1e3 / ms, "ms_per_solve": ms, "vs_fp64_path": ms64 / ms,
+            "iters": reps[-1]["iters"], "refinements": reps[-1]["refinements"],
+            "status": reps[-1]["status"], "true_relres_max": reps[-1]["true_relres"],
+            "err_vs_fp64_solution": float(err.max())}
+
+
+def batching_form(count=256):
+    """256 shifted 2D Laplacians in one batch vs one at a time (FP64 path)."""
+    import cacg_amd
+    from cacg_amd import batch as BT
+    systems = BT.batch_systems(count, 3)
+    b = BT.Batch(s=4, precision="fp32", hbm_bytes=48 << 30, maxiter=2000)
+    t0 = time.perf_counter()
+    ids = [b.submit(A, Bm, tol=1e-10) for A, Bm in systems]
+    lat, res, rounds = {}, {}, 0
+    while len(res) < len(ids):
+        rounds = b.step()["rounds"]
+        now = time.perf_counter() - t0
+        for rid in b.retired():
+            res[rid] = b.collect(rid)
+            lat[rid] = now
+    wall = time.perf_counter() - t0
+    b.close()
+    t1 = time.perf_counter()
+    for A, Bm in systems:
+        cacg_amd.bcbcg(A, Bm, np.zeros_like(Bm), 4, tol=1e-10, maxiter=2000)
+    seq = time.perf_counter() - t1
+    L = np.array([lat[i] for i in ids]) * 1e3
+    print(f"bench: batching form {count / wall:.1f} solves/s vs {count / seq:.1f} one at a time",
+          file=sys.stderr, flush=True)
+    return {"workload": f"{count} shifted 2D Laplacians (L log-uniform 32..256, shift U(0,0.1), "
+                        "m in {1,2,4,8,16,32}, seed 3), s=4, 1e-10, FP32 basis, all submitted at t=0",
+            "solves_per_s": count / wall, "p50_ms": float(np.percentile(L, 50)),
+            "p99_ms": float(np.percentile(L, 99)), "rounds": rounds,
+            "converged": int(sum(res[i]["status"] == "ok" for i in ids)),
+            "max_true_relres": float(max(res[i]["true_relres"] for i in ids)),
+            "one_at_a_time_fp64_solves_per_s": count / seq, "vs_one_at_a_time": seq / wall}
+
+
+def serving_form(count, rate):
+    """A Poisson stream: continuous batching vs one request at a time."""
+    from cacg_amd import batch as BT
+    stream = BT.serving_stream(count, rate, seed=4)
+    b = BT.Batch(s=4, precision="fp64", hbm_bytes=96 << 30, maxiter=3000)
+    res, wall = BT.serve_stream(stream, b, tol=1e-10)
+    b.close()
+    seq, wall_seq = BT.serve_sequential(stream, tol=1e-10, s=4, maxiter=3000)
+    lb = np.array([r["latency"] for r in res]) * 1e3
+    ls = np.array([r["latency"] for r in seq]) * 1e3
+    print(f"bench: serving form {count / wall:.2f} solves/s vs {count / wall_seq:.2f} sequential",
+          file=sys.stderr, flush=True)
+    return {"workload": f"{count} requests, Poisson {rate:g}/s (seed 4), 2D L 64..512 or 3D edge "
+                        "32..128, 1..32 RHS, s=4, 1e-10, FP64 basis",
+            "solves_per_s": count / wall, "p50_ms": float(np.percentile(lb, 50)),
+            "p99_ms": float(np.percentile(lb, 99)),
+            "converged": int(sum(r["status"] == "ok" for r in res)),
+            "sequential_fp64": {"solves_per_s": count / wall_seq,
+                                "p50_ms": float(np.percentile(ls, 50)),
+                                "p99_ms": float(np.percentile(ls, 99))}}
 
 
 def measured_traffic(kernel):
