@@ -111,6 +111,9 @@ def lib() -> ctypes.CDLL:
         "cacg_k_cvt_f64_f32": ([c_long, _vp, _vp, _vp], c_int),
         "cacg_k_qupdate_s32": ([c_int, c_int, _vp, _vp, _vp, _vp], c_int),
         "cacg_k_cvt_f32_f64": ([c_long, _vp, _vp, _vp], c_int),
+        "cacg_k_qupdate_zero": ([c_int, c_int, _vp, _vp, _vp], c_int),
+        "cacg_tsqr_workspace": ([c_int], c_long),
+        "cacg_k_tsqr": ([c_int, c_int, _vp, c_long, _vp, _vp, _vp, _vp, _vp], c_int),
         "cacg_k_gram_fast_f64f32": ([c_int, _vp, c_long, c_int, _vp, c_long, c_int, _vp, c_long,
                                      c_int, _vp, _vp, _vp, _vp], c_int),
         "cacg_k_fill_i32": ([_vp, c_long, c_int, _vp], c_int),
@@ -336,6 +339,17 @@ def k_qupdate_s32(S, Q, beta):
     db = DeviceBuffer.from_array(np.ascontiguousarray(beta, dtype=np.float64))
     _check(lib().cacg_k_qupdate_s32(n, sm, dS.ptr, dQ.ptr, db.ptr, None), "qupdate_s32")
     return dQ.download((n, sm))
+
+
+def k_tsqr(A):
+    """(Q, R, flag) of the tall-skinny QR of A (n x k, k <= 128)."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    n, k = A.shape
+    dA, dQ, dR = DeviceBuffer.from_array(A), DeviceBuffer(A.nbytes), DeviceBuffer(8 * k * k)
+    ws = DeviceBuffer(8 * lib().cacg_tsqr_workspace(k))
+    flag = DeviceBuffer.from_array(np.zeros(1, dtype=np.int32))
+    _check(lib().cacg_k_tsqr(n, k, dA.ptr, k, dQ.ptr, dR.ptr, ws.ptr, flag.ptr, None), "tsqr")
+    return dQ.download((n, k)), dR.download((k, k)), int(flag.download(1, np.int32)[0])
 
 
 def k_cvt_f32_f64(S):

@@ -80,7 +80,7 @@ __global__ __launch_bounds__(256) void qupdate_ref_kernel(int n, int sm, const T
             double acc = 0.0;
             for (int q = 0; q < sm; q++)
                 acc = add_rn(acc, mul_rn(B[(long)q * sm + c], qr[q]));
-            Q[i0 * sm + e] = (T)sub_rn((double)S[i0 * sm + e], acc);
+            Q[i0 * sm + e] = (T)sub_rn(S ? (double)S[i0 * sm + e] : 0.0, acc);
         }
         __syncthreads();
     }
@@ -193,7 +193,7 @@ __global__ __launch_bounds__(512) void qupdate_mfma_kernel(int n, const TS *__re
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const long i = i0 + lg + 4 * r;
-                sv[r] = i < n ? (double)S[i * SM + j * 16 + lc] : 0.0;
+                sv[r] = (i < n && S) ? (double)S[i * SM + j * 16 + lc] : 0.0;
             }
             dbl4 acc = {0.0, 0.0, 0.0, 0.0};
             const double *bcol = bl + (lg * KQ) * SM + j * 16 + lc;
@@ -511,6 +511,11 @@ extern "C" int cacg_k_qupdate(int n, int sm, const double *S, double *Q, const d
                               int mode, hipStream_t st)
 {
     return qupdate<double, double>(n, sm, S, Q, beta, mode, st);
+}
+
+extern "C" int cacg_k_qupdate_zero(int n, int sm, double *Q, const double *beta, hipStream_t st)
+{
+    return qupdate<double, double>(n, sm, (const double *)nullptr, Q, beta, CACG_MODE_FAST, st);
 }
 
 extern "C" int cacg_k_qupdate_s32(int n, int sm, const float *S, double *Q, const double *beta,

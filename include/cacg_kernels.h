@@ -99,6 +99,8 @@ int cacg_k_lu_solve(int N, const double *lhs, int k, const double *rhs, double *
  * (src/BLAS3.c:794-825 then 579-612). */
 int cacg_k_qupdate(int n, int sm, const double *S, double *Q, const double *beta, int mode,
                    hipStream_t st);
+/* Q = -Q beta in place (S = 0; fast order), the tall-skinny product of the QR */
+int cacg_k_qupdate_zero(int n, int sm, double *Q, const double *beta, hipStream_t st);
 /* FP32 basis S, FP64 Q (the precision form's update): Q = S - Q beta in FP64. */
 int cacg_k_qupdate_s32(int n, int sm, const float *S, double *Q, const double *beta,
                        hipStream_t st);
@@ -123,6 +125,19 @@ int cacg_k_colnorm2_fast(int n, int m, const double *R, double *d_norm2, double 
 /* d[0:count] = v, as a kernel (safe inside captured graphs, unlike a memset node) */
 int cacg_k_fill_i32(int *d, long count, int v, hipStream_t st);
 int cacg_k_fill_f64(double *d, long count, double v, hipStream_t st);
+
+/* Tall-skinny QR (no reference counterpart: the reference's TSQR.c is empty):
+ * A (n x k, row stride lda, k <= 128) = Q R with Q (n x k, contiguous) having
+ * orthonormal columns and R (k x k, row-major) upper triangular with a
+ * positive diagonal.  CholeskyQR2: two passes of G = Q^T Q (split-K FP64 MFMA,
+ * deterministic), single-workgroup LDS Cholesky and triangular inverse, and
+ * Q = Q R^{-1}; orthogonality O(u) for cond(A) up to ~1e8.  *d_flag is set if
+ * a Cholesky pivot falls below 64 u of its diagonal entry (A numerically rank
+ * deficient or beyond that conditioning).  d_work
+ * holds cacg_tsqr_workspace(k) doubles. */
+long cacg_tsqr_workspace(int k);
+int cacg_k_tsqr(int n, int k, const double *A, long lda, double *Q, double *R, double *d_work,
+                int *d_flag, hipStream_t st);
 
 /* ---- batched kernels (many independent systems per launch) ----------------
  * One descriptor per system slot, in device memory.  Work items are int2
