@@ -80,6 +80,9 @@ def lib() -> ctypes.CDLL:
         "cacg_solver_create": ([ctypes.POINTER(_vp), c_long, _lp, _lp, _dp, c_long, c_long,
                                 ctypes.POINTER(Opts)], c_int),
         "cacg_solver_solve_device": ([_vp, _vp, _vp, ctypes.POINTER(Report)], c_int),
+        "cacg_bcg": ([c_long, _lp, _lp, _dp, c_long, _dp, _dp, ctypes.POINTER(Opts),
+                      ctypes.POINTER(Report)], c_int),
+        "cacg_solver_solve_bcg_device": ([_vp, _vp, _vp, ctypes.POINTER(Report)], c_int),
         "cacg_solver_destroy": ([_vp], c_int),
         "cacg_solver_footprint": ([_vp], c_long),
         "cacg_device_count": ([], c_int),
@@ -118,6 +121,7 @@ def lib() -> ctypes.CDLL:
                                      c_int, _vp, _vp, _vp, _vp], c_int),
         "cacg_k_fill_i32": ([_vp, c_long, c_int, _vp], c_int),
         "cacg_k_fill_f64": ([_vp, c_long, c_double, _vp], c_int),
+        "cacg_k_scale_f64": ([_vp, c_long, c_double, _vp], c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -434,6 +438,33 @@ def bcbcg(A, B, X0, s, *, stop_mode=STOP_REL_PER_COLUMN, eps=0.0, tol=1e-8, maxi
     return out
 
 
+def bcg(A, B, X0, *, stop_mode=STOP_REL_PER_COLUMN, eps=0.0, tol=1e-8, maxiter=1000,
+        order="fast", print_period=0) -> dict:
+    """Host-buffer block CG through cacg_bcg() (the reference's bcg_v1,
+    src/bcgKernel.c:22-462).  res_hist[k] holds ||R(:,c)|| of the residual
+    iteration k started from (what its stop test reads); X has been updated
+    by the iteration that stops."""
+    B = np.ascontiguousarray(B, dtype=np.float64).reshape(A.n, -1)
+    m = B.shape[1]
+    X = np.ascontiguousarray(X0, dtype=np.float64).reshape(A.n, m).copy()
+    rp = np.ascontiguousarray(A.rowptr, dtype=np.int64)
+    ci = np.ascontiguousarray(A.col, dtype=np.int64)
+    va = np.ascontiguousarray(A.val, dtype=np.float64)
+    o = make_opts(stop_mode, eps, tol, maxiter, order, print_period)
+    hist = np.zeros((maxiter, m))
+    rep = Report()
+    rep.res_hist = hist.ctypes.data_as(_dp)
+    rep.res_hist_cap = maxiter
+    rc = lib().cacg_bcg(A.n, rp.ctypes.data_as(_lp), ci.ctypes.data_as(_lp), va.ctypes.data_as(_dp),
+                        m, B.ctypes.data_as(_dp), X.ctypes.data_as(_dp), ctypes.byref(o),
+                        ctypes.byref(rep))
+    if rc not in (0, 1, 2):
+        raise RuntimeError(f"cacg_bcg failed: {STATUS.get(rc, rc)}")
+    out = _report_dict(rep, hist, m)
+    out["X"] = X
+    return out
+
+
 class Solver:
     """Device-resident solver: matrix uploaded once, work blocks allocated once."""
 
@@ -460,6 +491,16 @@ class Solver:
         rep.res_hist = self.hist.ctypes.data_as(_dp)
         rep.res_hist_cap = self.maxiter
         rc = lib().cacg_solver_solve_device(self.h, dB.ptr, dX.ptr, ctypes.byref(rep))
+        if rc not in (0, 1, 2):
+            raise RuntimeError(f"solve failed: {STATUS.get(rc, rc)}")
+        return _report_dict(rep, self.hist, self.m)
+
+    def solve_bcg(self, dB: DeviceBuffer, dX: DeviceBuffer) -> dict:
+        """Block CG (bcg_v1) on this solver's matrix and buffers."""
+        rep = Report()
+        rep.res_hist = self.hist.ctypes.data_as(_dp)
+        rep.res_hist_cap = self.maxiter
+        rc = lib().cacg_solver_solve_bcg_device(self.h, dB.ptr, dX.ptr, ctypes.byref(rep))
         if rc not in (0, 1, 2):
             raise RuntimeError(f"solve failed: {STATUS.get(rc, rc)}")
         return _report_dict(rep, self.hist, self.m)

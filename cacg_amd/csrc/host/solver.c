@@ -3,6 +3,9 @@
  * GPU.  C11 over the HIP runtime's C API and the kernel launchers of
  * include/cacg_kernels.h.
  *
+ * Also the reference's plain block CG, bcg_v1 (src/bcgKernel.c:22-462), on the
+ * same kernels (cacg_solver_solve_bcg_device).
+ *
  * One outer iteration (the reference's src/bcbcgKernel.c:172-526):
  *   S = [S_0 | ... | S_{s-1}], Chebyshev basis of R     s - 1 SpMMs, fused recurrence
  *   k == 0: Q = S (buffer swap)                           src/bcbcgKernel.c:199-203
@@ -565,6 +568,140 @@ out:
     return status;
 }
 
+/* ---- block CG (src/bcgKernel.c:22-462) ---------------------------------------------- */
+
+/* O'Leary block CG on the solver's matrix: P, A P in the leading n x m parts of
+ * the Q and A Q buffers, P^T A P, R^T R and the coefficients in the m x m
+ * leading parts of the small blocks.  Per iteration (src/bcgKernel.c:184-443):
+ *   AP = A P; PtAP = P^T AP; alpha = PtAP^{-1} RtR      :192, :213, :239
+ *   X += P alpha; R -= AP alpha; ||R(:,c)||^2           :258-291, :375
+ *   stop test on the R the iteration started from        :301-365
+ *   RnRn = R^T R; beta = RtR^{-1} RnRn; RtR = RnRn        :389-419
+ *   P = R + P beta, as R - P (-beta): negation is exact   :428-443
+ * The X update comes before the stop test, as in the reference, so the
+ * iteration that stops has already moved X once more.  The new residual's
+ * norms are computed with the X/R update and read back once per iteration;
+ * the test of iteration k reads those of iteration k - 1. */
+int cacg_solver_solve_bcg_device(cacg_solver *sv, const double *d_B, double *d_X, cacg_report *rep)
+{
+    if (!sv || !d_B || !d_X || sv->opts.precision != CACG_PREC_FP64)
+        return CACG_EINVAL;
+    const cacg_opts *o = &sv->opts;
+    const int mode = o->order == CACG_ORDER_REFERENCE ? CACG_MODE_REFERENCE : CACG_MODE_FAST;
+    const int n = (int)sv->n, m = (int)sv->m;
+    hipStream_t st = sv->st;
+    double *h_norm2 = sv->h_pin, *h_bnorm2 = sv->h_pin + m;
+    int *h_flag = (int *)(sv->h_pin + 2 * m + 2);
+    double *P = sv->d_Q, *AP = sv->d_AQ, *R = sv->d_R;
+    double *PtAP = sv->d_QtAQ, *RtR = sv->d_QtR, *RnRn = sv->d_QtAS;
+    double *alpha = sv->d_alpha, *beta = sv->d_beta;
+    double *prev = (double *)malloc(sizeof(double) * (size_t)m);
+    int status = CACG_MAXITER;
+    long it = 0;
+    if (!prev)
+        return CACG_ENOMEM;
+    if (rep) {
+        double *keep = rep->res_hist;
+        long cap = rep->res_hist_cap;
+        memset(rep, 0, sizeof(*rep));
+        rep->res_hist = keep;
+        rep->res_hist_cap = cap;
+        rep->true_relres = -1.0;
+    }
+#define GRAMM(L_, R_, G_)                                                                        \
+    (mode == CACG_MODE_REFERENCE                                                                 \
+         ? cacg_k_gram_ref(n, L_, m, m, R_, m, m, NULL, 0, 0, G_, NULL, st)                      \
+         : cacg_k_gram_fast(n, L_, m, m, R_, m, m, NULL, 0, 0, G_, NULL, sv->d_gram_work, st))
+
+    sv->nev = 0;
+    mark(sv, CACG_PHASE_OTHER);
+    /* R = B - A X, P = R, RtR = R^T R, ||R(:,c)||^2 (src/bcgKernel.c:89-178) */
+    SCHK(cacg_k_fill_i32(sv->d_flag, 1, 0, st));
+    SCHK(cacg_k_spmm(&sv->A, m, d_X, m, R, m, CACG_EPI_RESID, 0.0, 0.0, d_B, m, NULL, 0, NULL, 0,
+                     st));
+    SCHK(hipMemcpyAsync(P, R, sizeof(double) * (size_t)n * m, hipMemcpyDeviceToDevice, st));
+    SCHK(GRAMM(R, R, RtR));
+    if (mode == CACG_MODE_REFERENCE)
+        SCHK(cacg_k_colnorm2_ref(n, m, R, m, sv->d_norm2, st));
+    else
+        SCHK(cacg_k_colnorm2_fast(n, m, R, sv->d_norm2, sv->d_xr_work, st));
+    SCHK(hipMemcpyAsync(prev, sv->d_norm2, sizeof(double) * m, hipMemcpyDeviceToHost, st));
+    if (o->stop_mode == CACG_STOP_REL_PER_COLUMN) {
+        if (mode == CACG_MODE_REFERENCE)
+            SCHK(cacg_k_colnorm2_ref(n, m, d_B, m, sv->d_norm2, st));
+        else
+            SCHK(cacg_k_colnorm2_fast(n, m, d_B, sv->d_norm2, sv->d_xr_work, st));
+        SCHK(hipMemcpyAsync(h_bnorm2, sv->d_norm2, sizeof(double) * m, hipMemcpyDeviceToHost, st));
+    }
+    mark(sv, CACG_PHASE_SETUP);
+    SCHK(hipStreamSynchronize(st));
+    harvest(sv, rep);
+
+    for (it = 0; it < o->maxiter; it++) {
+        mark(sv, CACG_PHASE_OTHER);
+        SCHK(cacg_k_spmm(&sv->A, m, P, m, AP, m, CACG_EPI_NONE, 0.0, 0.0, NULL, 0, NULL, 0, NULL, 0,
+                         st));
+        mark(sv, CACG_PHASE_AQ);
+        SCHK(GRAMM(P, AP, PtAP));
+        mark(sv, CACG_PHASE_GRAM);
+        SCHK(cacg_k_lu_solve(m, PtAP, m, RtR, alpha, sv->d_lu_work, sv->d_flag, mode, 0, st));
+        mark(sv, CACG_PHASE_SOLVE);
+        SCHK(cacg_k_xrupdate(n, m, m, P, AP, alpha, d_X, R, sv->d_norm2, sv->d_xr_work, mode, st));
+        mark(sv, CACG_PHASE_XRUPDATE);
+        SCHK(hipMemcpyAsync(h_norm2, sv->d_norm2, sizeof(double) * m, hipMemcpyDeviceToHost, st));
+        SCHK(hipMemcpyAsync(h_flag, sv->d_flag, sizeof(int), hipMemcpyDeviceToHost, st));
+        SCHK(hipStreamSynchronize(st));
+        harvest(sv, rep);
+
+        int done = 1, finite = 1;
+        for (int c = 0; c < m; c++) {
+            const double rn = sqrt(prev[c]);
+            if (rep && rep->res_hist && it < rep->res_hist_cap)
+                rep->res_hist[it * m + c] = rn;
+            if (!isfinite(rn))
+                finite = 0;
+            if (o->stop_mode == CACG_STOP_REL_PER_COLUMN && !(rn < o->tol * sqrt(h_bnorm2[c])))
+                done = 0;
+        }
+        const double r0 = sqrt(prev[0]);
+        if (o->stop_mode == CACG_STOP_REF_COL0_ABS)
+            done = r0 < o->eps; /* src/bcgKernel.c:301, 364-366 */
+        if (o->print_period > 0 &&
+            (it % o->print_period == 0 || done || it == o->maxiter - 1)) /* :302-305 */
+            printf("id:%d, Loop: %ld, R[%d]_norm=%30.30f\n", 0, it, 1, r0);
+        if (done) {
+            status = CACG_OK;
+            it++;
+            break;
+        }
+        if (!finite || *h_flag) {
+            status = CACG_BREAKDOWN;
+            it++;
+            break;
+        }
+        memcpy(prev, h_norm2, sizeof(double) * m);
+        mark(sv, CACG_PHASE_OTHER);
+        SCHK(GRAMM(R, R, RnRn));
+        mark(sv, CACG_PHASE_GRAM_AS);
+        SCHK(cacg_k_lu_solve(m, RtR, m, RnRn, beta, sv->d_lu_work, sv->d_flag, mode, 0, st));
+        mark(sv, CACG_PHASE_SOLVE);
+        double *t = RtR;
+        RtR = RnRn;
+        RnRn = t;
+        SCHK(cacg_k_scale_f64(beta, (long)m * m, -1.0, st));
+        SCHK(cacg_k_qupdate(n, m, R, P, beta, mode, st));
+        mark(sv, CACG_PHASE_QUPDATE);
+    }
+#undef GRAMM
+out:
+    free(prev);
+    if (rep) {
+        rep->status = status;
+        rep->iters = it;
+    }
+    return status;
+}
+
 /* ---- precision form: FP32 basis, FP64 refinement ----------------------------------- */
 
 /* 1 when the column norms (squared) meet the stop criterion */
@@ -795,17 +932,52 @@ int cacg_bcbcg(long n, const long *row_start, const long *col_idx, const double 
     return rc;
 }
 
-/* ---- drop-in entry points (src/cbcgKernel.h:21, src/bcbcgKernel.h:22) ------------ */
+int cacg_bcg(long n, const long *row_start, const long *col_idx, const double *val, long m,
+             const double *B, double *X, const cacg_opts *opts, cacg_report *report)
+{
+    if (!B || !X)
+        return CACG_EINVAL;
+    if (opts && opts->precision != CACG_PREC_FP64)
+        return CACG_EINVAL;
+    cacg_solver *sv = NULL;
+    int rc = cacg_solver_create(&sv, n, row_start, col_idx, val, m, 1, opts);
+    if (rc != CACG_OK)
+        return rc;
+    double *dB = NULL, *dX = NULL;
+    const size_t bytes = sizeof(double) * (size_t)n * (size_t)m;
+    if (hipMalloc((void **)&dB, bytes) != hipSuccess || hipMalloc((void **)&dX, bytes) != hipSuccess) {
+        hipFree(dB);
+        cacg_solver_destroy(sv);
+        return CACG_ENOMEM;
+    }
+    rc = CACG_EDEVICE;
+    if (hipMemcpy(dB, B, bytes, hipMemcpyHostToDevice) == hipSuccess &&
+        hipMemcpy(dX, X, bytes, hipMemcpyHostToDevice) == hipSuccess) {
+        rc = cacg_solver_solve_bcg_device(sv, dB, dX, report);
+        if (rc != CACG_EDEVICE && rc != CACG_EINVAL && rc != CACG_ENOMEM &&
+            hipMemcpy(X, dX, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = CACG_EDEVICE;
+    }
+    hipFree(dB);
+    hipFree(dX);
+    cacg_solver_destroy(sv);
+    return rc;
+}
+
+/* ---- drop-in entry points (src/cbcgKernel.h:21, src/bcbcgKernel.h:22, src/bcgKernel.h:21) */
+
+enum { DROPIN_CBCG, DROPIN_BCBCG, DROPIN_BCG };
 
 static void dropin(csrType_local mat, denseType B, denseType X, long s, double epsilon, int myid,
-                   int numprocs, long maxiter)
+                   int numprocs, long maxiter, int which)
 {
+    static const char *names[] = {"cbcg_v1", "bcbcg_v1", "bcg_v1"};
     g_last_iters = 0;
     if (numprocs != 1 || myid != 0 || mat.start != 0 || mat.num_rows != B.local_num_row ||
         B.local_num_row != B.global_num_row || X.local_num_row != B.local_num_row ||
         X.local_num_col != B.local_num_col || mat.num_cols != mat.num_rows || !B.data || !X.data) {
         fprintf(stderr, "cacg: %s: the whole system must be local to one process (numprocs == 1)\n",
-                maxiter == 1000 ? "cbcg_v1" : "bcbcg_v1");
+                names[which]);
         g_last_status = CACG_EINVAL;
         return;
     }
@@ -820,8 +992,12 @@ static void dropin(csrType_local mat, denseType B, denseType X, long s, double e
     o.print_period = (quiet && quiet[0] == '1') ? 0 : 10;
     cacg_report rep;
     memset(&rep, 0, sizeof(rep));
-    g_last_status = cacg_bcbcg(mat.num_rows, mat.row_start, mat.col_idx, mat.csrdata,
-                               B.local_num_col, s, B.data, X.data, &o, &rep);
+    if (which == DROPIN_BCG)
+        g_last_status = cacg_bcg(mat.num_rows, mat.row_start, mat.col_idx, mat.csrdata,
+                                 B.local_num_col, B.data, X.data, &o, &rep);
+    else
+        g_last_status = cacg_bcbcg(mat.num_rows, mat.row_start, mat.col_idx, mat.csrdata,
+                                   B.local_num_col, s, B.data, X.data, &o, &rep);
     g_last_iters = rep.iters;
     fflush(stdout);
 }
@@ -836,11 +1012,16 @@ void cbcg_v1(csrType_local mat, denseType B, denseType X, long s, double epsilon
         g_last_iters = 0;
         return;
     }
-    dropin(mat, B, X, s, epsilon, myid, numprocs, 1000);
+    dropin(mat, B, X, s, epsilon, myid, numprocs, 1000, DROPIN_CBCG);
 }
 
 void bcbcg_v1(csrType_local mat, denseType B, denseType X, long s, double epsilon, int myid,
               int numprocs)
 {
-    dropin(mat, B, X, s, epsilon, myid, numprocs, 100); /* src/bcbcgKernel.c:3 */
+    dropin(mat, B, X, s, epsilon, myid, numprocs, 100, DROPIN_BCBCG); /* src/bcbcgKernel.c:3 */
+}
+
+void bcg_v1(csrType_local mat, denseType B, denseType X, double epsilon, int myid, int numprocs)
+{
+    dropin(mat, B, X, 1, epsilon, myid, numprocs, 1000, DROPIN_BCG); /* src/bcgKernel.c:2 */
 }

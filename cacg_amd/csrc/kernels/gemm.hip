@@ -552,6 +552,24 @@ extern "C" int cacg_k_fill_i32(int *d, long count, int v, hipStream_t st)
     return 0;
 }
 
+namespace {
+__global__ __launch_bounds__(256) void scale_kernel(double *d, long count, double a)
+{
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < count; e += (long)gridDim.x * 256)
+        d[e] *= a;
+}
+}  // namespace
+
+extern "C" int cacg_k_scale_f64(double *d, long count, double a, hipStream_t st)
+{
+    if (count <= 0)
+        return 0;
+    hipLaunchKernelGGL(scale_kernel, dim3(cacg_grid(count, 256, 1024)), dim3(256), 0, st, d, count,
+                       a);
+    CACG_LAUNCH_CHECK();
+    return 0;
+}
+
 extern "C" int cacg_k_fill_f64(double *d, long count, double v, hipStream_t st)
 {
     if (count <= 0)

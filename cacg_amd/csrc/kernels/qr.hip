@@ -89,14 +89,6 @@ __global__ __launch_bounds__(QT) void chol_inv_kernel(int N, const double *__res
     }
 }
 
-__global__ __launch_bounds__(256) void negate_kernel(int cnt, const double *__restrict__ src,
-                                                     double *__restrict__ dst)
-{
-    const int e = blockIdx.x * 256 + threadIdx.x;
-    if (e < cnt)
-        dst[e] = -src[e];
-}
-
 __global__ __launch_bounds__(256) void copy_rows_kernel(long n, int k, const double *__restrict__ A,
                                                         long lda, double *__restrict__ Q)
 {
@@ -111,8 +103,8 @@ __global__ __launch_bounds__(256) void copy_rows_kernel(long n, int k, const dou
 
 extern "C" long cacg_tsqr_workspace(int k)
 {
-    // Gram partials, then G, R1, W, R2, Wneg
-    return cacg_gram_workspace(k, k) + 5L * k * k;
+    // Gram partials, then G, R1, W, R2
+    return cacg_gram_workspace(k, k) + 4L * k * k;
 }
 
 extern "C" int cacg_k_tsqr(int n, int k, const double *A, long lda, double *Q, double *R,
@@ -123,7 +115,6 @@ extern "C" int cacg_k_tsqr(int n, int k, const double *A, long lda, double *Q, d
     double *gw = d_work;
     double *G = gw + cacg_gram_workspace(k, k);
     double *R1 = G + (long)k * k, *W = R1 + (long)k * k, *R2 = W + (long)k * k;
-    double *Wn = R2 + (long)k * k;
     const size_t shm = (size_t)k * k * sizeof(double);
     cacg_allow_lds(chol_inv_kernel, shm);
     // Q = A (contiguous n x k)
@@ -140,9 +131,10 @@ extern "C" int cacg_k_tsqr(int n, int k, const double *A, long lda, double *Q, d
                            pass == 0 ? nullptr : R, d_flag);
         CACG_LAUNCH_CHECK();
         // Q = Q W, as Q = 0 - Q (-W) with the in-place tall-skinny update
-        hipLaunchKernelGGL(negate_kernel, dim3((k * k + 255) / 256), dim3(256), 0, st, k * k, W, Wn);
-        CACG_LAUNCH_CHECK();
-        rc = cacg_k_qupdate_zero(n, k, Q, Wn, st);
+        rc = cacg_k_scale_f64(W, (long)k * k, -1.0, st);
+        if (rc)
+            return rc;
+        rc = cacg_k_qupdate_zero(n, k, Q, W, st);
         if (rc)
             return rc;
     }

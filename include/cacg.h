@@ -66,6 +66,9 @@ void cbcg_v1(csrType_local mat, denseType B, denseType X, long s, double epsilon
              int numprocs);
 void bcbcg_v1(csrType_local mat, denseType B, denseType X, long s, double epsilon, int myid,
               int numprocs);
+/* src/bcgKernel.h:21 (src/bcgKernel.c:22-462): O'Leary block CG, 1000-iteration
+ * cap (src/bcgKernel.c:2); prints the reference's "id:0, Loop: ..." lines. */
+void bcg_v1(csrType_local mat, denseType B, denseType X, double epsilon, int myid, int numprocs);
 
 /* ---- status codes ------------------------------------------------------- */
 enum {
@@ -159,6 +162,13 @@ int cacg_bcbcg(long n, const long *row_start, const long *col_idx, const double 
                long m, long s, const double *B, double *X, const cacg_opts *opts,
                cacg_report *report);
 
+/* Host-buffer block CG (the reference's bcg_v1, src/bcgKernel.c:22-462):
+ * CACG_PREC_FP64 only.  The stop test of iteration k reads the residual that
+ * iteration started from, after X has been updated (src/bcgKernel.c:278-365),
+ * and report->res_hist[k] holds those norms. */
+int cacg_bcg(long n, const long *row_start, const long *col_idx, const double *val, long m,
+             const double *B, double *X, const cacg_opts *opts, cacg_report *report);
+
 /* Device-resident solver: the matrix is uploaded once (int32 indices when
  * nnz < 2^31) and the work blocks are allocated once for (m, s).  solve_device
  * takes device pointers to B and X (n x m row-major, X in/out) and runs on the
@@ -167,6 +177,12 @@ typedef struct cacg_solver cacg_solver;
 int cacg_solver_create(cacg_solver **out, long n, const long *row_start, const long *col_idx,
                        const double *val, long m, long s, const cacg_opts *opts);
 int cacg_solver_solve_device(cacg_solver *sv, const double *d_B, double *d_X, cacg_report *report);
+/* Block CG (bcg_v1) with the solver's matrix and stream, for a solver created
+ * with any s (it uses the leading n x m parts of the work blocks); phases:
+ * AQ = A P, GRAM = P^T A P, GRAM_AS = R^T R, SOLVE = both LU solves,
+ * XRUPDATE = X and R updates with the norms, QUPDATE = P = R + P beta. */
+int cacg_solver_solve_bcg_device(cacg_solver *sv, const double *d_B, double *d_X,
+                                 cacg_report *report);
 int cacg_solver_destroy(cacg_solver *sv);
 /* bytes of HBM the solver holds (matrix + work blocks) */
 long cacg_solver_footprint(const cacg_solver *sv);

@@ -125,6 +125,8 @@ int cacg_k_colnorm2_fast(int n, int m, const double *R, double *d_norm2, double 
 /* d[0:count] = v, as a kernel (safe inside captured graphs, unlike a memset node) */
 int cacg_k_fill_i32(int *d, long count, int v, hipStream_t st);
 int cacg_k_fill_f64(double *d, long count, double v, hipStream_t st);
+/* d[0:count] *= a in place (a = -1: exact negation) */
+int cacg_k_scale_f64(double *d, long count, double a, hipStream_t st);
 
 /* Tall-skinny QR (no reference counterpart: the reference's TSQR.c is empty):
  * A (n x k, row stride lda, k <= 128) = Q R with Q (n x k, contiguous) having

@@ -75,6 +75,9 @@ def lib():
         L.orc_bcbcg.argtypes = [ctypes.POINTER(_Csr), ctypes.c_long, ctypes.c_long, _dp, _dp,
                                 ctypes.POINTER(_Opts), _lp]
         L.orc_bcbcg.restype = ctypes.c_int
+        L.orc_bcg.argtypes = [ctypes.POINTER(_Csr), ctypes.c_long, _dp, _dp,
+                              ctypes.POINTER(_Opts), _lp]
+        L.orc_bcg.restype = ctypes.c_int
         L.orc_bcbcg_time_sample.argtypes = [ctypes.POINTER(_Csr), ctypes.c_long, ctypes.c_long,
                                             _dp, _dp, ctypes.c_int, ctypes.c_long, _dp, _dp]
         L.orc_bcbcg_time_sample.restype = ctypes.c_int
@@ -181,6 +184,29 @@ def cbcg(A, b, x0, s, **kw):
     return bcbcg(A, np.reshape(b, (A.n, 1)), np.reshape(x0, (A.n, 1)), s, **kw)
 
 
+def bcg(A, B, X0, *, stop_mode=STOP_REF_COL0_ABS, eps=0.0, tol=0.0, maxiter=1000, nranks=1,
+        record_x=False):
+    """bcg_v1 (src/bcgKernel.c:22-462), O'Leary block CG with the reference's
+    1000-iteration cap (src/bcgKernel.c:2).  res_hist[k] is ||R(:,c)|| of the
+    residual iteration k started from (the one its stop test reads), x_hist[k]
+    the X after iteration k's update."""
+    h = _CsrHold(A)
+    B = _c64(B).reshape(A.n, -1)
+    m = B.shape[1]
+    X = _c64(X0).reshape(A.n, m).copy()
+    res = np.zeros((maxiter, m))
+    xh = np.zeros((maxiter, A.n, m)) if record_x else None
+    o = _Opts(stop_mode, eps, tol, maxiter, nranks, _d(res),
+              _d(xh) if xh is not None else None, None)
+    it = ctypes.c_long(0)
+    st = lib().orc_bcg(ctypes.byref(h.s), m, _d(B), _d(X), ctypes.byref(o), ctypes.byref(it))
+    k = it.value
+    out = {"X": X, "iters": k, "status": STATUS[st], "res_hist": res[:k]}
+    if xh is not None:
+        out["x_hist"] = xh[:k]
+    return out
+
+
 def time_sample(A, B, X0, s, nranks, iters):
     """(t_setup, [t_iteration_0, ...]) wall seconds of a bounded sample of the
     solve at `nranks` emulated ranks (one thread each)."""
@@ -199,11 +225,13 @@ def reference_available() -> bool:
 
 
 def run_reference(A, B, X0, s, eps, solver="bcbcg", nranks=1):
-    """Run the reference's own bcbcg_v1 / cbcg_v1 at `nranks` MPI ranks
-    (mpiexec from the image's MPICH when nranks > 1).
+    """Run the reference's own bcbcg_v1 / cbcg_v1 / bcg_v1 at `nranks` MPI
+    ranks (mpiexec from the image's MPICH when nranks > 1).
 
     Returns dict(iters, r2 (iters,), X (final, n x m)) and, at one rank,
-    R and x_hist (iters x n x m) recorded after every outer iteration.
+    R and x_hist (iters x n x m) recorded at every outer iteration's stop test
+    (bcbcg / cbcg: after the R update; bcg: the R the iteration started from,
+    with X already updated).
     """
     if not reference_available():
         raise RuntimeError("reference build missing (make -C oracle ref)")
@@ -214,7 +242,7 @@ def run_reference(A, B, X0, s, eps, solver="bcbcg", nranks=1):
         fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
         with open(fin, "wb") as f:
             f.write(b"CACGIN01")
-            f.write(struct.pack("<5q", n, A.nnz, m, s, 1 if solver == "cbcg" else 2))
+            f.write(struct.pack("<5q", n, A.nnz, m, s, {"bcg": 0, "cbcg": 1, "bcbcg": 2}[solver]))
             f.write(struct.pack("<d", eps))
             f.write(np.ascontiguousarray(A.rowptr, dtype="<i8").tobytes())
             f.write(np.ascontiguousarray(A.col, dtype="<i8").tobytes())

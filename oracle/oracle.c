@@ -481,6 +481,91 @@ int orc_bcbcg(const orc_csr *A, long m, long s, const double *B, double *X,
     return status;
 }
 
+/* src/bcgKernel.c:22-462 */
+int orc_bcg(const orc_csr *A, long m, const double *B, double *X, const orc_opts *o, long *iters)
+{
+    const long n = A->n;
+    const int P = o->nranks > 0 ? o->nranks : 1;
+    if (n <= 0 || m <= 0 || o->maxiter <= 0)
+        return ORC_INVAL;
+    const size_t nm = (size_t)n * m, mm = (size_t)m * m;
+    double *R = calloc(nm, sizeof(double)), *AX = calloc(nm, sizeof(double));
+    double *Pm = calloc(nm, sizeof(double)), *AP = calloc(nm, sizeof(double));
+    double *Pa = calloc(nm, sizeof(double)), *APa = calloc(nm, sizeof(double));
+    double *Pb = calloc(nm, sizeof(double));
+    double *PtAP = calloc(mm, sizeof(double)), *RtR = calloc(mm, sizeof(double));
+    double *RnRn = calloc(mm, sizeof(double)), *alpha = calloc(mm, sizeof(double));
+    double *beta = calloc(mm, sizeof(double)), *bnorm = calloc((size_t)m, sizeof(double));
+    int status = ORC_INVAL;
+    long it = 0;
+    if (!R || !AX || !Pm || !AP || !Pa || !APa || !Pb || !PtAP || !RtR || !RnRn || !alpha ||
+        !beta || !bnorm)
+        goto out;
+    for (long c = 0; c < m; c++)
+        bnorm[c] = sqrt(orc_col_norm2sq(n, m, B, c));
+    spmm_ranked(A, X, m, AX, P);                                /* :89 */
+    orc_lincomb2((long)nm, 1.0, B, -1.0, AX, R);                /* :127 */
+    memcpy(Pm, R, nm * sizeof(double));                         /* :138 */
+    gram_ranked(n, R, m, R, m, RtR, P);                         /* :178 */
+    status = ORC_MAXITER;
+    for (it = 0; it < o->maxiter; it++) {
+        spmm_ranked(A, Pm, m, AP, P);                           /* :192 */
+        gram_ranked(n, Pm, m, AP, m, PtAP, P);                  /* :213 */
+        orc_lu_solve(m, PtAP, m, RtR, alpha);                   /* :239 */
+        gemm_ranked(n, Pm, m, alpha, m, Pa, P);                 /* :258 */
+        orc_lincomb2((long)nm, 1.0, X, 1.0, Pa, X);             /* :278 */
+        gemm_ranked(n, AP, m, alpha, m, APa, P);                /* :291 */
+        int done = 1, finite = 1;
+        for (long c = 0; c < m; c++) {
+            const double rn = sqrt(orc_col_norm2sq(n, m, R, c));
+            if (o->res_hist)
+                o->res_hist[it * m + c] = rn;
+            if (!isfinite(rn))
+                finite = 0;
+            if (o->stop_mode == ORC_STOP_REL_PER_COLUMN && !(rn < o->tol * bnorm[c]))
+                done = 0;
+        }
+        if (o->x_hist)
+            memcpy(o->x_hist + it * nm, X, nm * sizeof(double));
+        if (o->stop_mode == ORC_STOP_REF_COL0_ABS)              /* :301-365 */
+            done = sqrt(orc_col_norm2sq(n, m, R, 0)) < o->eps;
+        if (done) {
+            status = ORC_OK;
+            it++;
+            break;
+        }
+        if (!finite) {
+            status = ORC_BREAKDOWN;
+            it++;
+            break;
+        }
+        orc_lincomb2((long)nm, 1.0, R, -1.0, APa, R);           /* :375 */
+        gram_ranked(n, R, m, R, m, RnRn, P);                    /* :389 */
+        orc_lu_solve(m, RtR, m, RnRn, beta);                    /* :404 */
+        double *t = RtR;                                        /* :419 */
+        RtR = RnRn;
+        RnRn = t;
+        gemm_ranked(n, Pm, m, beta, m, Pb, P);                  /* :428 */
+        orc_lincomb2((long)nm, 1.0, R, 1.0, Pb, Pm);            /* :443 */
+    }
+out:
+    *iters = it;
+    free(R);
+    free(AX);
+    free(Pm);
+    free(AP);
+    free(Pa);
+    free(APa);
+    free(Pb);
+    free(PtAP);
+    free(RtR);
+    free(RnRn);
+    free(alpha);
+    free(beta);
+    free(bnorm);
+    return status;
+}
+
 static double now_s(void)
 {
     struct timespec t;

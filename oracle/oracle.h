@@ -68,6 +68,16 @@ double orc_col_norm2sq(long n, long m, const double *R, long col);
 int orc_bcbcg(const orc_csr *A, long m, long s, const double *B, double *X,
               const orc_opts *o, long *iters);
 
+/* src/bcgKernel.c:22-462, O'Leary block CG (m RHS):
+ *   R = B - A X, P = R, RtR = R^T R; per iteration
+ *   AP = A P, PtAP = P^T AP, alpha = PtAP^{-1} RtR, X += P alpha,
+ *   stop test on ||R(:,0)|| of the R the iteration started from,
+ *   R -= AP alpha, beta = RtR^{-1} (R^T R), RtR = R^T R, P = R + P beta.
+ * res_hist[it] holds ||R(:,c)|| of that starting R (what the stop test reads);
+ * x_hist[it] the X after the iteration's update.  Returns ORC_*; *iters =
+ * iterations performed (the last one updates X and stops). */
+int orc_bcg(const orc_csr *A, long m, const double *B, double *X, const orc_opts *o, long *iters);
+
 /* Wall-clock seconds of a bounded sample: setup (bounds, R = B - AX, work
  * allocation) and each of `iters` outer iterations (t_iters[iters]), no stop
  * test.  For the CPU baseline only. */

@@ -2,8 +2,9 @@
  * Golden-data harness for the reference solver (TEST INFRASTRUCTURE ONLY).
  *
  * Links the reference's own solver objects, compiled from
- * /root/reference/src by oracle/ref/Makefile, and calls bcbcg_v1 / cbcg_v1
- * (src/bcbcgKernel.h:22, src/cbcgKernel.h:21) once, at one rank or under
+ * /root/reference/src by oracle/ref/Makefile, and calls bcbcg_v1 / cbcg_v1 /
+ * bcg_v1 (src/bcbcgKernel.h:22, src/cbcgKernel.h:21, src/bcgKernel.h:21)
+ * once, at one rank or under
  * `mpiexec -n P`.  Every rank reads the whole input and keeps its row block
  * the way src/SparseMatrixDistribution.c:64-108 and src/GetRHS.c:124-190 lay
  * it out: n/P rows per rank, the last rank takes the remainder, local
@@ -13,12 +14,14 @@
  * The reference never writes X (src/DenseMatIO.c:3-7 is empty), so the link
  * wraps norm2square_dist_denseMat_col_n (-Wl,--wrap): the solver calls it once
  * per outer iteration right after the X and R updates
- * (src/bcbcgKernel.c:447, src/cbcgKernel.c:214) with R as its argument; the
+ * (src/bcbcgKernel.c:447, src/cbcgKernel.c:214) with R as its argument (bcg_v1:
+ * src/bcgKernel.c:301, after the X update at :278 and before the R update at
+ * :375, so it sees the residual the iteration started from); the
  * wrapper forwards the call unchanged and records the global ||R(:,0)||^2 and,
  * at one rank, R and X.  After the solver returns, X is gathered to rank 0.
  *
  * Input file (little-endian):  "CACGIN01", int64 n, nnz, m, s, solver
- * (1 = cbcg_v1, 2 = bcbcg_v1), float64 epsilon, int64 rowptr[n+1],
+ * (0 = bcg_v1, 1 = cbcg_v1, 2 = bcbcg_v1), float64 epsilon, int64 rowptr[n+1],
  * int64 col[nnz], float64 val[nnz], float64 B[n*m], float64 X0[n*m].
  * Output file (rank 0): "CACGOUT2", int64 iterations, int64 P, int64 n,
  * int64 m, float64 r2[iterations], then at P == 1 per iteration float64
@@ -36,6 +39,7 @@ void bcbcg_v1(csrType_local mat, denseType B, denseType X, long s, double epsilo
               int numprocs);
 void cbcg_v1(csrType_local mat, denseType B, denseType X, long s, double epsilon, int myid,
              int numprocs);
+void bcg_v1(csrType_local mat, denseType B, denseType X, double epsilon, int myid, int numprocs);
 void __real_norm2square_dist_denseMat_col_n(denseType mat, double *result, long n, int myid,
                                             int numprocs);
 
@@ -142,7 +146,9 @@ int main(int argc, char **argv)
         g_snap = fopen(snap_path, "wb");
     }
 
-    if (solver == 1)
+    if (solver == 0)
+        bcg_v1(A, B, X, eps, myid, np);
+    else if (solver == 1)
         cbcg_v1(A, B, X, s, eps, myid, np);
     else
         bcbcg_v1(A, B, X, s, eps, myid, np);

@@ -1,19 +1,23 @@
 """Write the golden fixtures of the s-step Chebyshev (block) CG path.
 
-Runs the reference's own bcbcg_v1 / cbcg_v1 (compiled from
+Runs the reference's own bcbcg_v1 / cbcg_v1 / bcg_v1 (compiled from
 /root/reference/src by oracle/ref/Makefile and driven by
 oracle/ref/ref_harness.c) at one MPI rank and at four, on seeded inputs from
 cacg_amd.problems, and stores its outputs as small .npz fixtures plus a
 manifest.  Only data are committed: inputs are regenerated from the manifest
 (their SHA-256 is stored to catch drift), outputs are the reference's.
 
-    python tests/golden/make_golden.py      # needs oracle/_ref (this container)
+    python tests/golden/make_golden.py [name ...]   # needs oracle/_ref (this container)
+
+With names, only those fixtures are (re)written and the manifest keeps the
+other entries.
 
 Per fixture:
   r2_p1        ||R(:,0)||^2 after each outer iteration, reference at 1 rank
-  rnorm_cols   ||R(:,c)|| after each iteration (sequential row-order sum of the
-               reference's own R snapshots, the reference's norm loop applied
-               to every column)
+  rnorm_cols   ||R(:,c)|| at each iteration's stop test (sequential row-order
+               sum of the reference's own R snapshots, the reference's norm
+               loop applied to every column); bcbcg / cbcg: the R after the
+               iteration's update, bcg: the R the iteration started from
   X            final X at 1 rank
   x_hist       X after iterations 1 and 2 (small cases only)
   r2_p4, X_p4_reldiff   the reference at 4 ranks: its per-iteration ||R(:,0)||^2
@@ -56,6 +60,14 @@ CASES = {
     "bcbcg_3d12_m16_s8": ("lap3d", 12, 0.0, 16, 8, "bcbcg", 1e-8, "uniform", 1, "zero"),
     "bcbcg_spd16_m4_s4": ("spd2d", 16, 2.0, 4, 4, "bcbcg", 1e-8, "normal", 2, "zero"),
     "bcbcg_shift16_m8_s4": ("lap2d", 16, 0.05, 8, 4, "bcbcg", 1e-10, "uniform", 3, "zero"),
+    # bcg_v1 (block CG, no basis: s is unused and recorded as 1)
+    "bcg_2d16_m4": ("lap2d", 16, 0.0, 4, 1, "bcg", 1e-8, "uniform", 1, "zero"),
+    "bcg_2d32_m1": ("lap2d", 32, 0.0, 1, 1, "bcg", 1e-8, "uniform", 0, "zero"),
+    "bcg_2d16_m4_x1": ("lap2d", 16, 0.0, 4, 1, "bcg", 1e-8, "uniform", 1, "ones"),
+    "bcg_3d8_m4": ("lap3d", 8, 0.0, 4, 1, "bcg", 1e-8, "uniform", 1, "zero"),
+    "bcg_spd16_m4": ("spd2d", 16, 2.0, 4, 1, "bcg", 1e-8, "normal", 2, "zero"),
+    "bcg_2d32_m16_tol1e-10": ("lap2d", 32, 0.0, 16, 1, "bcg", 1e-10, "uniform", 1, "zero"),
+    "bcg_shift16_m8": ("lap2d", 16, 0.05, 8, 1, "bcg", 1e-10, "uniform", 3, "zero"),
 }
 
 
@@ -85,8 +97,17 @@ def main():
     if not oracle.reference_available():
         raise SystemExit("oracle/_ref/ref_harness is not built (make -C oracle ref)")
     oracle.build()
+    names = sys.argv[1:] or list(CASES)
+    unknown = [n for n in names if n not in CASES]
+    if unknown:
+        raise SystemExit(f"unknown fixtures: {unknown}")
+    mpath = os.path.join(HERE, "manifest.json")
     manifest = {}
-    for name, spec in CASES.items():
+    if sys.argv[1:] and os.path.exists(mpath):
+        with open(mpath) as f:
+            manifest = json.load(f)
+    for name in names:
+        spec = CASES[name]
         kind, size, param, m, s, solver, tol, rhs_kind, rhs_seed, x0 = spec
         A, B, X0 = make_problem(kind, size, param, m, rhs_kind, rhs_seed, x0)
         eps = tol * float(np.sqrt(oracle.col_norm2sq(B, 0)))
@@ -107,12 +128,12 @@ def main():
             "matrix": kind, "size": size, "param": param, "m": m, "s": s, "solver": solver,
             "tol": tol, "eps": eps, "rhs": rhs_kind, "rhs_seed": rhs_seed, "x0": x0,
             "n": A.n, "nnz": A.nnz, "iters_p1": k, "iters_p4": r4["iters"],
-            "maxiter": 1000 if solver == "cbcg" else 100,
+            "maxiter": 100 if solver == "bcbcg" else 1000,
             "input_sha256": input_digest(A, B, X0),
         }
         print(f"{name}: n={A.n} m={m} s={s} iters p1={k} p4={r4['iters']} "
               f"X p4 reldiff={float(data['X_p4_reldiff']):.1e}")
-    with open(os.path.join(HERE, "manifest.json"), "w") as f:
+    with open(mpath, "w") as f:
         json.dump(manifest, f, indent=1, sort_keys=True)
 
 

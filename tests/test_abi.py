@@ -42,6 +42,7 @@ def test_library_loads_and_exports_every_declared_function():
     L = cacg_amd.lib()
     names = declared_functions()
     assert "bcbcg_v1" in names and "cbcg_v1" in names and "cacg_k_gram_fast" in names
+    assert "bcg_v1" in names and "cacg_bcg" in names and "cacg_solver_solve_bcg_device" in names
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
 
@@ -143,6 +144,10 @@ def test_dropin_rejects_multirank_call_without_device():
     assert L.cacg_last_status() == 3       # CACG_EINVAL
     L.cbcg_v1(A, B, B, 2, 1.0, 1, 1)       # myid = 1
     assert L.cacg_last_status() == 3
+    L.bcg_v1.argtypes = [CsrType, DenseType, DenseType, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+    L.bcg_v1.restype = None
+    L.bcg_v1(A, B, B, 1.0, 0, 4)           # numprocs = 4
+    assert L.cacg_last_status() == 3
     assert L.cacg_status_string(3) == b"invalid argument"
 
 
@@ -166,6 +171,14 @@ def test_host_api_rejects_bad_arguments_without_device():
     assert rc == 3
     rc = L.cacg_bcbcg(A.n, rp.ctypes.data_as(lp), col.ctypes.data_as(lp), val.ctypes.data_as(dp),
                       1, 0, B.ctypes.data_as(dp), X.ctypes.data_as(dp), None, None)   # s = 0
+    assert rc == 3
+    # block CG: bad row pointers, and the FP32 form (FP64 only) are refused
+    rc = L.cacg_bcg(A.n, rp_bad.ctypes.data_as(lp), col.ctypes.data_as(lp), val.ctypes.data_as(dp),
+                    1, B.ctypes.data_as(dp), X.ctypes.data_as(dp), None, None)
+    assert rc == 3
+    o = cacg_amd.make_opts(precision="fp32")
+    rc = L.cacg_bcg(A.n, rp.ctypes.data_as(lp), col.ctypes.data_as(lp), val.ctypes.data_as(dp),
+                    1, B.ctypes.data_as(dp), X.ctypes.data_as(dp), ctypes.byref(o), None)
     assert rc == 3
 
 
