@@ -22,7 +22,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcacg.so")
 
-STATUS = {0: "ok", 1: "maxiter", 2: "breakdown", 3: "einval", 4: "enomem", 5: "edevice"}
+STATUS = {0: "ok", 1: "maxiter", 2: "breakdown", 3: "einval", 4: "enomem", 5: "edevice",
+          6: "eio"}
 STOP_REF_COL0_ABS, STOP_REL_PER_COLUMN = 0, 1
 ORDER_REFERENCE, ORDER_FAST = 0, 1
 MODE = {"reference": 0, "fast": 1}
@@ -122,6 +123,12 @@ def lib() -> ctypes.CDLL:
         "cacg_k_fill_i32": ([_vp, c_long, c_int, _vp], c_int),
         "cacg_k_fill_f64": ([_vp, c_long, c_double, _vp], c_int),
         "cacg_k_scale_f64": ([_vp, c_long, c_double, _vp], c_int),
+        "cacg_mtx_read": ([ctypes.c_char_p, _lp, _lp, _lp, ctypes.POINTER(_lp),
+                           ctypes.POINTER(_lp), ctypes.POINTER(_dp)], c_int),
+        "cacg_mtx_write": ([ctypes.c_char_p, c_long, c_long, _lp, _lp, _dp], c_int),
+        "cacg_csv_read": ([ctypes.c_char_p, c_long, c_long, ctypes.POINTER(_dp)], c_int),
+        "cacg_csv_write": ([ctypes.c_char_p, c_long, c_long, _dp], c_int),
+        "cacg_io_free": ([_vp], None),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -436,6 +443,63 @@ def bcbcg(A, B, X0, s, *, stop_mode=STOP_REL_PER_COLUMN, eps=0.0, tol=1e-8, maxi
     out = _report_dict(rep, hist, m)
     out["X"] = X
     return out
+
+
+# ---- input formats (include/cacg_io.h) ---------------------------------------------
+
+def read_mtx(path) -> dict:
+    """A Matrix Market coordinate file as the reference's reader and COO->CSR
+    leave it (cacg_mtx_read): dict(num_rows, num_cols, rowptr, col, val); for a
+    square matrix also 'csr', a problems.CSR."""
+    nr, nc, nnz = ctypes.c_long(), ctypes.c_long(), ctypes.c_long()
+    rp, ci, va = _lp(), _lp(), _dp()
+    rc = lib().cacg_mtx_read(os.fsencode(path), ctypes.byref(nr), ctypes.byref(nc),
+                             ctypes.byref(nnz), ctypes.byref(rp), ctypes.byref(ci), ctypes.byref(va))
+    if rc != 0:
+        raise (OSError if rc == 6 else ValueError)(f"cacg_mtx_read({path}): {STATUS.get(rc, rc)}")
+    try:
+        n, k = nr.value, nnz.value
+        out = {"num_rows": n, "num_cols": nc.value,
+               "rowptr": np.ctypeslib.as_array(rp, (n + 1,)).copy(),
+               "col": np.ctypeslib.as_array(ci, (k,)).copy() if k else np.zeros(0, np.int64),
+               "val": np.ctypeslib.as_array(va, (k,)).copy() if k else np.zeros(0)}
+    finally:
+        for p in (rp, ci, va):
+            lib().cacg_io_free(ctypes.cast(p, _vp))
+    if out["num_rows"] == out["num_cols"]:
+        from .problems import CSR
+        out["csr"] = CSR(out["num_rows"], out["rowptr"], out["col"], out["val"])
+    return out
+
+
+def write_mtx(path, num_rows, num_cols, rowptr, col, val) -> None:
+    rp = np.ascontiguousarray(rowptr, dtype=np.int64)
+    ci = np.ascontiguousarray(col, dtype=np.int64)
+    va = np.ascontiguousarray(val, dtype=np.float64)
+    rc = lib().cacg_mtx_write(os.fsencode(path), num_rows, num_cols, rp.ctypes.data_as(_lp),
+                              ci.ctypes.data_as(_lp), va.ctypes.data_as(_dp))
+    if rc != 0:
+        raise OSError(f"cacg_mtx_write({path}): {STATUS.get(rc, rc)}")
+
+
+def read_csv(path, rows, cols) -> np.ndarray:
+    """rows x cols values as the reference's parseCSV reads them (cacg_csv_read)."""
+    d = _dp()
+    rc = lib().cacg_csv_read(os.fsencode(path), rows, cols, ctypes.byref(d))
+    if rc != 0:
+        raise (OSError if rc == 6 else ValueError)(f"cacg_csv_read({path}): {STATUS.get(rc, rc)}")
+    try:
+        return np.ctypeslib.as_array(d, (rows, cols)).copy()
+    finally:
+        lib().cacg_io_free(ctypes.cast(d, _vp))
+
+
+def write_csv(path, data) -> None:
+    a = np.ascontiguousarray(data, dtype=np.float64)
+    a = a.reshape(a.shape[0], -1)
+    rc = lib().cacg_csv_write(os.fsencode(path), a.shape[0], a.shape[1], a.ctypes.data_as(_dp))
+    if rc != 0:
+        raise OSError(f"cacg_csv_write({path}): {STATUS.get(rc, rc)}")
 
 
 def bcg(A, B, X0, *, stop_mode=STOP_REL_PER_COLUMN, eps=0.0, tol=1e-8, maxiter=1000,

@@ -1,5 +1,6 @@
 """Build cacg_amd/libcacg.so: HIP kernels for gfx950 (hipcc) plus the C11 host
-driver (gcc), linked against the HIP runtime.  Run as `python -m cacg_amd.build`
+driver (gcc), linked against the HIP runtime; and cacg_amd/cacg, the command
+line program on it.  Run as `python -m cacg_amd.build`
 or through __graft_entry__.build().  Always recompiles (a few seconds per
 kernel file, in parallel)."""
 
@@ -15,6 +16,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcacg.so")
+CLI = os.path.join(HERE, "cacg")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
@@ -69,6 +71,11 @@ def build(verbose: bool = False) -> str:
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", *objs, "-o", tmp,
           f"-Wl,-rpath,{ROCM}/lib"])
     os.replace(tmp, LIB)
+    # the command line program of the reference (src/main.c), on libcacg.so
+    tmp = CLI + ".tmp"
+    _run(["gcc", *C_FLAGS, os.path.join(CSRC, "cli", "cacg_main.c"), "-o", tmp, "-L", HERE,
+          "-lcacg", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, CLI)
     return LIB
 
 

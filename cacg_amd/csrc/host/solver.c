@@ -80,6 +80,7 @@ const char *cacg_status_string(int st)
     case CACG_EINVAL: return "invalid argument";
     case CACG_ENOMEM: return "device allocation failed";
     case CACG_EDEVICE: return "HIP runtime error or no gfx950 device";
+    case CACG_EIO: return "file could not be opened, read or written";
     default: return "unknown status";
     }
 }

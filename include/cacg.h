@@ -77,7 +77,8 @@ enum {
     CACG_BREAKDOWN = 2,   /* non-finite residual or zero / non-finite pivot in the coefficient solve */
     CACG_EINVAL = 3,      /* bad argument */
     CACG_ENOMEM = 4,      /* device allocation failed */
-    CACG_EDEVICE = 5      /* HIP runtime error or no usable gfx950 device */
+    CACG_EDEVICE = 5,     /* HIP runtime error or no usable gfx950 device */
+    CACG_EIO = 6          /* a file could not be opened, read or written (include/cacg_io.h) */
 };
 
 int cacg_last_status(void);
