@@ -40,7 +40,12 @@ class Opts(ctypes.Structure):
     _fields_ = [("stop_mode", ctypes.c_int), ("eps", ctypes.c_double), ("tol", ctypes.c_double),
                 ("maxiter", ctypes.c_long), ("order", ctypes.c_int), ("precision", ctypes.c_int),
                 ("print_period", ctypes.c_int), ("inner_tol", ctypes.c_double),
-                ("inner_maxiter", ctypes.c_long), ("refine_restart", ctypes.c_int)]
+                ("inner_maxiter", ctypes.c_long), ("refine_restart", ctypes.c_int),
+                ("bounds", ctypes.c_int), ("lanczos_steps", ctypes.c_int),
+                ("bound_lo", ctypes.c_double), ("bound_hi", ctypes.c_double)]
+
+
+BOUNDS = {"gerschgorin": 0, "lanczos": 1, "given": 2}
 
 
 class Report(ctypes.Structure):
@@ -129,6 +134,9 @@ def lib() -> ctypes.CDLL:
         "cacg_csv_read": ([ctypes.c_char_p, c_long, c_long, ctypes.POINTER(_dp)], c_int),
         "cacg_csv_write": ([ctypes.c_char_p, c_long, c_long, _dp], c_int),
         "cacg_io_free": ([_vp], None),
+        "cacg_lanczos_workspace": ([c_long], c_long),
+        "cacg_k_lanczos": ([ctypes.POINTER(DCsr), c_int, _vp, _vp, _vp], c_int),
+        "cacg_lanczos_interval": ([_dp, _dp, c_int, _dp, _dp], c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -363,6 +371,29 @@ def k_tsqr(A):
     return dQ.download((n, k)), dR.download((k, k)), int(flag.download(1, np.int32)[0])
 
 
+def k_lanczos(A, steps):
+    """(alpha, beta) of `steps` device Lanczos steps on A (cacg_k_lanczos)."""
+    dA = DeviceCSR(A)
+    ws = DeviceBuffer(8 * lib().cacg_lanczos_workspace(A.n))
+    ab = DeviceBuffer(16 * steps)
+    _check(lib().cacg_k_lanczos(ctypes.byref(dA.s), steps, ws.ptr, ab.ptr, None), "lanczos")
+    out = ab.download(2 * steps)
+    return out[:steps].copy(), out[steps:].copy()
+
+
+def lanczos_interval(alpha, beta, gersh):
+    """The interval CACG_BOUNDS_LANCZOS uses (cacg_lanczos_interval)."""
+    a = np.ascontiguousarray(alpha, dtype=np.float64)
+    b = np.ascontiguousarray(beta, dtype=np.float64)
+    g = np.ascontiguousarray(gersh, dtype=np.float64)
+    out = np.zeros(2)
+    rc = lib().cacg_lanczos_interval(a.ctypes.data_as(_dp), b.ctypes.data_as(_dp), len(a),
+                                     g.ctypes.data_as(_dp), out.ctypes.data_as(_dp))
+    if rc != 0:
+        raise ValueError(f"cacg_lanczos_interval: {STATUS.get(rc, rc)}")
+    return out
+
+
 def k_cvt_f32_f64(S):
     S = np.ascontiguousarray(S, dtype=np.float32)
     dS, dQ = DeviceBuffer.from_array(S), DeviceBuffer(8 * S.size)
@@ -387,8 +418,10 @@ PRECISION = {"fp64": 0, "fp32": 1}
 
 def make_opts(stop_mode=STOP_REL_PER_COLUMN, eps=0.0, tol=1e-8, maxiter=100, order="fast",
               print_period=0, precision="fp64", inner_tol=0.0, inner_maxiter=0,
-              refine_restart=0) -> Opts:
-    """precision 'fp32' is the FP32-basis form with FP64 refinement."""
+              refine_restart=0, bounds="gerschgorin", lanczos_steps=20, bound_lo=0.0,
+              bound_hi=0.0) -> Opts:
+    """precision 'fp32' is the FP32-basis form with FP64 refinement; bounds
+    'gerschgorin' (the reference's), 'lanczos' or 'given' ([bound_lo, bound_hi])."""
     o = Opts()
     lib().cacg_default_opts(ctypes.byref(o))
     o.stop_mode = stop_mode
@@ -401,6 +434,10 @@ def make_opts(stop_mode=STOP_REL_PER_COLUMN, eps=0.0, tol=1e-8, maxiter=100, ord
     o.inner_tol = inner_tol
     o.inner_maxiter = inner_maxiter
     o.refine_restart = int(refine_restart)
+    o.bounds = BOUNDS[bounds] if isinstance(bounds, str) else int(bounds)
+    o.lanczos_steps = int(lanczos_steps)
+    o.bound_lo = bound_lo
+    o.bound_hi = bound_hi
     return o
 
 
@@ -421,7 +458,8 @@ def _report_dict(rep: Report, hist: np.ndarray, m: int) -> dict:
 
 def bcbcg(A, B, X0, s, *, stop_mode=STOP_REL_PER_COLUMN, eps=0.0, tol=1e-8, maxiter=100,
           order="fast", print_period=0, precision="fp64", inner_tol=0.0, inner_maxiter=0,
-          refine_restart=0) -> dict:
+          refine_restart=0, bounds="gerschgorin", lanczos_steps=20, bound_lo=0.0,
+          bound_hi=0.0) -> dict:
     """Host-buffer solve through cacg_bcbcg().  Returns dict with X and the report."""
     B = np.ascontiguousarray(B, dtype=np.float64).reshape(A.n, -1)
     m = B.shape[1]
@@ -430,7 +468,7 @@ def bcbcg(A, B, X0, s, *, stop_mode=STOP_REL_PER_COLUMN, eps=0.0, tol=1e-8, maxi
     ci = np.ascontiguousarray(A.col, dtype=np.int64)
     va = np.ascontiguousarray(A.val, dtype=np.float64)
     o = make_opts(stop_mode, eps, tol, maxiter, order, print_period, precision, inner_tol,
-                  inner_maxiter, refine_restart)
+                  inner_maxiter, refine_restart, bounds, lanczos_steps, bound_lo, bound_hi)
     hist = np.zeros((maxiter, m))
     rep = Report()
     rep.res_hist = hist.ctypes.data_as(_dp)

@@ -59,6 +59,10 @@ struct cacg_solver {
     double *d_norm2, *d_bounds, *d_gwork, *d_gram_work, *d_xr_work, *d_lu_work;
     int *d_flag;
     double *h_pin;   /* pinned: [0, m) column norms^2, [m, 2m) ||B(:,c)||^2, [2m, 2m+2) bounds, then the flag */
+    /* CACG_BOUNDS_LANCZOS: work space, coefficients, the interval once computed */
+    double *d_lz_work, *d_lz_ab, *h_lz;
+    int lz_steps, have_lz;
+    double lz_iv[2];
     hipEvent_t ev[MAXEV];
     int ev_phase[MAXEV];
     int nev;
@@ -98,6 +102,81 @@ void cacg_default_opts(cacg_opts *o)
     o->inner_tol = 0.0;
     o->inner_maxiter = 0;
     o->refine_restart = 0;
+    o->bounds = CACG_BOUNDS_GERSCHGORIN;
+    o->lanczos_steps = 20;
+    o->bound_lo = 0.0;
+    o->bound_hi = 0.0;
+}
+
+/* number of eigenvalues of the tridiagonal (a, b) below x (Sturm sequence of
+ * the LDL^T pivots) */
+static int sturm_below(const double *a, const double *b, int k, double x)
+{
+    int c = 0;
+    double d = 1.0;
+    for (int i = 0; i < k; i++) {
+        d = a[i] - x - (i > 0 ? b[i - 1] * b[i - 1] / d : 0.0);
+        if (d == 0.0)
+            d = -1e-300;
+        if (d < 0.0)
+            c++;
+    }
+    return c;
+}
+
+/* the idx-th smallest eigenvalue of the tridiagonal by bisection in [lo, hi] */
+static double tridiag_eig(const double *a, const double *b, int k, int idx, double lo, double hi)
+{
+    for (int it = 0; it < 200; it++) {
+        const double mid = 0.5 * (lo + hi);
+        if (mid <= lo || mid >= hi)
+            break;
+        if (sturm_below(a, b, k, mid) > idx)
+            hi = mid;
+        else
+            lo = mid;
+    }
+    return 0.5 * (lo + hi);
+}
+
+int cacg_lanczos_interval(const double *alpha, const double *beta, int k, const double g[2],
+                          double out[2])
+{
+    if (!alpha || !beta || !g || !out || k <= 0)
+        return CACG_EINVAL;
+    /* cut T at the first negligible beta: the Krylov space is invariant there */
+    int kk = k;
+    for (int j = 0; j < k; j++) {
+        if (!isfinite(alpha[j]) || !isfinite(beta[j]))
+            return CACG_EINVAL;
+        if (j + 1 < k &&
+            !(fabs(beta[j]) > 1e-13 * (fabs(alpha[j]) + (j > 0 ? fabs(beta[j - 1]) : 0.0)))) {
+            kk = j + 1;
+            break;
+        }
+    }
+    double lo = alpha[0], hi = alpha[0];
+    for (int i = 0; i < kk; i++) {
+        const double r = (i > 0 ? fabs(beta[i - 1]) : 0.0) + (i + 1 < kk ? fabs(beta[i]) : 0.0);
+        lo = fmin(lo, alpha[i] - r);
+        hi = fmax(hi, alpha[i] + r);
+    }
+    const double pad = 1e-12 * (fabs(lo) + fabs(hi)) + 1e-300;
+    const double tmin = tridiag_eig(alpha, beta, kk, 0, lo - pad, hi + pad);
+    const double tmax = tridiag_eig(alpha, beta, kk, kk - 1, lo - pad, hi + pad);
+    const double w = tmax - tmin;
+    double a = tmin - 0.05 * w, b = tmax + 0.05 * w;
+    if (a < 0.5 * tmin)
+        a = 0.5 * tmin;
+    if (b > g[1] && g[1] > tmax)
+        b = g[1];
+    if (!(b > a)) { /* a one-point spectrum: any interval around it */
+        a = tmin - 0.5 * fabs(tmin) - 1.0;
+        b = tmax + 0.5 * fabs(tmax) + 1.0;
+    }
+    out[0] = a;
+    out[1] = b;
+    return CACG_OK;
 }
 
 #define HIPCHK(call)                                                                             \
@@ -221,7 +300,7 @@ int cacg_solver_destroy(cacg_solver *sv)
                     sv->d_Q,      sv->d_AQ,    sv->d_QtAQ,  sv->d_QtAS,    sv->d_QtR,
                     sv->d_alpha,  sv->d_beta,  sv->d_norm2, sv->d_bounds,  sv->d_gwork,
                     sv->d_gram_work, sv->d_xr_work, sv->d_lu_work, sv->d_flag,
-                    sv->f_val,    sv->f_S};
+                    sv->f_val,    sv->f_S,     sv->d_lz_work, sv->d_lz_ab};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++)
         if (ptrs[i])
             hipFree(ptrs[i]);
@@ -232,6 +311,7 @@ int cacg_solver_destroy(cacg_solver *sv)
         hipStreamDestroy(sv->st);
     if (sv->h_pin)
         hipHostFree(sv->h_pin);
+    free(sv->h_lz);
     free(sv);
     return CACG_OK;
 }
@@ -269,7 +349,12 @@ int cacg_solver_create(cacg_solver **out, long n, const long *row_start, const l
         if (opts->maxiter <= 0 || (opts->precision != CACG_PREC_FP64 && !p32) ||
             (p32 && opts->order != CACG_ORDER_FAST) || opts->inner_tol < 0.0 ||
             opts->inner_maxiter < 0 || opts->tol < 0.0 ||
-            (opts->refine_restart != 0 && opts->refine_restart != 1))
+            (opts->refine_restart != 0 && opts->refine_restart != 1) || opts->bounds < 0 ||
+            opts->bounds > CACG_BOUNDS_GIVEN || opts->lanczos_steps < 0 ||
+            opts->lanczos_steps > 1000 ||
+            (opts->bounds == CACG_BOUNDS_GIVEN &&
+             !(isfinite(opts->bound_lo) && isfinite(opts->bound_hi) &&
+               opts->bound_lo < opts->bound_hi)))
             return CACG_EINVAL;
     }
     int rc = check_device();
@@ -329,6 +414,17 @@ int cacg_solver_create(cacg_solver **out, long n, const long *row_start, const l
     rc |= dalloc(sv, (void **)&sv->d_xr_work, cacg_xrupdate_workspace((int)m) * (long)sizeof(double));
     rc |= dalloc(sv, (void **)&sv->d_lu_work, sm * sm * (long)sizeof(double));
     rc |= dalloc(sv, (void **)&sv->d_flag, (long)sizeof(int));
+    if (sv->opts.bounds == CACG_BOUNDS_LANCZOS) {
+        int k = sv->opts.lanczos_steps > 0 ? sv->opts.lanczos_steps : 20;
+        if (k > n)
+            k = (int)n;
+        sv->lz_steps = k;
+        rc |= dalloc(sv, (void **)&sv->d_lz_work, cacg_lanczos_workspace(n) * (long)sizeof(double));
+        rc |= dalloc(sv, (void **)&sv->d_lz_ab, 2L * k * (long)sizeof(double));
+        sv->h_lz = (double *)malloc(sizeof(double) * 2 * (size_t)k);
+        if (!sv->h_lz)
+            rc = CACG_ENOMEM;
+    }
     if (rc != CACG_OK) {
         cacg_solver_destroy(sv);
         return CACG_ENOMEM;
@@ -416,6 +512,42 @@ static int gram(cacg_solver *sv, int mode, const double *L, const double *R1, lo
                             sv->st);
 }
 
+/* enqueue what the Chebyshev interval needs beyond Gerschgorin: the Lanczos
+ * coefficients, on a solver's first solve with CACG_BOUNDS_LANCZOS */
+static int interval_enqueue(cacg_solver *sv)
+{
+    if (sv->opts.bounds != CACG_BOUNDS_LANCZOS || sv->have_lz)
+        return 0;
+    int rc = cacg_k_lanczos(&sv->A, sv->lz_steps, sv->d_lz_work, sv->d_lz_ab, sv->st);
+    if (rc)
+        return rc;
+    return (int)hipMemcpyAsync(sv->h_lz, sv->d_lz_ab, sizeof(double) * 2 * (size_t)sv->lz_steps,
+                               hipMemcpyDeviceToHost, sv->st);
+}
+
+/* after the stream synchronised: the interval of this solve from the
+ * Gerschgorin interval g */
+static int interval_finish(cacg_solver *sv, const double g[2], double iv[2])
+{
+    if (sv->opts.bounds == CACG_BOUNDS_GIVEN) {
+        iv[0] = sv->opts.bound_lo;
+        iv[1] = sv->opts.bound_hi;
+    } else if (sv->opts.bounds == CACG_BOUNDS_LANCZOS) {
+        if (!sv->have_lz) {
+            if (cacg_lanczos_interval(sv->h_lz, sv->h_lz + sv->lz_steps, sv->lz_steps, g,
+                                      sv->lz_iv) != CACG_OK)
+                return CACG_BREAKDOWN;
+            sv->have_lz = 1;
+        }
+        iv[0] = sv->lz_iv[0];
+        iv[1] = sv->lz_iv[1];
+    } else {
+        iv[0] = g[0];
+        iv[1] = g[1];
+    }
+    return CACG_OK;
+}
+
 #define SCHK(call)                                                                               \
     do {                                                                                         \
         int e_ = (int)(call);                                                                    \
@@ -458,6 +590,7 @@ int cacg_solver_solve_device(cacg_solver *sv, const double *d_B, double *d_X, ca
     /* setup: bounds, R = B - A X (src/bcbcgKernel.c:106-154) */
     SCHK(cacg_k_fill_i32(sv->d_flag, 1, 0, st));
     SCHK(cacg_k_gerschgorin(&sv->A, sv->d_gwork, sv->d_bounds, st));
+    SCHK(interval_enqueue(sv));
     SCHK(cacg_k_spmm(&sv->A, m, d_X, m, sv->d_R, m, CACG_EPI_RESID, 0.0, 0.0, d_B, m, NULL, 0,
                      NULL, 0, st));
     if (o->stop_mode == CACG_STOP_REL_PER_COLUMN) {
@@ -476,13 +609,18 @@ int cacg_solver_solve_device(cacg_solver *sv, const double *d_B, double *d_X, ca
     SCHK(hipMemcpyAsync(h_bounds, sv->d_bounds, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
     SCHK(hipStreamSynchronize(st));
     harvest(sv, rep);
+    double iv[2];
+    status = interval_finish(sv, h_bounds, iv);
+    if (status != CACG_OK)
+        goto out;
+    status = CACG_MAXITER;
     if (rep) {
-        rep->bounds[0] = h_bounds[0];
-        rep->bounds[1] = h_bounds[1];
+        rep->bounds[0] = iv[0];
+        rep->bounds[1] = iv[1];
     }
     /* src/bcbcgKernel.c:113-115 */
-    const double sa = 2.0 / (h_bounds[1] - h_bounds[0]);
-    const double sb = -(h_bounds[1] + h_bounds[0]) / (h_bounds[1] - h_bounds[0]);
+    const double sa = 2.0 / (iv[1] - iv[0]);
+    const double sb = -(iv[1] + iv[0]) / (iv[1] - iv[0]);
     const double sa2 = 2.0 * sa, sb2 = 2.0 * sb;
 
     for (it = 0; it < o->maxiter; it++) {
@@ -755,18 +893,24 @@ static int solve_refine(cacg_solver *sv, const double *d_B, double *d_X, cacg_re
     mark(sv, CACG_PHASE_OTHER);
     SCHK(cacg_k_fill_i32(sv->d_flag, 1, 0, st));
     SCHK(cacg_k_gerschgorin(&sv->A, sv->d_gwork, sv->d_bounds, st));
+    SCHK(interval_enqueue(sv));
     SCHK(cacg_k_colnorm2_fast(n, m, d_B, sv->d_norm2, sv->d_xr_work, st));
     SCHK(hipMemcpyAsync(h_bnorm2, sv->d_norm2, sizeof(double) * m, hipMemcpyDeviceToHost, st));
     SCHK(hipMemcpyAsync(h_bounds, sv->d_bounds, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
     mark(sv, CACG_PHASE_SETUP);
     SCHK(hipStreamSynchronize(st));
     harvest(sv, rep);
+    double iv[2];
+    status = interval_finish(sv, h_bounds, iv);
+    if (status != CACG_OK)
+        goto out;
+    status = CACG_MAXITER;
     if (rep) {
-        rep->bounds[0] = h_bounds[0];
-        rep->bounds[1] = h_bounds[1];
+        rep->bounds[0] = iv[0];
+        rep->bounds[1] = iv[1];
     }
-    const double sa = 2.0 / (h_bounds[1] - h_bounds[0]);
-    const double sb = -(h_bounds[1] + h_bounds[0]) / (h_bounds[1] - h_bounds[0]);
+    const double sa = 2.0 / (iv[1] - iv[0]);
+    const double sb = -(iv[1] + iv[0]) / (iv[1] - iv[0]);
     const double sa2 = 2.0 * sa, sb2 = 2.0 * sb;
     for (int c = 0; c < m; c++)
         fin[c] = o->stop_mode == CACG_STOP_REF_COL0_ABS ? (c == 0 ? o->eps : 0.0)

@@ -101,6 +101,13 @@ enum { CACG_ORDER_REFERENCE = 0, CACG_ORDER_FAST = 1 };
  * cacg_opts.inner_tol) and the stop test is on it.  No reference counterpart
  * (the reference is FP64 only); requires CACG_ORDER_FAST. */
 enum { CACG_PREC_FP64 = 0, CACG_PREC_FP32_REFINE = 1 };
+/* The Chebyshev interval [lo, hi] of the basis recurrence.
+ * CACG_BOUNDS_GERSCHGORIN: the reference's Gerschgorin discs (src/common.c:53-106).
+ * CACG_BOUNDS_LANCZOS: extreme Ritz values of lanczos_steps Lanczos steps on the
+ * device (no reference counterpart), widened as cacg_lanczos_interval states;
+ * computed once per solver and reused by its later solves.
+ * CACG_BOUNDS_GIVEN: [bound_lo, bound_hi] from the caller. */
+enum { CACG_BOUNDS_GERSCHGORIN = 0, CACG_BOUNDS_LANCZOS = 1, CACG_BOUNDS_GIVEN = 2 };
 
 typedef struct {
     int stop_mode;       /* CACG_STOP_* */
@@ -123,6 +130,9 @@ typedef struct {
     double inner_tol;
     long inner_maxiter;
     int refine_restart;
+    int bounds;          /* CACG_BOUNDS_* (default GERSCHGORIN, the reference's) */
+    int lanczos_steps;   /* CACG_BOUNDS_LANCZOS: 1..1000 steps; 0 selects 20 */
+    double bound_lo, bound_hi;  /* CACG_BOUNDS_GIVEN: the interval, lo < hi */
 } cacg_opts;
 
 typedef struct {
@@ -156,6 +166,16 @@ enum {
 };
 
 void cacg_default_opts(cacg_opts *o);
+
+/* The interval CACG_BOUNDS_LANCZOS uses, from the Lanczos coefficients
+ * (alpha_0..k-1, beta_0..k-1) and the Gerschgorin interval g: the extreme
+ * eigenvalues t_min <= t_max of the tridiagonal T (Sturm bisection; T is cut
+ * at the first negligible beta), widened to lo = t_min - 0.05 (t_max - t_min)
+ * clipped at (t_min / 2, ...) and hi = t_max + 0.05 (t_max - t_min) clipped at
+ * g[1], the Gerschgorin upper bound.  Host code; returns CACG_EINVAL for
+ * k <= 0 or non-finite coefficients. */
+int cacg_lanczos_interval(const double *alpha, const double *beta, int k, const double g[2],
+                          double out[2]);
 
 /* Host-buffer solve: A from a csrType_local-like triple (row_start/col_idx of
  * n + 1 / nnz longs, values), B and X n x m row-major; X in/out.  s >= 1. */

@@ -125,6 +125,14 @@ int cacg_k_colnorm2_fast(int n, int m, const double *R, double *d_norm2, double 
 /* d[0:count] = v, as a kernel (safe inside captured graphs, unlike a memset node) */
 int cacg_k_fill_i32(int *d, long count, int v, hipStream_t st);
 int cacg_k_fill_f64(double *d, long count, double v, hipStream_t st);
+/* Lanczos coefficients for the Chebyshev interval: `steps` steps of the
+ * symmetric Lanczos recurrence on A from a fixed start vector (no reference
+ * counterpart; the reference uses Gerschgorin discs).  d_ab receives
+ * alpha_0..steps-1 then beta_0..steps-1; d_work holds
+ * cacg_lanczos_workspace(n) doubles. */
+long cacg_lanczos_workspace(long n);
+int cacg_k_lanczos(const cacg_dcsr *A, int steps, double *d_work, double *d_ab, hipStream_t st);
+
 /* d[0:count] *= a in place (a = -1: exact negation) */
 int cacg_k_scale_f64(double *d, long count, double a, hipStream_t st);
 

@@ -90,11 +90,13 @@ OPTS_C = r"""
 #include <stdio.h>
 #include "cacg.h"
 int main(void) {
-  printf("opts %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(cacg_opts),
+  printf("opts %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(cacg_opts),
          offsetof(cacg_opts, stop_mode), offsetof(cacg_opts, eps), offsetof(cacg_opts, tol),
          offsetof(cacg_opts, maxiter), offsetof(cacg_opts, order), offsetof(cacg_opts, precision),
          offsetof(cacg_opts, print_period), offsetof(cacg_opts, inner_tol),
-         offsetof(cacg_opts, inner_maxiter), offsetof(cacg_opts, refine_restart));
+         offsetof(cacg_opts, inner_maxiter), offsetof(cacg_opts, refine_restart),
+         offsetof(cacg_opts, bounds), offsetof(cacg_opts, lanczos_steps),
+         offsetof(cacg_opts, bound_lo), offsetof(cacg_opts, bound_hi));
   printf("report %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(cacg_report),
          offsetof(cacg_report, status), offsetof(cacg_report, iters), offsetof(cacg_report, bounds),
          offsetof(cacg_report, res_hist), offsetof(cacg_report, res_hist_cap),
@@ -184,7 +186,10 @@ def test_host_api_rejects_bad_arguments_without_device():
 
 @pytest.mark.parametrize("bad", [dict(precision="fp32", order="reference"), dict(precision=7),
                                  dict(precision="fp32", inner_tol=-1.0), dict(maxiter=0),
-                                 dict(precision="fp32", refine_restart=2)])
+                                 dict(precision="fp32", refine_restart=2), dict(bounds=3),
+                                 dict(bounds="given", bound_lo=1.0, bound_hi=1.0),
+                                 dict(bounds="lanczos", lanczos_steps=-1),
+                                 dict(bounds="lanczos", lanczos_steps=1001)])
 def test_options_rejected_without_device(bad):
     """Invalid options are refused (CACG_EINVAL) before any device is touched:
     the FP32-basis form exists in the fast order only."""
