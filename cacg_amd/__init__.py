@@ -101,6 +101,8 @@ def lib() -> ctypes.CDLL:
         "cacg_k_gerschgorin": ([ctypes.POINTER(DCsr), _vp, _vp, _vp], c_int),
         "cacg_k_spmm": ([ctypes.POINTER(DCsr), c_int, _vp, c_long, _vp, c_long, c_int, c_double,
                          c_double, _vp, c_long, _vp, c_long, _vp, c_long, _vp], c_int),
+        "cacg_k_spmm_x": ([ctypes.POINTER(DCsr), c_int, _vp, c_long, _vp, c_long, c_int, c_double,
+                           c_double, _vp, c_long, _vp, c_long, _vp, c_long, _vp, _vp], c_int),
         "cacg_k_gram_ref": ([c_int, _vp, c_long, c_int, _vp, c_long, c_int, _vp, c_long, c_int,
                              _vp, _vp, _vp], c_int),
         "cacg_gram_workspace": ([c_int, c_int], c_long),
@@ -223,8 +225,13 @@ def k_gerschgorin(A) -> np.ndarray:
     return out.download(2)
 
 
-def k_spmm(A, X, epi="none", a=0.0, b=0.0, p1=None, p2=None, want_copy0=False):
-    """Y = epilogue(A X) for X n x k; returns Y (and copy0 for 'cheb1')."""
+class SpmmSched(ctypes.Structure):
+    _fields_ = [("grid", ctypes.c_int), ("chunk_rows", ctypes.c_int)]
+
+
+def k_spmm(A, X, epi="none", a=0.0, b=0.0, p1=None, p2=None, want_copy0=False, sched=None):
+    """Y = epilogue(A X) for X n x k; returns Y (and copy0 for 'cheb1').
+    sched = (grid, chunk_rows) selects a launch schedule (cacg_k_spmm_x)."""
     dA = DeviceCSR(A)
     X = np.ascontiguousarray(X, dtype=np.float64).reshape(A.n, -1)
     k = X.shape[1]
@@ -233,9 +240,10 @@ def k_spmm(A, X, epi="none", a=0.0, b=0.0, p1=None, p2=None, want_copy0=False):
     d1 = DeviceBuffer.from_array(np.ascontiguousarray(p1, dtype=np.float64)) if p1 is not None else None
     d2 = DeviceBuffer.from_array(np.ascontiguousarray(p2, dtype=np.float64)) if p2 is not None else None
     dC = DeviceBuffer(X.nbytes) if epi == "cheb1" else None
-    rc = lib().cacg_k_spmm(ctypes.byref(dA.s), k, dX.ptr, k, dY.ptr, k, EPI[epi], a, b,
-                           d1.ptr if d1 else None, k, d2.ptr if d2 else None, k,
-                           dC.ptr if dC else None, k, None)
+    sc = ctypes.byref(SpmmSched(*sched)) if sched is not None else None
+    rc = lib().cacg_k_spmm_x(ctypes.byref(dA.s), k, dX.ptr, k, dY.ptr, k, EPI[epi], a, b,
+                             d1.ptr if d1 else None, k, d2.ptr if d2 else None, k,
+                             dC.ptr if dC else None, k, sc, None)
     _check(rc, "spmm")
     Y = dY.download((A.n, k))
     if want_copy0:

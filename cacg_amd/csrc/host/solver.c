@@ -59,6 +59,11 @@ struct cacg_solver {
     double *d_norm2, *d_bounds, *d_gwork, *d_gram_work, *d_xr_work, *d_lu_work;
     int *d_flag;
     double *h_pin;   /* pinned: [0, m) column norms^2, [m, 2m) ||B(:,c)||^2, [2m, 2m+2) bounds, then the flag */
+    /* SpMM schedules of the basis steps and of A Q, picked by timing on the
+     * first solve (tuned = 1) */
+    cacg_spmm_sched sch_basis, sch_aq;
+    int tuned;
+    hipEvent_t tev[2];
     /* CACG_BOUNDS_LANCZOS: work space, coefficients, the interval once computed */
     double *d_lz_work, *d_lz_ab, *h_lz;
     int lz_steps, have_lz;
@@ -307,6 +312,9 @@ int cacg_solver_destroy(cacg_solver *sv)
     for (int i = 0; i < MAXEV; i++)
         if (sv->ev[i])
             hipEventDestroy(sv->ev[i]);
+    for (int i = 0; i < 2; i++)
+        if (sv->tev[i])
+            hipEventDestroy(sv->tev[i]);
     if (sv->st)
         hipStreamDestroy(sv->st);
     if (sv->h_pin)
@@ -384,6 +392,11 @@ int cacg_solver_create(cacg_solver **out, long n, const long *row_start, const l
     }
     for (int i = 0; i < MAXEV; i++)
         if (hipEventCreate(&sv->ev[i]) != hipSuccess) {
+            cacg_solver_destroy(sv);
+            return CACG_EDEVICE;
+        }
+    for (int i = 0; i < 2; i++)
+        if (hipEventCreate(&sv->tev[i]) != hipSuccess) {
             cacg_solver_destroy(sv);
             return CACG_EDEVICE;
         }
@@ -512,6 +525,82 @@ static int gram(cacg_solver *sv, int mode, const double *L, const double *R1, lo
                             sv->st);
 }
 
+/* ---- SpMM schedule autotuner ------------------------------------------------------- */
+
+#define TUNE_MIN_ROWS (1L << 17) /* below this the launches are latency-bound: defaults */
+
+/* device time of `reps` launches of one SpMM shape under schedule sc, after
+ * one untimed launch; a negative value when a launch failed */
+static float time_spmm(cacg_solver *sv, int f32, int k, const void *x, void *y, const void *p1,
+                       const void *p2, int epi, const cacg_spmm_sched *sc, int reps)
+{
+    const long ld = sv->sm;
+    float ms = -1.f;
+    for (int r = -1; r < reps; r++) {
+        if (r == 0 && hipEventRecord(sv->tev[0], sv->st) != hipSuccess)
+            return -1.f;
+        int rc = f32 ? cacg_k_spmm_f32_x(&sv->A32, k, (const float *)x, ld, (float *)y, ld, epi,
+                                         0.5, 0.25, (const float *)p1, ld, (const float *)p2, ld,
+                                         sc, sv->st)
+                     : cacg_k_spmm_x(&sv->A, k, (const double *)x, ld, (double *)y, ld, epi, 0.5,
+                                     0.25, (const double *)p1, ld, (const double *)p2, ld, NULL, 0,
+                                     sc, sv->st);
+        if (rc)
+            return -1.f;
+    }
+    if (hipEventRecord(sv->tev[1], sv->st) != hipSuccess ||
+        hipEventSynchronize(sv->tev[1]) != hipSuccess ||
+        hipEventElapsedTime(&ms, sv->tev[0], sv->tev[1]) != hipSuccess)
+        return -1.f;
+    return ms;
+}
+
+/* Pick the tile order and grid of the two SpMM shapes a solve runs (a basis
+ * step: k = m columns of the n x sm basis with the Chebyshev epilogue; A Q:
+ * k = sm) by timing a few schedules on this matrix, once per solver.  The
+ * work blocks serve as scratch (zero-filled first; the solve overwrites
+ * them).  A schedule changes no result, so neither does the choice. */
+static int tune_spmm(cacg_solver *sv)
+{
+    static const cacg_spmm_sched cand[] = {{0, 0},    {4096, 0},    {8192, 0},   {0, 1024},
+                                           {0, 2048}, {0, 4096},    {4096, 2048}, {8192, 4096}};
+    const int nc = (int)(sizeof(cand) / sizeof(cand[0]));
+    sv->tuned = 1;
+    if (sv->n < TUNE_MIN_ROWS)
+        return 0;
+    const int m = (int)sv->m, sm = (int)sv->sm;
+    const int f32 = sv->opts.precision == CACG_PREC_FP32_REFINE;
+    const long nsm = sv->n * sv->sm;
+    int rc = cacg_k_fill_f64(sv->d_Q, nsm, 0.0, sv->st);
+    rc = rc ? rc : cacg_k_fill_f64(sv->d_AQ, nsm, 0.0, sv->st);
+    if (!rc && f32)
+        rc = (int)hipMemsetAsync(sv->f_S, 0, sizeof(float) * (size_t)nsm, sv->st);
+    else if (!rc)
+        rc = cacg_k_fill_f64(sv->d_S, nsm, 0.0, sv->st);
+    if (rc)
+        return rc;
+    /* basis step: x, p1 one block of S, p2 another, y a block of Q (FP64) or of the
+     * FP32 basis (the precision form) */
+    const void *bx = f32 ? (const void *)sv->f_S : (const void *)sv->d_S;
+    const void *bp2 = f32 ? (const void *)(sv->f_S + (sm > m ? m : 0)) : (const void *)sv->d_AQ;
+    void *by = f32 ? (void *)(sv->f_S + (sm > 2 * m ? 2 * m : 0)) : (void *)sv->d_Q;
+    float best_b = -1.f, best_a = -1.f;
+    for (int c = 0; c < nc; c++) {
+        const float tb = time_spmm(sv, f32, m, bx, by, bx, bp2, CACG_EPI_CHEB, &cand[c], 2);
+        if (tb >= 0.f && (best_b < 0.f || tb < best_b)) {
+            best_b = tb;
+            sv->sch_basis = cand[c];
+        }
+        const float ta = time_spmm(sv, 0, sm, sv->d_Q, sv->d_AQ, NULL, NULL, CACG_EPI_NONE,
+                                   &cand[c], 2);
+        if (ta >= 0.f && (best_a < 0.f || ta < best_a)) {
+            best_a = ta;
+            sv->sch_aq = cand[c];
+        }
+    }
+    return (best_b < 0.f || best_a < 0.f) ? (int)hipErrorLaunchFailure : 0;
+}
+
 /* enqueue what the Chebyshev interval needs beyond Gerschgorin: the Lanczos
  * coefficients, on a solver's first solve with CACG_BOUNDS_LANCZOS */
 static int interval_enqueue(cacg_solver *sv)
@@ -585,6 +674,8 @@ int cacg_solver_solve_device(cacg_solver *sv, const double *d_B, double *d_X, ca
         rep->true_relres = -1.0;
     }
 
+    if (!sv->tuned)
+        SCHK(tune_spmm(sv));
     sv->nev = 0;
     mark(sv, CACG_PHASE_OTHER);
     /* setup: bounds, R = B - A X (src/bcbcgKernel.c:106-154) */
@@ -629,13 +720,13 @@ int cacg_solver_solve_device(cacg_solver *sv, const double *d_B, double *d_X, ca
         if (s == 1) {
             SCHK(hipMemcpyAsync(S, sv->d_R, sizeof(double) * (size_t)n * m, hipMemcpyDeviceToDevice, st));
         } else {
-            SCHK(cacg_k_spmm(&sv->A, m, sv->d_R, m, S + m, sm, CACG_EPI_CHEB1, sa, sb, NULL, 0, NULL,
-                             0, S, sm, st));
+            SCHK(cacg_k_spmm_x(&sv->A, m, sv->d_R, m, S + m, sm, CACG_EPI_CHEB1, sa, sb, NULL, 0,
+                               NULL, 0, S, sm, &sv->sch_basis, st));
             mark(sv, CACG_PHASE_BASIS1);
             for (int j = 2; j < s; j++)
-                SCHK(cacg_k_spmm(&sv->A, m, S + (long)(j - 1) * m, sm, S + (long)j * m, sm,
-                                 CACG_EPI_CHEB, sa2, sb2, S + (long)(j - 1) * m, sm,
-                                 S + (long)(j - 2) * m, sm, NULL, 0, st));
+                SCHK(cacg_k_spmm_x(&sv->A, m, S + (long)(j - 1) * m, sm, S + (long)j * m, sm,
+                                   CACG_EPI_CHEB, sa2, sb2, S + (long)(j - 1) * m, sm,
+                                   S + (long)(j - 2) * m, sm, NULL, 0, &sv->sch_basis, st));
         }
         if (s != 2)
             mark(sv, s == 1 ? CACG_PHASE_BASIS1 : CACG_PHASE_BASIS);
@@ -653,8 +744,8 @@ int cacg_solver_solve_device(cacg_solver *sv, const double *d_B, double *d_X, ca
             SCHK(cacg_k_qupdate(n, sm, S, Q, sv->d_beta, mode, st));
             mark(sv, CACG_PHASE_QUPDATE);
         }
-        SCHK(cacg_k_spmm(&sv->A, sm, Q, sm, sv->d_AQ, sm, CACG_EPI_NONE, 0.0, 0.0, NULL, 0, NULL, 0,
-                         NULL, 0, st));
+        SCHK(cacg_k_spmm_x(&sv->A, sm, Q, sm, sv->d_AQ, sm, CACG_EPI_NONE, 0.0, 0.0, NULL, 0, NULL,
+                           0, NULL, 0, &sv->sch_aq, st));
         mark(sv, CACG_PHASE_AQ);
         SCHK(gram(sv, mode, Q, sv->d_AQ, sm, sm, sv->d_R, m, m, sv->d_QtAQ, sv->d_QtR));
         mark(sv, CACG_PHASE_GRAM);
@@ -889,6 +980,8 @@ static int solve_refine(cacg_solver *sv, const double *d_B, double *d_X, cacg_re
         rep->true_relres = -1.0;
     }
 
+    if (!sv->tuned)
+        SCHK(tune_spmm(sv));
     sv->nev = 0;
     mark(sv, CACG_PHASE_OTHER);
     SCHK(cacg_k_fill_i32(sv->d_flag, 1, 0, st));
@@ -959,12 +1052,14 @@ static int solve_refine(cacg_solver *sv, const double *d_B, double *d_X, cacg_re
             if (s == 1) {
                 SCHK(cacg_k_cvt_f64_f32((long)n * m, sv->d_R, S, st));
             } else {
-                SCHK(cacg_k_cheb1_f32(&sv->A32, m, sv->d_R, m, S + m, sm, S, sm, sa, sb, st));
+                SCHK(cacg_k_cheb1_f32_x(&sv->A32, m, sv->d_R, m, S + m, sm, S, sm, sa, sb,
+                                        &sv->sch_basis, st));
                 mark(sv, CACG_PHASE_BASIS1);
                 for (int j = 2; j < s; j++)
-                    SCHK(cacg_k_spmm_f32(&sv->A32, m, S + (long)(j - 1) * m, sm, S + (long)j * m,
-                                         sm, CACG_EPI_CHEB, sa2, sb2, S + (long)(j - 1) * m, sm,
-                                         S + (long)(j - 2) * m, sm, st));
+                    SCHK(cacg_k_spmm_f32_x(&sv->A32, m, S + (long)(j - 1) * m, sm,
+                                           S + (long)j * m, sm, CACG_EPI_CHEB, sa2, sb2,
+                                           S + (long)(j - 1) * m, sm, S + (long)(j - 2) * m, sm,
+                                           &sv->sch_basis, st));
             }
             if (s != 2)
                 mark(sv, s == 1 ? CACG_PHASE_BASIS1 : CACG_PHASE_BASIS);
@@ -981,8 +1076,8 @@ static int solve_refine(cacg_solver *sv, const double *d_B, double *d_X, cacg_re
                 SCHK(cacg_k_qupdate_s32(n, sm, S, Q, sv->d_beta, st));
             }
             mark(sv, CACG_PHASE_QUPDATE);
-            SCHK(cacg_k_spmm(&sv->A, sm, Q, sm, AQ, sm, CACG_EPI_NONE, 0.0, 0.0, NULL, 0, NULL, 0,
-                             NULL, 0, st));
+            SCHK(cacg_k_spmm_x(&sv->A, sm, Q, sm, AQ, sm, CACG_EPI_NONE, 0.0, 0.0, NULL, 0, NULL,
+                               0, NULL, 0, &sv->sch_aq, st));
             mark(sv, CACG_PHASE_AQ);
             SCHK(cacg_k_gram_fast(n, Q, sm, sm, AQ, sm, sm, sv->d_R, m, m, sv->d_QtAQ, sv->d_QtR,
                                   sv->d_gram_work, st));

@@ -13,13 +13,20 @@
 // issued before the in-order accumulation, for memory-level parallelism.
 //
 // Tile order: hardware workgroup ids are dealt round-robin over the 8 XCDs,
-// so workgroup h runs on XCD h % 8.  Each XCD owns a contiguous band of row
-// tiles, and its workgroups take the band's tiles interleaved (workgroup j of
-// the XCD: tiles j, j + Gx, j + 2 Gx, ...), so the XCD sweeps its band as one
-// compact front: the rows that gather the same x rows (neighbours i +- 1,
-// i +- a grid line, i +- a grid plane for a stencil matrix) run close together
-// in time under the same L2.  Operands used once (the matrix, S_{j-2}, the
-// outputs) are loaded and stored non-temporally so they do not evict the
+// so workgroup h runs on XCD h % 8.  Two orders, chosen per launch:
+//   bands (P = 0): each XCD owns a contiguous band of row tiles, and its
+//     workgroups take the band's tiles interleaved (workgroup j of the XCD:
+//     tiles j, j + Gx, j + 2 Gx, ...), so the XCD sweeps its band as one
+//     compact front: rows that gather the same x rows (neighbours i +- 1,
+//     i +- a grid line) run close together in time under the same L2;
+//   chunks (P > 0): chunks of P tiles are dealt round-robin over the XCDs and
+//     each XCD sweeps its chunks in order, so an XCD owns a slice of every
+//     grid plane and the x rows a plane away (i +- a grid plane) are gathered
+//     again under the same L2 one chunk later, instead of a plane later.
+// The order changes no result (each row is summed by one lane group, in CSR
+// order); which is faster depends on the matrix, so the solver times both on
+// its own matrix (cacg_spmm_sched).  Operands used once (the matrix, S_{j-2},
+// the outputs) are loaded and stored non-temporally so they do not evict the
 // gathered rows from L2.
 #include <float.h>
 #include <stdint.h>
@@ -82,7 +89,7 @@ __global__ __launch_bounds__(256) void spmm_kernel(
     int n, const int *__restrict__ rowptr, const int *__restrict__ colidx,
     const TV *__restrict__ val, int k, const TX *__restrict__ x, long ldx, TY *__restrict__ y,
     long ldy, double ad, double bd, const TY *__restrict__ p1, long ld1, const TY *__restrict__ p2,
-    long ld2, TY *__restrict__ copy0, long ldc)
+    long ld2, TY *__restrict__ copy0, long ldc, int P)
 {
     constexpr int NG = 256 / LPR, TR = NG * RPG;
     constexpr int CAP = TR * 8 > 256 ? TR * 8 : 256;
@@ -93,13 +100,24 @@ __global__ __launch_bounds__(256) void spmm_kernel(
     const TV a = (TV)ad, b = (TV)bd;
     const int t = threadIdx.x, g = t / LPR, lane = t % LPR;
     const int ntiles = (n + TR - 1) / TR;
-    // band of this workgroup's XCD and the workgroup's slot in it
+    // this workgroup's XCD and its slot among the XCD's workgroups
     const int nx = gridDim.x >= 8 ? 8 : 1;
     const int xcd = blockIdx.x % nx, gx = gridDim.x / nx, j = blockIdx.x / nx;
     const int band = (ntiles + nx - 1) / nx;
-    const int tb = xcd * band;
-    const int te = (tb + band) < ntiles ? (tb + band) : ntiles;
-    for (int tile = tb + j; tile < te; tile += gx) {
+    // the XCD's tiles in sweep order: lt-th of its band, or of its chunks;
+    // tile ids grow with lt in both orders, so the first one past the end ends
+    // the loop
+    for (int lt = j;; lt += gx) {
+        int tile;
+        if (P == 0) {
+            if (lt >= band)
+                break;
+            tile = xcd * band + lt;
+        } else {
+            tile = ((lt / P) * nx + xcd) * P + lt % P;
+        }
+        if (tile >= ntiles)
+            break;
         const int r0 = tile * TR;
         const int rows = (n - r0) < TR ? (n - r0) : TR;
         for (int e = t; e <= rows; e += 256)
@@ -186,19 +204,24 @@ __global__ __launch_bounds__(256) void spmm_kernel(
 template <typename TV, typename TX, typename TY, int LPR, int VW>
 int launch_spmm(int n, const int *rowptr, const int *col, const TV *val, int k, const TX *x,
                 long ldx, TY *y, long ldy, int epi, double a, double b, const TY *p1, long ld1,
-                const TY *p2, long ld2, TY *copy0, long ldc, hipStream_t st)
+                const TY *p2, long ld2, TY *copy0, long ldc, const cacg_spmm_sched *sc,
+                hipStream_t st)
 {
     constexpr int RPG = LPR >= 16 ? 2 : 1;
     constexpr int TR = (256 / LPR) * RPG;
-    const int gmax = LPR >= 16 ? 4096 : 8192;
+    const int gmax = sc && sc->grid > 0 ? sc->grid : (LPR >= 16 ? 4096 : 8192);
     const int ntiles = (n + TR - 1) / TR;
-    // a multiple of 8 so every XCD gets the same number of workgroups for its band
+    // a multiple of 8 so every XCD gets the same number of workgroups
     int G = ntiles < gmax ? ntiles : gmax;
     if (G >= 8)
         G &= ~7;
+    // chunk order: chunk_rows rounded to whole tiles
+    int P = 0;
+    if (sc && sc->chunk_rows > 0)
+        P = sc->chunk_rows >= TR ? sc->chunk_rows / TR : 1;
 #define CACG_SPMM_CASE(E)                                                                        \
     hipLaunchKernelGGL((spmm_kernel<TV, TX, TY, LPR, VW, RPG, E>), dim3(G), dim3(256), 0, st, n,  \
-                       rowptr, col, val, k, x, ldx, y, ldy, a, b, p1, ld1, p2, ld2, copy0, ldc)
+                       rowptr, col, val, k, x, ldx, y, ldy, a, b, p1, ld1, p2, ld2, copy0, ldc, P)
     constexpr bool f64 = sizeof(TV) == 8 && sizeof(TX) == 8 && sizeof(TY) == 8;
     constexpr bool f32 = sizeof(TV) == 4 && sizeof(TX) == 4 && sizeof(TY) == 4;
     if constexpr (f64) {
@@ -235,9 +258,10 @@ int launch_spmm(int n, const int *rowptr, const int *col, const TV *val, int k, 
 template <typename TV, typename TX, typename TY>
 int dispatch_spmm(int n, const int *rowptr, const int *col, const TV *val, int k, const TX *x,
                   long ldx, TY *y, long ldy, int epi, double a, double b, const TY *p1, long ld1,
-                  const TY *p2, long ld2, TY *copy0, long ldc, hipStream_t st)
+                  const TY *p2, long ld2, TY *copy0, long ldc, const cacg_spmm_sched *sc,
+                  hipStream_t st)
 {
-#define CACG_SPMM_ARGS n, rowptr, col, val, k, x, ldx, y, ldy, epi, a, b, p1, ld1, p2, ld2, copy0, ldc, st
+#define CACG_SPMM_ARGS n, rowptr, col, val, k, x, ldx, y, ldy, epi, a, b, p1, ld1, p2, ld2, copy0, ldc, sc, st
     const bool vec4 = k % 4 == 0 && ((uintptr_t)x & 15) == 0 && (ldx * (long)sizeof(TX)) % 16 == 0;
     if (vec4) {
         // LPR = k / 4 rounded up to a power of two, at most 32 (wider k loops)
@@ -337,40 +361,66 @@ __global__ __launch_bounds__(256) void gersh_final(int nblk, const double *__res
 
 }  // namespace
 
-extern "C" int cacg_k_spmm(const cacg_dcsr *A, int k, const double *x, long ldx, double *y,
-                           long ldy, int epi, double a, double b, const double *p1, long ld1,
-                           const double *p2, long ld2, double *copy0, long ldc, hipStream_t st)
+extern "C" int cacg_k_spmm_x(const cacg_dcsr *A, int k, const double *x, long ldx, double *y,
+                             long ldy, int epi, double a, double b, const double *p1, long ld1,
+                             const double *p2, long ld2, double *copy0, long ldc,
+                             const cacg_spmm_sched *sc, hipStream_t st)
 {
-    if (!A || k <= 0 || A->n < 0)
+    if (!A || k <= 0 || A->n < 0 || (sc && (sc->grid < 0 || sc->chunk_rows < 0)))
         return (int)hipErrorInvalidValue;
     if (A->n == 0)
         return 0;
     return dispatch_spmm<double, double, double>(A->n, A->rowptr, A->col, A->val, k, x, ldx, y,
-                                                 ldy, epi, a, b, p1, ld1, p2, ld2, copy0, ldc, st);
+                                                 ldy, epi, a, b, p1, ld1, p2, ld2, copy0, ldc, sc,
+                                                 st);
+}
+
+extern "C" int cacg_k_spmm(const cacg_dcsr *A, int k, const double *x, long ldx, double *y,
+                           long ldy, int epi, double a, double b, const double *p1, long ld1,
+                           const double *p2, long ld2, double *copy0, long ldc, hipStream_t st)
+{
+    return cacg_k_spmm_x(A, k, x, ldx, y, ldy, epi, a, b, p1, ld1, p2, ld2, copy0, ldc, nullptr,
+                         st);
+}
+
+extern "C" int cacg_k_spmm_f32_x(const cacg_scsr *A, int k, const float *x, long ldx, float *y,
+                                 long ldy, int epi, double a, double b, const float *p1, long ld1,
+                                 const float *p2, long ld2, const cacg_spmm_sched *sc,
+                                 hipStream_t st)
+{
+    if (!A || k <= 0 || A->n < 0 || (epi != CACG_EPI_NONE && epi != CACG_EPI_CHEB) ||
+        (sc && (sc->grid < 0 || sc->chunk_rows < 0)))
+        return (int)hipErrorInvalidValue;
+    if (A->n == 0)
+        return 0;
+    return dispatch_spmm<float, float, float>(A->n, A->rowptr, A->col, A->val, k, x, ldx, y, ldy,
+                                              epi, a, b, p1, ld1, p2, ld2, NULL, 0, sc, st);
 }
 
 extern "C" int cacg_k_spmm_f32(const cacg_scsr *A, int k, const float *x, long ldx, float *y,
                                long ldy, int epi, double a, double b, const float *p1, long ld1,
                                const float *p2, long ld2, hipStream_t st)
 {
-    if (!A || k <= 0 || A->n < 0 || (epi != CACG_EPI_NONE && epi != CACG_EPI_CHEB))
-        return (int)hipErrorInvalidValue;
-    if (A->n == 0)
-        return 0;
-    return dispatch_spmm<float, float, float>(A->n, A->rowptr, A->col, A->val, k, x, ldx, y, ldy,
-                                              epi, a, b, p1, ld1, p2, ld2, NULL, 0, st);
+    return cacg_k_spmm_f32_x(A, k, x, ldx, y, ldy, epi, a, b, p1, ld1, p2, ld2, nullptr, st);
 }
 
-extern "C" int cacg_k_cheb1_f32(const cacg_scsr *A, int k, const double *r, long ldr, float *s1,
-                                long ld1, float *s0, long ld0, double a, double b, hipStream_t st)
+extern "C" int cacg_k_cheb1_f32_x(const cacg_scsr *A, int k, const double *r, long ldr,
+                                  float *s1, long ld1, float *s0, long ld0, double a, double b,
+                                  const cacg_spmm_sched *sc, hipStream_t st)
 {
-    if (!A || k <= 0 || A->n < 0 || !s0)
+    if (!A || k <= 0 || A->n < 0 || !s0 || (sc && (sc->grid < 0 || sc->chunk_rows < 0)))
         return (int)hipErrorInvalidValue;
     if (A->n == 0)
         return 0;
     return dispatch_spmm<float, double, float>(A->n, A->rowptr, A->col, A->val, k, r, ldr, s1,
                                                ld1, CACG_EPI_CHEB1, a, b, NULL, 0, NULL, 0, s0,
-                                               ld0, st);
+                                               ld0, sc, st);
+}
+
+extern "C" int cacg_k_cheb1_f32(const cacg_scsr *A, int k, const double *r, long ldr, float *s1,
+                                long ld1, float *s0, long ld0, double a, double b, hipStream_t st)
+{
+    return cacg_k_cheb1_f32_x(A, k, r, ldr, s1, ld1, s0, ld0, a, b, nullptr, st);
 }
 
 namespace {

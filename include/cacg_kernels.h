@@ -61,6 +61,26 @@ int cacg_k_spmm_f32(const cacg_scsr *A, int k, const float *x, long ldx, float *
                     long ld2, hipStream_t st);
 int cacg_k_cheb1_f32(const cacg_scsr *A, int k, const double *r, long ldr, float *s1, long ld1,
                      float *s0, long ld0, double a, double b, hipStream_t st);
+
+/* Launch schedule of the SpMMs above (no effect on any result: each output is
+ * summed by one lane group in CSR order).  grid: workgroups (0: 8192 for
+ * k <= 32, 4096 above); chunk_rows: 0 deals each XCD a contiguous band of row
+ * tiles, > 0 deals chunks of that many rows round-robin over the XCDs (an XCD
+ * then owns a slice of every grid plane of a stencil matrix).  The _x forms
+ * take one (NULL: the defaults); the forms above use the defaults. */
+typedef struct {
+    int grid;
+    int chunk_rows;
+} cacg_spmm_sched;
+int cacg_k_spmm_x(const cacg_dcsr *A, int k, const double *x, long ldx, double *y, long ldy,
+                  int epi, double a, double b, const double *p1, long ld1, const double *p2,
+                  long ld2, double *copy0, long ldc, const cacg_spmm_sched *sc, hipStream_t st);
+int cacg_k_spmm_f32_x(const cacg_scsr *A, int k, const float *x, long ldx, float *y, long ldy,
+                      int epi, double a, double b, const float *p1, long ld1, const float *p2,
+                      long ld2, const cacg_spmm_sched *sc, hipStream_t st);
+int cacg_k_cheb1_f32_x(const cacg_scsr *A, int k, const double *r, long ldr, float *s1, long ld1,
+                       float *s0, long ld0, double a, double b, const cacg_spmm_sched *sc,
+                       hipStream_t st);
 /* dst[e] = fl32(src[e]), e < count (the FP32 copy of the matrix values) */
 int cacg_k_cvt_f64_f32(long count, const double *src, float *dst, hipStream_t st);
 /* dst[e] = src[e] widened to FP64 (Q = S of a fresh FP32 basis) */
