@@ -130,6 +130,10 @@ def lib() -> ctypes.CDLL:
         "cacg_k_fill_i32": ([_vp, c_long, c_int, _vp], c_int),
         "cacg_k_fill_f64": ([_vp, c_long, c_double, _vp], c_int),
         "cacg_k_scale_f64": ([_vp, c_long, c_double, _vp], c_int),
+        "cacg_k_split_supported": ([c_int, c_int], c_int),
+        "cacg_k_qxupdate": ([c_int, c_int, c_int, _vp, _vp, _vp, _vp, _vp, _vp], c_int),
+        "cacg_k_xrupdate_split": ([c_int, c_int, c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, c_int,
+                                   _vp], c_int),
         "cacg_mtx_read": ([ctypes.c_char_p, _lp, _lp, _lp, ctypes.POINTER(_lp),
                            ctypes.POINTER(_lp), ctypes.POINTER(_dp)], c_int),
         "cacg_mtx_write": ([ctypes.c_char_p, c_long, c_long, _lp, _lp, _dp], c_int),
@@ -315,6 +319,33 @@ def k_xrupdate(Q, AQ, alpha, X, R, mode="reference"):
     _check(lib().cacg_k_xrupdate(n, sm, m, dQ.ptr, dAQ.ptr, da.ptr, dX.ptr, dR.ptr, dn.ptr, ws.ptr,
                                  MODE[mode], None), "xrupdate")
     return dX.download((n, m)), dR.download((n, m)), dn.download(m)
+
+
+def k_qxupdate(S, Q, beta, alpha, X):
+    """Deferred fast form: X += Q alpha (old Q), then Q = S - Q beta; returns (Q, X)."""
+    S = np.ascontiguousarray(S, dtype=np.float64)
+    n, sm = S.shape
+    m = alpha.shape[1]
+    dS, dQ, db = DeviceBuffer.from_array(S), DeviceBuffer.from_array(Q), DeviceBuffer.from_array(beta)
+    da, dX = DeviceBuffer.from_array(alpha), DeviceBuffer.from_array(X)
+    _check(lib().cacg_k_qxupdate(n, sm, m, dS.ptr, dQ.ptr, db.ptr, da.ptr, dX.ptr, None), "qxupdate")
+    return dQ.download((n, sm)), dX.download((n, m))
+
+
+def k_xrupdate_split(Q, AQ, alpha, X, R, which):
+    """which 1: X += Q alpha, returns X; which 2: R -= AQ alpha, returns (R, norms^2)."""
+    AQ = np.ascontiguousarray(AQ, dtype=np.float64)
+    n, sm = AQ.shape
+    m = alpha.shape[1]
+    dQ, dAQ, da = DeviceBuffer.from_array(Q), DeviceBuffer.from_array(AQ), DeviceBuffer.from_array(alpha)
+    dX, dR = DeviceBuffer.from_array(X), DeviceBuffer.from_array(R)
+    dn = DeviceBuffer(8 * m)
+    ws = DeviceBuffer(8 * lib().cacg_xrupdate_workspace(m))
+    _check(lib().cacg_k_xrupdate_split(n, sm, m, dQ.ptr, dAQ.ptr, da.ptr, dX.ptr, dR.ptr, dn.ptr,
+                                       ws.ptr, which, None), "xrupdate_split")
+    if which == 1:
+        return dX.download((n, m))
+    return dR.download((n, m)), dn.download(m)
 
 
 def k_spmm_f32(A, X, epi="none", a=0.0, b=0.0, p1=None, p2=None):

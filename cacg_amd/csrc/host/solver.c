@@ -15,6 +15,8 @@
  *   [QtAQ | QtR] = Q^T [AQ | R]                           :307, :330 (one pass over Q)
  *   alpha = QtAQ^{-1} QtR                                 :344
  *   X += Q alpha; R -= AQ alpha; ||R(:,c)||^2             :376-447
+ * (fast order: X += Q alpha is deferred into the next iteration's Q update,
+ * which reads the same Q rows anyway, and applied once after the last one.)
  * The m column norms come back to the host once per iteration (one small
  * device-to-host copy, the only synchronisation in the loop) for the stop test.
  *
@@ -659,6 +661,8 @@ int cacg_solver_solve_device(cacg_solver *sv, const double *d_B, double *d_X, ca
     const cacg_opts *o = &sv->opts;
     const int mode = o->order == CACG_ORDER_REFERENCE ? CACG_MODE_REFERENCE : CACG_MODE_FAST;
     const int n = (int)sv->n, m = (int)sv->m, s = (int)sv->s, sm = (int)sv->sm;
+    /* fast order: the X update deferred into the next Q update (Q read once) */
+    const int split = mode == CACG_MODE_FAST && cacg_k_split_supported(sm, m);
     hipStream_t st = sv->st;
     double *h_norm2 = sv->h_pin, *h_bnorm2 = sv->h_pin + m, *h_bounds = sv->h_pin + 2 * m;
     int *h_flag = (int *)(sv->h_pin + 2 * m + 2);
@@ -741,7 +745,10 @@ int cacg_solver_solve_device(cacg_solver *sv, const double *d_B, double *d_X, ca
             SCHK(cacg_k_lu_solve(sm, sv->d_QtAQ, sm, sv->d_QtAS, sv->d_beta, sv->d_lu_work,
                                  sv->d_flag, mode, 1, st));
             mark(sv, CACG_PHASE_SOLVE);
-            SCHK(cacg_k_qupdate(n, sm, S, Q, sv->d_beta, mode, st));
+            if (split) /* X += Q alpha of the previous iteration, then Q = S - Q beta */
+                SCHK(cacg_k_qxupdate(n, sm, m, S, Q, sv->d_beta, sv->d_alpha, d_X, st));
+            else
+                SCHK(cacg_k_qupdate(n, sm, S, Q, sv->d_beta, mode, st));
             mark(sv, CACG_PHASE_QUPDATE);
         }
         SCHK(cacg_k_spmm_x(&sv->A, sm, Q, sm, sv->d_AQ, sm, CACG_EPI_NONE, 0.0, 0.0, NULL, 0, NULL,
@@ -752,8 +759,12 @@ int cacg_solver_solve_device(cacg_solver *sv, const double *d_B, double *d_X, ca
         SCHK(cacg_k_lu_solve(sm, sv->d_QtAQ, m, sv->d_QtR, sv->d_alpha, sv->d_lu_work, sv->d_flag,
                              mode, 0, st));
         mark(sv, CACG_PHASE_SOLVE);
-        SCHK(cacg_k_xrupdate(n, sm, m, Q, sv->d_AQ, sv->d_alpha, d_X, sv->d_R, sv->d_norm2,
-                             sv->d_xr_work, mode, st));
+        if (split) /* R and its norms now; X with the next Q update */
+            SCHK(cacg_k_xrupdate_split(n, sm, m, NULL, sv->d_AQ, sv->d_alpha, NULL, sv->d_R,
+                                       sv->d_norm2, sv->d_xr_work, 2, st));
+        else
+            SCHK(cacg_k_xrupdate(n, sm, m, Q, sv->d_AQ, sv->d_alpha, d_X, sv->d_R, sv->d_norm2,
+                                 sv->d_xr_work, mode, st));
         mark(sv, CACG_PHASE_XRUPDATE);
         SCHK(hipMemcpyAsync(h_norm2, sv->d_norm2, sizeof(double) * m, hipMemcpyDeviceToHost, st));
         SCHK(hipMemcpyAsync(h_flag, sv->d_flag, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -786,6 +797,10 @@ int cacg_solver_solve_device(cacg_solver *sv, const double *d_B, double *d_X, ca
             it++;
             break;
         }
+    }
+    if (split && it > 0) { /* the last iteration's X += Q alpha */
+        SCHK(cacg_k_xrupdate_split(n, sm, m, Q, NULL, sv->d_alpha, d_X, NULL, NULL, NULL, 1, st));
+        SCHK(hipStreamSynchronize(st));
     }
 out:
     /* leave the two basis-sized buffers in allocation roles for the next solve */

@@ -156,17 +156,34 @@ __global__ __launch_bounds__(64) void colnorm2_ref_kernel(int n, int m, const do
 // rows per 16-row tile handled by one wave; K = SM, KQ = SM / 4 per lane group
 // TS, T = storage types of S and Q (double, or float for an FP32 basis); the
 // products and sums are FP64 either way
-template <typename TS, typename T, int SM>
+// NTX > 0: also X += Q alpha with the Q rows before their update (the X
+// update of the previous iteration, deferred so Q is read once): alpha is
+// SM x m, m <= 16 NTX, held in registers; the products run in the same
+// operand order as xrupdate_mfma_kernel's, so X gets the same bits.
+template <typename TS, typename T, int SM, int NTX>
 __global__ __launch_bounds__(512) void qupdate_mfma_kernel(int n, const TS *__restrict__ S, T *Q,
-                                                           const double *__restrict__ beta)
+                                                           const double *__restrict__ beta, int m,
+                                                           const double *__restrict__ alpha,
+                                                           double *__restrict__ X)
 {
     constexpr int KQ = SM / 4, NTL = SM / 16;
+    constexpr int NA = NTX > 0 ? NTX : 1;
     extern __shared__ double bl[];  // SM x SM
     for (int e = threadIdx.x; e < SM * SM; e += 512)
         bl[e] = beta[e];
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int lg = lane >> 4, lc = lane & 15;
+    double alf[NA][KQ];
+    if (NTX > 0) {
+#pragma unroll
+        for (int j = 0; j < NA; j++) {
+            const int c = j * 16 + lc;
+#pragma unroll
+            for (int tt = 0; tt < KQ; tt++)
+                alf[j][tt] = c < m ? alpha[(long)(lg * KQ + tt) * m + c] : 0.0;
+        }
+    }
     const int nw = gridDim.x * 8;
     const int wid = xcd_remap(blockIdx.x, gridDim.x) * 8 + wave;
     const int ntiles = (n + 15) / 16;
@@ -207,11 +224,30 @@ __global__ __launch_bounds__(512) void qupdate_mfma_kernel(int n, const TS *__re
                     Q[i * SM + j * 16 + lc] = (T)(sv[r] - acc[r]);
             }
         }
+        if (NTX > 0) {
+#pragma unroll
+            for (int j = 0; j < NA; j++) {
+                dbl4 ax = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int tt = 0; tt < KQ; tt++)
+                    ax = mfma16x16x4(av[tt], alf[j][tt], ax);
+                const int c = j * 16 + lc;
+                if (c < m) {
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const long i = i0 + lg + 4 * r;
+                        if (i < n)
+                            X[i * m + c] += ax[r];
+                    }
+                }
+            }
+        }
     }
 }
 
 // NTM = output column tiles (m <= 16 * NTM); norms: per-block partials
-template <typename T, int SM, int NTM>
+// DOX: X += Q alpha; DOR: R -= AQ alpha and the column norms of the new R
+template <typename T, int SM, int NTM, bool DOX, bool DOR>
 __global__ __launch_bounds__(256) void xrupdate_mfma_kernel(int n, int m, const T *__restrict__ Q,
                                                             const T *__restrict__ AQ,
                                                             const double *__restrict__ alpha,
@@ -248,8 +284,10 @@ __global__ __launch_bounds__(256) void xrupdate_mfma_kernel(int n, int m, const 
         const long arow = i0 + lc;
         double av[KQ], aq[KQ];
         if (arow < n) {
-            load_row<T, KQ>(Q + arow * SM + lg * KQ, av);
-            load_row<T, KQ>(AQ + arow * SM + lg * KQ, aq);
+            if (DOX)
+                load_row<T, KQ>(Q + arow * SM + lg * KQ, av);
+            if (DOR)
+                load_row<T, KQ>(AQ + arow * SM + lg * KQ, aq);
         } else {
 #pragma unroll
             for (int u = 0; u < KQ; u++) {
@@ -262,8 +300,10 @@ __global__ __launch_bounds__(256) void xrupdate_mfma_kernel(int n, int m, const 
             dbl4 ax = {0.0, 0.0, 0.0, 0.0}, ar = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int tt = 0; tt < KQ; tt++) {
-                ax = mfma16x16x4(av[tt], alf[j][tt], ax);
-                ar = mfma16x16x4(aq[tt], alf[j][tt], ar);
+                if (DOX)
+                    ax = mfma16x16x4(av[tt], alf[j][tt], ax);
+                if (DOR)
+                    ar = mfma16x16x4(aq[tt], alf[j][tt], ar);
             }
             const int c = j * 16 + lc;
             if (c < m) {
@@ -271,15 +311,20 @@ __global__ __launch_bounds__(256) void xrupdate_mfma_kernel(int n, int m, const 
                 for (int r = 0; r < 4; r++) {
                     const long i = i0 + lg + 4 * r;
                     if (i < n) {
-                        X[i * m + c] += ax[r];
-                        const double rn = R[i * m + c] - ar[r];
-                        R[i * m + c] = rn;
-                        nrm[j] += rn * rn;
+                        if (DOX)
+                            X[i * m + c] += ax[r];
+                        if (DOR) {
+                            const double rn = R[i * m + c] - ar[r];
+                            R[i * m + c] = rn;
+                            nrm[j] += rn * rn;
+                        }
                     }
                 }
             }
         }
     }
+    if (!DOR)
+        return;
 #pragma unroll
     for (int j = 0; j < NTM; j++)
         red[wave][lane][j] = nrm[j];
@@ -353,8 +398,9 @@ __global__ __launch_bounds__(256) void colnorm2_final_kernel(const double *__res
 
 constexpr int XR_BLOCKS = 1024;  // fixed: the norm partials depend on it
 
-template <typename TS, typename T, int SM>
-int launch_qupdate_mfma(int n, const TS *S, T *Q, const double *beta, hipStream_t st)
+template <typename TS, typename T, int SM, int NTX = 0>
+int launch_qupdate_mfma(int n, const TS *S, T *Q, const double *beta, hipStream_t st, int m = 0,
+                        const double *alpha = nullptr, double *X = nullptr)
 {
     const int ntiles = (n + 15) / 16;
     int G = (ntiles + 7) / 8;
@@ -362,20 +408,22 @@ int launch_qupdate_mfma(int n, const TS *S, T *Q, const double *beta, hipStream_
         G = 256;
     if (G >= 8)
         G &= ~7;
-    cacg_allow_lds(qupdate_mfma_kernel<TS, T, SM>, (size_t)SM * SM * sizeof(double));
-    hipLaunchKernelGGL((qupdate_mfma_kernel<TS, T, SM>), dim3(G), dim3(512),
-                       (size_t)SM * SM * sizeof(double), st, n, S, Q, beta);
+    cacg_allow_lds(qupdate_mfma_kernel<TS, T, SM, NTX>, (size_t)SM * SM * sizeof(double));
+    hipLaunchKernelGGL((qupdate_mfma_kernel<TS, T, SM, NTX>), dim3(G), dim3(512),
+                       (size_t)SM * SM * sizeof(double), st, n, S, Q, beta, m, alpha, X);
     CACG_LAUNCH_CHECK();
     return 0;
 }
 
-template <typename T, int SM, int NTM>
+template <typename T, int SM, int NTM, bool DOX = true, bool DOR = true>
 int launch_xrupdate_mfma(int n, int m, const T *Q, const T *AQ, const double *alpha, double *X,
                          double *R, double *d_norm2, double *work, hipStream_t st)
 {
-    hipLaunchKernelGGL((xrupdate_mfma_kernel<T, SM, NTM>), dim3(XR_BLOCKS), dim3(256), 0, st, n, m,
-                       Q, AQ, alpha, X, R, work);
+    hipLaunchKernelGGL((xrupdate_mfma_kernel<T, SM, NTM, DOX, DOR>), dim3(XR_BLOCKS), dim3(256), 0,
+                       st, n, m, Q, AQ, alpha, X, R, work);
     CACG_LAUNCH_CHECK();
+    if (!DOR)
+        return 0;
     hipLaunchKernelGGL(colnorm2_final_kernel, dim3(m), dim3(256), 0, st, work, XR_BLOCKS,
                        m, d_norm2);
     CACG_LAUNCH_CHECK();
@@ -511,6 +559,76 @@ extern "C" int cacg_k_qupdate(int n, int sm, const double *S, double *Q, const d
                               int mode, hipStream_t st)
 {
     return qupdate<double, double>(n, sm, S, Q, beta, mode, st);
+}
+
+namespace {
+// the widths with MFMA Q / X / R updates: sm a multiple of 16 up to 128, m <= 32
+bool split_supported(int sm, int m)
+{
+    return sm >= 16 && sm <= 128 && sm % 16 == 0 && m >= 1 && m <= 32;
+}
+}  // namespace
+
+extern "C" int cacg_k_split_supported(int sm, int m) { return split_supported(sm, m) ? 1 : 0; }
+
+extern "C" int cacg_k_qxupdate(int n, int sm, int m, const double *S, double *Q,
+                               const double *beta, const double *alpha, double *X, hipStream_t st)
+{
+    if (n < 0 || !split_supported(sm, m))
+        return (int)hipErrorInvalidValue;
+    if (n == 0)
+        return 0;
+#define CACG_QX(SMV)                                                                              \
+    case SMV:                                                                                     \
+        return m <= 16 ? launch_qupdate_mfma<double, double, SMV, 1>(n, S, Q, beta, st, m, alpha, X) \
+                       : launch_qupdate_mfma<double, double, SMV, 2>(n, S, Q, beta, st, m, alpha, X);
+    switch (sm) {
+        CACG_QX(128)
+        CACG_QX(112)
+        CACG_QX(96)
+        CACG_QX(80)
+        CACG_QX(64)
+        CACG_QX(48)
+        CACG_QX(32)
+        CACG_QX(16)
+    default: break;
+    }
+#undef CACG_QX
+    return (int)hipErrorInvalidValue;
+}
+
+extern "C" int cacg_k_xrupdate_split(int n, int sm, int m, const double *Q, const double *AQ,
+                                     const double *alpha, double *X, double *R, double *d_norm2,
+                                     double *d_work, int which, hipStream_t st)
+{
+    if (n < 0 || !split_supported(sm, m) || (which != 1 && which != 2))
+        return (int)hipErrorInvalidValue;
+    if (n == 0)
+        return 0;
+#define CACG_XS(SMV)                                                                              \
+    case SMV:                                                                                     \
+        if (which == 1)                                                                           \
+            return m <= 16 ? launch_xrupdate_mfma<double, SMV, 1, true, false>(                   \
+                                 n, m, Q, AQ, alpha, X, R, d_norm2, d_work, st)                   \
+                           : launch_xrupdate_mfma<double, SMV, 2, true, false>(                   \
+                                 n, m, Q, AQ, alpha, X, R, d_norm2, d_work, st);                  \
+        return m <= 16 ? launch_xrupdate_mfma<double, SMV, 1, false, true>(                       \
+                             n, m, Q, AQ, alpha, X, R, d_norm2, d_work, st)                       \
+                       : launch_xrupdate_mfma<double, SMV, 2, false, true>(                       \
+                             n, m, Q, AQ, alpha, X, R, d_norm2, d_work, st);
+    switch (sm) {
+        CACG_XS(128)
+        CACG_XS(112)
+        CACG_XS(96)
+        CACG_XS(80)
+        CACG_XS(64)
+        CACG_XS(48)
+        CACG_XS(32)
+        CACG_XS(16)
+    default: break;
+    }
+#undef CACG_XS
+    return (int)hipErrorInvalidValue;
 }
 
 extern "C" int cacg_k_qupdate_zero(int n, int sm, double *Q, const double *beta, hipStream_t st)

@@ -119,6 +119,19 @@ int cacg_k_lu_solve(int N, const double *lhs, int k, const double *rhs, double *
  * (src/BLAS3.c:794-825 then 579-612). */
 int cacg_k_qupdate(int n, int sm, const double *S, double *Q, const double *beta, int mode,
                    hipStream_t st);
+/* Fast order with the X update deferred by one iteration, so Q is read once
+ * per iteration (no reference counterpart; results are the bits of the
+ * undeferred fast order), for cacg_k_split_supported(sm, m): sm a multiple of
+ * 16 up to 128 and m <= 32.
+ *   cacg_k_qxupdate      : X += Q alpha with the old Q, then Q = S - Q beta;
+ *   cacg_k_xrupdate_split: which 1: X += Q alpha only (the last iteration's);
+ *                          which 2: R -= AQ alpha and ||R(:,c)||^2 only. */
+int cacg_k_split_supported(int sm, int m);
+int cacg_k_qxupdate(int n, int sm, int m, const double *S, double *Q, const double *beta,
+                    const double *alpha, double *X, hipStream_t st);
+int cacg_k_xrupdate_split(int n, int sm, int m, const double *Q, const double *AQ,
+                          const double *alpha, double *X, double *R, double *d_norm2,
+                          double *d_work, int which, hipStream_t st);
 /* Q = -Q beta in place (S = 0; fast order), the tall-skinny product of the QR */
 int cacg_k_qupdate_zero(int n, int sm, double *Q, const double *beta, hipStream_t st);
 /* FP32 basis S, FP64 Q (the precision form's update): Q = S - Q beta in FP64. */

@@ -218,3 +218,32 @@ def test_xrupdate(gpu_lib, oracle_lib, n, sm, m):
     Xf, Rf, nf = gpu_lib.k_xrupdate(Qi, AQi, ai, Xi, Ri, mode="fast")
     assert np.array_equal(Xf, Xe) and np.array_equal(Rf, Re)
     assert np.array_equal(nf, (Re.astype(np.int64) ** 2).sum(axis=0).astype(np.float64))
+
+
+# ---- deferred X update (fast order): the same bits as the undeferred kernels ------
+
+@pytest.mark.parametrize("sm,m", [(128, 16), (64, 8), (96, 32), (48, 20), (16, 1)])
+def test_deferred_x_update_kernels_match_undeferred_bitwise(gpu_lib, sm, m):
+    rng = np.random.default_rng(sm * 100 + m)
+    n = 5003                                       # ragged last tile
+    S, Q, AQ = (rng.standard_normal((n, sm)) for _ in range(3))
+    beta, alpha = rng.standard_normal((sm, sm)), rng.standard_normal((sm, m))
+    X, R = rng.standard_normal((n, m)), rng.standard_normal((n, m))
+    print(f"deferred x update sm={sm} m={m}", flush=True)
+    assert gpu_lib.lib().cacg_k_split_supported(sm, m) == 1
+    Xw, Rw, nw = gpu_lib.k_xrupdate(Q, AQ, alpha, X, R, mode="fast")
+    Qw = gpu_lib.k_qupdate(S, Q, beta, mode="fast")
+    Qd, Xd = gpu_lib.k_qxupdate(S, Q, beta, alpha, X)
+    assert np.array_equal(Qd, Qw) and np.array_equal(Xd, Xw)
+    assert np.array_equal(gpu_lib.k_xrupdate_split(Q, AQ, alpha, X, R, 1), Xw)
+    Rd, nd = gpu_lib.k_xrupdate_split(Q, AQ, alpha, X, R, 2)
+    assert np.array_equal(Rd, Rw) and np.array_equal(nd, nw)
+    # against plain float64 products
+    assert np.allclose(Xd, X + Q @ alpha, rtol=1e-12, atol=1e-11)
+    assert np.allclose(Qd, S - Q @ beta, rtol=1e-12, atol=1e-11)
+
+
+def test_deferred_forms_refuse_unsupported_widths(gpu_lib):
+    L = gpu_lib.lib()
+    assert L.cacg_k_split_supported(40, 4) == 0 and L.cacg_k_split_supported(128, 33) == 0
+    assert L.cacg_k_qxupdate(10, 40, 4, None, None, None, None, None, None) != 0
