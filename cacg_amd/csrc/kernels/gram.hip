@@ -306,6 +306,16 @@ __global__ __launch_bounds__(256) void gram_reduce_kernel(const double *__restri
 //   <12, 2, 3, 16, 2, 3>, <16, 2, 2, 16, 2, 2>: a 128-row G against 144 and
 //   128 columns, as 12 and 16 waves of 2 x 3 and 2 x 2 tiles (measured on
 //   gfx950: 58 and 56 TFLOP/s against 42 and 50 for the general shape).
+// LDS row stride (doubles) of a staged k-column block: k rounded up to 16
+// mod 32.  A wave reads 4 rows x 16 consecutive doubles per MFMA operand;
+// with a stride of 16 mod 32 doubles, rows 0, 1 (and 2, 3) fall into the two
+// disjoint halves of the 64 four-byte LDS banks, so a 64-lane read takes the
+// 2 cycles its 512 bytes need (a stride of k + 2 overlapped all four rows).
+__host__ __device__ inline int lds_row_stride(int k)
+{
+    return k + (48 - k % 32) % 32;
+}
+
 template <typename TL, typename T1, int NW, int MT, int NT, int SR, int LDG, int VPT>
 __global__ __launch_bounds__(NW * 64) void gram_lds_kernel(int n, SrcT<TL, T1> s,
                                                            double *__restrict__ part, int kl_pad,
@@ -314,7 +324,7 @@ __global__ __launch_bounds__(NW * 64) void gram_lds_kernel(int n, SrcT<TL, T1> s
     constexpr int NTH = NW * 64;
     extern __shared__ double lds[];
     const int kr = s.kr1 + s.kr2;
-    const int LDP = s.kl + 2, RDP = kr + 2;          // padded LDS row strides
+    const int LDP = lds_row_stride(s.kl), RDP = lds_row_stride(kr);  // padded LDS row strides
     const int BUF = SR * (LDP + RDP);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int lr = lane >> 4, lc = lane & 15;
@@ -491,7 +501,7 @@ int launch_gram_wide(int n, const SrcT<TL, T1> &s, const GramPlan &p, double *wo
     const int kl_pad = p.TM * 16, kr_pad = p.TN * 16;
     if (!staged_ok || p.TM != 8 || p.chunk % SRW != 0 || (p.TN != 8 && p.TN != 9))
         return 0;
-    const size_t shm = 2 * (size_t)SRW * (s.kl + 2 + kr + 2) * sizeof(double);
+    const size_t shm = 2 * (size_t)SRW * (lds_row_stride(s.kl) + lds_row_stride(kr)) * sizeof(double);
     if (p.TN == 9) {
         if (stage_vectors(s, SRW) > 12 * 64 * 3)
             return 0;
@@ -525,7 +535,8 @@ int launch_gram_mfma(int n, const SrcT<TL, T1> &s, const GramPlan &p, double *G1
     if (launch_gram_wide(n, s, p, work, aligned, st)) {
         // launched
     } else if (staged) {
-        const size_t shm = 2 * (size_t)SR * (s.kl + 2 + kr + 2) * sizeof(double);
+        const size_t shm =
+            2 * (size_t)SR * (lds_row_stride(s.kl) + lds_row_stride(kr)) * sizeof(double);
         cacg_allow_lds(gram_lds_kernel<TL, T1, 4, MT, NT, SR, LDG, VPT>, shm);
         hipLaunchKernelGGL((gram_lds_kernel<TL, T1, 4, MT, NT, SR, LDG, VPT>), dim3(p.gx, p.gy),
                            dim3(256), shm, st, n, s, work, kl_pad, kr_pad, p.chunk, p.GN, p.NG);
