@@ -177,6 +177,10 @@ def main():
         "qupdate": ("qupdate", "mfma", M.qupdate_flops(n, sm), 1),
         "xrupdate": ("xrupdate", "hbm", M.xrupdate_bytes(n, sm, m), 1),
     }
+    if args.order == "fast" and cacg_amd.lib().cacg_k_split_supported(sm, m):
+        # the X update rides on the next Q update; the X/R update updates R only
+        table["qupdate"] = ("qupdate", "mfma", M.qxupdate_flops(n, sm, m), 1)
+        table["xrupdate"] = ("xrupdate", "hbm", M.rupdate_bytes(n, sm, m), 1)
     kernel_rates = {}
     for name, (phase, bound, work, per_call) in table.items():
         launches = calls[phase] * per_call

@@ -44,6 +44,16 @@ def xrupdate_bytes(n, sm, m, w=8):
     return 2 * n * sm * w + 4 * n * m * w   # read Q, AQ; read+write X, R
 
 
+# fast order with the X update deferred (cacg_k_qxupdate / cacg_k_xrupdate_split):
+# the Q update also forms X += Q alpha, the X/R update only R -= AQ alpha
+def qxupdate_flops(n, sm, m):
+    return 2.0 * n * sm * sm + 2.0 * n * sm * m
+
+
+def rupdate_bytes(n, sm, m, w=8):
+    return n * sm * w + 2 * n * m * w       # read AQ; read+write R
+
+
 def lu_flops(sm, k):
     return 2.0 * (2.0 / 3.0) * sm ** 3 + 2.0 * sm * sm * k
 
