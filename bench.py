@@ -236,6 +236,7 @@ def main():
     }
     sv.close()
     if not args.no_forms:
+        out["dropin_host_arrays"] = dropin_host_arrays(A, B, m, s, args, out["ms_per_step"])
         out["precision_form"] = precision_form(A, B, X, dB, dX, m, s, args, out["ms_per_step"])
         out["batching_form"] = batching_form()
         out["serving_form"] = serving_form(args.serve_requests, args.serve_rate)
@@ -246,6 +247,26 @@ def main():
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))
+
+
+def dropin_host_arrays(A, B, m, s, args, ms_device):
+    """The same solve through the host-array API (cacg_bcbcg, what bcbcg_v1 calls): solver
+    creation (HBM allocation, int32 CSR upload, SpMM autotuning), upload of B and X, the solve,
+    download of X, teardown, timed end to end (PCIe included)."""
+    import cacg_amd
+    lat = []
+    for k in range(2):
+        t = time.perf_counter()
+        r = cacg_amd.bcbcg(A, B, np.zeros_like(B), s, tol=args.tol, maxiter=args.maxiter,
+                           order=args.order)
+        if k > 0:
+            lat.append(time.perf_counter() - t)
+    ms = float(np.mean(lat) * 1e3)
+    print(f"bench: host-array drop-in {ms:.1f} ms per solve", file=sys.stderr, flush=True)
+    return {"workload": "same solve through cacg_bcbcg with host arrays: create, upload A/B/X, "
+                        "solve, download X, destroy",
+            "ms_per_solve": ms, "solves_per_s": 1e3 / ms, "device_only_ms": ms_device,
+            "status": r["status"], "iters": r["iters"]}
 
 
 def precision_form(A, B, X64, dB, dX, m, s, args, ms64):

@@ -1026,7 +1026,16 @@ static int solve_refine(cacg_solver *sv, const double *d_B, double *d_X, cacg_re
 
     int fresh = 1;   /* next iteration starts the recurrence (Q = S) */
     long since = 0;  /* iterations since the last true residual */
+    /* X += Q alpha deferred into the next Q update (split) and flushed wherever X
+     * is read: before a true residual and at the end */
+    const int split = cacg_k_split_supported(sm, m);
+    int xpend = 0;
     for (;;) {
+        if (xpend) {
+            SCHK(cacg_k_xrupdate_split(n, sm, m, Q, NULL, sv->d_alpha, d_X, NULL, NULL, NULL, 1,
+                                       st));
+            xpend = 0;
+        }
         /* true residual in FP64 with the FP64 matrix; it replaces the recursive one */
         mark(sv, CACG_PHASE_OTHER);
         SCHK(cacg_k_spmm(&sv->A, m, d_X, m, sv->d_R, m, CACG_EPI_RESID, 0.0, 0.0, d_B, m, NULL, 0,
@@ -1079,6 +1088,11 @@ static int solve_refine(cacg_solver *sv, const double *d_B, double *d_X, cacg_re
             if (s != 2)
                 mark(sv, s == 1 ? CACG_PHASE_BASIS1 : CACG_PHASE_BASIS);
             if (fresh) {
+                if (xpend) { /* Q is about to be replaced: apply its pending X update */
+                    SCHK(cacg_k_xrupdate_split(n, sm, m, Q, NULL, sv->d_alpha, d_X, NULL, NULL,
+                                               NULL, 1, st));
+                    xpend = 0;
+                }
                 SCHK(cacg_k_cvt_f32_f64((long)n * sm, S, Q, st));
                 fresh = 0;
             } else {
@@ -1088,7 +1102,11 @@ static int solve_refine(cacg_solver *sv, const double *d_B, double *d_X, cacg_re
                 SCHK(cacg_k_lu_solve(sm, sv->d_QtAQ, sm, sv->d_QtAS, sv->d_beta, sv->d_lu_work,
                                      sv->d_flag, CACG_MODE_FAST, 1, st));
                 mark(sv, CACG_PHASE_SOLVE);
-                SCHK(cacg_k_qupdate_s32(n, sm, S, Q, sv->d_beta, st));
+                if (xpend)
+                    SCHK(cacg_k_qxupdate_s32(n, sm, m, S, Q, sv->d_beta, sv->d_alpha, d_X, st));
+                else
+                    SCHK(cacg_k_qupdate_s32(n, sm, S, Q, sv->d_beta, st));
+                xpend = 0;
             }
             mark(sv, CACG_PHASE_QUPDATE);
             SCHK(cacg_k_spmm_x(&sv->A, sm, Q, sm, AQ, sm, CACG_EPI_NONE, 0.0, 0.0, NULL, 0, NULL,
@@ -1100,8 +1118,14 @@ static int solve_refine(cacg_solver *sv, const double *d_B, double *d_X, cacg_re
             SCHK(cacg_k_lu_solve(sm, sv->d_QtAQ, m, sv->d_QtR, sv->d_alpha, sv->d_lu_work,
                                  sv->d_flag, CACG_MODE_FAST, 0, st));
             mark(sv, CACG_PHASE_SOLVE);
-            SCHK(cacg_k_xrupdate(n, sm, m, Q, AQ, sv->d_alpha, d_X, sv->d_R, sv->d_norm2,
-                                 sv->d_xr_work, CACG_MODE_FAST, st));
+            if (split) {
+                SCHK(cacg_k_xrupdate_split(n, sm, m, NULL, AQ, sv->d_alpha, NULL, sv->d_R,
+                                           sv->d_norm2, sv->d_xr_work, 2, st));
+                xpend = 1;
+            } else {
+                SCHK(cacg_k_xrupdate(n, sm, m, Q, AQ, sv->d_alpha, d_X, sv->d_R, sv->d_norm2,
+                                     sv->d_xr_work, CACG_MODE_FAST, st));
+            }
             mark(sv, CACG_PHASE_XRUPDATE);
             SCHK(hipMemcpyAsync(h_norm2, sv->d_norm2, sizeof(double) * m, hipMemcpyDeviceToHost, st));
             SCHK(hipMemcpyAsync(h_flag, sv->d_flag, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1133,6 +1157,10 @@ static int solve_refine(cacg_solver *sv, const double *d_B, double *d_X, cacg_re
             status = CACG_BREAKDOWN;
             break;
         }
+    }
+    if (xpend) {
+        SCHK(cacg_k_xrupdate_split(n, sm, m, Q, NULL, sv->d_alpha, d_X, NULL, NULL, NULL, 1, st));
+        SCHK(hipStreamSynchronize(st));
     }
     if (rep && o->stop_mode == CACG_STOP_REL_PER_COLUMN) {
         double worst = 0.0;

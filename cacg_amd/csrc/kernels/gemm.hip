@@ -571,8 +571,10 @@ bool split_supported(int sm, int m)
 
 extern "C" int cacg_k_split_supported(int sm, int m) { return split_supported(sm, m) ? 1 : 0; }
 
-extern "C" int cacg_k_qxupdate(int n, int sm, int m, const double *S, double *Q,
-                               const double *beta, const double *alpha, double *X, hipStream_t st)
+namespace {
+template <typename TS>
+int qxupdate(int n, int sm, int m, const TS *S, double *Q, const double *beta,
+             const double *alpha, double *X, hipStream_t st)
 {
     if (n < 0 || !split_supported(sm, m))
         return (int)hipErrorInvalidValue;
@@ -580,8 +582,8 @@ extern "C" int cacg_k_qxupdate(int n, int sm, int m, const double *S, double *Q,
         return 0;
 #define CACG_QX(SMV)                                                                              \
     case SMV:                                                                                     \
-        return m <= 16 ? launch_qupdate_mfma<double, double, SMV, 1>(n, S, Q, beta, st, m, alpha, X) \
-                       : launch_qupdate_mfma<double, double, SMV, 2>(n, S, Q, beta, st, m, alpha, X);
+        return m <= 16 ? launch_qupdate_mfma<TS, double, SMV, 1>(n, S, Q, beta, st, m, alpha, X) \
+                       : launch_qupdate_mfma<TS, double, SMV, 2>(n, S, Q, beta, st, m, alpha, X);
     switch (sm) {
         CACG_QX(128)
         CACG_QX(112)
@@ -595,6 +597,20 @@ extern "C" int cacg_k_qxupdate(int n, int sm, int m, const double *S, double *Q,
     }
 #undef CACG_QX
     return (int)hipErrorInvalidValue;
+}
+}  // namespace
+
+extern "C" int cacg_k_qxupdate(int n, int sm, int m, const double *S, double *Q,
+                               const double *beta, const double *alpha, double *X, hipStream_t st)
+{
+    return qxupdate<double>(n, sm, m, S, Q, beta, alpha, X, st);
+}
+
+extern "C" int cacg_k_qxupdate_s32(int n, int sm, int m, const float *S, double *Q,
+                                   const double *beta, const double *alpha, double *X,
+                                   hipStream_t st)
+{
+    return qxupdate<float>(n, sm, m, S, Q, beta, alpha, X, st);
 }
 
 extern "C" int cacg_k_xrupdate_split(int n, int sm, int m, const double *Q, const double *AQ,

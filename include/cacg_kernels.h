@@ -129,6 +129,9 @@ int cacg_k_qupdate(int n, int sm, const double *S, double *Q, const double *beta
 int cacg_k_split_supported(int sm, int m);
 int cacg_k_qxupdate(int n, int sm, int m, const double *S, double *Q, const double *beta,
                     const double *alpha, double *X, hipStream_t st);
+/* the same with the FP32 basis S (the precision form) */
+int cacg_k_qxupdate_s32(int n, int sm, int m, const float *S, double *Q, const double *beta,
+                        const double *alpha, double *X, hipStream_t st);
 int cacg_k_xrupdate_split(int n, int sm, int m, const double *Q, const double *AQ,
                           const double *alpha, double *X, double *R, double *d_norm2,
                           double *d_work, int which, hipStream_t st);
