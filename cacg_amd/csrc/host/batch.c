@@ -15,7 +15,11 @@
  *   Q = S - Q beta; AQ = A Q; [QtAQ | QtR], alpha; X += Q alpha;
  *   then for every admitted system R = B - A X (FP64) and its column norms.
  * The recomputed R replaces the recursive update R -= AQ alpha, which keeps
- * the stop test on the true residual and needs no extra pass.  Newly
+ * the stop test on the true residual and needs no extra pass.  Systems of
+ * SOLO_ROWS rows or more instead run the single-system kernels on streams of
+ * their own, iterate on the recursive residual as the single-system solver
+ * does, and recompute the true residual in one extra round when the recursive
+ * one meets the tolerance (or the cap): the true residual still decides.  Newly
  * admitted systems join at the residual step.  Work lists of {slot, tile}
  * items are built on the host each round and uploaded in one copy.
  */
@@ -38,6 +42,8 @@ typedef struct {
     int state, status, slot;
     int joined;          /* 0: admitted this round (residual only), 1: iterating */
     int fresh;           /* the next iteration starts the recurrence (Q = S) */
+    int verify;          /* solo: this round recomputes the true residual only */
+    int xpend;           /* solo: X += Q alpha of the last iteration still to apply */
     long n, nnz, m, iters, round_submitted, rounds_queued;
     double tol, sa, sb, true_relres;
     int *rp, *ci;        /* host copies, int32 */
@@ -50,9 +56,16 @@ typedef struct {
     long off, size;
 } fblk;
 
+/* streams for the solo systems (default priority), so that several of them and
+ * the batched launches run at once; with the main stream they use the four
+ * hardware queues a process gets */
+#define NSOLO 3
+
 struct cacg_batch {
     cacg_batch_opts o;
     hipStream_t st;
+    hipStream_t sst[NSOLO];
+    hipEvent_t sev[NSOLO + 1];
     char *arena;
     long arena_bytes, arena_used;
     fblk *fl;            /* free list, sorted by offset */
@@ -93,6 +106,16 @@ struct cacg_batch {
 
 static long al256(long x) { return (x + 255) & ~255L; }
 
+/* Systems of at least this many rows iterate with the single-system kernels
+ * (vector-per-row SpMM, wide split-K Gram, MFMA tall-skinny updates) on their
+ * arena block instead of joining the batched launches, which are built for
+ * many small systems: alone, a 2-million-row system runs 3-4.5 x slower per
+ * iteration through them.  The choice depends on the system's shape only, so a
+ * result is still the same alone and in any mix. */
+#define SOLO_ROWS (1L << 17)
+
+static int is_solo(long n) { return n >= SOLO_ROWS; }
+
 /* ---- per-system layout in its arena block ----------------------------------------- */
 
 typedef struct {
@@ -127,7 +150,15 @@ static layout make_layout(long n, long nnz, long m, int s, int precision)
     PUT(lu, sm * sm * 8);
     PUT(QtR, sm * m * 8);
     PUT(alpha, sm * m * 8);
-    PUT(gpart, nch * sm * (sm + m) * 8);
+    {
+        long gp = nch * sm * (sm + m);
+        if (is_solo(n)) { /* the single-system Gram's split-K partials */
+            const long ws = cacg_gram_workspace((int)sm, (int)(sm + m));
+            if (ws > gp)
+                gp = ws;
+        }
+        PUT(gpart, gp * 8);
+    }
     PUT(npart, ntile * m * 8);
 #undef PUT
     L.total = o;
@@ -247,6 +278,14 @@ int cacg_batch_destroy(cacg_batch *b)
         hipHostFree(b->h_flags);
     if (b->h_items)
         hipHostFree(b->h_items);
+    for (int i = 0; i < NSOLO; i++)
+        if (b->sst[i]) {
+            hipStreamSynchronize(b->sst[i]);
+            hipStreamDestroy(b->sst[i]);
+        }
+    for (int i = 0; i <= NSOLO; i++)
+        if (b->sev[i])
+            hipEventDestroy(b->sev[i]);
     if (b->st)
         hipStreamDestroy(b->st);
     free(b);
@@ -323,6 +362,16 @@ int cacg_batch_create(cacg_batch **out, const cacg_batch_opts *opts)
         cacg_batch_destroy(b);
         return CACG_EDEVICE;
     }
+    for (int i = 0; i < NSOLO; i++)
+        if (hipStreamCreate(&b->sst[i]) != hipSuccess) {
+            cacg_batch_destroy(b);
+            return CACG_EDEVICE;
+        }
+    for (int i = 0; i <= NSOLO; i++)
+        if (hipEventCreateWithFlags(&b->sev[i], hipEventDisableTiming) != hipSuccess) {
+            cacg_batch_destroy(b);
+            return CACG_EDEVICE;
+        }
     if (hipMalloc((void **)&b->arena, (size_t)b->arena_bytes) != hipSuccess ||
         hipMalloc((void **)&b->d_sys, sizeof(cacg_bsys) * ns) != hipSuccess ||
         hipMalloc((void **)&b->d_norms, sizeof(double) * ns * CACG_BATCH_MAX_M) != hipSuccess ||
@@ -588,6 +637,112 @@ static int gram_phase(cacg_batch *b, const int2 *di, const long *o, const long *
     return rc;
 }
 
+/* one outer iteration of a solo system with the single-system kernels (fast
+ * order), the same sequence as the batched round: basis from the true R; Q = S
+ * or QtAS, beta (the previous alpha solve's factor), Q = S - Q beta; AQ;
+ * [QtAQ | QtR], alpha; X += Q alpha */
+/* X += Q alpha of a solo system's last iteration (the deferred X update) */
+static int solo_flush_x(const cacg_bsys *d, hipStream_t st)
+{
+    KCHK(cacg_k_xrupdate_split(d->n, d->sm, d->m, d->Q, NULL, d->alpha, d->X, NULL, NULL, NULL, 1,
+                               st));
+    return CACG_OK;
+}
+
+/* Solo systems iterate as the single-system solver does: the recursive
+ * residual R -= AQ alpha drives the next basis and a provisional stop test,
+ * and X += Q alpha rides on the next Q update (*xpend) where the widths allow;
+ * a round of solo_residual then recomputes the true residual, which decides. */
+static int solo_iteration(const cacg_bsys *d, int fresh, int s, int f32, int *xpend,
+                          hipStream_t st)
+{
+    const int split = cacg_k_split_supported(d->sm, d->m);
+    const int n = d->n, m = d->m, sm = d->sm;
+    const cacg_dcsr A = {n, d->rowptr, d->col, d->val};
+    const cacg_scsr A32 = {n, d->rowptr, d->col, d->val32};
+    const double sa2 = 2.0 * d->sa, sb2 = 2.0 * d->sb;
+    if (f32) {
+        float *S = (float *)d->S;
+        if (s == 1) {
+            KCHK(cacg_k_cvt_f64_f32((long)n * m, d->R, S, st));
+        } else {
+            KCHK(cacg_k_cheb1_f32(&A32, m, d->R, m, S + m, sm, S, sm, d->sa, d->sb, st));
+            for (int j = 2; j < s; j++)
+                KCHK(cacg_k_spmm_f32(&A32, m, S + (long)(j - 1) * m, sm, S + (long)j * m, sm,
+                                     CACG_EPI_CHEB, sa2, sb2, S + (long)(j - 1) * m, sm,
+                                     S + (long)(j - 2) * m, sm, st));
+        }
+        if (fresh) {
+            if (*xpend && solo_flush_x(d, st) != CACG_OK)
+                return CACG_EDEVICE;
+            KCHK(cacg_k_cvt_f32_f64((long)n * sm, S, d->Q, st));
+        } else {
+            KCHK(cacg_k_gram_fast_f64f32(n, d->AQ, sm, sm, S, sm, sm, NULL, 0, 0, d->QtAS, NULL,
+                                         d->gpart, st));
+            KCHK(cacg_k_lu_solve(sm, d->QtAQ, sm, d->QtAS, d->beta, d->lu, d->flag,
+                                 CACG_MODE_FAST, 1, st));
+            if (*xpend)
+                KCHK(cacg_k_qxupdate_s32(n, sm, m, S, d->Q, d->beta, d->alpha, d->X, st));
+            else
+                KCHK(cacg_k_qupdate_s32(n, sm, S, d->Q, d->beta, st));
+        }
+        *xpend = 0;
+    } else {
+        double *S = (double *)d->S;
+        if (s == 1) {
+            HCHK(hipMemcpyAsync(S, d->R, sizeof(double) * (size_t)n * m, hipMemcpyDeviceToDevice,
+                                st));
+        } else {
+            KCHK(cacg_k_spmm(&A, m, d->R, m, S + m, sm, CACG_EPI_CHEB1, d->sa, d->sb, NULL, 0, NULL,
+                             0, S, sm, st));
+            for (int j = 2; j < s; j++)
+                KCHK(cacg_k_spmm(&A, m, S + (long)(j - 1) * m, sm, S + (long)j * m, sm,
+                                 CACG_EPI_CHEB, sa2, sb2, S + (long)(j - 1) * m, sm,
+                                 S + (long)(j - 2) * m, sm, NULL, 0, st));
+        }
+        if (fresh) {
+            if (*xpend && solo_flush_x(d, st) != CACG_OK)
+                return CACG_EDEVICE;
+            HCHK(hipMemcpyAsync(d->Q, S, sizeof(double) * (size_t)n * sm,
+                                hipMemcpyDeviceToDevice, st));
+        } else {
+            KCHK(cacg_k_gram_fast(n, d->AQ, sm, sm, S, sm, sm, NULL, 0, 0, d->QtAS, NULL, d->gpart,
+                                  st));
+            KCHK(cacg_k_lu_solve(sm, d->QtAQ, sm, d->QtAS, d->beta, d->lu, d->flag,
+                                 CACG_MODE_FAST, 1, st));
+            if (*xpend)
+                KCHK(cacg_k_qxupdate(n, sm, m, S, d->Q, d->beta, d->alpha, d->X, st));
+            else
+                KCHK(cacg_k_qupdate(n, sm, S, d->Q, d->beta, CACG_MODE_FAST, st));
+        }
+        *xpend = 0;
+    }
+    KCHK(cacg_k_spmm(&A, sm, d->Q, sm, d->AQ, sm, CACG_EPI_NONE, 0.0, 0.0, NULL, 0, NULL, 0, NULL,
+                     0, st));
+    KCHK(cacg_k_gram_fast(n, d->Q, sm, sm, d->AQ, sm, sm, d->R, m, m, d->QtAQ, d->QtR, d->gpart,
+                          st));
+    KCHK(cacg_k_lu_solve(sm, d->QtAQ, m, d->QtR, d->alpha, d->lu, d->flag, CACG_MODE_FAST, 0, st));
+    if (split) { /* R and its norms now, X with the next Q update */
+        KCHK(cacg_k_xrupdate_split(n, sm, m, NULL, d->AQ, d->alpha, NULL, d->R, d->norm2, d->npart,
+                                   2, st));
+        *xpend = 1;
+    } else {
+        KCHK(cacg_k_xrupdate(n, sm, m, d->Q, d->AQ, d->alpha, d->X, d->R, d->norm2, d->npart,
+                             CACG_MODE_FAST, st));
+    }
+    return CACG_OK;
+}
+
+/* the true residual R = B - A X of a solo system and its column norms */
+static int solo_residual(const cacg_bsys *d, hipStream_t st)
+{
+    const cacg_dcsr A = {d->n, d->rowptr, d->col, d->val};
+    KCHK(cacg_k_spmm(&A, d->m, d->X, d->m, d->R, d->m, CACG_EPI_RESID, 0.0, 0.0, d->B, d->m, NULL,
+                     0, NULL, 0, st));
+    KCHK(cacg_k_colnorm2_fast(d->n, d->m, d->R, d->norm2, d->npart, st));
+    return CACG_OK;
+}
+
 int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
 {
     if (!b)
@@ -617,7 +772,8 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
         b->qhead++;
     }
 
-    /* work lists */
+    /* work lists (solo systems take no part in the batched launches) */
+#define BATCHED(i) (b->slot[i] && !is_solo(b->h_sys[i].n))
     long need = 0;
     int nact = 0, nit = 0, max_sm = 1;
     for (int i = 0; i < b->nslots; i++) {
@@ -626,6 +782,8 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
             continue;
         const cacg_bsys *d = &b->h_sys[i];
         nact++;
+        if (is_solo(d->n))
+            continue;
         need += 1 + d->ntile;
         if (r->joined) {
             nit++;
@@ -643,11 +801,11 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
      *        then 1) | Gram0 blocks | LU0 | Gram1 blocks | LU1 | all tiles | all systems */
     long o_it = k;
     for (int i = 0; i < b->nslots; i++)
-        if (b->slot[i] && b->slot[i]->joined)
+        if (BATCHED(i) && b->slot[i]->joined)
             k = put_range(it, k, i, b->h_sys[i].ntile);
     const long n_it = k - o_it, o_fr = k;
     for (int i = 0; i < b->nslots; i++)
-        if (b->slot[i] && b->slot[i]->joined && b->slot[i]->fresh)
+        if (BATCHED(i) && b->slot[i]->joined && b->slot[i]->fresh)
             k = put_range(it, k, i, b->h_sys[i].ntile);
     const long n_fr = k - o_fr;
     /* Q and X updates as MFMA per padded width smp = 16 ceil(sm / 16), split by
@@ -659,7 +817,8 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
             for (int i = 0; i < b->nslots; i++) {
                 const breq *r = b->slot[i];
                 const int sm = b->h_sys[i].sm;
-                if (r && r->joined && !r->fresh && (sm + 15) / 16 == c && (sm == 16 * c) == ex)
+                if (BATCHED(i) && r->joined && !r->fresh && (sm + 15) / 16 == c &&
+                    (sm == 16 * c) == ex)
                     k = put_range(it, k, i, b->h_sys[i].nchunk);
             }
             n_qc[c][ex] = k - o_qc[c][ex];
@@ -668,7 +827,7 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
                 for (int i = 0; i < b->nslots; i++) {
                     const breq *r = b->slot[i];
                     const int sm = b->h_sys[i].sm;
-                    if (r && r->joined && (sm + 15) / 16 == c && (sm == 16 * c) == ex &&
+                    if (BATCHED(i) && r->joined && (sm + 15) / 16 == c && (sm == 16 * c) == ex &&
                         (b->h_sys[i].m > 16) == h)
                         k = put_range(it, k, i, b->h_sys[i].nchunk);
                 }
@@ -683,7 +842,7 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
             o_gc[w][c] = k;
             for (int i = 0; i < b->nslots; i++) {
                 const breq *r = b->slot[i];
-                if (!r || !r->joined || (w == 0 && r->fresh))
+                if (!BATCHED(i) || !r->joined || (w == 0 && r->fresh))
                     continue;
                 int tm = 0, tn = 0, cls = 0;
                 if (gram_class(&b->h_sys[i], w, &tm, &tn))
@@ -695,7 +854,7 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
         }
     const long o_b0 = k;
     for (int i = 0; i < b->nslots; i++)
-        if (b->slot[i] && b->slot[i]->joined && !b->slot[i]->fresh) {
+        if (BATCHED(i) && b->slot[i]->joined && !b->slot[i]->fresh) {
             const cacg_bsys *d = &b->h_sys[i];
             k = put_range(it, k, i, (int)(((long)d->sm * d->sm + 255) / 256));
         }
@@ -707,29 +866,63 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
             o_lu[w][c] = k;
             for (int i = 0; i < b->nslots; i++) {
                 const breq *r = b->slot[i];
-                if (r && r->joined && (w == 1 || !r->fresh) && ((b->h_sys[i].sm > 64) == c))
+                if (BATCHED(i) && r->joined && (w == 1 || !r->fresh) &&
+                    ((b->h_sys[i].sm > 64) == c))
                     k = put_range(it, k, i, 1);
             }
             n_lu[w][c] = k - o_lu[w][c];
         }
     const long o_b1 = k;
     for (int i = 0; i < b->nslots; i++)
-        if (b->slot[i] && b->slot[i]->joined) {
+        if (BATCHED(i) && b->slot[i]->joined) {
             const cacg_bsys *d = &b->h_sys[i];
             k = put_range(it, k, i, (int)(((long)d->sm * (d->sm + d->m) + 255) / 256));
         }
     const long n_b1 = k - o_b1, o_all = k;
     for (int i = 0; i < b->nslots; i++)
-        if (b->slot[i])
+        if (BATCHED(i))
             k = put_range(it, k, i, b->h_sys[i].ntile);
     const long n_all = k - o_all, o_sys = k;
     for (int i = 0; i < b->nslots; i++)
-        if (b->slot[i])
+        if (BATCHED(i))
             k = put_range(it, k, i, 1);
     const long n_sys = k - o_sys;
     if (k > 0)
         HCHK(hipMemcpyAsync(b->d_items, it, sizeof(int2) * (size_t)k, hipMemcpyHostToDevice, st));
     const int2 *di = b->d_items;
+
+    /* solo systems: one iteration (when joined) and the true residual, dealt
+     * round-robin over the solo streams, which first wait for what the main
+     * stream queued so far (admission uploads) */
+    int nsolo = 0;
+    for (int i = 0; i < b->nslots; i++)
+        if (b->slot[i] && is_solo(b->h_sys[i].n))
+            nsolo++;
+    if (nsolo > 0) {
+        HCHK(hipEventRecord(b->sev[NSOLO], st));
+        for (int q = 0; q < NSOLO && q < nsolo; q++)
+            HCHK(hipStreamWaitEvent(b->sst[q], b->sev[NSOLO], 0));
+        int q = 0;
+        for (int i = 0; i < b->nslots; i++) {
+            const breq *r = b->slot[i];
+            if (!r || !is_solo(b->h_sys[i].n))
+                continue;
+            hipStream_t ss = b->sst[q];
+            q = (q + 1) % NSOLO;
+            breq *rw = b->slot[i];
+            int rc;
+            if (r->joined && !r->verify) {
+                rc = solo_iteration(&b->h_sys[i], r->fresh, s, f32, &rw->xpend, ss);
+            } else { /* admitted, or verifying: X complete, then the true residual */
+                rc = rw->xpend ? solo_flush_x(&b->h_sys[i], ss) : CACG_OK;
+                rw->xpend = 0;
+                if (rc == CACG_OK)
+                    rc = solo_residual(&b->h_sys[i], ss);
+            }
+            if (rc != CACG_OK)
+                return rc;
+        }
+    }
 
     /* one outer iteration of the joined systems */
     if (n_it > 0) {
@@ -758,8 +951,14 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
                     KCHK(cacg_kb_xupdate_class(b->d_sys, di + o_xc[c][ex][h], (int)n_xc[c][ex][h],
                                                16 * c, ex, h ? 32 : 16, st));
     }
-    /* the true residual of every admitted system */
+#undef BATCHED
+    /* the true residual of every batched system */
     KCHK(cacg_kb_resid(b->d_sys, di + o_all, (int)n_all, di + o_sys, (int)n_sys, st));
+    /* the solo streams' work joins the main stream before the norms come back */
+    for (int q2 = 0; q2 < NSOLO && q2 < nsolo; q2++) {
+        HCHK(hipEventRecord(b->sev[q2], b->sst[q2]));
+        HCHK(hipStreamWaitEvent(st, b->sev[q2], 0));
+    }
     int hi_slot = 0;
     for (int i = 0; i < b->nslots; i++)
         if (b->slot[i])
@@ -780,11 +979,15 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
         if (!r)
             continue;
         const double *nr = b->h_norms + (long)i * CACG_BATCH_MAX_M;
-        if (r->joined) {
+        const int solo = is_solo(r->n);
+        /* a solo system's iteration rounds report its recursive residual */
+        const int recursive = solo && r->joined && !r->verify;
+        if (r->joined && !(solo && r->verify)) {
             r->iters++;
             r->fresh = 0;
         }
         r->joined = 1;
+        r->verify = 0;
         int done = 1, finite = 1;
         for (long c = 0; c < r->m; c++) {
             if (!isfinite(nr[c]))
@@ -793,12 +996,24 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
                 done = 0;
         }
         int status = -1;
-        if (done)
+        if (recursive) { /* provisional: the true residual decides next round */
+            if (!finite || b->h_flags[i])
+                status = CACG_BREAKDOWN;
+            else if (done || r->iters >= b->o.maxiter)
+                r->verify = 1;
+        } else if (done) {
             status = CACG_OK;
-        else if (!finite || b->h_flags[i])
+        } else if (!finite || b->h_flags[i]) {
             status = CACG_BREAKDOWN;
-        else if (r->iters >= b->o.maxiter)
+        } else if (r->iters >= b->o.maxiter) {
             status = CACG_MAXITER;
+        }
+        if (status >= 0 && r->xpend) { /* the X update still pending */
+            const int rc = solo_flush_x(&b->h_sys[i], st);
+            if (rc != CACG_OK)
+                return rc;
+            r->xpend = 0;
+        }
         if (status >= 0) {
             const int rc = retire(b, r, status, nr);
             if (rc != CACG_OK)

@@ -129,3 +129,22 @@ def test_batching_form_sample(gpu_lib):
     for (A, B), r in zip(systems, res):
         assert r["status"] == "ok"
         assert _relres(A, B, r["X"]) < TOL
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_large_systems_alone_equal_mix_and_meet_tol(gpu_lib, precision):
+    # two systems above the row count that runs the single-system kernels,
+    # mixed with small batched ones: same bits alone and in the mix
+    big = [(P.laplacian_2d(400, 0.01), 4), (P.laplacian_3d(52, 0.02), 8)]
+    systems = [(A, P.uniform(200 + k, A.n * m).reshape(A.n, m)) for k, (A, m) in enumerate(big)]
+    systems += _mix()[:3]
+    print(f"batch large+small {precision}", flush=True)
+    mixed = _run(systems, s=4, precision=precision, hbm_bytes=4 << 30)
+    for (A, B), r in zip(systems, mixed):
+        assert r["status"] == "ok"
+        rr = _relres(A, B, r["X"])
+        assert rr < TOL and abs(rr - r["true_relres"]) <= 1e-3 * rr
+    for k in range(2):
+        alone = _run([systems[k]], s=4, precision=precision, hbm_bytes=4 << 30)[0]
+        assert alone["iters"] == mixed[k]["iters"]
+        assert np.array_equal(alone["X"], mixed[k]["X"]), k
