@@ -84,6 +84,101 @@ struct vecld<float, 4> {
 // FP64 solver uses <double, double, double>; the FP32 basis <float, float,
 // float>, and <float, double, float> for its first step, which reads the FP64
 // residual and writes its FP32 copy S_0.
+// The XCD's tiles in sweep order: the lt-th tile of its band (P == 0) or of
+// its chunks of P tiles; -1 past the end.  Tile ids grow with lt in both
+// orders, so the first -1 ends a workgroup's sweep.
+__device__ __forceinline__ int sweep_tile(int lt, int P, int nx, int xcd, int band, int ntiles)
+{
+    int tile;
+    if (P == 0) {
+        if (lt >= band)
+            return -1;
+        tile = xcd * band + lt;
+    } else {
+        tile = ((lt / P) * nx + xcd) * P + lt % P;
+    }
+    return tile < ntiles ? tile : -1;
+}
+
+// The rows of one tile (rows r0 .. r0 + rows - 1) whose row pointers are in
+// s_rp and, when staged, (col, val) pairs in s_col / s_val (offsets from
+// pbase): lane group g takes rows g, g + NG, ..., each lane VW columns.
+template <typename TV, typename TX, typename TY, int LPR, int VW, int RPG, int EPI>
+__device__ __forceinline__ void tile_rows(const int *s_rp, const int *s_col, const TV *s_val,
+                                          bool staged, int pbase, int r0, int rows, int g,
+                                          int lane, const int *__restrict__ colidx,
+                                          const TV *__restrict__ val, int k,
+                                          const TX *__restrict__ x, long ldx, TY *__restrict__ y,
+                                          long ldy, TV a, TV b, const TY *__restrict__ p1,
+                                          long ld1, const TY *__restrict__ p2, long ld2,
+                                          TY *__restrict__ copy0, long ldc)
+{
+    constexpr int NG = 256 / LPR;
+    constexpr int UNR = 8;
+    for (int cb = 0; cb < k; cb += LPR * VW) {
+        const int c0 = cb + lane * VW;
+        if (c0 >= k)
+            continue;
+        for (int rr = 0; rr < RPG; rr++) {
+            const int li = rr * NG + g;
+            if (li >= rows)
+                continue;
+            const long i = r0 + li;
+            const int pb = s_rp[li] - pbase, pe = s_rp[li + 1] - pbase;
+            TV acc[VW], e1[VW], e2[VW];
+#pragma unroll
+            for (int v = 0; v < VW; v++) {
+                acc[v] = (TV)0;
+                if (EPI == CACG_EPI_RESID) {
+                    e1[v] = (TV)__builtin_nontemporal_load(p1 + i * ld1 + c0 + v);
+                } else if (EPI == CACG_EPI_CHEB1) {
+                    e1[v] = (TV)x[i * ldx + c0 + v];
+                } else if (EPI == CACG_EPI_CHEB) {
+                    e1[v] = (TV)p1[i * ld1 + c0 + v];
+                    e2[v] = (TV)__builtin_nontemporal_load(p2 + i * ld2 + c0 + v);
+                }
+            }
+            for (int p = pb; p < pe; p += UNR) {
+                int cc[UNR];
+                TV vv[UNR];
+#pragma unroll
+                for (int u = 0; u < UNR; u++) {
+                    const bool in = p + u < pe;
+                    cc[u] = in ? (staged ? s_col[p + u] : colidx[pbase + p + u]) : 0;
+                    vv[u] = in ? (staged ? s_val[p + u] : val[pbase + p + u]) : (TV)0;
+                }
+                vecld<TX, VW> xv[UNR];
+#pragma unroll
+                for (int u = 0; u < UNR; u++)
+                    if (p + u < pe)
+                        xv[u].load(x + (long)cc[u] * ldx + c0);
+#pragma unroll
+                for (int u = 0; u < UNR; u++)
+                    if (p + u < pe) {
+#pragma unroll
+                        for (int v = 0; v < VW; v++)
+                            acc[v] = add_rn(acc[v], mul_rn(vv[u], (TV)xv[u].v[v]));
+                    }
+            }
+#pragma unroll
+            for (int v = 0; v < VW; v++) {
+                TV out;
+                if (EPI == CACG_EPI_NONE) {
+                    out = acc[v];
+                } else if (EPI == CACG_EPI_RESID) {
+                    out = sub_rn(e1[v], acc[v]);
+                } else if (EPI == CACG_EPI_CHEB1) {
+                    out = add_rn(mul_rn(a, acc[v]), mul_rn(b, e1[v]));
+                    __builtin_nontemporal_store((TY)e1[v], copy0 + i * ldc + c0 + v);
+                } else {
+                    out = sub_rn(add_rn(mul_rn(a, acc[v]), mul_rn(b, e1[v])), e2[v]);
+                }
+                __builtin_nontemporal_store((TY)out, y + i * ldy + c0 + v);
+            }
+        }
+    }
+}
+
 template <typename TV, typename TX, typename TY, int LPR, int VW, int RPG, int EPI>
 __global__ __launch_bounds__(256) void spmm_kernel(
     int n, const int *__restrict__ rowptr, const int *__restrict__ colidx,
@@ -93,30 +188,17 @@ __global__ __launch_bounds__(256) void spmm_kernel(
 {
     constexpr int NG = 256 / LPR, TR = NG * RPG;
     constexpr int CAP = TR * 8 > 256 ? TR * 8 : 256;
-    constexpr int UNR = 8;
     __shared__ int s_rp[TR + 1];
     __shared__ int s_col[CAP];
     __shared__ TV s_val[CAP];
-    const TV a = (TV)ad, b = (TV)bd;
     const int t = threadIdx.x, g = t / LPR, lane = t % LPR;
     const int ntiles = (n + TR - 1) / TR;
-    // this workgroup's XCD and its slot among the XCD's workgroups
     const int nx = gridDim.x >= 8 ? 8 : 1;
     const int xcd = blockIdx.x % nx, gx = gridDim.x / nx, j = blockIdx.x / nx;
     const int band = (ntiles + nx - 1) / nx;
-    // the XCD's tiles in sweep order: lt-th of its band, or of its chunks;
-    // tile ids grow with lt in both orders, so the first one past the end ends
-    // the loop
     for (int lt = j;; lt += gx) {
-        int tile;
-        if (P == 0) {
-            if (lt >= band)
-                break;
-            tile = xcd * band + lt;
-        } else {
-            tile = ((lt / P) * nx + xcd) * P + lt % P;
-        }
-        if (tile >= ntiles)
+        const int tile = sweep_tile(lt, P, nx, xcd, band, ntiles);
+        if (tile < 0)
             break;
         const int r0 = tile * TR;
         const int rows = (n - r0) < TR ? (n - r0) : TR;
@@ -132,69 +214,115 @@ __global__ __launch_bounds__(256) void spmm_kernel(
                 s_val[e] = __builtin_nontemporal_load(val + pbase + e);
             }
         __syncthreads();
-        for (int cb = 0; cb < k; cb += LPR * VW) {
-            const int c0 = cb + lane * VW;
-            if (c0 >= k)
-                continue;
-            for (int rr = 0; rr < RPG; rr++) {
-                const int li = rr * NG + g;
-                if (li >= rows)
-                    continue;
-                const long i = r0 + li;
-                const int pb = s_rp[li] - pbase, pe = s_rp[li + 1] - pbase;
-                TV acc[VW], e1[VW], e2[VW];
+        tile_rows<TV, TX, TY, LPR, VW, RPG, EPI>(s_rp, s_col, s_val, staged, pbase, r0, rows, g,
+                                                 lane, colidx, val, k, x, ldx, y, ldy, (TV)ad,
+                                                 (TV)bd, p1, ld1, p2, ld2, copy0, ldc);
+        __syncthreads();
+    }
+}
+
+// The same with the tile staging pipelined (LPR 4 and 8: k = 16, 32): while a
+// tile is gathered, the (col, val) pairs of the next tile and the row
+// pointers of the one after are already in flight into registers, so the
+// chain row pointer -> (col, val) -> x is paid once per workgroup instead of
+// once per tile (8% on the 128^3 Laplacian at k = 16; no gain for wide rows,
+// which keep spmm_kernel, and not used for LPR <= 2, whose doubled LDS
+// staging would halve occupancy).  Two LDS buffers alternate between tiles.
+template <typename TV, typename TX, typename TY, int LPR, int VW, int RPG, int EPI>
+__global__ __launch_bounds__(256) void spmm_pipe_kernel(
+    int n, const int *__restrict__ rowptr, const int *__restrict__ colidx,
+    const TV *__restrict__ val, int k, const TX *__restrict__ x, long ldx, TY *__restrict__ y,
+    long ldy, double ad, double bd, const TY *__restrict__ p1, long ld1, const TY *__restrict__ p2,
+    long ld2, TY *__restrict__ copy0, long ldc, int P)
+{
+    constexpr int NG = 256 / LPR, TR = NG * RPG;
+    constexpr int CAP = TR * 8 > 256 ? TR * 8 : 256;
+    constexpr int PER = CAP / 256;  // staged pairs per thread
+    static_assert(CAP % 256 == 0, "CAP must be a multiple of the workgroup size");
+    __shared__ int s_rp[2][TR + 1];
+    __shared__ int s_col[2][CAP];
+    __shared__ TV s_val[2][CAP];
+    const int t = threadIdx.x, g = t / LPR, lane = t % LPR;
+    const int ntiles = (n + TR - 1) / TR;
+    const int nx = gridDim.x >= 8 ? 8 : 1;
+    const int xcd = blockIdx.x % nx, gx = gridDim.x / nx, j = blockIdx.x / nx;
+    const int band = (ntiles + nx - 1) / nx;
+    int lt = j;
+    int tile = sweep_tile(lt, P, nx, xcd, band, ntiles);
+    if (tile < 0)
+        return;
+    auto rows_of = [&](int tl) { return (n - tl * TR) < TR ? (n - tl * TR) : TR; };
+    // registers in flight: entry t of a tile's row pointers, and PER (col, val) pairs
+    int rp_reg = 0;
+    int col_reg[PER];
+    TV val_reg[PER];
+    // prologue: tile 0 staged in buffer 0, the row pointers of tile 1 in flight
+    int cur = 0;
+    int rows = rows_of(tile);
+    if (t <= rows)
+        s_rp[0][t] = rowptr[tile * TR + t];
+    __syncthreads();
+    {
+        const int pb = s_rp[0][0], tn = s_rp[0][rows] - pb;
+        if (tn <= CAP)
+            for (int e = t; e < tn; e += 256) {
+                s_col[0][e] = __builtin_nontemporal_load(colidx + pb + e);
+                s_val[0][e] = __builtin_nontemporal_load(val + pb + e);
+            }
+    }
+    int tile_n = sweep_tile(lt + gx, P, nx, xcd, band, ntiles);
+    if (tile_n >= 0 && t <= rows_of(tile_n))
+        rp_reg = rowptr[tile_n * TR + t];
+    __syncthreads();
+    for (;;) {
+        const int nxt = cur ^ 1;
+        const int r0 = tile * TR;
+        const int pbase = s_rp[cur][0];
+        const bool staged = s_rp[cur][rows] - pbase <= CAP;
+        // the next tile's row pointers to LDS, then its (col, val) pairs and the
+        // row pointers of the tile after it into registers
+        int rows_n = 0, pbn = 0, tnn = 0;
+        if (tile_n >= 0) {
+            rows_n = rows_of(tile_n);
+            if (t <= rows_n)
+                s_rp[nxt][t] = rp_reg;
+        }
+        __syncthreads();
+        const int tile_nn = tile_n >= 0 ? sweep_tile(lt + 2 * gx, P, nx, xcd, band, ntiles) : -1;
+        if (tile_n >= 0) {
+            pbn = s_rp[nxt][0];
+            tnn = s_rp[nxt][rows_n] - pbn;
 #pragma unroll
-                for (int v = 0; v < VW; v++) {
-                    acc[v] = (TV)0;
-                    if (EPI == CACG_EPI_RESID) {
-                        e1[v] = (TV)__builtin_nontemporal_load(p1 + i * ld1 + c0 + v);
-                    } else if (EPI == CACG_EPI_CHEB1) {
-                        e1[v] = (TV)x[i * ldx + c0 + v];
-                    } else if (EPI == CACG_EPI_CHEB) {
-                        e1[v] = (TV)p1[i * ld1 + c0 + v];
-                        e2[v] = (TV)__builtin_nontemporal_load(p2 + i * ld2 + c0 + v);
-                    }
-                }
-                for (int p = pb; p < pe; p += UNR) {
-                    int cc[UNR];
-                    TV vv[UNR];
+            for (int u = 0; u < PER; u++) {
+                const int e = t + 256 * u;
+                const bool in = tnn <= CAP && e < tnn;
+                col_reg[u] = in ? __builtin_nontemporal_load(colidx + pbn + e) : 0;
+                val_reg[u] = in ? __builtin_nontemporal_load(val + pbn + e) : (TV)0;
+            }
+            if (tile_nn >= 0 && t <= rows_of(tile_nn))
+                rp_reg = rowptr[tile_nn * TR + t];
+        }
+        tile_rows<TV, TX, TY, LPR, VW, RPG, EPI>(s_rp[cur], s_col[cur], s_val[cur], staged, pbase,
+                                                 r0, rows, g, lane, colidx, val, k, x, ldx, y, ldy,
+                                                 (TV)ad, (TV)bd, p1, ld1, p2, ld2, copy0, ldc);
+        if (tile_n < 0)
+            break;
+        if (tnn <= CAP) {
 #pragma unroll
-                    for (int u = 0; u < UNR; u++) {
-                        const bool in = p + u < pe;
-                        cc[u] = in ? (staged ? s_col[p + u] : colidx[pbase + p + u]) : 0;
-                        vv[u] = in ? (staged ? s_val[p + u] : val[pbase + p + u]) : (TV)0;
-                    }
-                    vecld<TX, VW> xv[UNR];
-#pragma unroll
-                    for (int u = 0; u < UNR; u++)
-                        if (p + u < pe)
-                            xv[u].load(x + (long)cc[u] * ldx + c0);
-#pragma unroll
-                    for (int u = 0; u < UNR; u++)
-                        if (p + u < pe) {
-#pragma unroll
-                            for (int v = 0; v < VW; v++)
-                                acc[v] = add_rn(acc[v], mul_rn(vv[u], (TV)xv[u].v[v]));
-                        }
-                }
-#pragma unroll
-                for (int v = 0; v < VW; v++) {
-                    TV out;
-                    if (EPI == CACG_EPI_NONE) {
-                        out = acc[v];
-                    } else if (EPI == CACG_EPI_RESID) {
-                        out = sub_rn(e1[v], acc[v]);
-                    } else if (EPI == CACG_EPI_CHEB1) {
-                        out = add_rn(mul_rn(a, acc[v]), mul_rn(b, e1[v]));
-                        __builtin_nontemporal_store((TY)e1[v], copy0 + i * ldc + c0 + v);
-                    } else {
-                        out = sub_rn(add_rn(mul_rn(a, acc[v]), mul_rn(b, e1[v])), e2[v]);
-                    }
-                    __builtin_nontemporal_store((TY)out, y + i * ldy + c0 + v);
+            for (int u = 0; u < PER; u++) {
+                const int e = t + 256 * u;
+                if (e < tnn) {
+                    s_col[nxt][e] = col_reg[u];
+                    s_val[nxt][e] = val_reg[u];
                 }
             }
         }
         __syncthreads();
+        cur = nxt;
+        lt += gx;
+        tile = tile_n;
+        rows = rows_n;
+        tile_n = tile_nn;
     }
 }
 
@@ -220,8 +348,16 @@ int launch_spmm(int n, const int *rowptr, const int *col, const TV *val, int k, 
     if (sc && sc->chunk_rows > 0)
         P = sc->chunk_rows >= TR ? sc->chunk_rows / TR : 1;
 #define CACG_SPMM_CASE(E)                                                                        \
-    hipLaunchKernelGGL((spmm_kernel<TV, TX, TY, LPR, VW, RPG, E>), dim3(G), dim3(256), 0, st, n,  \
-                       rowptr, col, val, k, x, ldx, y, ldy, a, b, p1, ld1, p2, ld2, copy0, ldc, P)
+    do {                                                                                         \
+        if constexpr (LPR == 4 || LPR == 8)                                                      \
+            hipLaunchKernelGGL((spmm_pipe_kernel<TV, TX, TY, LPR, VW, RPG, E>), dim3(G), dim3(256), \
+                               0, st, n, rowptr, col, val, k, x, ldx, y, ldy, a, b, p1, ld1, p2,    \
+                               ld2, copy0, ldc, P);                                              \
+        else                                                                                     \
+            hipLaunchKernelGGL((spmm_kernel<TV, TX, TY, LPR, VW, RPG, E>), dim3(G), dim3(256), 0, \
+                               st, n, rowptr, col, val, k, x, ldx, y, ldy, a, b, p1, ld1, p2, ld2, \
+                               copy0, ldc, P);                                                   \
+    } while (0)
     constexpr bool f64 = sizeof(TV) == 8 && sizeof(TX) == 8 && sizeof(TY) == 8;
     constexpr bool f32 = sizeof(TV) == 4 && sizeof(TX) == 4 && sizeof(TY) == 4;
     if constexpr (f64) {
