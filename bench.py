@@ -23,7 +23,9 @@ Beside the headline line (unless --no-forms), the same JSON object carries
 the two further forms, each measured in this run:
   precision_form : the same workload with the FP32 basis and FP64 refinement,
                    its solves/s against the FP64 path and its solution's
-                   error against the FP64 solution;
+                   error against the FP64 solution; then the same Laplacian
+                   and its random SPD variant (seed 2) with 16 RHS N(0,1)
+                   from seed 2, each against its own FP64 run;
   batching_form  : 256 shifted 2D Laplacians (L log-uniform in [32, 256],
                    shift U(0, 0.1), m from {1, 2, 4, 8, 16, 32}, seed 3,
                    s = 4, 1e-10) in one batch against the same systems solved
@@ -291,12 +293,56 @@ def precision_form(A, B, X64, dB, dX, m, s, args, ms64):
     ms = float(np.mean(lat) * 1e3)
     print(f"bench: precision form {ms:.1f} ms per solve, {reps[-1]['iters']} iterations",
           file=sys.stderr, flush=True)
-    return {"workload": "same system, FP32 matrix copy / basis / Chebyshev SpMM, FP64 Q, A Q, Gram, "
-                        "LU, X, R; true residual recomputed in FP64 (residual replacement)",
-            "solves_per_s": 1e3 / ms, "ms_per_solve": ms, "vs_fp64_path": ms64 / ms,
-            "iters": reps[-1]["iters"], "refinements": reps[-1]["refinements"],
-            "status": reps[-1]["status"], "true_relres_max": reps[-1]["true_relres"],
-            "err_vs_fp64_solution": float(err.max())}
+    out = {"workload": "same system, FP32 matrix copy / basis / Chebyshev SpMM, FP64 Q, A Q, Gram, "
+                       "LU, X, R; true residual recomputed in FP64 (residual replacement)",
+           "solves_per_s": 1e3 / ms, "ms_per_solve": ms, "vs_fp64_path": ms64 / ms,
+           "iters": reps[-1]["iters"], "refinements": reps[-1]["refinements"],
+           "status": reps[-1]["status"], "true_relres_max": reps[-1]["true_relres"],
+           "err_vs_fp64_solution": float(err.max())}
+    # the 128^3 Laplacian and its random SPD variant (same pattern, off-diagonals
+    # -U(0.5, 1.5) seed 2, diagonal 1.01 x absolute row sum), 16 RHS N(0, 1) seed 2,
+    # each against its own FP64 run
+    from cacg_amd import problems as P
+    out["normal_rhs"] = {}
+    for name, mk in (("laplacian", lambda: A), ("random_spd", lambda: P.random_spd_like(A, 2))):
+        Am = mk()
+        Bn = P.rhs_normal(Am.n, m, 2)
+        r64 = _timed_solves(Am, Bn, m, s, args, "fp64")
+        r32 = _timed_solves(Am, Bn, m, s, args, "fp32")
+        e = np.linalg.norm(r32["X"] - r64["X"], axis=0) / np.linalg.norm(r64["X"], axis=0)
+        out["normal_rhs"][name] = {
+            "fp64_ms_per_solve": r64["ms"], "fp64_iters": r64["rep"]["iters"],
+            "ms_per_solve": r32["ms"], "iters": r32["rep"]["iters"],
+            "refinements": r32["rep"]["refinements"], "vs_fp64_path": r64["ms"] / r32["ms"],
+            "status": r32["rep"]["status"], "true_relres_max": r32["rep"]["true_relres"],
+            "err_vs_fp64_solution": float(e.max())}
+        print(f"bench: precision form, {name} N(0,1) RHS: FP64 {r64['ms']:.1f} ms "
+              f"({r64['rep']['iters']} it), FP32/refined {r32['ms']:.1f} ms "
+              f"({r32['rep']['iters']} it)", file=sys.stderr, flush=True)
+    return out
+
+
+def _timed_solves(A, B, m, s, args, precision, reps=3):
+    """Mean wall time of `reps` solves from x0 = 0 after one untimed solve."""
+    import cacg_amd
+    L = cacg_amd.lib()
+    maxiter = args.maxiter if precision == "fp64" else 3 * args.maxiter
+    sv = cacg_amd.Solver(A, m, s, tol=args.tol, maxiter=maxiter, order="fast",
+                         precision=precision)
+    dB = cacg_amd.DeviceBuffer.from_array(B)
+    dX = cacg_amd.DeviceBuffer(B.nbytes)
+    lat = []
+    for k in range(1 + reps):
+        L.cacg_k_fill_f64(dX.ptr, A.n * m, 0.0, None)
+        L.cacg_dev_sync()
+        t = time.perf_counter()
+        rep = sv.solve(dB, dX)
+        L.cacg_dev_sync()
+        if k > 0:
+            lat.append(time.perf_counter() - t)
+    X = dX.download(B.shape)
+    sv.close()
+    return {"ms": float(np.mean(lat) * 1e3), "rep": rep, "X": X}
 
 
 def batching_form(count=256):

@@ -204,14 +204,23 @@ __global__ __launch_bounds__(512) void qupdate_mfma_kernel(int n, const TS *__re
             for (int u = 0; u < KQ; u++)
                 av[u] = 0.0;
         }
-#pragma unroll 1
-        for (int j = 0; j < NTL; j++) {
-            double sv[4];
+        // S for one 16-column tile: FP64 S by guarded loads (they stay in
+        // flight until the subtraction); FP32 S from a clamped row without a
+        // branch, loaded a tile ahead into alternating registers and widened at
+        // the subtraction (a guarded FP32 load is widened, and so waited for, on
+        // the spot).  The FP32 basis is never absent, so it has no null test.
+        auto load_s = [&](int j, TS *dst) {
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const long i = i0 + lg + 4 * r;
-                sv[r] = (i < n && S) ? (double)S[i * SM + j * 16 + lc] : 0.0;
+                if constexpr (sizeof(TS) == 4)
+                    dst[r] = S[(i < n ? i : n - 1) * SM + j * 16 + lc];
+                else
+                    dst[r] = (i < n && S) ? S[i * SM + j * 16 + lc] : (TS)0;
             }
+        };
+        // Q(:, tile j) = S - Q beta for the rows of this tile
+        auto update_tile = [&](int j, const TS *sv) {
             dbl4 acc = {0.0, 0.0, 0.0, 0.0};
             const double *bcol = bl + (lg * KQ) * SM + j * 16 + lc;
 #pragma unroll
@@ -221,7 +230,42 @@ __global__ __launch_bounds__(512) void qupdate_mfma_kernel(int n, const TS *__re
             for (int r = 0; r < 4; r++) {
                 const long i = i0 + lg + 4 * r;
                 if (i < n)
-                    Q[i * SM + j * 16 + lc] = (T)(sv[r] - acc[r]);
+                    Q[i * SM + j * 16 + lc] = (T)((double)sv[r] - acc[r]);
+            }
+        };
+        if constexpr (sizeof(TS) == 4 && NTL % 2 == 0) {
+            TS sa[4], sb[4];
+            load_s(0, sa);
+#pragma unroll 1
+            for (int j = 0; j < NTL; j += 2) {
+                if (j + 1 < NTL)
+                    load_s(j + 1, sb);
+                update_tile(j, sa);
+                if (j + 1 >= NTL)
+                    break;
+                if (j + 2 < NTL)
+                    load_s(j + 2, sa);
+                update_tile(j + 1, sb);
+            }
+        } else if constexpr (sizeof(TS) == 4) {
+            // odd tile counts: one tile ahead through a register copy
+            TS sv[4], svn[4];
+            load_s(0, sv);
+#pragma unroll 1
+            for (int j = 0; j < NTL; j++) {
+                if (j + 1 < NTL)
+                    load_s(j + 1, svn);
+                update_tile(j, sv);
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    sv[r] = svn[r];
+            }
+        } else {
+#pragma unroll 1
+            for (int j = 0; j < NTL; j++) {
+                TS sv[4];
+                load_s(j, sv);
+                update_tile(j, sv);
             }
         }
         if (NTX > 0) {

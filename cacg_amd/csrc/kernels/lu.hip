@@ -19,7 +19,14 @@
 // Backward substitution: the reference subtracts x[q] u_iq in ascending q,
 // which makes each row wait for all later rows; mode REFERENCE keeps that
 // order (one sequential chain per column, x in LDS, U from the scratch copy).
-// Mode FAST runs it column-oriented (descending q, barrier per step).
+// Mode FAST runs it column-oriented (descending q).
+//
+// With the factor in LDS it is stored column-major with an odd stride
+// (element (r, c) at c (N + 1) + r), so the reads of a factor column by
+// consecutive rows are consecutive words, and fast-order substitutions run
+// wave-local: every right-hand-side column is solved by the lanes of one
+// wave, which pass finished rows to each other by readlane or lane shuffle,
+// with no barrier (lu_subst_wave).
 #include <math.h>
 
 #include "kcommon.hpp"
@@ -44,15 +51,119 @@ __host__ __device__ inline int lu_columns(int N, int k, bool in_lds)
     return kc;
 }
 
-// dynamic LDS of a solve: factor, broadcast rows (2 x kc), pivot row/column
-// (2 x 2 x 128), 2 reciprocal pivots, N reciprocal diagonal entries; without
-// the LDS factor an N x kc block instead of the factor
+// dynamic LDS of a solve: factor (N x (N + 1)), broadcast rows (2 x kc),
+// pivot row/column (2 x 2 x 128), 2 reciprocal pivots, N reciprocal diagonal
+// entries; without the LDS factor an N x kc block instead of the factor
 __host__ __device__ inline size_t lu_lds_bytes(int N, int k, bool in_lds)
 {
     const int kc = lu_columns(N, k, in_lds);
     if (in_lds)
-        return ((size_t)N * N + 2 * (size_t)kc + 512 + 2 + N) * sizeof(double);
+        return ((size_t)N * (N + 1) + 2 * (size_t)kc + 512 + 2 + N) * sizeof(double);
     return ((size_t)N * kc + 2 * (size_t)kc + N) * sizeof(double);
+}
+
+// position of factor element (r, c): column-major with stride N + 1 in LDS,
+// row-major in the global scratch block
+template <bool IN_LDS>
+__device__ __forceinline__ long fidx(int N, int r, int c)
+{
+    return IN_LDS ? (long)c * (N + 1) + r : (long)r * N + c;
+}
+
+// v of lane src (src the same on every lane of the wave)
+__device__ __forceinline__ double readlane_f64(double v, int src)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, src);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// v of lane src of this lane's group of LPC lanes
+template <int LPC>
+__device__ __forceinline__ double group_bcast(double v, int src)
+{
+    if constexpr (LPC == 64)
+        return readlane_f64(v, src);
+    else
+        return __shfl(v, src, LPC);
+}
+
+// Fast-order forward and backward substitutions with the factor in LDS
+// (a: column-major, stride N + 1; dinv: reciprocal diagonal of L), N <= 128.
+// Right-hand-side column `col` is solved by LPC lanes of one wave (64 / LPC
+// columns per wave, 1024 / LPC per pass); lane lc holds rows lc + LPC r in
+// registers.  Row q is finished by its lane and handed to the others of the
+// group (readlane, or a lane shuffle for LPC < 64), which update their rows:
+// ascending q for L y = rhs with the reciprocal pivot applied last, then
+// descending q for U x = y (unit U).  Each element receives exactly the
+// operations of the barrier-per-step form, in the same order, so the result
+// is the same bits; nothing is shared between waves, so no barrier is needed.
+template <int LPC>
+__device__ __forceinline__ void lu_subst_wave(int N, const double *__restrict__ a,
+                                              const double *__restrict__ dinv, int k,
+                                              const double *__restrict__ rhs,
+                                              double *__restrict__ x)
+{
+    constexpr int RPL = 128 / LPC;  // rows per lane
+    constexpr int WC = 64 / LPC;    // columns per wave
+    const int R = N + 1;
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    const int lc = l % LPC;
+    for (int c0 = 0; c0 < k; c0 += (LU_T / 64) * WC) {
+        const int col = c0 + w * WC + l / LPC;
+        const bool has = col < k;
+        double y[RPL];
+#pragma unroll
+        for (int r = 0; r < RPL; r++) {
+            const int i = lc + LPC * r;
+            y[r] = (has && i < N) ? rhs[(long)i * k + col] : 0.0;
+        }
+#pragma unroll
+        for (int rq = 0; rq < RPL; rq++) {
+#pragma unroll 1
+            for (int lq = 0; lq < LPC; lq++) {
+                const int q = lq + LPC * rq;
+                if (q >= N)
+                    break;
+                if (lc == lq)
+                    y[rq] = mul_rn(y[rq], dinv[q]);
+                const double yq = group_bcast<LPC>(y[rq], lq);
+                const double *Lq = a + (long)q * R;
+#pragma unroll
+                for (int r = rq; r < RPL; r++) {
+                    const int i = lc + LPC * r;
+                    if (i > q && i < N)
+                        y[r] = sub_rn(y[r], mul_rn(yq, Lq[i]));
+                }
+            }
+        }
+#pragma unroll
+        for (int rq = RPL - 1; rq >= 0; rq--) {
+#pragma unroll 1
+            for (int lq = LPC - 1; lq >= 0; lq--) {
+                const int q = lq + LPC * rq;
+                if (q >= N)
+                    continue;
+                const double xq = group_bcast<LPC>(y[rq], lq);
+                const double *Uq = a + (long)q * R;
+#pragma unroll
+                for (int r = 0; r <= rq; r++) {
+                    const int i = lc + LPC * r;
+                    if (i < q)
+                        y[r] = sub_rn(y[r], mul_rn(xq, Uq[i]));
+                }
+            }
+        }
+        if (has) {
+#pragma unroll
+            for (int r = 0; r < RPL; r++) {
+                const int i = lc + LPC * r;
+                if (i < N)
+                    x[(long)i * k + col] = y[r];
+            }
+        }
+    }
 }
 
 // The solve of one system by one workgroup of LU_T lanes; lds is the dynamic
@@ -65,7 +176,7 @@ __device__ __forceinline__ void lu_solve_body(int N, const double *__restrict__ 
                                               int *flag, int mode, int kc, int reuse, double *lds)
 {
     double *a = IN_LDS ? lds : gw;
-    double *rowbuf = IN_LDS ? lds + (long)N * N : lds;
+    double *rowbuf = IN_LDS ? lds + (long)N * (N + 1) : lds;
     double *xs = IN_LDS ? lds : lds + 2 * kc;
     const int t = threadIdx.x;
 
@@ -74,7 +185,7 @@ __device__ __forceinline__ void lu_solve_body(int N, const double *__restrict__ 
     if (reuse) {
         if (IN_LDS) {
             for (long e = t; e < (long)N * N; e += LU_T)
-                a[e] = gw[e];
+                a[fidx<true>(N, (int)(e / N), (int)(e % N))] = gw[e];
         }
         __syncthreads();
     } else if (IN_LDS) {
@@ -92,7 +203,7 @@ __device__ __forceinline__ void lu_solve_body(int N, const double *__restrict__ 
                 const int r = tr + 32 * i, c = tc + 32 * j;
                 v[i][j] = (r < N && c < N) ? lhs[(long)r * N + c] : 0.0;
             }
-        double *ub = lds + (long)N * N + 2 * kc;  // 2 x 128: unscaled row p
+        double *ub = lds + (long)N * (N + 1) + 2 * kc;  // 2 x 128: unscaled row p
         double *lb = ub + 256;                     // 2 x 128: column p
         double *ib = lb + 256;                     // 2: reciprocal pivot
         // pivot p = 32 ip + pr: ip is a compile-time index into the register block
@@ -156,7 +267,7 @@ __device__ __forceinline__ void lu_solve_body(int N, const double *__restrict__ 
             for (int j = 0; j < 4; j++) {
                 const int r = tr + 32 * i, c = tc + 32 * j;
                 if (r < N && c < N) {
-                    a[(long)r * N + c] = v[i][j];
+                    a[fidx<true>(N, r, c)] = v[i][j];
                     gw[(long)r * N + c] = v[i][j];
                 }
             }
@@ -187,10 +298,25 @@ __device__ __forceinline__ void lu_solve_body(int N, const double *__restrict__ 
     if (t == 0 && bad)
         *flag = 1;
     // reciprocal diagonal of L for the forward pass (the reference's 1.0 / L_ii)
-    double *dinv = IN_LDS ? lds + (long)N * N + 2 * kc + 512 + 2 : rowbuf + 2 * kc + (long)N * kc;
+    double *dinv = IN_LDS ? lds + (long)N * (N + 1) + 2 * kc + 512 + 2
+                          : rowbuf + 2 * kc + (long)N * kc;
     for (int q = t; q < N; q += LU_T)
-        dinv[q] = 1.0 / a[(long)q * N + q];
+        dinv[q] = 1.0 / a[fidx<IN_LDS>(N, q, q)];
     __syncthreads();
+
+    if constexpr (IN_LDS) {
+        if (mode == CACG_MODE_FAST) {
+            if (k <= 16)
+                lu_subst_wave<64>(N, a, dinv, k, rhs, x);
+            else if (k <= 32)
+                lu_subst_wave<32>(N, a, dinv, k, rhs, x);
+            else if (k <= 64)
+                lu_subst_wave<16>(N, a, dinv, k, rhs, x);
+            else
+                lu_subst_wave<8>(N, a, dinv, k, rhs, x);
+            return;
+        }
+    }
 
     // ---- substitutions, kc columns at a time
     for (int c0 = 0; c0 < k; c0 += kc) {
@@ -201,7 +327,7 @@ __device__ __forceinline__ void lu_solve_body(int N, const double *__restrict__ 
         if (IN_LDS && mode == CACG_MODE_REFERENCE && c0 > 0) {
             // the previous chunk's backward pass reused the factor's LDS
             for (long e = t; e < (long)N * N; e += LU_T)
-                a[e] = gw[e];
+                a[fidx<true>(N, (int)(e / N), (int)(e % N))] = gw[e];
             __syncthreads();
         }
         double y[JMAX];
@@ -230,7 +356,7 @@ __device__ __forceinline__ void lu_solve_body(int N, const double *__restrict__ 
                     for (int j = jq; j < JMAX; j++) {
                         const int i = g + G * j;
                         if ((j > jq || g > gq) && i < N)
-                            y[j] = sub_rn(y[j], mul_rn(yq, a[(long)i * N + q]));
+                            y[j] = sub_rn(y[j], mul_rn(yq, a[fidx<IN_LDS>(N, i, q)]));
                     }
                 }
             }
@@ -287,7 +413,7 @@ __device__ __forceinline__ void lu_solve_body(int N, const double *__restrict__ 
                         for (int j = 0; j <= jq; j++) {
                             const int i = g + G * j;
                             if (j < jq || g < gq)
-                                y[j] = sub_rn(y[j], mul_rn(xq, a[(long)i * N + q]));
+                                y[j] = sub_rn(y[j], mul_rn(xq, a[fidx<IN_LDS>(N, i, q)]));
                         }
                     }
                 }

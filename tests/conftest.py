@@ -51,3 +51,33 @@ def gpu_lib():
     if L.cacg_device_count() < 1:
         pytest.fail("no HIP device visible to a test marked gpu")
     return cacg_amd
+
+
+def lu_restated(lhs, rhs, backward="reference"):
+    """The small solve in numpy, operation for operation: Crout factorisation
+    (unit-diagonal U, no pivoting; src/DLSsolvers.c:106-208), forward
+    substitution with ascending subtractions and the reciprocal pivot applied
+    last, then the backward substitution either in the reference's order
+    (row by row from the bottom, ascending subtractions) or in the fast order
+    (column by column, descending q).  numpy rounds every product and every
+    difference separately, as the kernels do."""
+    a = np.array(lhs, dtype=np.float64)
+    N = a.shape[0]
+    for p in range(N):
+        inv = 1.0 / a[p, p]
+        a[p, p + 1:] = a[p, p + 1:] * inv
+        a[p + 1:, p + 1:] = a[p + 1:, p + 1:] - a[p + 1:, p:p + 1] * a[p:p + 1, p + 1:]
+    y = np.array(rhs, dtype=np.float64)
+    for i in range(N):
+        for q in range(i):
+            y[i] = y[i] - y[q] * a[i, q]
+        y[i] = y[i] * (1.0 / a[i, i])
+    x = y.copy()
+    if backward == "reference":
+        for i in range(N - 2, -1, -1):
+            for q in range(i + 1, N):
+                x[i] = x[i] - x[q] * a[i, q]
+    else:
+        for q in range(N - 1, 0, -1):
+            x[:q] = x[:q] - a[:q, q:q + 1] * x[q:q + 1]
+    return x

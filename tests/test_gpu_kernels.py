@@ -173,6 +173,25 @@ def test_lu_solve(gpu_lib, oracle_lib, N, k):
     assert np.array_equal(x2, oracle_lib.lu_solve(A, rhs2))
 
 
+@pytest.mark.parametrize("N,k", [(1, 1), (4, 4), (9, 3), (16, 16), (64, 16), (100, 7), (128, 1),
+                                 (128, 16), (128, 17), (128, 32), (128, 33), (128, 64),
+                                 (128, 65), (128, 128), (128, 300)])
+def test_lu_solve_fast_order_bitwise(gpu_lib, N, k):
+    # the fast order (wave-local substitutions with the factor in LDS) against
+    # its numpy restatement, bit for bit, fresh and with the kept factor
+    from conftest import lu_restated
+    A = _spd(N, 3 * N + k)
+    rhs = np.random.default_rng(N + k).standard_normal((N, k))
+    print(f"lu fast N={N} k={k}", flush=True)
+    x, flag = gpu_lib.k_lu_solve(A, rhs, mode="fast")
+    assert flag == 0
+    assert np.array_equal(x, lu_restated(A, rhs, "fast"))
+    rhs2 = np.random.default_rng(N + k + 1).standard_normal((N, max(1, k // 3)))
+    x1, x2, flag = gpu_lib.k_lu_solve(A, rhs, mode="fast", rhs2=rhs2)
+    assert flag == 0 and np.array_equal(x1, x)
+    assert np.array_equal(x2, lu_restated(A, rhs2, "fast"))
+
+
 def test_lu_zero_pivot_sets_flag(gpu_lib):
     A = _spd(8, 1)
     A[0, 0] = 0.0

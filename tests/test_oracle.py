@@ -112,6 +112,22 @@ def test_lu_solve_against_lapack(oracle_lib):
     assert np.allclose(A @ x, rhs, rtol=0, atol=1e-11)
 
 
+def test_lu_restatement_is_the_reference_solve(oracle_lib):
+    # the numpy restatement the fast-order kernel test uses: with the reference's
+    # backward order it is the oracle's solve bit for bit (factor, forward pass);
+    # with the fast order it is the same solve to rounding
+    from conftest import lu_restated
+    rng = np.random.default_rng(5)
+    for N, k in [(1, 1), (5, 3), (17, 4), (40, 40)]:
+        M = rng.standard_normal((N, N))
+        A = M @ M.T + N * np.eye(N)
+        rhs = rng.standard_normal((N, k))
+        assert np.array_equal(lu_restated(A, rhs, "reference"), oracle_lib.lu_solve(A, rhs))
+        xf = lu_restated(A, rhs, "fast")
+        assert np.allclose(xf, np.linalg.solve(A, rhs), rtol=1e-11, atol=1e-13)
+        assert not np.array_equal(xf, lu_restated(A, rhs, "reference")) or N < 5
+
+
 def test_lu_solve_triangular_exact(oracle_lib):
     # A = L U with integer L (lower, diag 1 or 2) and unit upper U: Crout recovers
     # L and U exactly, and the substitutions solve an integer system exactly
