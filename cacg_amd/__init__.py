@@ -132,6 +132,9 @@ def lib() -> ctypes.CDLL:
         "cacg_k_scale_f64": ([_vp, c_long, c_double, _vp], c_int),
         "cacg_k_split_supported": ([c_int, c_int], c_int),
         "cacg_k_qxupdate": ([c_int, c_int, c_int, _vp, _vp, _vp, _vp, _vp, _vp], c_int),
+        "cacg_k_qxupdate_blk": ([c_int, c_int, c_int, _vp, c_long, _vp, _vp, _vp, _vp, _vp], c_int),
+        "cacg_k_gram_fast_blk": ([c_int, _vp, c_long, c_int, _vp, c_int, c_long, c_int, _vp, _vp,
+                                  _vp], c_int),
         "cacg_k_xrupdate_split": ([c_int, c_int, c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, c_int,
                                    _vp], c_int),
         "cacg_mtx_read": ([ctypes.c_char_p, _lp, _lp, _lp, ctypes.POINTER(_lp),
@@ -330,6 +333,46 @@ def k_qxupdate(S, Q, beta, alpha, X):
     da, dX = DeviceBuffer.from_array(alpha), DeviceBuffer.from_array(X)
     _check(lib().cacg_k_qxupdate(n, sm, m, dS.ptr, dQ.ptr, db.ptr, da.ptr, dX.ptr, None), "qxupdate")
     return dQ.download((n, sm)), dX.download((n, m))
+
+
+def to_blocks(S, m, gap=0):
+    """S (n x s m, row-major) as its blocks S_j (n x m each) one after the
+    other, block j at j * (n m + gap) elements; returns the flat array."""
+    S = np.ascontiguousarray(S, dtype=np.float64)
+    n, sm = S.shape
+    bs = n * m + gap
+    out = np.full(bs * (sm // m), np.nan)
+    for j in range(sm // m):
+        out[j * bs:j * bs + n * m] = S[:, j * m:(j + 1) * m].reshape(-1)
+    return out, bs
+
+
+def k_qxupdate_blk(S, Q, beta, alpha, X, gap=0):
+    """k_qxupdate with S passed block-major (blocks of m columns, gap elements
+    of padding between blocks); returns (Q, X)."""
+    S = np.ascontiguousarray(S, dtype=np.float64)
+    n, sm = S.shape
+    m = alpha.shape[1]
+    Sb, bs = to_blocks(S, m, gap)
+    dS, dQ, db = DeviceBuffer.from_array(Sb), DeviceBuffer.from_array(Q), DeviceBuffer.from_array(beta)
+    da, dX = DeviceBuffer.from_array(alpha), DeviceBuffer.from_array(X)
+    _check(lib().cacg_k_qxupdate_blk(n, sm, m, dS.ptr, bs, dQ.ptr, db.ptr, da.ptr, dX.ptr, None),
+           "qxupdate_blk")
+    return dQ.download((n, sm)), dX.download((n, m))
+
+
+def k_gram_blk(L, R1, bw, gap=0):
+    """Fast-order L^T R1 with R1 passed block-major (blocks of bw columns)."""
+    L = np.ascontiguousarray(L, dtype=np.float64)
+    n, kl = L.shape
+    kr1 = R1.shape[1]
+    Rb, bs = to_blocks(R1, bw, gap)
+    dL, dR = DeviceBuffer.from_array(L), DeviceBuffer.from_array(Rb)
+    g1 = DeviceBuffer(8 * kl * kr1)
+    ws = DeviceBuffer(8 * lib().cacg_gram_workspace(kl, kr1))
+    _check(lib().cacg_k_gram_fast_blk(n, dL.ptr, kl, kl, dR.ptr, bw, bs, kr1, g1.ptr, ws.ptr, None),
+           "gram_fast_blk")
+    return g1.download((kl, kr1))
 
 
 def k_xrupdate_split(Q, AQ, alpha, X, R, which):

@@ -66,6 +66,8 @@ struct cacg_solver {
     cacg_spmm_sched sch_basis, sch_aq;
     int tuned;
     hipEvent_t tev[2];
+    /* 1: keep the fast order's basis row-major (CACG_BASIS_ROWS=1 at creation) */
+    int basis_rows;
     /* CACG_BOUNDS_LANCZOS: work space, coefficients, the interval once computed */
     double *d_lz_work, *d_lz_ab, *h_lz;
     int lz_steps, have_lz;
@@ -402,6 +404,10 @@ int cacg_solver_create(cacg_solver **out, long n, const long *row_start, const l
             cacg_solver_destroy(sv);
             return CACG_EDEVICE;
         }
+    {
+        const char *br = getenv("CACG_BASIS_ROWS");
+        sv->basis_rows = br && br[0] == '1';
+    }
     const long nm = n * m, nsm = n * sv->sm, sm = sv->sm;
     long gram_ws = cacg_gram_workspace((int)sm, (int)(sm + m));
     rc = CACG_OK;
@@ -533,10 +539,9 @@ static int gram(cacg_solver *sv, int mode, const double *L, const double *R1, lo
 
 /* device time of `reps` launches of one SpMM shape under schedule sc, after
  * one untimed launch; a negative value when a launch failed */
-static float time_spmm(cacg_solver *sv, int f32, int k, const void *x, void *y, const void *p1,
-                       const void *p2, int epi, const cacg_spmm_sched *sc, int reps)
+static float time_spmm(cacg_solver *sv, int f32, int k, long ld, const void *x, void *y,
+                       const void *p1, const void *p2, int epi, const cacg_spmm_sched *sc, int reps)
 {
-    const long ld = sv->sm;
     float ms = -1.f;
     for (int r = -1; r < reps; r++) {
         if (r == 0 && hipEventRecord(sv->tev[0], sv->st) != hipSuccess)
@@ -562,7 +567,7 @@ static float time_spmm(cacg_solver *sv, int f32, int k, const void *x, void *y, 
  * k = sm) by timing a few schedules on this matrix, once per solver.  The
  * work blocks serve as scratch (zero-filled first; the solve overwrites
  * them).  A schedule changes no result, so neither does the choice. */
-static int tune_spmm(cacg_solver *sv)
+static int tune_spmm(cacg_solver *sv, int blk)
 {
     static const cacg_spmm_sched cand[] = {{0, 0},    {4096, 0},    {8192, 0},   {0, 1024},
                                            {0, 2048}, {0, 4096},    {4096, 2048}, {8192, 4096}};
@@ -588,12 +593,14 @@ static int tune_spmm(cacg_solver *sv)
     void *by = f32 ? (void *)(sv->f_S + (sm > 2 * m ? 2 * m : 0)) : (void *)sv->d_Q;
     float best_b = -1.f, best_a = -1.f;
     for (int c = 0; c < nc; c++) {
-        const float tb = time_spmm(sv, f32, m, bx, by, bx, bp2, CACG_EPI_CHEB, &cand[c], 2);
+        /* the basis steps' leading dimension: m for a block-major basis */
+        const float tb = time_spmm(sv, f32, m, blk ? m : sm, bx, by, bx, bp2, CACG_EPI_CHEB,
+                                   &cand[c], 2);
         if (tb >= 0.f && (best_b < 0.f || tb < best_b)) {
             best_b = tb;
             sv->sch_basis = cand[c];
         }
-        const float ta = time_spmm(sv, 0, sm, sv->d_Q, sv->d_AQ, NULL, NULL, CACG_EPI_NONE,
+        const float ta = time_spmm(sv, 0, sm, sm, sv->d_Q, sv->d_AQ, NULL, NULL, CACG_EPI_NONE,
                                    &cand[c], 2);
         if (ta >= 0.f && (best_a < 0.f || ta < best_a)) {
             best_a = ta;
@@ -663,6 +670,12 @@ int cacg_solver_solve_device(cacg_solver *sv, const double *d_B, double *d_X, ca
     const int n = (int)sv->n, m = (int)sv->m, s = (int)sv->s, sm = (int)sv->sm;
     /* fast order: the X update deferred into the next Q update (Q read once) */
     const int split = mode == CACG_MODE_FAST && cacg_k_split_supported(sm, m);
+    /* fast order: from iteration 1 on, the basis is stored block-major (S_j as
+     * n x m blocks n m elements apart), so each basis step reads and writes
+     * whole contiguous rows; iteration 0 writes it row-major into Q (Q = S) */
+    const int blk = split && m % 4 == 0 && s >= 2 && !sv->basis_rows &&
+                    cacg_gram_blk_supported(n, sm, sm, m);
+    const long bs = (long)n * m;
     hipStream_t st = sv->st;
     double *h_norm2 = sv->h_pin, *h_bnorm2 = sv->h_pin + m, *h_bounds = sv->h_pin + 2 * m;
     int *h_flag = (int *)(sv->h_pin + 2 * m + 2);
@@ -679,7 +692,7 @@ int cacg_solver_solve_device(cacg_solver *sv, const double *d_B, double *d_X, ca
     }
 
     if (!sv->tuned)
-        SCHK(tune_spmm(sv));
+        SCHK(tune_spmm(sv, blk));
     sv->nev = 0;
     mark(sv, CACG_PHASE_OTHER);
     /* setup: bounds, R = B - A X (src/bcbcgKernel.c:106-154) */
@@ -720,32 +733,38 @@ int cacg_solver_solve_device(cacg_solver *sv, const double *d_B, double *d_X, ca
 
     for (it = 0; it < o->maxiter; it++) {
         mark(sv, CACG_PHASE_OTHER);
-        /* Chebyshev basis (src/common.c:182-256) */
+        /* Chebyshev basis (src/common.c:182-256): block j at SB + j * bo, rows
+         * lb apart; the first iteration's basis is its Q */
+        const int bm = blk && it > 0;
+        double *SB = it == 0 ? Q : S;
+        const long bo = bm ? bs : m, lb = bm ? m : sm;
         if (s == 1) {
-            SCHK(hipMemcpyAsync(S, sv->d_R, sizeof(double) * (size_t)n * m, hipMemcpyDeviceToDevice, st));
+            SCHK(hipMemcpyAsync(SB, sv->d_R, sizeof(double) * (size_t)n * m, hipMemcpyDeviceToDevice, st));
         } else {
-            SCHK(cacg_k_spmm_x(&sv->A, m, sv->d_R, m, S + m, sm, CACG_EPI_CHEB1, sa, sb, NULL, 0,
-                               NULL, 0, S, sm, &sv->sch_basis, st));
+            SCHK(cacg_k_spmm_x(&sv->A, m, sv->d_R, m, SB + bo, lb, CACG_EPI_CHEB1, sa, sb, NULL, 0,
+                               NULL, 0, SB, lb, &sv->sch_basis, st));
             mark(sv, CACG_PHASE_BASIS1);
             for (int j = 2; j < s; j++)
-                SCHK(cacg_k_spmm_x(&sv->A, m, S + (long)(j - 1) * m, sm, S + (long)j * m, sm,
-                                   CACG_EPI_CHEB, sa2, sb2, S + (long)(j - 1) * m, sm,
-                                   S + (long)(j - 2) * m, sm, NULL, 0, &sv->sch_basis, st));
+                SCHK(cacg_k_spmm_x(&sv->A, m, SB + (j - 1) * bo, lb, SB + j * bo, lb,
+                                   CACG_EPI_CHEB, sa2, sb2, SB + (j - 1) * bo, lb,
+                                   SB + (j - 2) * bo, lb, NULL, 0, &sv->sch_basis, st));
         }
         if (s != 2)
             mark(sv, s == 1 ? CACG_PHASE_BASIS1 : CACG_PHASE_BASIS);
-        if (it == 0) {
-            double *t = Q;
-            Q = S;
-            S = t;
-        } else {
-            SCHK(gram(sv, mode, sv->d_AQ, S, sm, sm, NULL, 0, 0, sv->d_QtAS, NULL));
+        if (it > 0) {
+            if (bm)
+                SCHK(cacg_k_gram_fast_blk(n, sv->d_AQ, sm, sm, S, m, bs, sm, sv->d_QtAS,
+                                          sv->d_gram_work, st));
+            else
+                SCHK(gram(sv, mode, sv->d_AQ, S, sm, sm, NULL, 0, 0, sv->d_QtAS, NULL));
             mark(sv, CACG_PHASE_GRAM_AS);
             /* Q^T A Q of the previous iteration: its factor is still in d_lu_work */
             SCHK(cacg_k_lu_solve(sm, sv->d_QtAQ, sm, sv->d_QtAS, sv->d_beta, sv->d_lu_work,
                                  sv->d_flag, mode, 1, st));
             mark(sv, CACG_PHASE_SOLVE);
-            if (split) /* X += Q alpha of the previous iteration, then Q = S - Q beta */
+            if (split && bm) /* X += Q alpha of the previous iteration, then Q = S - Q beta */
+                SCHK(cacg_k_qxupdate_blk(n, sm, m, S, bs, Q, sv->d_beta, sv->d_alpha, d_X, st));
+            else if (split)
                 SCHK(cacg_k_qxupdate(n, sm, m, S, Q, sv->d_beta, sv->d_alpha, d_X, st));
             else
                 SCHK(cacg_k_qupdate(n, sm, S, Q, sv->d_beta, mode, st));
@@ -996,7 +1015,7 @@ static int solve_refine(cacg_solver *sv, const double *d_B, double *d_X, cacg_re
     }
 
     if (!sv->tuned)
-        SCHK(tune_spmm(sv));
+        SCHK(tune_spmm(sv, 0));
     sv->nev = 0;
     mark(sv, CACG_PHASE_OTHER);
     SCHK(cacg_k_fill_i32(sv->d_flag, 1, 0, st));

@@ -164,7 +164,8 @@ template <typename TS, typename T, int SM, int NTX>
 __global__ __launch_bounds__(512) void qupdate_mfma_kernel(int n, const TS *__restrict__ S, T *Q,
                                                            const double *__restrict__ beta, int m,
                                                            const double *__restrict__ alpha,
-                                                           double *__restrict__ X)
+                                                           double *__restrict__ X, long sbs,
+                                                           int sbw)
 {
     constexpr int KQ = SM / 4, NTL = SM / 16;
     constexpr int NA = NTX > 0 ? NTX : 1;
@@ -209,14 +210,19 @@ __global__ __launch_bounds__(512) void qupdate_mfma_kernel(int n, const TS *__re
         // branch, loaded a tile ahead into alternating registers and widened at
         // the subtraction (a guarded FP32 load is widened, and so waited for, on
         // the spot).  The FP32 basis is never absent, so it has no null test.
+        // S row-major (sbs == 0) or block-major: blocks of sbw columns, sbs
+        // elements apart
         auto load_s = [&](int j, TS *dst) {
+            const int col = j * 16 + lc;
+            const long co = sbs ? (long)(col / sbw) * sbs + col % sbw : (long)col;
+            const long rs = sbs ? (long)sbw : (long)SM;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const long i = i0 + lg + 4 * r;
                 if constexpr (sizeof(TS) == 4)
-                    dst[r] = S[(i < n ? i : n - 1) * SM + j * 16 + lc];
+                    dst[r] = S[(i < n ? i : n - 1) * rs + co];
                 else
-                    dst[r] = (i < n && S) ? S[i * SM + j * 16 + lc] : (TS)0;
+                    dst[r] = (i < n && S) ? S[i * rs + co] : (TS)0;
             }
         };
         // Q(:, tile j) = S - Q beta for the rows of this tile
@@ -444,7 +450,8 @@ constexpr int XR_BLOCKS = 1024;  // fixed: the norm partials depend on it
 
 template <typename TS, typename T, int SM, int NTX = 0>
 int launch_qupdate_mfma(int n, const TS *S, T *Q, const double *beta, hipStream_t st, int m = 0,
-                        const double *alpha = nullptr, double *X = nullptr)
+                        const double *alpha = nullptr, double *X = nullptr, long sbs = 0,
+                        int sbw = 1)
 {
     const int ntiles = (n + 15) / 16;
     int G = (ntiles + 7) / 8;
@@ -454,7 +461,7 @@ int launch_qupdate_mfma(int n, const TS *S, T *Q, const double *beta, hipStream_
         G &= ~7;
     cacg_allow_lds(qupdate_mfma_kernel<TS, T, SM, NTX>, (size_t)SM * SM * sizeof(double));
     hipLaunchKernelGGL((qupdate_mfma_kernel<TS, T, SM, NTX>), dim3(G), dim3(512),
-                       (size_t)SM * SM * sizeof(double), st, n, S, Q, beta, m, alpha, X);
+                       (size_t)SM * SM * sizeof(double), st, n, S, Q, beta, m, alpha, X, sbs, sbw);
     CACG_LAUNCH_CHECK();
     return 0;
 }
@@ -618,7 +625,7 @@ extern "C" int cacg_k_split_supported(int sm, int m) { return split_supported(sm
 namespace {
 template <typename TS>
 int qxupdate(int n, int sm, int m, const TS *S, double *Q, const double *beta,
-             const double *alpha, double *X, hipStream_t st)
+             const double *alpha, double *X, hipStream_t st, long sbs = 0)
 {
     if (n < 0 || !split_supported(sm, m))
         return (int)hipErrorInvalidValue;
@@ -626,8 +633,9 @@ int qxupdate(int n, int sm, int m, const TS *S, double *Q, const double *beta,
         return 0;
 #define CACG_QX(SMV)                                                                              \
     case SMV:                                                                                     \
-        return m <= 16 ? launch_qupdate_mfma<TS, double, SMV, 1>(n, S, Q, beta, st, m, alpha, X) \
-                       : launch_qupdate_mfma<TS, double, SMV, 2>(n, S, Q, beta, st, m, alpha, X);
+        return m <= 16                                                                            \
+                   ? launch_qupdate_mfma<TS, double, SMV, 1>(n, S, Q, beta, st, m, alpha, X, sbs, m) \
+                   : launch_qupdate_mfma<TS, double, SMV, 2>(n, S, Q, beta, st, m, alpha, X, sbs, m);
     switch (sm) {
         CACG_QX(128)
         CACG_QX(112)
@@ -648,6 +656,15 @@ extern "C" int cacg_k_qxupdate(int n, int sm, int m, const double *S, double *Q,
                                const double *beta, const double *alpha, double *X, hipStream_t st)
 {
     return qxupdate<double>(n, sm, m, S, Q, beta, alpha, X, st);
+}
+
+extern "C" int cacg_k_qxupdate_blk(int n, int sm, int m, const double *S, long sbs, double *Q,
+                                   const double *beta, const double *alpha, double *X,
+                                   hipStream_t st)
+{
+    if (sbs < (long)n * m)
+        return (int)hipErrorInvalidValue;
+    return qxupdate<double>(n, sm, m, S, Q, beta, alpha, X, st, sbs);
 }
 
 extern "C" int cacg_k_qxupdate_s32(int n, int sm, int m, const float *S, double *Q,

@@ -109,7 +109,11 @@ GramPlan plan_gram(int n, int kl, int kr)
 
 // L and R1 are FP64 (the FP64 solver) or FP32 (the FP32 basis: Q, AQ, S);
 // R2 (the residual R) is always FP64.  Loaded values are widened to FP64 and
-// every product and sum is FP64.
+// every product and sum is FP64.  R1 may be block-major (bs1 > 0): column c
+// of row i at R1[(c / ld1) * bs1 + i * ld1 + c % ld1], blocks of ld1 columns
+// bs1 elements apart (the basis S stored as its blocks S_0, S_1, ...); the
+// LDS-staged kernels read it (their 16-byte vectors stay inside a block),
+// the unstaged one does not.
 template <typename TL, typename T1>
 struct SrcT {
     const TL *L;
@@ -121,7 +125,15 @@ struct SrcT {
     const double *R2;
     long ld2;
     int kr2;
+    long bs1;
 };
+
+// offset of column c of R1 from its row start
+template <typename TL, typename T1>
+__device__ __forceinline__ long r1_col(const SrcT<TL, T1> &s, int c)
+{
+    return s.bs1 ? (long)(c / s.ld1) * s.bs1 + c % s.ld1 : (long)c;
+}
 
 template <typename TL, typename T1, int MT, int NT>
 __global__ __launch_bounds__(256) void gram_mfma_kernel(int n, SrcT<TL, T1> s,
@@ -365,7 +377,7 @@ __global__ __launch_bounds__(NW * 64) void gram_lds_kernel(int n, SrcT<TL, T1> s
             isf[u] = sizeof(TL) == 4;
         } else if ((v -= vl) < v1) {
             const int row = v / (s.kr1 / E1), c = E1 * (v % (s.kr1 / E1));
-            src[u] = reinterpret_cast<const char *>(s.R1 + c);
+            src[u] = reinterpret_cast<const char *>(s.R1 + r1_col(s, c));
             ldb[u] = s.ld1 * (long)sizeof(T1);
             srow[u] = row;
             loff[u] = SR * LDP + row * RDP + c;
@@ -529,11 +541,14 @@ int launch_gram_mfma(int n, const SrcT<TL, T1> &s, const GramPlan &p, double *G1
     auto al16 = [](const void *q) { return ((uintptr_t)q & 15) == 0; };
     // 16-byte vectors: column counts and row strides in whole vectors, aligned rows
     const bool aligned = p.KS == 1 && s.kl % EL == 0 && s.kr1 % E1 == 0 && s.kr2 % 2 == 0 &&
-                         s.ldl % EL == 0 && s.ld1 % E1 == 0 && (s.kr2 == 0 || s.ld2 % 2 == 0) &&
+                         s.ldl % EL == 0 && s.ld1 % E1 == 0 && s.bs1 % E1 == 0 &&
+                         (s.kr2 == 0 || s.ld2 % 2 == 0) &&
                          al16(s.L) && al16(s.R1) && (s.kr2 == 0 || al16(s.R2));
     const bool staged = aligned && stage_vectors(s, SR) <= 256 * VPT && p.chunk % SR == 0;
     if (launch_gram_wide(n, s, p, work, aligned, st)) {
         // launched
+    } else if (s.bs1 && !staged) {
+        return (int)hipErrorInvalidValue;  // a block-major R1 needs the staged kernels
     } else if (staged) {
         const size_t shm =
             2 * (size_t)SR * (lds_row_stride(s.kl) + lds_row_stride(kr)) * sizeof(double);
@@ -558,12 +573,12 @@ int launch_gram_mfma(int n, const SrcT<TL, T1> &s, const GramPlan &p, double *G1
 template <typename TL, typename T1>
 int gram_fast(int n, const TL *L, long ldl, int kl, const T1 *R1, long ld1, int kr1,
               const double *R2, long ld2, int kr2, double *G1, double *G2, double *d_work,
-              hipStream_t st)
+              hipStream_t st, long bs1 = 0)
 {
     if (n < 0 || kl <= 0 || kr1 < 0 || kr2 < 0 || kr1 + kr2 <= 0)
         return (int)hipErrorInvalidValue;
     const GramPlan p = plan_gram(n, kl, kr1 + kr2);
-    const SrcT<TL, T1> s = {L, ldl, kl, R1, ld1, kr1, R2, ld2, kr2};
+    const SrcT<TL, T1> s = {L, ldl, kl, R1, ld1, kr1, R2, ld2, kr2, bs1};
     // MT in {1, 2}, NT in 1..9
 #define CACG_GRAM_NT(MTV)                                                                         \
     switch (p.NT) {                                                                               \
@@ -612,6 +627,32 @@ extern "C" int cacg_k_gram_fast(int n, const double *L, long ldl, int kl, const 
 {
     return gram_fast<double, double>(n, L, ldl, kl, R1, ld1, kr1, R2, ld2, kr2, G1, G2, d_work,
                                      st);
+}
+
+extern "C" int cacg_gram_blk_supported(int n, int kl, int kr1, int bw)
+{
+    if (n <= 0 || kl <= 0 || bw <= 0 || kr1 <= 0 || kr1 % bw != 0 || bw % 2 != 0 || kl % 2 != 0)
+        return 0;
+    // the conditions under which launch_gram_mfma stages the operands through
+    // LDS (16-byte aligned L, R1 and ldl = kl assumed)
+    const GramPlan p = plan_gram(n, kl, kr1);
+    const SrcT<double, double> s = {nullptr, kl, kl, nullptr, bw, kr1, nullptr, 1, 0, (long)n * bw};
+    if (p.KS != 1)
+        return 0;
+    const bool wide = p.TM == 8 && p.chunk % 16 == 0 && (p.TN == 8 || p.TN == 9) &&
+                      stage_vectors(s, 16) <= (p.TN == 9 ? 12 * 64 * 3 : 16 * 64 * 2);
+    const bool staged = stage_vectors(s, 8) <= 256 * 5 && p.chunk % 8 == 0;
+    return wide || staged;
+}
+
+extern "C" int cacg_k_gram_fast_blk(int n, const double *L, long ldl, int kl, const double *R1,
+                                    int bw, long bs, int kr1, double *G1, double *d_work,
+                                    hipStream_t st)
+{
+    if (bw <= 0 || bs < (long)n * bw || kr1 % bw != 0)
+        return (int)hipErrorInvalidValue;
+    return gram_fast<double, double>(n, L, ldl, kl, R1, bw, kr1, nullptr, 1, 0, G1, nullptr,
+                                     d_work, st, bs);
 }
 
 extern "C" int cacg_k_gram_fast_f64f32(int n, const double *L, long ldl, int kl, const float *R1,

@@ -98,6 +98,17 @@ long cacg_gram_workspace(int kl, int kr);
 int cacg_k_gram_fast(int n, const double *L, long ldl, int kl, const double *R1, long ld1, int kr1,
                      const double *R2, long ld2, int kr2, double *G1, double *G2, double *d_work,
                      hipStream_t st);
+/* The fast order with R1 block-major (column c of row i at
+ * R1[(c / bw) * bs + i * bw + c % bw]: the basis S stored as its blocks S_j,
+ * n x bw each, bs elements apart) and no R2; kr1 a multiple of bw.  Only the
+ * shapes of the LDS-staged kernels (16-byte aligned rows: bw, bs even, R1
+ * 16-byte aligned); others return hipErrorInvalidValue.  Same bits as
+ * cacg_k_gram_fast on the row-major copy. */
+int cacg_k_gram_fast_blk(int n, const double *L, long ldl, int kl, const double *R1, int bw,
+                         long bs, int kr1, double *G1, double *d_work, hipStream_t st);
+/* 1 when cacg_k_gram_fast_blk takes this shape (ldl = kl, bs = n bw, 16-byte
+ * aligned L and R1), else 0 */
+int cacg_gram_blk_supported(int n, int kl, int kr1, int bw);
 /* The fast order with FP64 L and FP32 R1 (Q^T A S = AQ^T S against an FP32
  * basis): R1 widened to FP64 on load, FP64 products and accumulation. */
 int cacg_k_gram_fast_f64f32(int n, const double *L, long ldl, int kl, const float *R1, long ld1,
@@ -129,6 +140,10 @@ int cacg_k_qupdate(int n, int sm, const double *S, double *Q, const double *beta
 int cacg_k_split_supported(int sm, int m);
 int cacg_k_qxupdate(int n, int sm, int m, const double *S, double *Q, const double *beta,
                     const double *alpha, double *X, hipStream_t st);
+/* cacg_k_qxupdate with S block-major (blocks of m columns, sbs >= n m elements
+ * apart); same bits */
+int cacg_k_qxupdate_blk(int n, int sm, int m, const double *S, long sbs, double *Q,
+                        const double *beta, const double *alpha, double *X, hipStream_t st);
 /* the same with the FP32 basis S (the precision form) */
 int cacg_k_qxupdate_s32(int n, int sm, int m, const float *S, double *Q, const double *beta,
                         const double *alpha, double *X, hipStream_t st);

@@ -262,6 +262,23 @@ def test_deferred_x_update_kernels_match_undeferred_bitwise(gpu_lib, sm, m):
     assert np.allclose(Qd, S - Q @ beta, rtol=1e-12, atol=1e-11)
 
 
+@pytest.mark.parametrize("n,sm,m,gap", [(5003, 128, 16, 0), (4099, 64, 8, 2), (3001, 96, 32, 0),
+                                        (2000, 48, 4, 6), (70001, 128, 16, 0)])
+def test_block_major_basis_kernels_same_bits(gpu_lib, n, sm, m, gap):
+    # the basis stored as its blocks (the fast order from iteration 1 on): the
+    # Q^T A S Gram and the deferred Q update give the bits of the row-major forms
+    rng = np.random.default_rng(n + sm + m)
+    S, Q, AQ = (rng.standard_normal((n, sm)) for _ in range(3))
+    beta, alpha = rng.standard_normal((sm, sm)), rng.standard_normal((sm, m))
+    X = rng.standard_normal((n, m))
+    print(f"block-major basis n={n} sm={sm} m={m}", flush=True)
+    Qr, Xr = gpu_lib.k_qxupdate(S, Q, beta, alpha, X)
+    Qb, Xb = gpu_lib.k_qxupdate_blk(S, Q, beta, alpha, X, gap)
+    assert np.array_equal(Qb, Qr) and np.array_equal(Xb, Xr)
+    G = gpu_lib.k_gram(AQ, S, mode="fast")
+    assert np.array_equal(gpu_lib.k_gram_blk(AQ, S, m, gap), G)
+
+
 def test_deferred_forms_refuse_unsupported_widths(gpu_lib):
     L = gpu_lib.lib()
     assert L.cacg_k_split_supported(40, 4) == 0 and L.cacg_k_split_supported(128, 33) == 0

@@ -192,3 +192,27 @@ def test_solver_reuse_is_deterministic(gpu_lib):
         xs.append(dX.download(B.shape))
     assert np.array_equal(xs[0], xs[1])
     sv.close()
+
+
+@pytest.mark.parametrize("grid,m,s", [(20, 16, 8), (24, 8, 4), (16, 4, 3), (48, 16, 8)])
+def test_block_major_basis_gives_the_same_bits(gpu_lib, monkeypatch, grid, m, s):
+    # the fast order keeps its basis block-major from iteration 1 on; with
+    # CACG_BASIS_ROWS=1 (read at solver creation) it stays row-major: every
+    # iterate, the residual history and X are the same bits
+    A = P.laplacian_3d(grid)
+    B = P.rhs_uniform(A.n, m, 2)
+    out = {}
+    for rows in ("0", "1"):
+        monkeypatch.setenv("CACG_BASIS_ROWS", rows)
+        print(f"basis rows={rows} grid={grid} m={m} s={s}", flush=True)
+        sv = gpu_lib.Solver(A, m, s, tol=1e-10, maxiter=200, order="fast")
+        dB = gpu_lib.DeviceBuffer.from_array(B)
+        dX = gpu_lib.DeviceBuffer.from_array(np.zeros_like(B))
+        rep = sv.solve(dB, dX)
+        assert rep["status"] == "ok"
+        out[rows] = (dX.download(B.shape), rep)
+        sv.close()
+    (xb, rb), (xr, rr) = out["0"], out["1"]
+    assert rb["iters"] == rr["iters"] > 1
+    assert np.array_equal(rb["res_hist"], rr["res_hist"])
+    assert np.array_equal(xb, xr)
