@@ -11,6 +11,8 @@
 // a system runs in an order fixed by that system alone (its rows, tiles,
 // chunks and class), so a system's results are the same bits whatever else
 // shares the launch.
+#include <type_traits>
+
 #include "kcommon.hpp"
 
 namespace {
@@ -120,94 +122,227 @@ __global__ __launch_bounds__(256) void baq_kernel(const cacg_bsys *__restrict__ 
     }
 }
 
+// LDS row stride of k doubles: 16 mod 32, so a wave's operand read (4 rows x
+// 16 doubles) falls on disjoint halves of the 64 banks
+constexpr int bstride(int k)
+{
+    return k + (48 - k % 32) % 32;
+}
+
+// waves of the Gram kernel of tile class TM
+constexpr int gram_waves(int TM)
+{
+    return TM >= 4 ? 8 : 4;
+}
+
 // MFMA Gram partial of one 2048-row chunk for systems with TM = ceil(sm / 16)
 // row tiles and TN = ceil(kr / 16) column tiles (columns past sm and kr are
-// zeros in LDS and are not written).  NW = min(TM, 4)
-// waves each own MT = ceil(TM / NW) M tiles x TN N tiles; stages of 16 rows
-// of L and R are loaded by all 256 lanes into LDS (rows past the chunk as
-// zeros) and read as MFMA operands (contraction index = row).  Per tile the
-// k-steps run in row order, so the partial depends on the system alone.
-template <typename TB, int TM, int TN>
-__global__ __launch_bounds__(256) void bgram_mfma_kernel(const cacg_bsys *__restrict__ sys,
-                                                         const int2 *__restrict__ items, int which)
+// zeros in LDS and are not written); WHICH 0: Q^T A S = AQ^T S, 1: Q^T [AQ | R].
+// NW waves split the TM x TN tiles into GMG x GNG groups of MT x NT tiles.
+// Stages of GS rows of L and of [R1 | R2] are loaded with 16-byte vectors
+// where a row's width allows (single elements otherwise) into registers one
+// stage ahead, stored to one of two LDS buffers and read as MFMA operands
+// (contraction index = row).  Per tile the k-steps run in row order from
+// zero (rows past the chunk are zeros), so the partial depends on the system
+// alone.
+template <typename TB, int TM, int TN, int WHICH>
+__global__ __launch_bounds__(gram_waves(TM) * 64) void bgram_mfma_kernel(
+    const cacg_bsys *__restrict__ sys, const int2 *__restrict__ items)
 {
-    constexpr int NW = TM < 4 ? TM : 4, MT = (TM + NW - 1) / NW;
-    constexpr int KL = TM * 16, LP = KL + 2, RP = TN * 16 + 2;
-    __shared__ double Ls[GS * LP];
-    __shared__ double Rs[GS * RP];
+    constexpr int NW = gram_waves(TM), NTH = NW * 64;
+    constexpr int GMG = TM < NW ? TM : NW, GNG = NW / GMG;
+    constexpr int MT = (TM + GMG - 1) / GMG, NT = (TN + GNG - 1) / GNG;
+    constexpr int KL = TM * 16, KR = TN * 16, LP = bstride(KL), RP = bstride(KR);
+    constexpr int BUF = GS * (LP + RP);
+    // register slots per lane and stage: L and R1 up to GS x KL elements, R2 up
+    // to GS x 32, one element or one 16-byte vector each
+    constexpr int SLL = (GS * KL + NTH - 1) / NTH, SL2 = (GS * MM + NTH - 1) / NTH;
+    using T1 = typename std::conditional<WHICH == 1, double, TB>::type;
+    extern __shared__ double lds[];
     const int2 it = items[blockIdx.x];
     const cacg_bsys &d = sys[it.x];
     const int n = d.n, m = d.m, sm = d.sm;
-    const int kr1 = sm, kr = sm + (which ? m : 0);
-    const double *L = which ? d.Q : d.AQ;
-    const TB *S = reinterpret_cast<const TB *>(d.S);
+    const int kr = sm + (WHICH ? m : 0);
+    const double *L = WHICH ? d.Q : d.AQ;
+    const T1 *R1 = WHICH ? reinterpret_cast<const T1 *>(d.AQ) : reinterpret_cast<const T1 *>(d.S);
     const int c0 = it.y * GR, c1 = (c0 + GR) < n ? (c0 + GR) : n;
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63, lr = lane >> 4, lc = lane & 15;
-    dbl4 acc[MT][TN];
+    const int gm = wave % GMG, gn = wave / GMG;
+    const bool active = gn < GNG;
+
+    // element widths of one load: 2 doubles / 4 floats where the row width
+    // (and so every row start) allows 16-byte alignment
+    constexpr int V1 = 16 / (int)sizeof(T1);
+    const int wl = sm % 2 == 0 ? 2 : 1, w1 = sm % V1 == 0 ? V1 : 1;
+    const int w2 = m % 2 == 0 ? 2 : 1;
+    const int nl = sm / wl, n1 = sm / w1, n2 = WHICH ? m / w2 : 0;
+    int lrow[SLL], lcol[SLL], r1row[SLL], r1col[SLL], r2row[SL2], r2col[SL2];
+#pragma unroll
+    for (int u = 0; u < SLL; u++) {
+        const int v = t + NTH * u;
+        lrow[u] = v < GS * nl ? v / nl : -1;
+        lcol[u] = v < GS * nl ? (v % nl) * wl : 0;
+        r1row[u] = v < GS * n1 ? v / n1 : -1;
+        r1col[u] = v < GS * n1 ? (v % n1) * w1 : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < SL2; u++) {
+        const int v = t + NTH * u;
+        r2row[u] = v < GS * n2 ? v / n2 : -1;
+        r2col[u] = v < GS * n2 ? (v % n2) * w2 : 0;
+    }
+    using V1T = typename std::conditional<sizeof(T1) == 4, float4, double2>::type;
+    double2 gl[SLL], g2[SL2];
+    V1T g1[SLL];
+    auto gload = [&](int rb) {
+#pragma unroll
+        for (int u = 0; u < SLL; u++) {
+            const long i = rb + lrow[u];
+            gl[u] = double2{0.0, 0.0};
+            if (lrow[u] >= 0 && i < c1) {
+                const double *p = L + i * sm + lcol[u];
+                if (wl == 2)
+                    gl[u] = *reinterpret_cast<const double2 *>(p);
+                else
+                    gl[u] = double2{p[0], 0.0};
+            }
+            const long i1 = rb + r1row[u];
+            if constexpr (sizeof(T1) == 4)
+                g1[u] = float4{0.f, 0.f, 0.f, 0.f};
+            else
+                g1[u] = double2{0.0, 0.0};
+            if (r1row[u] >= 0 && i1 < c1) {
+                const T1 *p = R1 + i1 * sm + r1col[u];
+                if (w1 == V1)
+                    g1[u] = *reinterpret_cast<const V1T *>(p);
+                else if constexpr (sizeof(T1) == 4)
+                    g1[u] = float4{p[0], 0.f, 0.f, 0.f};
+                else
+                    g1[u] = double2{p[0], 0.0};
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < SL2; u++) {
+            const long i = rb + r2row[u];
+            g2[u] = double2{0.0, 0.0};
+            if (r2row[u] >= 0 && i < c1) {
+                const double *p = d.R + i * m + r2col[u];
+                if (w2 == 2)
+                    g2[u] = *reinterpret_cast<const double2 *>(p);
+                else
+                    g2[u] = double2{p[0], 0.0};
+            }
+        }
+    };
+    auto lstore = [&](int buf) {
+        double *Ls = lds + buf * BUF, *Rs = Ls + GS * LP;
+#pragma unroll
+        for (int u = 0; u < SLL; u++) {
+            if (lrow[u] >= 0) {
+                Ls[lrow[u] * LP + lcol[u]] = gl[u].x;
+                if (wl == 2)
+                    Ls[lrow[u] * LP + lcol[u] + 1] = gl[u].y;
+            }
+            if (r1row[u] >= 0) {
+                double *q = Rs + r1row[u] * RP + r1col[u];
+                if constexpr (sizeof(T1) == 4) {
+                    q[0] = (double)g1[u].x;
+                    if (w1 == 4) {
+                        q[1] = (double)g1[u].y;
+                        q[2] = (double)g1[u].z;
+                        q[3] = (double)g1[u].w;
+                    }
+                } else {
+                    q[0] = g1[u].x;
+                    if (w1 == 2)
+                        q[1] = g1[u].y;
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < SL2; u++) {
+            if (r2row[u] >= 0) {
+                double *q = Rs + r2row[u] * RP + sm + r2col[u];
+                q[0] = g2[u].x;
+                if (w2 == 2)
+                    q[1] = g2[u].y;
+            }
+        }
+    };
+
+    // the padding columns (past sm in L, past kr in R) stay zero in both buffers
+    for (int e = t; e < 2 * GS; e += NTH) {
+        double *Ls = lds + (e / GS) * BUF, *Rs = Ls + GS * LP;
+        const int r = e % GS;
+        for (int c = sm; c < KL; c++)
+            Ls[r * LP + c] = 0.0;
+        for (int c = kr; c < KR; c++)
+            Rs[r * RP + c] = 0.0;
+    }
+    dbl4 acc[MT][NT];
 #pragma unroll
     for (int i = 0; i < MT; i++)
 #pragma unroll
-        for (int j = 0; j < TN; j++)
+        for (int j = 0; j < NT; j++)
             acc[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
-    for (int rb = c0; rb < c1; rb += GS) {
-        const int rows = (c1 - rb) < GS ? (c1 - rb) : GS;
-        for (int e = t; e < GS * KL; e += 256) {
-            const int r = e / KL, a = e - r * KL;
-            Ls[r * LP + a] = (r < rows && a < sm) ? L[(long)(rb + r) * sm + a] : 0.0;
-        }
-        for (int e = t; e < GS * kr; e += 256) {
-            const int r = e / kr, b = e - r * kr;
-            double v = 0.0;
-            if (r < rows) {
-                const long i = rb + r;
-                if (b < kr1)
-                    v = which ? d.AQ[i * sm + b] : (double)S[i * sm + b];
-                else
-                    v = d.R[i * m + (b - kr1)];
-            }
-            Rs[r * RP + b] = v;
-        }
-        __syncthreads();
-        if (wave < NW) {
+    const int nst = (c1 - c0 + GS - 1) / GS;
+    gload(c0);
+    lstore(0);
+    if (nst > 1)
+        gload(c0 + GS);
+    __syncthreads();
+    for (int stg = 0; stg < nst; stg++) {
+        const int cur = stg & 1;
+        if (active) {
+            const double *Ls = lds + cur * BUF, *Rs = Ls + GS * LP;
 #pragma unroll
             for (int ks = 0; ks < GS / 4; ks++) {
                 const int row = ks * 4 + lr;
-                double av[MT], bv[TN];
+                double av[MT], bv[NT];
 #pragma unroll
                 for (int i = 0; i < MT; i++) {
-                    const int a = (wave * MT + i) * 16 + lc;
+                    const int a = (gm * MT + i) * 16 + lc;
                     av[i] = a < KL ? Ls[row * LP + a] : 0.0;
                 }
 #pragma unroll
-                for (int j = 0; j < TN; j++) {
-                    const int b = j * 16 + lc;
-                    bv[j] = b < kr ? Rs[row * RP + b] : 0.0;
+                for (int j = 0; j < NT; j++) {
+                    const int b = (gn * NT + j) * 16 + lc;
+                    bv[j] = b < KR ? Rs[row * RP + b] : 0.0;
                 }
 #pragma unroll
                 for (int i = 0; i < MT; i++)
 #pragma unroll
-                    for (int j = 0; j < TN; j++)
+                    for (int j = 0; j < NT; j++)
                         acc[i][j] = mfma16x16x4(av[i], bv[j], acc[i][j]);
             }
         }
+        if (stg + 1 < nst) {
+            lstore(cur ^ 1);
+            if (stg + 2 < nst)
+                gload(c0 + (stg + 2) * GS);
+        }
         __syncthreads();
     }
-    if (wave >= NW)
+    if (!active)
         return;
     double *gp = d.gpart + (long)it.y * sm * kr;
 #pragma unroll
     for (int i = 0; i < MT; i++) {
-        const int tm = wave * MT + i;
+        const int tm = gm * MT + i;
         if (tm >= TM)
             continue;
 #pragma unroll
-        for (int j = 0; j < TN; j++)
+        for (int j = 0; j < NT; j++) {
+            const int tn = gn * NT + j;
+            if (tn >= TN)
+                continue;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int a = tm * 16 + lr + 4 * r, b = j * 16 + lc;
+                const int a = tm * 16 + lr + 4 * r, b = tn * 16 + lc;
                 if (a < sm && b < kr)
                     gp[(long)a * kr + b] = acc[i][j][r];
             }
+        }
     }
 }
 
@@ -449,26 +584,31 @@ extern "C" int cacg_kb_aq(const cacg_bsys *d_sys, const int2 *d_tiles, int count
 }
 
 namespace {
+template <typename TB, int TM, int TN, int WHICH>
+void launch_bgram(const cacg_bsys *d_sys, const int2 *d_chunks, int nchunks, hipStream_t st)
+{
+    constexpr int NW = gram_waves(TM);
+    const size_t shm = 2 * (size_t)GS * (bstride(TM * 16) + bstride(TN * 16)) * sizeof(double);
+    cacg_allow_lds(bgram_mfma_kernel<TB, TM, TN, WHICH>, shm);
+    hipLaunchKernelGGL((bgram_mfma_kernel<TB, TM, TN, WHICH>), dim3(nchunks), dim3(NW * 64), shm,
+                       st, d_sys, d_chunks);
+}
+
 template <typename TB, int TM>
 int launch_gram_class(const cacg_bsys *d_sys, const int2 *d_chunks, int nchunks, int which, int tn,
                       hipStream_t st)
 {
-    switch (tn - TM) {
-    case 0:
-        hipLaunchKernelGGL((bgram_mfma_kernel<TB, TM, TM>), dim3(nchunks), dim3(256), 0, st, d_sys,
-                           d_chunks, which);
-        break;
-    case 1:
-        hipLaunchKernelGGL((bgram_mfma_kernel<TB, TM, TM + 1>), dim3(nchunks), dim3(256), 0, st,
-                           d_sys, d_chunks, which);
-        break;
-    case 2:
-        hipLaunchKernelGGL((bgram_mfma_kernel<TB, TM, TM + 2>), dim3(nchunks), dim3(256), 0, st,
-                           d_sys, d_chunks, which);
-        break;
-    default:
+    // which 0: kr = sm (TN = TM); which 1: kr = sm + m, m <= 32 (TN - TM in 0..2)
+    if (which == 0 && tn == TM)
+        launch_bgram<TB, TM, TM, 0>(d_sys, d_chunks, nchunks, st);
+    else if (which == 1 && tn == TM)
+        launch_bgram<TB, TM, TM, 1>(d_sys, d_chunks, nchunks, st);
+    else if (which == 1 && tn == TM + 1)
+        launch_bgram<TB, TM, TM + 1, 1>(d_sys, d_chunks, nchunks, st);
+    else if (which == 1 && tn == TM + 2)
+        launch_bgram<TB, TM, TM + 2, 1>(d_sys, d_chunks, nchunks, st);
+    else
         return (int)hipErrorInvalidValue;
-    }
     CACG_LAUNCH_CHECK();
     return 0;
 }
