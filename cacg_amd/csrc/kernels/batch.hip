@@ -135,6 +135,14 @@ constexpr int gram_waves(int TM)
     return TM >= 4 ? 8 : 4;
 }
 
+// rows per stage of the Gram kernel: the narrow classes do little work per
+// row, so their stages are longer (fewer load round trips and barriers; the
+// rows are still taken in order)
+constexpr int gram_stage(int TM)
+{
+    return TM <= 2 ? 64 : 16;
+}
+
 // MFMA Gram partial of one 2048-row chunk for systems with TM = ceil(sm / 16)
 // row tiles and TN = ceil(kr / 16) column tiles (columns past sm and kr are
 // zeros in LDS and are not written); WHICH 0: Q^T A S = AQ^T S, 1: Q^T [AQ | R].
@@ -149,7 +157,7 @@ template <typename TB, int TM, int TN, int WHICH>
 __global__ __launch_bounds__(gram_waves(TM) * 64) void bgram_mfma_kernel(
     const cacg_bsys *__restrict__ sys, const int2 *__restrict__ items)
 {
-    constexpr int NW = gram_waves(TM), NTH = NW * 64;
+    constexpr int NW = gram_waves(TM), NTH = NW * 64, GS = gram_stage(TM);
     constexpr int GMG = TM < NW ? TM : NW, GNG = NW / GMG;
     constexpr int MT = (TM + GMG - 1) / GMG, NT = (TN + GNG - 1) / GNG;
     constexpr int KL = TM * 16, KR = TN * 16, LP = bstride(KL), RP = bstride(KR);
@@ -588,7 +596,8 @@ template <typename TB, int TM, int TN, int WHICH>
 void launch_bgram(const cacg_bsys *d_sys, const int2 *d_chunks, int nchunks, hipStream_t st)
 {
     constexpr int NW = gram_waves(TM);
-    const size_t shm = 2 * (size_t)GS * (bstride(TM * 16) + bstride(TN * 16)) * sizeof(double);
+    const size_t shm =
+        2 * (size_t)gram_stage(TM) * (bstride(TM * 16) + bstride(TN * 16)) * sizeof(double);
     cacg_allow_lds(bgram_mfma_kernel<TB, TM, TN, WHICH>, shm);
     hipLaunchKernelGGL((bgram_mfma_kernel<TB, TM, TN, WHICH>), dim3(nchunks), dim3(NW * 64), shm,
                        st, d_sys, d_chunks);
