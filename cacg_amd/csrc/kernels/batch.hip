@@ -59,11 +59,94 @@ __device__ __forceinline__ TV row_dot(const int *__restrict__ rowptr, const int 
     return acc;
 }
 
+// Vector path of the tile SpMMs: the tile's row pointers and (col, val)
+// pairs staged in LDS (when at most CAPB), each lane producing 4 adjacent
+// columns of one row from 16-byte gathers of x, 4 nonzeros gathered ahead of
+// their in-order accumulation.  Each output is the CSR-order sum of separately
+// rounded products, the bits of row_dot.
+constexpr int CAPB = 1024;
+
+template <typename TV>
+struct TileCsr {
+    int rp[BT + 1];
+    int col[CAPB];
+    TV val[CAPB];
+};
+
+// stage rows r0 .. r0 + rows - 1; returns whether the pairs fit (else the
+// caller uses row_dot); ends with a barrier
+template <typename TV>
+__device__ __forceinline__ bool stage_tile(TileCsr<TV> &tc, const int *__restrict__ rowptr,
+                                           const int *__restrict__ col,
+                                           const TV *__restrict__ val, long r0, int rows)
+{
+    for (int e = threadIdx.x; e <= rows; e += 256)
+        tc.rp[e] = rowptr[r0 + e];
+    __syncthreads();
+    const int pb = tc.rp[0], tn = tc.rp[rows] - pb;
+    const bool staged = tn <= CAPB;
+    if (staged)
+        for (int e = threadIdx.x; e < tn; e += 256) {
+            tc.col[e] = col[pb + e];
+            tc.val[e] = val[pb + e];
+        }
+    __syncthreads();
+    return staged;
+}
+
+// 4 consecutive elements of x at p (16-byte aligned), converted to TV
+template <typename TV, typename TX>
+__device__ __forceinline__ void load4(const TX *p, TV *o)
+{
+    if constexpr (sizeof(TX) == 4) {
+        const float4 v = *reinterpret_cast<const float4 *>(p);
+        o[0] = (TV)v.x;
+        o[1] = (TV)v.y;
+        o[2] = (TV)v.z;
+        o[3] = (TV)v.w;
+    } else {
+        const double2 v0 = reinterpret_cast<const double2 *>(p)[0];
+        const double2 v1 = reinterpret_cast<const double2 *>(p)[1];
+        o[0] = (TV)v0.x;
+        o[1] = (TV)v0.y;
+        o[2] = (TV)v1.x;
+        o[3] = (TV)v1.y;
+    }
+}
+
+// acc[0..3] = sums over row r of the staged tile of val * x[col * ldx + c0 + 0..3]
+template <typename TV, typename TX>
+__device__ __forceinline__ void row_dot4(const TileCsr<TV> &tc, int r, const TX *__restrict__ x,
+                                         long ldx, int c0, TV *acc)
+{
+    const int pb = tc.rp[0], p0 = tc.rp[r] - pb, p1 = tc.rp[r + 1] - pb;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        acc[k] = (TV)0;
+    for (int p = p0; p < p1; p += 4) {
+        TV xv[4][4], vv[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const bool in = p + u < p1;
+            vv[u] = in ? tc.val[p + u] : (TV)0;
+            if (in)
+                load4<TV, TX>(x + (long)tc.col[p + u] * ldx + c0, xv[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (p + u < p1)
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    acc[k] = add_rn(acc[k], mul_rn(vv[u], xv[u][k]));
+    }
+}
+
 // basis step j for one tile (see cacg_kb_basis); TB = basis type
 template <typename TB>
 __global__ __launch_bounds__(256) void bbasis_kernel(const cacg_bsys *__restrict__ sys,
                                                      const int2 *__restrict__ items, int j)
 {
+    __shared__ TileCsr<TB> tc;
     const int2 it = items[blockIdx.x];
     const cacg_bsys &d = sys[it.x];
     const int n = d.n, m = d.m, sm = d.sm;
@@ -73,13 +156,46 @@ __global__ __launch_bounds__(256) void bbasis_kernel(const cacg_bsys *__restrict
     const TB *val = sizeof(TB) == 4 ? reinterpret_cast<const TB *>(d.val32)
                                     : reinterpret_cast<const TB *>(d.val);
     const TB a1 = (TB)d.sa, b1 = (TB)d.sb, a2 = (TB)(2.0 * d.sa), b2 = (TB)(2.0 * d.sb);
+    if (j == 0) {
+        for (int e = threadIdx.x; e < rows * m; e += 256) {
+            const int r = e / m, c = e - r * m;
+            const long i = r0 + r;
+            S[i * sm + c] = (TB)d.R[i * m + c];
+        }
+        return;
+    }
+    // the vector path: m a multiple of 4 (16-byte aligned rows and blocks)
+    const bool vec = m % 4 == 0 && stage_tile<TB>(tc, d.rowptr, d.col, val, r0, rows);
+    if (vec) {
+        const int q = m / 4;
+        for (int e = threadIdx.x; e < rows * q; e += 256) {
+            const int r = e / q, c0 = 4 * (e - r * q);
+            const long i = r0 + r;
+            TB acc[4];
+            if (j == 1) {
+                row_dot4<TB, double>(tc, r, d.R, m, c0, acc);
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const TB e1 = (TB)d.R[i * m + c0 + k];
+                    S[i * sm + c0 + k] = e1;
+                    S[i * sm + m + c0 + k] = add_rn(mul_rn(a1, acc[k]), mul_rn(b1, e1));
+                }
+            } else {
+                const TB *xs = S + (long)(j - 1) * m;
+                row_dot4<TB, TB>(tc, r, xs, sm, c0, acc);
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const TB e1 = xs[i * sm + c0 + k], e2 = S[i * sm + (long)(j - 2) * m + c0 + k];
+                    S[i * sm + (long)j * m + c0 + k] =
+                        sub_rn(add_rn(mul_rn(a2, acc[k]), mul_rn(b2, e1)), e2);
+                }
+            }
+        }
+        return;
+    }
     for (int e = threadIdx.x; e < rows * m; e += 256) {
         const int r = e / m, c = e - r * m;
         const long i = r0 + r;
-        if (j == 0) {
-            S[i * sm + c] = (TB)d.R[i * m + c];
-            continue;
-        }
         if (j == 1) {
             const TB acc = row_dot<TB, double>(d.rowptr, d.col, val, d.R, m, c, i);
             const TB e1 = (TB)d.R[i * m + c];
@@ -110,11 +226,24 @@ __global__ __launch_bounds__(256) void bq_from_s_kernel(const cacg_bsys *__restr
 __global__ __launch_bounds__(256) void baq_kernel(const cacg_bsys *__restrict__ sys,
                                                   const int2 *__restrict__ items)
 {
+    __shared__ TileCsr<double> tc;
     const int2 it = items[blockIdx.x];
     const cacg_bsys &d = sys[it.x];
     const int sm = d.sm;
     const long r0 = (long)it.y * BT;
     const int rows = tile_rows(d.n, it.y);
+    if (sm % 4 == 0 && stage_tile<double>(tc, d.rowptr, d.col, d.val, r0, rows)) {
+        const int q = sm / 4;
+        for (int e = threadIdx.x; e < rows * q; e += 256) {
+            const int r = e / q, c0 = 4 * (e - r * q);
+            double acc[4];
+            row_dot4<double, double>(tc, r, d.Q, sm, c0, acc);
+            double *y = d.AQ + (r0 + r) * sm + c0;
+            reinterpret_cast<double2 *>(y)[0] = double2{acc[0], acc[1]};
+            reinterpret_cast<double2 *>(y)[1] = double2{acc[2], acc[3]};
+        }
+        return;
+    }
     for (int e = threadIdx.x; e < rows * sm; e += 256) {
         const int r = e / sm, c = e - r * sm;
         const long i = r0 + r;
