@@ -421,9 +421,10 @@ long cacg_batch_submit(cacg_batch *b, long n, const long *row_start, const long 
     r->ci = (int *)malloc(sizeof(int) * (size_t)(nnz > 0 ? nnz : 1));
     r->val = (double *)malloc(sizeof(double) * (size_t)(nnz > 0 ? nnz : 1));
     r->B = (double *)malloc(sizeof(double) * (size_t)(n * m));
-    r->X = (double *)malloc(sizeof(double) * (size_t)(n * m));
+    /* X0 = 0 (no X0 given) is not stored: admission zeroes X on the device */
+    r->X = X0 ? (double *)malloc(sizeof(double) * (size_t)(n * m)) : NULL;
     r->bn2 = (double *)calloc((size_t)m, sizeof(double));
-    if (!r->rp || !r->ci || !r->val || !r->B || !r->X || !r->bn2) {
+    if (!r->rp || !r->ci || !r->val || !r->B || (X0 && !r->X) || !r->bn2) {
         free_req(r);
         return -CACG_ENOMEM;
     }
@@ -436,8 +437,6 @@ long cacg_batch_submit(cacg_batch *b, long n, const long *row_start, const long 
     memcpy(r->B, B, sizeof(double) * (size_t)(n * m));
     if (X0)
         memcpy(r->X, X0, sizeof(double) * (size_t)(n * m));
-    else
-        memset(r->X, 0, sizeof(double) * (size_t)(n * m));
     for (long i = 0; i < n; i++)
         for (long c = 0; c < m; c++)
             r->bn2[c] += B[i * m + c] * B[i * m + c];
@@ -546,7 +545,10 @@ static int admit(cacg_batch *b, breq *r, int slot)
     }
     ACHK(hipMemcpyAsync((void *)d->B, r->B, sizeof(double) * (size_t)(n * m),
                         hipMemcpyHostToDevice, st));
-    ACHK(hipMemcpyAsync(d->X, r->X, sizeof(double) * (size_t)(n * m), hipMemcpyHostToDevice, st));
+    if (r->X)
+        ACHK(hipMemcpyAsync(d->X, r->X, sizeof(double) * (size_t)(n * m), hipMemcpyHostToDevice, st));
+    else
+        AKCHK(cacg_k_fill_f64(d->X, n * m, 0.0, st));
     AKCHK(cacg_k_fill_i32(d->flag, 1, 0, st));
     ACHK(hipMemcpyAsync(b->d_sys + slot, d, sizeof(*d), hipMemcpyHostToDevice, st));
     r->off = off;
@@ -567,6 +569,11 @@ static int retire(cacg_batch *b, breq *r, int status, const double *norm2)
         const double q = sqrt(norm2[c]) / sqrt(r->bn2[c]);
         if (!(q <= worst))
             worst = q;
+    }
+    if (!r->X) { /* submitted without X0 */
+        r->X = (double *)malloc(sizeof(double) * (size_t)(r->n * r->m));
+        if (!r->X)
+            return CACG_ENOMEM;
     }
     HCHK(hipMemcpyAsync(r->X, b->h_sys[r->slot].X, sizeof(double) * (size_t)(r->n * r->m),
                         hipMemcpyDeviceToHost, b->st));
