@@ -233,6 +233,7 @@ void cacg_batch_default_opts(cacg_batch_opts *o)
     o->s = 4;
     o->precision = CACG_PREC_FP32_REFINE;
     o->maxiter = 1000;
+    o->max_large = 0;
 }
 
 static void free_req(breq *r)
@@ -326,6 +327,7 @@ int cacg_batch_create(cacg_batch **out, const cacg_batch_opts *opts)
     if (o.slots == 0)
         o.slots = 1024;
     if (o.slots < 1 || o.slots > 4096 || o.s < 1 || o.maxiter <= 0 || o.hbm_bytes < 0 ||
+        o.max_large < 0 ||
         (o.precision != CACG_PREC_FP64 && o.precision != CACG_PREC_FP32_REFINE))
         return CACG_EINVAL;
     int cnt = 0;
@@ -758,14 +760,20 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
     const int s = b->o.s, f32 = b->o.precision == CACG_PREC_FP32_REFINE;
     int admitted = 0, retired = 0;
 
-    /* admission, FIFO: the head waits for a slot and memory */
-    int free_slot = 0;
+    /* admission, FIFO: the head waits for a slot and memory (and, for a large
+     * system, for the large systems in flight to drop below max_large) */
+    int free_slot = 0, nlarge = 0;
+    for (int i = 0; i < b->nslots; i++)
+        if (b->slot[i] && is_solo(b->slot[i]->n))
+            nlarge++;
     while (b->qhead < b->nreq) {
         breq *r = b->req[b->qhead];
         if (!r || r->state != RQ_QUEUED) {
             b->qhead++;
             continue;
         }
+        if (is_solo(r->n) && b->o.max_large > 0 && nlarge >= b->o.max_large)
+            break;
         while (free_slot < b->nslots && b->slot[free_slot])
             free_slot++;
         if (free_slot == b->nslots)
@@ -776,6 +784,8 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
         if (ok == 0)
             break;
         admitted++;
+        if (is_solo(r->n))
+            nlarge++;
         b->qhead++;
     }
 

@@ -36,6 +36,9 @@ typedef struct {
     int precision;   /* CACG_PREC_FP64, or CACG_PREC_FP32_REFINE: FP32 matrix copy, basis and
                         Chebyshev SpMMs; Q, A Q, Gram, LU, X and R FP64 */
     long maxiter;    /* outer-iteration cap per system */
+    int max_large;   /* systems of 2^17 rows or more (each runs the single-system kernels on
+                        its own stream) in flight at most; 0: no limit.  A queued large system
+                        at the head waits for one to retire (FIFO: later requests wait too) */
 } cacg_batch_opts;
 
 typedef struct {

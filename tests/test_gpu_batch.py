@@ -148,3 +148,26 @@ def test_large_systems_alone_equal_mix_and_meet_tol(gpu_lib, precision):
         alone = _run([systems[k]], s=4, precision=precision, hbm_bytes=4 << 30)[0]
         assert alone["iters"] == mixed[k]["iters"]
         assert np.array_equal(alone["X"], mixed[k]["X"]), k
+
+
+def test_max_large_caps_concurrent_large_systems(gpu_lib):
+    # max_large = 1: the second large system waits at the head of the queue
+    # (and the small one behind it too) until the first retires; the results
+    # are the bits of the uncapped run
+    big = [(P.laplacian_2d(400, 0.02), 2), (P.laplacian_3d(52, 0.05), 4)]
+    systems = [(A, P.uniform(300 + k, A.n * m).reshape(A.n, m)) for k, (A, m) in enumerate(big)]
+    systems += _mix()[:1]
+    print("batch max_large", flush=True)
+    b = BT.Batch(s=4, precision="fp64", hbm_bytes=4 << 30, max_large=1)
+    ids = [b.submit(A, B, tol=TOL) for A, B in systems]
+    st = b.step()
+    assert st["active"] == 1 and st["queued"] == 2
+    res = b.run()
+    b.close()
+    capped = [res[i] for i in ids]
+    free = _run(systems, s=4, precision="fp64", hbm_bytes=4 << 30)
+    assert capped[1]["rounds_queued"] > 0 and free[1]["rounds_queued"] == 0
+    for x, y in zip(capped, free):
+        assert x["status"] == "ok" and np.array_equal(x["X"], y["X"])
+    with pytest.raises(RuntimeError):
+        BT.Batch(s=4, max_large=-1)
