@@ -31,7 +31,7 @@ PRECISION = {"fp64": 0, "fp32": 1}
 class BatchOpts(ctypes.Structure):
     _fields_ = [("hbm_bytes", ctypes.c_long), ("slots", ctypes.c_int), ("s", ctypes.c_int),
                 ("precision", ctypes.c_int), ("maxiter", ctypes.c_long),
-                ("max_large", ctypes.c_int)]
+                ("max_large", ctypes.c_int), ("graph", ctypes.c_int)]
 
 
 class BatchResult(ctypes.Structure):
@@ -42,7 +42,8 @@ class BatchResult(ctypes.Structure):
 class BatchStats(ctypes.Structure):
     _fields_ = [("active", ctypes.c_int), ("queued", ctypes.c_int), ("finished", ctypes.c_int),
                 ("admitted", ctypes.c_int), ("retired", ctypes.c_int), ("rounds", ctypes.c_long),
-                ("arena_used", ctypes.c_long), ("arena_bytes", ctypes.c_long)]
+                ("arena_used", ctypes.c_long), ("arena_bytes", ctypes.c_long),
+                ("graph_rounds", ctypes.c_long)]
 
 
 def lib():
@@ -75,12 +76,12 @@ class Batch:
     """A batch of device-resident systems advanced together (cacg_batch_*)."""
 
     def __init__(self, s=4, precision="fp32", slots=1024, hbm_bytes=0, maxiter=1000,
-                 max_large=0):
+                 max_large=0, graph=True):
         o = BatchOpts()
         lib().cacg_batch_default_opts(ctypes.byref(o))
         o.hbm_bytes, o.slots, o.s = int(hbm_bytes), int(slots), int(s)
         o.precision, o.maxiter = PRECISION[precision], int(maxiter)
-        o.max_large = int(max_large)
+        o.max_large, o.graph = int(max_large), int(bool(graph))
         h = _vp()
         rc = lib().cacg_batch_create(ctypes.byref(h), ctypes.byref(o))
         if rc != 0:
@@ -111,7 +112,8 @@ class Batch:
         st = self.stats
         return {"active": st.active, "queued": st.queued, "finished": st.finished,
                 "admitted": st.admitted, "retired": st.retired, "rounds": st.rounds,
-                "arena_used": st.arena_used, "arena_bytes": st.arena_bytes}
+                "arena_used": st.arena_used, "arena_bytes": st.arena_bytes,
+                "graph_rounds": st.graph_rounds}
 
     def retired(self) -> list:
         """Ids retired since the last call."""

@@ -61,6 +61,19 @@ typedef struct {
  * hardware queues a process gets */
 #define NSOLO 3
 
+/* a round's work lists: offsets and counts into the uploaded {slot, item} array
+ * (order: iterating tiles | fresh tiles | Q-update tiles | X-update tiles |
+ * Gram chunks by class | Gram0 blocks | LU0 | Gram1 blocks | LU1 | all tiles |
+ * all systems), with what else fixes the round's launches */
+typedef struct {
+    long k;
+    long o_it, n_it, o_fr, n_fr;
+    long o_qc[9][2], n_qc[9][2], o_xc[9][2][2], n_xc[9][2][2];
+    long o_gc[2][25], n_gc[2][25];
+    long o_b0, n_b0, o_lu[2][2], n_lu[2][2], o_b1, n_b1, o_all, n_all, o_sys, n_sys;
+    int s, f32, max_sm, pad_;
+} wlists;
+
 struct cacg_batch {
     cacg_batch_opts o;
     hipStream_t st;
@@ -82,6 +95,13 @@ struct cacg_batch {
     long rounds;
     long *ret;           /* ids retired and not yet reported by cacg_batch_retired */
     long nret, capret;
+    /* the last uploaded work lists (the graph key) and the graph of their launches */
+    wlists g_wl;
+    int2 *g_items, *g_ditems;
+    long g_cap;
+    int g_valid;
+    hipGraphExec_t g_exec;
+    long graph_rounds;
 };
 
 #define HCHK(call)                                                                               \
@@ -234,6 +254,7 @@ void cacg_batch_default_opts(cacg_batch_opts *o)
     o->precision = CACG_PREC_FP32_REFINE;
     o->maxiter = 1000;
     o->max_large = 0;
+    o->graph = 1;
 }
 
 static void free_req(breq *r)
@@ -255,6 +276,9 @@ int cacg_batch_destroy(cacg_batch *b)
         return CACG_OK;
     if (b->st)
         hipStreamSynchronize(b->st);
+    if (b->g_exec)
+        hipGraphExecDestroy(b->g_exec);
+    free(b->g_items);
     for (long i = 0; i < b->nreq; i++)
         free_req(b->req[i]);
     free(b->req);
@@ -327,7 +351,7 @@ int cacg_batch_create(cacg_batch **out, const cacg_batch_opts *opts)
     if (o.slots == 0)
         o.slots = 1024;
     if (o.slots < 1 || o.slots > 4096 || o.s < 1 || o.maxiter <= 0 || o.hbm_bytes < 0 ||
-        o.max_large < 0 ||
+        o.max_large < 0 || (o.graph != 0 && o.graph != 1) ||
         (o.precision != CACG_PREC_FP64 && o.precision != CACG_PREC_FP32_REFINE))
         return CACG_EINVAL;
     int cnt = 0;
@@ -752,6 +776,98 @@ static int solo_residual(const cacg_bsys *d, hipStream_t st)
     return CACG_OK;
 }
 
+/* the batched launches of a round, in order, on the main stream: basis steps,
+ * Q = S for fresh systems, QtAS, beta, Q = S - Q beta (X += Q alpha deferred in),
+ * A Q, [QtAQ | QtR], alpha, X += Q alpha; then R = B - A X and the norms */
+static int batched_launches(cacg_batch *b, const wlists *L)
+{
+    hipStream_t st = b->st;
+    const int2 *di = b->d_items;
+    const int s = L->s, f32 = L->f32;
+    if (L->n_it > 0) {
+        if (s == 1) {
+            KCHK(cacg_kb_basis(b->d_sys, di + L->o_it, (int)L->n_it, 0, f32, st));
+        } else {
+            KCHK(cacg_kb_basis(b->d_sys, di + L->o_it, (int)L->n_it, 1, f32, st));
+            for (int j = 2; j < s; j++)
+                KCHK(cacg_kb_basis(b->d_sys, di + L->o_it, (int)L->n_it, j, f32, st));
+        }
+        KCHK(cacg_kb_q_from_s(b->d_sys, di + L->o_fr, (int)L->n_fr, f32, st));
+        KCHK(gram_phase(b, di, L->o_gc[0], L->n_gc[0], di + L->o_b0, L->n_b0, 0, f32));
+        KCHK(cacg_k_lu_batch(b->d_sys, di + L->o_lu[0][0], (int)L->n_lu[0][0], 0, 64, st));
+        KCHK(cacg_k_lu_batch(b->d_sys, di + L->o_lu[0][1], (int)L->n_lu[0][1], 0, L->max_sm, st));
+        for (int c = 1; c <= 8; c++)
+            for (int ex = 0; ex < 2; ex++)
+                KCHK(cacg_kb_qupdate_class(b->d_sys, di + L->o_qc[c][ex], (int)L->n_qc[c][ex],
+                                           16 * c, ex, f32, st));
+        KCHK(cacg_kb_aq(b->d_sys, di + L->o_it, (int)L->n_it, st));
+        KCHK(gram_phase(b, di, L->o_gc[1], L->n_gc[1], di + L->o_b1, L->n_b1, 1, f32));
+        KCHK(cacg_k_lu_batch(b->d_sys, di + L->o_lu[1][0], (int)L->n_lu[1][0], 1, 64, st));
+        KCHK(cacg_k_lu_batch(b->d_sys, di + L->o_lu[1][1], (int)L->n_lu[1][1], 1, L->max_sm, st));
+        for (int c = 1; c <= 8; c++)
+            for (int ex = 0; ex < 2; ex++)
+                for (int h = 0; h < 2; h++)
+                    KCHK(cacg_kb_xupdate_class(b->d_sys, di + L->o_xc[c][ex][h],
+                                               (int)L->n_xc[c][ex][h], 16 * c, ex, h ? 32 : 16,
+                                               st));
+    }
+    KCHK(cacg_kb_resid(b->d_sys, di + L->o_all, (int)L->n_all, di + L->o_sys, (int)L->n_sys, st));
+    return CACG_OK;
+}
+
+static void drop_graph(cacg_batch *b)
+{
+    if (b->g_exec)
+        (void)hipGraphExecDestroy(b->g_exec);
+    b->g_exec = NULL;
+    b->g_valid = 0;
+}
+
+/* remember the lists just uploaded (the key a later round is compared with) */
+static void keep_lists(cacg_batch *b, const wlists *L, const int2 *it)
+{
+    if (!b->o.graph)
+        return;
+    if (L->k > b->g_cap) {
+        free(b->g_items);
+        b->g_cap = 0;
+        b->g_items = (int2 *)malloc(sizeof(int2) * (size_t)L->k);
+        if (!b->g_items)
+            return;   /* no key, no graph: the rounds launch one by one */
+        b->g_cap = L->k;
+    }
+    memcpy(b->g_items, it, sizeof(int2) * (size_t)L->k);
+    b->g_wl = *L;
+    b->g_ditems = b->d_items;
+    b->g_valid = 1;
+}
+
+/* capture the batched launches of a round into b->g_exec (nothing runs during
+ * the capture).  The launches read the slot descriptors and work lists from
+ * device memory, so a replay serves any later round with the same lists.  If
+ * capture or instantiation fails, the stream has left capture mode, graphs are
+ * switched off for this batch and the caller launches one by one. */
+static void capture_round(cacg_batch *b, const wlists *L)
+{
+    hipGraph_t g = NULL;
+    if (hipStreamBeginCapture(b->st, hipStreamCaptureModeRelaxed) != hipSuccess) {
+        (void)hipGetLastError();
+        b->o.graph = 0;
+        return;
+    }
+    const int rc = batched_launches(b, L);
+    const hipError_t e = hipStreamEndCapture(b->st, &g);
+    if (rc == CACG_OK && e == hipSuccess && g &&
+        hipGraphInstantiate(&b->g_exec, g, NULL, NULL, 0) != hipSuccess)
+        b->g_exec = NULL;
+    if (g)
+        (void)hipGraphDestroy(g);
+    if (!b->g_exec) {
+        (void)hipGetLastError();
+        b->o.graph = 0;
+    }
+}
+
 int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
 {
     if (!b)
@@ -813,24 +929,24 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
     if (grow_items(b, need) != CACG_OK)
         return CACG_ENOMEM;
     int2 *it = b->h_items;
+    wlists wl;
+    memset(&wl, 0, sizeof(wl));   /* compared bytewise: padding included */
     long k = 0;
-    /* order: iterating tiles | fresh tiles | update tiles | Gram chunks by class (which 0,
-     *        then 1) | Gram0 blocks | LU0 | Gram1 blocks | LU1 | all tiles | all systems */
-    long o_it = k;
+    wl.o_it = k;
     for (int i = 0; i < b->nslots; i++)
         if (BATCHED(i) && b->slot[i]->joined)
             k = put_range(it, k, i, b->h_sys[i].ntile);
-    const long n_it = k - o_it, o_fr = k;
+    wl.n_it = k - wl.o_it;
+    wl.o_fr = k;
     for (int i = 0; i < b->nslots; i++)
         if (BATCHED(i) && b->slot[i]->joined && b->slot[i]->fresh)
             k = put_range(it, k, i, b->h_sys[i].ntile);
-    const long n_fr = k - o_fr;
+    wl.n_fr = k - wl.o_fr;
     /* Q and X updates as MFMA per padded width smp = 16 ceil(sm / 16), split by
      * sm == smp (16-byte loads) and, for X, by m <= 16 */
-    long o_qc[9][2], n_qc[9][2], o_xc[9][2][2], n_xc[9][2][2];
     for (int c = 1; c <= 8; c++)
         for (int ex = 0; ex < 2; ex++) {
-            o_qc[c][ex] = k;
+            wl.o_qc[c][ex] = k;
             for (int i = 0; i < b->nslots; i++) {
                 const breq *r = b->slot[i];
                 const int sm = b->h_sys[i].sm;
@@ -838,9 +954,9 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
                     (sm == 16 * c) == ex)
                     k = put_range(it, k, i, b->h_sys[i].nchunk);
             }
-            n_qc[c][ex] = k - o_qc[c][ex];
+            wl.n_qc[c][ex] = k - wl.o_qc[c][ex];
             for (int h = 0; h < 2; h++) {
-                o_xc[c][ex][h] = k;
+                wl.o_xc[c][ex][h] = k;
                 for (int i = 0; i < b->nslots; i++) {
                     const breq *r = b->slot[i];
                     const int sm = b->h_sys[i].sm;
@@ -848,15 +964,14 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
                         (b->h_sys[i].m > 16) == h)
                         k = put_range(it, k, i, b->h_sys[i].nchunk);
                 }
-                n_xc[c][ex][h] = k - o_xc[c][ex][h];
+                wl.n_xc[c][ex][h] = k - wl.o_xc[c][ex][h];
             }
         }
     /* Gram chunks per tile class: [which][1 + 3 (tm - 1) + tn - tm]; [which][0] stays empty
      * (every admissible system has a class) */
-    long o_gc[2][25], n_gc[2][25];
     for (int w = 0; w < 2; w++)
         for (int c = 0; c < 25; c++) {
-            o_gc[w][c] = k;
+            wl.o_gc[w][c] = k;
             for (int i = 0; i < b->nslots; i++) {
                 const breq *r = b->slot[i];
                 if (!BATCHED(i) || !r->joined || (w == 0 && r->fresh))
@@ -867,46 +982,58 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
                 if (cls == c)
                     k = put_range(it, k, i, b->h_sys[i].nchunk);
             }
-            n_gc[w][c] = k - o_gc[w][c];
+            wl.n_gc[w][c] = k - wl.o_gc[w][c];
         }
-    const long o_b0 = k;
+    wl.o_b0 = k;
     for (int i = 0; i < b->nslots; i++)
         if (BATCHED(i) && b->slot[i]->joined && !b->slot[i]->fresh) {
             const cacg_bsys *d = &b->h_sys[i];
             k = put_range(it, k, i, (int)(((long)d->sm * d->sm + 255) / 256));
         }
-    const long n_b0 = k - o_b0;
+    wl.n_b0 = k - wl.o_b0;
     /* LU solves in two LDS classes: sm <= 64 (several workgroups per CU), larger */
-    long o_lu[2][2], n_lu[2][2];
     for (int w = 0; w < 2; w++)
         for (int c = 0; c < 2; c++) {
-            o_lu[w][c] = k;
+            wl.o_lu[w][c] = k;
             for (int i = 0; i < b->nslots; i++) {
                 const breq *r = b->slot[i];
                 if (BATCHED(i) && r->joined && (w == 1 || !r->fresh) &&
                     ((b->h_sys[i].sm > 64) == c))
                     k = put_range(it, k, i, 1);
             }
-            n_lu[w][c] = k - o_lu[w][c];
+            wl.n_lu[w][c] = k - wl.o_lu[w][c];
         }
-    const long o_b1 = k;
+    wl.o_b1 = k;
     for (int i = 0; i < b->nslots; i++)
         if (BATCHED(i) && b->slot[i]->joined) {
             const cacg_bsys *d = &b->h_sys[i];
             k = put_range(it, k, i, (int)(((long)d->sm * (d->sm + d->m) + 255) / 256));
         }
-    const long n_b1 = k - o_b1, o_all = k;
+    wl.n_b1 = k - wl.o_b1;
+    wl.o_all = k;
     for (int i = 0; i < b->nslots; i++)
         if (BATCHED(i))
             k = put_range(it, k, i, b->h_sys[i].ntile);
-    const long n_all = k - o_all, o_sys = k;
+    wl.n_all = k - wl.o_all;
+    wl.o_sys = k;
     for (int i = 0; i < b->nslots; i++)
         if (BATCHED(i))
             k = put_range(it, k, i, 1);
-    const long n_sys = k - o_sys;
-    if (k > 0)
+    wl.n_sys = k - wl.o_sys;
+    wl.k = k;
+    wl.s = s;
+    wl.f32 = f32;
+    wl.max_sm = max_sm;
+    /* the same lists as the last upload (the device copy still holds them): the
+     * round's launches repeat the last round's, and run as a graph */
+    const int same = k > 0 && b->o.graph && b->g_valid && b->g_ditems == b->d_items &&
+                     memcmp(&wl, &b->g_wl, sizeof(wl)) == 0 &&
+                     memcmp(it, b->g_items, sizeof(int2) * (size_t)k) == 0;
+    if (!same && k > 0) {
         HCHK(hipMemcpyAsync(b->d_items, it, sizeof(int2) * (size_t)k, hipMemcpyHostToDevice, st));
-    const int2 *di = b->d_items;
+        drop_graph(b);
+        keep_lists(b, &wl, it);
+    }
 
     /* solo systems: one iteration (when joined) and the true residual, dealt
      * round-robin over the solo streams, which first wait for what the main
@@ -940,37 +1067,21 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
                 return rc;
         }
     }
-
-    /* one outer iteration of the joined systems */
-    if (n_it > 0) {
-        if (s == 1) {
-            KCHK(cacg_kb_basis(b->d_sys, di + o_it, (int)n_it, 0, f32, st));
-        } else {
-            KCHK(cacg_kb_basis(b->d_sys, di + o_it, (int)n_it, 1, f32, st));
-            for (int j = 2; j < s; j++)
-                KCHK(cacg_kb_basis(b->d_sys, di + o_it, (int)n_it, j, f32, st));
-        }
-        KCHK(cacg_kb_q_from_s(b->d_sys, di + o_fr, (int)n_fr, f32, st));
-        KCHK(gram_phase(b, di, o_gc[0], n_gc[0], di + o_b0, n_b0, 0, f32));
-        KCHK(cacg_k_lu_batch(b->d_sys, di + o_lu[0][0], (int)n_lu[0][0], 0, 64, st));
-        KCHK(cacg_k_lu_batch(b->d_sys, di + o_lu[0][1], (int)n_lu[0][1], 0, max_sm, st));
-        for (int c = 1; c <= 8; c++)
-            for (int ex = 0; ex < 2; ex++)
-                KCHK(cacg_kb_qupdate_class(b->d_sys, di + o_qc[c][ex], (int)n_qc[c][ex], 16 * c, ex,
-                                           f32, st));
-        KCHK(cacg_kb_aq(b->d_sys, di + o_it, (int)n_it, st));
-        KCHK(gram_phase(b, di, o_gc[1], n_gc[1], di + o_b1, n_b1, 1, f32));
-        KCHK(cacg_k_lu_batch(b->d_sys, di + o_lu[1][0], (int)n_lu[1][0], 1, 64, st));
-        KCHK(cacg_k_lu_batch(b->d_sys, di + o_lu[1][1], (int)n_lu[1][1], 1, max_sm, st));
-        for (int c = 1; c <= 8; c++)
-            for (int ex = 0; ex < 2; ex++)
-                for (int h = 0; h < 2; h++)
-                    KCHK(cacg_kb_xupdate_class(b->d_sys, di + o_xc[c][ex][h], (int)n_xc[c][ex][h],
-                                               16 * c, ex, h ? 32 : 16, st));
-    }
 #undef BATCHED
-    /* the true residual of every batched system */
-    KCHK(cacg_kb_resid(b->d_sys, di + o_all, (int)n_all, di + o_sys, (int)n_sys, st));
+
+    /* one outer iteration of the joined batched systems and the true residual of
+     * every batched one: replayed from the graph when the lists repeat (captured on
+     * the first repeat), else launched one by one */
+    if (same && !b->g_exec)
+        capture_round(b, &wl);
+    if (same && b->g_exec) {
+        HCHK(hipGraphLaunch(b->g_exec, st));
+        b->graph_rounds++;
+    } else {
+        const int rc = batched_launches(b, &wl);
+        if (rc != CACG_OK)
+            return rc;
+    }
     /* the solo streams' work joins the main stream before the norms come back */
     for (int q2 = 0; q2 < NSOLO && q2 < nsolo; q2++) {
         HCHK(hipEventRecord(b->sev[q2], b->sst[q2]));
@@ -1055,6 +1166,7 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
         stats->rounds = b->rounds;
         stats->arena_used = b->arena_used;
         stats->arena_bytes = b->arena_bytes;
+        stats->graph_rounds = b->graph_rounds;
     }
     return CACG_OK;
 }

@@ -39,6 +39,9 @@ typedef struct {
     int max_large;   /* systems of 2^17 rows or more (each runs the single-system kernels on
                         its own stream) in flight at most; 0: no limit.  A queued large system
                         at the head waits for one to retire (FIFO: later requests wait too) */
+    int graph;       /* 1 (default): when a round's batched launches repeat the previous
+                        round's (same work lists), capture them once in a HIP graph and replay
+                        it, with no work-list upload; 0: launch them one by one.  Same bits */
 } cacg_batch_opts;
 
 typedef struct {
@@ -56,6 +59,7 @@ typedef struct {
     int retired;         /* retired in the last round */
     long rounds;         /* rounds so far */
     long arena_used, arena_bytes;
+    long graph_rounds;   /* rounds so far whose batched launches ran as a graph replay */
 } cacg_batch_stats;
 
 void cacg_batch_default_opts(cacg_batch_opts *o);

@@ -1,5 +1,8 @@
-"""Host-side parts of the batching engine, no GPU: the footprint formula
-and refusal to create a batch without a device."""
+"""Host-side parts of the batching engine, no GPU: the footprint formula, the
+Python mirrors of the header's structs, refusal of bad options and of a batch
+without a device."""
+
+import ctypes
 
 import pytest
 
@@ -30,3 +33,55 @@ def test_create_without_device_fails_cleanly():
         pytest.skip("a device is visible")
     with pytest.raises(RuntimeError):
         BT.Batch()
+
+
+BATCH_C = r"""
+#include <stddef.h>
+#include <stdio.h>
+#include "cacg_batch.h"
+int main(void)
+{
+  printf("opts %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(cacg_batch_opts),
+         offsetof(cacg_batch_opts, hbm_bytes), offsetof(cacg_batch_opts, slots),
+         offsetof(cacg_batch_opts, s), offsetof(cacg_batch_opts, precision),
+         offsetof(cacg_batch_opts, maxiter), offsetof(cacg_batch_opts, max_large),
+         offsetof(cacg_batch_opts, graph));
+  printf("stats %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(cacg_batch_stats),
+         offsetof(cacg_batch_stats, active), offsetof(cacg_batch_stats, queued),
+         offsetof(cacg_batch_stats, finished), offsetof(cacg_batch_stats, admitted),
+         offsetof(cacg_batch_stats, retired), offsetof(cacg_batch_stats, rounds),
+         offsetof(cacg_batch_stats, arena_used), offsetof(cacg_batch_stats, arena_bytes),
+         offsetof(cacg_batch_stats, graph_rounds));
+  printf("result %zu %zu %zu %zu %zu\n", sizeof(cacg_batch_result),
+         offsetof(cacg_batch_result, status), offsetof(cacg_batch_result, iters),
+         offsetof(cacg_batch_result, rounds_queued), offsetof(cacg_batch_result, true_relres));
+  return 0;
+}
+"""
+
+
+def test_python_batch_structs_match_header(tmp_path):
+    import os
+    import subprocess
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    src, exe = tmp_path / "b.c", tmp_path / "b"
+    src.write_text(BATCH_C)
+    subprocess.run(["gcc", "-std=gnu11", "-D__HIP_PLATFORM_AMD__", "-I", inc, "-I",
+                    "/opt/rocm/include", str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    for line, S in zip(out, (BT.BatchOpts, BT.BatchStats, BT.BatchResult)):
+        want = [ctypes.sizeof(S)] + [getattr(S, f).offset for f, _ in S._fields_]
+        assert line.split()[1:] == [str(v) for v in want], line
+
+
+@pytest.mark.parametrize("field,value", [("graph", 2), ("graph", -1), ("max_large", -1),
+                                         ("s", 0), ("slots", 5000), ("maxiter", 0)])
+def test_batch_options_rejected_without_device(field, value):
+    """Invalid batch options are refused (CACG_EINVAL) before any device is touched."""
+    o = BT.BatchOpts()
+    BT.lib().cacg_batch_default_opts(ctypes.byref(o))
+    assert o.graph == 1 and o.max_large == 0
+    setattr(o, field, value)
+    h = ctypes.c_void_p()
+    assert BT.lib().cacg_batch_create(ctypes.byref(h), ctypes.byref(o)) == 3
+    assert not h.value

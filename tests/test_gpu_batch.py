@@ -171,3 +171,42 @@ def test_max_large_caps_concurrent_large_systems(gpu_lib):
         assert x["status"] == "ok" and np.array_equal(x["X"], y["X"])
     with pytest.raises(RuntimeError):
         BT.Batch(s=4, max_large=-1)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_graph_replays_give_the_launch_by_launch_bits(gpu_lib, precision):
+    # Two batches driven in lockstep, one replaying captured rounds (graph) and one
+    # launching every kernel on its own; submissions arrive mid-run (the lists change:
+    # the graph is dropped and captured again) and a large system iterates on its own
+    # streams beside the replays.  Every round retires the same systems, and every
+    # result has the same bits and iteration count.
+    systems = _mix()
+    A0 = P.laplacian_2d(400, 0.02)
+    systems.insert(2, (A0, P.uniform(400, A0.n * 2).reshape(A0.n, 2)))
+    print(f"batch graph {precision}", flush=True)
+    runs = [BT.Batch(s=4, precision=precision, hbm_bytes=2 << 30, graph=g) for g in (False, True)]
+    ids = [[], []]
+    res = [{}, {}]
+    for rnd in range(100000):
+        if rnd == 0 or rnd == 4:
+            part = systems[:6] if rnd == 0 else systems[6:]
+            for k in range(2):
+                ids[k] += [runs[k].submit(A, B, tol=TOL) for A, B in part]
+        st = [b.step() for b in runs]
+        got = [sorted(b.retired()) for b in runs]
+        assert got[0] == got[1], rnd
+        for k in range(2):
+            for rid in got[k]:
+                res[k][rid] = runs[k].collect(rid)
+        keys = ("active", "queued", "finished", "retired", "rounds")
+        assert [st[0][q] for q in keys] == [st[1][q] for q in keys], rnd
+        if rnd > 4 and st[0]["active"] == 0 and st[0]["queued"] == 0:
+            break
+    assert st[0]["graph_rounds"] == 0 and st[1]["graph_rounds"] > 10, st[1]
+    for b in runs:
+        b.close()
+    assert ids[0] == ids[1] and len(res[0]) == len(systems)
+    for i in ids[0]:
+        x, y = res[0][i], res[1][i]
+        assert x["status"] == "ok" and y["status"] == "ok"
+        assert x["iters"] == y["iters"] and np.array_equal(x["X"], y["X"]), i
