@@ -74,6 +74,25 @@ typedef struct {
     int s, f32, max_sm, pad_;
 } wlists;
 
+/* Online SpMM schedule tuning of a solo system: its first NCAND iterations each
+ * run their basis steps and A Q under one candidate schedule (tile order and
+ * grid, cacg_spmm_sched), timed by events on the system's stream, and the
+ * fastest of each shape is kept for the rest of the solve.  The timed launches
+ * are the solve's own (a schedule changes no bit), so tuning costs no extra
+ * work; the candidates are those of the single-system solver's offline tuner. */
+static const cacg_spmm_sched CAND[] = {{0, 0},    {4096, 0},    {8192, 0},    {0, 1024},
+                                       {0, 2048}, {0, 4096},    {4096, 2048}, {8192, 4096}};
+#define NCAND ((int)(sizeof(CAND) / sizeof(CAND[0])))
+
+typedef struct {
+    hipEvent_t ev[4];          /* basis steps start / end, A Q start / end */
+    int have_ev;
+    int timing;                /* the candidate timed this round, -1: none */
+    int next;                  /* the next candidate; NCAND: done */
+    float best_b, best_a;
+    cacg_spmm_sched sb, sa;    /* in use: basis steps, A Q */
+} stune;
+
 struct cacg_batch {
     cacg_batch_opts o;
     hipStream_t st;
@@ -102,6 +121,7 @@ struct cacg_batch {
     int g_valid;
     hipGraphExec_t g_exec;
     long graph_rounds;
+    stune *tune;                /* per slot */
 };
 
 #define HCHK(call)                                                                               \
@@ -255,6 +275,7 @@ void cacg_batch_default_opts(cacg_batch_opts *o)
     o->maxiter = 1000;
     o->max_large = 0;
     o->graph = 1;
+    o->tune = 1;
 }
 
 static void free_req(breq *r)
@@ -284,6 +305,13 @@ int cacg_batch_destroy(cacg_batch *b)
     free(b->req);
     free(b->ret);
     free(b->slot);
+    if (b->tune) {
+        for (int i = 0; i < b->nslots; i++)
+            if (b->tune[i].have_ev)
+                for (int e = 0; e < 4; e++)
+                    hipEventDestroy(b->tune[i].ev[e]);
+        free(b->tune);
+    }
     free(b->fl);
     if (b->arena)
         hipFree(b->arena);
@@ -352,6 +380,7 @@ int cacg_batch_create(cacg_batch **out, const cacg_batch_opts *opts)
         o.slots = 1024;
     if (o.slots < 1 || o.slots > 4096 || o.s < 1 || o.maxiter <= 0 || o.hbm_bytes < 0 ||
         o.max_large < 0 || (o.graph != 0 && o.graph != 1) ||
+        (o.tune != 0 && o.tune != 1) ||
         (o.precision != CACG_PREC_FP64 && o.precision != CACG_PREC_FP32_REFINE))
         return CACG_EINVAL;
     int cnt = 0;
@@ -374,9 +403,10 @@ int cacg_batch_create(cacg_batch **out, const cacg_batch_opts *opts)
     b->capfl = 64;
     b->fl = (fblk *)malloc(sizeof(fblk) * (size_t)b->capfl);
     b->slot = (breq **)calloc((size_t)b->nslots, sizeof(breq *));
+    b->tune = (stune *)calloc((size_t)b->nslots, sizeof(stune));
     b->capreq = 1024;
     b->req = (breq **)calloc((size_t)b->capreq, sizeof(breq *));
-    if (!b->fl || !b->slot || !b->req) {
+    if (!b->fl || !b->slot || !b->tune || !b->req) {
         cacg_batch_destroy(b);
         return CACG_ENOMEM;
     }
@@ -585,6 +615,11 @@ static int admit(cacg_batch *b, breq *r, int slot)
     r->iters = 0;
     r->rounds_queued = b->rounds - r->round_submitted;
     b->slot[slot] = r;
+    stune *t = &b->tune[slot];
+    t->timing = -1;
+    t->next = b->o.tune ? 0 : NCAND;
+    t->best_b = t->best_a = -1.f;
+    t->sb = t->sa = CAND[0];
     return 1;
 }
 
@@ -687,6 +722,7 @@ static int solo_flush_x(const cacg_bsys *d, hipStream_t st)
  * and X += Q alpha rides on the next Q update (*xpend) where the widths allow;
  * a round of solo_residual then recomputes the true residual, which decides. */
 static int solo_iteration(const cacg_bsys *d, int fresh, int s, int f32, int *xpend,
+                          const cacg_spmm_sched *sb, const cacg_spmm_sched *sa, hipEvent_t *ev,
                           hipStream_t st)
 {
     const int split = cacg_k_split_supported(d->sm, d->m);
@@ -694,17 +730,21 @@ static int solo_iteration(const cacg_bsys *d, int fresh, int s, int f32, int *xp
     const cacg_dcsr A = {n, d->rowptr, d->col, d->val};
     const cacg_scsr A32 = {n, d->rowptr, d->col, d->val32};
     const double sa2 = 2.0 * d->sa, sb2 = 2.0 * d->sb;
+    if (ev)
+        HCHK(hipEventRecord(ev[0], st));
     if (f32) {
         float *S = (float *)d->S;
         if (s == 1) {
             KCHK(cacg_k_cvt_f64_f32((long)n * m, d->R, S, st));
         } else {
-            KCHK(cacg_k_cheb1_f32(&A32, m, d->R, m, S + m, sm, S, sm, d->sa, d->sb, st));
+            KCHK(cacg_k_cheb1_f32_x(&A32, m, d->R, m, S + m, sm, S, sm, d->sa, d->sb, sb, st));
             for (int j = 2; j < s; j++)
-                KCHK(cacg_k_spmm_f32(&A32, m, S + (long)(j - 1) * m, sm, S + (long)j * m, sm,
-                                     CACG_EPI_CHEB, sa2, sb2, S + (long)(j - 1) * m, sm,
-                                     S + (long)(j - 2) * m, sm, st));
+                KCHK(cacg_k_spmm_f32_x(&A32, m, S + (long)(j - 1) * m, sm, S + (long)j * m, sm,
+                                       CACG_EPI_CHEB, sa2, sb2, S + (long)(j - 1) * m, sm,
+                                       S + (long)(j - 2) * m, sm, sb, st));
         }
+        if (ev)
+            HCHK(hipEventRecord(ev[1], st));
         if (fresh) {
             if (*xpend && solo_flush_x(d, st) != CACG_OK)
                 return CACG_EDEVICE;
@@ -726,13 +766,15 @@ static int solo_iteration(const cacg_bsys *d, int fresh, int s, int f32, int *xp
             HCHK(hipMemcpyAsync(S, d->R, sizeof(double) * (size_t)n * m, hipMemcpyDeviceToDevice,
                                 st));
         } else {
-            KCHK(cacg_k_spmm(&A, m, d->R, m, S + m, sm, CACG_EPI_CHEB1, d->sa, d->sb, NULL, 0, NULL,
-                             0, S, sm, st));
+            KCHK(cacg_k_spmm_x(&A, m, d->R, m, S + m, sm, CACG_EPI_CHEB1, d->sa, d->sb, NULL, 0,
+                               NULL, 0, S, sm, sb, st));
             for (int j = 2; j < s; j++)
-                KCHK(cacg_k_spmm(&A, m, S + (long)(j - 1) * m, sm, S + (long)j * m, sm,
-                                 CACG_EPI_CHEB, sa2, sb2, S + (long)(j - 1) * m, sm,
-                                 S + (long)(j - 2) * m, sm, NULL, 0, st));
+                KCHK(cacg_k_spmm_x(&A, m, S + (long)(j - 1) * m, sm, S + (long)j * m, sm,
+                                   CACG_EPI_CHEB, sa2, sb2, S + (long)(j - 1) * m, sm,
+                                   S + (long)(j - 2) * m, sm, NULL, 0, sb, st));
         }
+        if (ev)
+            HCHK(hipEventRecord(ev[1], st));
         if (fresh) {
             if (*xpend && solo_flush_x(d, st) != CACG_OK)
                 return CACG_EDEVICE;
@@ -750,8 +792,12 @@ static int solo_iteration(const cacg_bsys *d, int fresh, int s, int f32, int *xp
         }
         *xpend = 0;
     }
-    KCHK(cacg_k_spmm(&A, sm, d->Q, sm, d->AQ, sm, CACG_EPI_NONE, 0.0, 0.0, NULL, 0, NULL, 0, NULL,
-                     0, st));
+    if (ev)
+        HCHK(hipEventRecord(ev[2], st));
+    KCHK(cacg_k_spmm_x(&A, sm, d->Q, sm, d->AQ, sm, CACG_EPI_NONE, 0.0, 0.0, NULL, 0, NULL, 0, NULL,
+                       0, sa, st));
+    if (ev)
+        HCHK(hipEventRecord(ev[3], st));
     KCHK(cacg_k_gram_fast(n, d->Q, sm, sm, d->AQ, sm, sm, d->R, m, m, d->QtAQ, d->QtR, d->gpart,
                           st));
     KCHK(cacg_k_lu_solve(sm, d->QtAQ, m, d->QtR, d->alpha, d->lu, d->flag, CACG_MODE_FAST, 0, st));
@@ -1056,7 +1102,27 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
             breq *rw = b->slot[i];
             int rc;
             if (r->joined && !r->verify) {
-                rc = solo_iteration(&b->h_sys[i], r->fresh, s, f32, &rw->xpend, ss);
+                stune *t = &b->tune[i];
+                t->timing = -1;
+                if (t->next < NCAND && !t->have_ev) {
+                    int e = 0;
+                    while (e < 4 && hipEventCreate(&t->ev[e]) == hipSuccess)
+                        e++;
+                    if (e < 4) { /* no events: keep the default schedules */
+                        (void)hipGetLastError();
+                        while (e > 0)
+                            hipEventDestroy(t->ev[--e]);
+                        t->next = NCAND;
+                    } else {
+                        t->have_ev = 1;
+                    }
+                }
+                if (t->next < NCAND)
+                    t->timing = t->next;
+                const cacg_spmm_sched *sb = t->timing >= 0 ? &CAND[t->timing] : &t->sb;
+                const cacg_spmm_sched *sa = t->timing >= 0 ? &CAND[t->timing] : &t->sa;
+                rc = solo_iteration(&b->h_sys[i], r->fresh, s, f32, &rw->xpend, sb, sa,
+                                    t->timing >= 0 ? t->ev : NULL, ss);
             } else { /* admitted, or verifying: X complete, then the true residual */
                 rc = rw->xpend ? solo_flush_x(&b->h_sys[i], ss) : CACG_OK;
                 rw->xpend = 0;
@@ -1100,6 +1166,27 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
     }
     HCHK(hipStreamSynchronize(st));
     b->rounds++;
+    /* the solo systems' timed candidates (their streams joined the main one) */
+    for (int i = 0; i < hi_slot; i++) {
+        stune *t = &b->tune[i];
+        if (!b->slot[i] || t->timing < 0)
+            continue;
+        float tb = 0.f, ta = 0.f;
+        const hipError_t eb = hipEventElapsedTime(&tb, t->ev[0], t->ev[1]);
+        const hipError_t ea = hipEventElapsedTime(&ta, t->ev[2], t->ev[3]);
+        if (eb != hipSuccess || ea != hipSuccess)
+            (void)hipGetLastError();   /* an untimed candidate is skipped, not an error */
+        if (eb == hipSuccess && (t->best_b < 0.f || tb < t->best_b)) {
+            t->best_b = tb;
+            t->sb = CAND[t->timing];
+        }
+        if (ea == hipSuccess && (t->best_a < 0.f || ta < t->best_a)) {
+            t->best_a = ta;
+            t->sa = CAND[t->timing];
+        }
+        t->next = t->timing + 1;
+        t->timing = -1;
+    }
 
     /* stop tests on the true residuals; retire */
     for (int i = 0; i < hi_slot; i++) {

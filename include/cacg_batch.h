@@ -42,6 +42,10 @@ typedef struct {
     int graph;       /* 1 (default): when a round's batched launches repeat the previous
                         round's (same work lists), capture them once in a HIP graph and replay
                         it, with no work-list upload; 0: launch them one by one.  Same bits */
+    int tune;        /* 1 (default): the first iterations of a system of 2^17 rows or more run
+                        its SpMMs under candidate schedules (tile order, grid), timed on its
+                        stream, and the rest under the fastest; 0: default schedules.  Same
+                        bits */
 } cacg_batch_opts;
 
 typedef struct {

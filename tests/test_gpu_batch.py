@@ -134,7 +134,9 @@ def test_batching_form_sample(gpu_lib):
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
 def test_large_systems_alone_equal_mix_and_meet_tol(gpu_lib, precision):
     # two systems above the row count that runs the single-system kernels,
-    # mixed with small batched ones: same bits alone and in the mix
+    # mixed with small batched ones: same bits alone and in the mix, and with the
+    # online schedule tuning off (each run times its own candidates, under
+    # different concurrent work, so the kept schedules may differ)
     big = [(P.laplacian_2d(400, 0.01), 4), (P.laplacian_3d(52, 0.02), 8)]
     systems = [(A, P.uniform(200 + k, A.n * m).reshape(A.n, m)) for k, (A, m) in enumerate(big)]
     systems += _mix()[:3]
@@ -148,6 +150,10 @@ def test_large_systems_alone_equal_mix_and_meet_tol(gpu_lib, precision):
         alone = _run([systems[k]], s=4, precision=precision, hbm_bytes=4 << 30)[0]
         assert alone["iters"] == mixed[k]["iters"]
         assert np.array_equal(alone["X"], mixed[k]["X"]), k
+    untuned = _run(systems[:2], s=4, precision=precision, hbm_bytes=4 << 30, tune=False)
+    for k in range(2):
+        assert untuned[k]["iters"] == mixed[k]["iters"]
+        assert np.array_equal(untuned[k]["X"], mixed[k]["X"]), k
 
 
 def test_max_large_caps_concurrent_large_systems(gpu_lib):
