@@ -233,12 +233,12 @@ def k_gerschgorin(A) -> np.ndarray:
 
 
 class SpmmSched(ctypes.Structure):
-    _fields_ = [("grid", ctypes.c_int), ("chunk_rows", ctypes.c_int)]
+    _fields_ = [("grid", ctypes.c_int), ("chunk_rows", ctypes.c_int), ("passes", ctypes.c_int)]
 
 
 def k_spmm(A, X, epi="none", a=0.0, b=0.0, p1=None, p2=None, want_copy0=False, sched=None):
     """Y = epilogue(A X) for X n x k; returns Y (and copy0 for 'cheb1').
-    sched = (grid, chunk_rows) selects a launch schedule (cacg_k_spmm_x)."""
+    sched = (grid, chunk_rows[, passes]) selects a launch schedule (cacg_k_spmm_x)."""
     dA = DeviceCSR(A)
     X = np.ascontiguousarray(X, dtype=np.float64).reshape(A.n, -1)
     k = X.shape[1]

@@ -80,8 +80,9 @@ typedef struct {
  * fastest of each shape is kept for the rest of the solve.  The timed launches
  * are the solve's own (a schedule changes no bit), so tuning costs no extra
  * work; the candidates are those of the single-system solver's offline tuner. */
-static const cacg_spmm_sched CAND[] = {{0, 0},    {4096, 0},    {8192, 0},    {0, 1024},
-                                       {0, 2048}, {0, 4096},    {4096, 2048}, {8192, 4096}};
+static const cacg_spmm_sched CAND[] = {
+    {0, 0, 0},       {4096, 0, 0},    {8192, 0, 0},    {0, 1024, 0},   {0, 2048, 0},
+    {0, 4096, 0},    {4096, 2048, 0}, {8192, 4096, 0}, {16384, 512, 4}, {32768, 512, 4}};
 #define NCAND ((int)(sizeof(CAND) / sizeof(CAND[0])))
 
 typedef struct {

@@ -569,8 +569,10 @@ static float time_spmm(cacg_solver *sv, int f32, int k, long ld, const void *x, 
  * them).  A schedule changes no result, so neither does the choice. */
 static int tune_spmm(cacg_solver *sv, int blk)
 {
-    static const cacg_spmm_sched cand[] = {{0, 0},    {4096, 0},    {8192, 0},   {0, 1024},
-                                           {0, 2048}, {0, 4096},    {4096, 2048}, {8192, 4096}};
+    static const cacg_spmm_sched cand[] = {
+        {0, 0, 0},       {4096, 0, 0},    {8192, 0, 0},       {0, 1024, 0},
+        {0, 2048, 0},    {0, 4096, 0},    {4096, 2048, 0},    {8192, 4096, 0},
+        {16384, 512, 4}, {32768, 512, 4}};
     const int nc = (int)(sizeof(cand) / sizeof(cand[0]));
     sv->tuned = 1;
     if (sv->n < TUNE_MIN_ROWS)

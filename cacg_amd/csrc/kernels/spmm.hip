@@ -23,7 +23,13 @@
 //   chunks (P > 0): chunks of P tiles are dealt round-robin over the XCDs and
 //     each XCD sweeps its chunks in order, so an XCD owns a slice of every
 //     grid plane and the x rows a plane away (i +- a grid plane) are gathered
-//     again under the same L2 one chunk later, instead of a plane later.
+//     again under the same L2 one chunk later, instead of a plane later;
+//   chunks in Q passes (P > 0, Q > 1): the XCD's chunks l = 0, 1, ... are swept
+//     as Q passes, pass h taking chunks h, h + Q, h + 2 Q, ...  When an XCD owns
+//     Q chunks of every grid plane, a pass walks one narrow column of the grid
+//     plane by plane, so the three planes' rows a row gathers (its own, and
+//     +- a plane) are a Q-th of the chunk-order working set: at k = 128 the
+//     XCD's slice of three planes is 6 MB, which its 4 MB L2 cannot hold.
 // The order changes no result (each row is summed by one lane group, in CSR
 // order); which is faster depends on the matrix, so the solver times both on
 // its own matrix (cacg_spmm_sched).  Operands used once (the matrix, S_{j-2},
@@ -86,18 +92,50 @@ struct vecld<float, 4> {
 // FP64 solver uses <double, double, double>; the FP32 basis <float, float,
 // float>, and <float, double, float> for its first step, which reads the FP64
 // residual and writes its FP32 copy S_0.
+// The lt-th tile of an XCD's chunks of P tiles swept in Q passes: passes
+// r .. Q-1 hold F chunks each, then passes 0 .. r-1 hold F + 1, for the XCD's
+// L = F Q + r chunks.  Recomputed per tile from the launch constants (a few
+// scalar operations per tile) rather than kept live: the extra scalar registers
+// made hipcc spill SGPRs in the k = 32 kernel and then fail instruction
+// selection.
+__device__ __forceinline__ int passes_tile(int lt, int P, int Q, int nx, int xcd, int ntiles)
+{
+    const int nch = (ntiles + P - 1) / P;
+    const int L = nch > xcd ? (nch - 1 - xcd) / nx + 1 : 0;
+    const int u = lt / P;
+    if (u >= L)
+        return -1;
+    const int F = L / Q, r = L % Q, u1 = (Q - r) * F;
+    int h, i;
+    if (u < u1) {
+        h = r + u / F;
+        i = u % F;
+    } else {
+        h = (u - u1) / (F + 1);
+        i = (u - u1) % (F + 1);
+    }
+    return ((i * Q + h) * nx + xcd) * P + lt % P;
+}
+
 // The XCD's tiles in sweep order: the lt-th tile of its band (P == 0) or of
-// its chunks of P tiles; -1 past the end.  Tile ids grow with lt in both
-// orders, so the first -1 ends a workgroup's sweep.
-__device__ __forceinline__ int sweep_tile(int lt, int P, int nx, int xcd, int band, int ntiles)
+// its chunks of P tiles (in Q passes when Q > 1); -1 past the end.  Only the
+// last chunk can be partial, and every order visits it last (the passes start
+// at pass L % Q, so the pass holding chunk L - 1 comes last), so the first -1
+// ends a workgroup's sweep.
+__device__ __forceinline__ int sweep_tile(int lt, int P, int Q, int nx, int xcd, int band,
+                                          int ntiles)
 {
     int tile;
     if (P == 0) {
         if (lt >= band)
             return -1;
         tile = xcd * band + lt;
-    } else {
+    } else if (Q <= 1) {
         tile = ((lt / P) * nx + xcd) * P + lt % P;
+    } else {
+        tile = passes_tile(lt, P, Q, nx, xcd, ntiles);
+        if (tile < 0)
+            return -1;
     }
     return tile < ntiles ? tile : -1;
 }
@@ -195,7 +233,7 @@ __global__ __launch_bounds__(256) void spmm_kernel(
     int n, const int *__restrict__ rowptr, const int *__restrict__ colidx,
     const TV *__restrict__ val, int k, const TX *__restrict__ x, long ldx, TY *__restrict__ y,
     long ldy, double ad, double bd, const TY *__restrict__ p1, long ld1, const TY *__restrict__ p2,
-    long ld2, TY *__restrict__ copy0, long ldc, int P)
+    long ld2, TY *__restrict__ copy0, long ldc, int P, int Q)
 {
     constexpr int NG = 256 / LPR, TR = NG * RPG;
     constexpr int CAP = TR * 8 > 256 ? TR * 8 : 256;
@@ -208,7 +246,7 @@ __global__ __launch_bounds__(256) void spmm_kernel(
     const int xcd = blockIdx.x % nx, gx = gridDim.x / nx, j = blockIdx.x / nx;
     const int band = (ntiles + nx - 1) / nx;
     for (int lt = j;; lt += gx) {
-        const int tile = sweep_tile(lt, P, nx, xcd, band, ntiles);
+        const int tile = sweep_tile(lt, P, Q, nx, xcd, band, ntiles);
         if (tile < 0)
             break;
         const int r0 = tile * TR;
@@ -244,7 +282,7 @@ __global__ __launch_bounds__(256) void spmm_pipe_kernel(
     int n, const int *__restrict__ rowptr, const int *__restrict__ colidx,
     const TV *__restrict__ val, int k, const TX *__restrict__ x, long ldx, TY *__restrict__ y,
     long ldy, double ad, double bd, const TY *__restrict__ p1, long ld1, const TY *__restrict__ p2,
-    long ld2, TY *__restrict__ copy0, long ldc, int P)
+    long ld2, TY *__restrict__ copy0, long ldc, int P, int Q)
 {
     constexpr int NG = 256 / LPR, TR = NG * RPG;
     constexpr int CAP = TR * 8 > 256 ? TR * 8 : 256;
@@ -253,13 +291,17 @@ __global__ __launch_bounds__(256) void spmm_pipe_kernel(
     __shared__ int s_rp[2][TR + 1];
     __shared__ int s_col[2][CAP];
     __shared__ TV s_val[2][CAP];
+    // one pass: the multi-pass order is not built into the pipelined form (with
+    // it, hipcc spills SGPRs in these kernels and fails instruction selection
+    // in the FP64 k = 17..32 one)
+    Q = 1;
     const int t = threadIdx.x, g = t / LPR, lane = t % LPR;
     const int ntiles = (n + TR - 1) / TR;
     const int nx = gridDim.x >= 8 ? 8 : 1;
     const int xcd = blockIdx.x % nx, gx = gridDim.x / nx, j = blockIdx.x / nx;
     const int band = (ntiles + nx - 1) / nx;
     int lt = j;
-    int tile = sweep_tile(lt, P, nx, xcd, band, ntiles);
+    int tile = sweep_tile(lt, P, Q, nx, xcd, band, ntiles);
     if (tile < 0)
         return;
     auto rows_of = [&](int tl) { return (n - tl * TR) < TR ? (n - tl * TR) : TR; };
@@ -281,7 +323,7 @@ __global__ __launch_bounds__(256) void spmm_pipe_kernel(
                 s_val[0][e] = __builtin_nontemporal_load(val + pb + e);
             }
     }
-    int tile_n = sweep_tile(lt + gx, P, nx, xcd, band, ntiles);
+    int tile_n = sweep_tile(lt + gx, P, Q, nx, xcd, band, ntiles);
     if (tile_n >= 0 && t <= rows_of(tile_n))
         rp_reg = rowptr[tile_n * TR + t];
     __syncthreads();
@@ -299,7 +341,7 @@ __global__ __launch_bounds__(256) void spmm_pipe_kernel(
                 s_rp[nxt][t] = rp_reg;
         }
         __syncthreads();
-        const int tile_nn = tile_n >= 0 ? sweep_tile(lt + 2 * gx, P, nx, xcd, band, ntiles) : -1;
+        const int tile_nn = tile_n >= 0 ? sweep_tile(lt + 2 * gx, P, Q, nx, xcd, band, ntiles) : -1;
         if (tile_n >= 0) {
             pbn = s_rp[nxt][0];
             tnn = s_rp[nxt][rows_n] - pbn;
@@ -355,19 +397,21 @@ int launch_spmm(int n, const int *rowptr, const int *col, const TV *val, int k, 
     if (G >= 8)
         G &= ~7;
     // chunk order: chunk_rows rounded to whole tiles
-    int P = 0;
-    if (sc && sc->chunk_rows > 0)
+    int P = 0, Q = 1;
+    if (sc && sc->chunk_rows > 0) {
         P = sc->chunk_rows >= TR ? sc->chunk_rows / TR : 1;
+        Q = sc->passes > 1 ? sc->passes : 1;
+    }
 #define CACG_SPMM_CASE(E)                                                                        \
     do {                                                                                         \
         if constexpr (LPR == 4 || LPR == 8)                                                      \
             hipLaunchKernelGGL((spmm_pipe_kernel<TV, TX, TY, LPR, VW, RPG, E>), dim3(G), dim3(256), \
                                0, st, n, rowptr, col, val, k, x, ldx, y, ldy, a, b, p1, ld1, p2,    \
-                               ld2, copy0, ldc, P);                                              \
+                               ld2, copy0, ldc, P, Q);                                           \
         else                                                                                     \
             hipLaunchKernelGGL((spmm_kernel<TV, TX, TY, LPR, VW, RPG, E>), dim3(G), dim3(256), 0, \
                                st, n, rowptr, col, val, k, x, ldx, y, ldy, a, b, p1, ld1, p2, ld2, \
-                               copy0, ldc, P);                                                   \
+                               copy0, ldc, P, Q);                                                \
     } while (0)
     constexpr bool f64 = sizeof(TV) == 8 && sizeof(TX) == 8 && sizeof(TY) == 8;
     constexpr bool f32 = sizeof(TV) == 4 && sizeof(TX) == 4 && sizeof(TY) == 4;
@@ -513,7 +557,7 @@ extern "C" int cacg_k_spmm_x(const cacg_dcsr *A, int k, const double *x, long ld
                              const double *p2, long ld2, double *copy0, long ldc,
                              const cacg_spmm_sched *sc, hipStream_t st)
 {
-    if (!A || k <= 0 || A->n < 0 || (sc && (sc->grid < 0 || sc->chunk_rows < 0)))
+    if (!A || k <= 0 || A->n < 0 || (sc && (sc->grid < 0 || sc->chunk_rows < 0 || sc->passes < 0)))
         return (int)hipErrorInvalidValue;
     if (A->n == 0)
         return 0;
@@ -536,7 +580,7 @@ extern "C" int cacg_k_spmm_f32_x(const cacg_scsr *A, int k, const float *x, long
                                  hipStream_t st)
 {
     if (!A || k <= 0 || A->n < 0 || (epi != CACG_EPI_NONE && epi != CACG_EPI_CHEB) ||
-        (sc && (sc->grid < 0 || sc->chunk_rows < 0)))
+        (sc && (sc->grid < 0 || sc->chunk_rows < 0 || sc->passes < 0)))
         return (int)hipErrorInvalidValue;
     if (A->n == 0)
         return 0;
@@ -555,7 +599,7 @@ extern "C" int cacg_k_cheb1_f32_x(const cacg_scsr *A, int k, const double *r, lo
                                   float *s1, long ld1, float *s0, long ld0, double a, double b,
                                   const cacg_spmm_sched *sc, hipStream_t st)
 {
-    if (!A || k <= 0 || A->n < 0 || !s0 || (sc && (sc->grid < 0 || sc->chunk_rows < 0)))
+    if (!A || k <= 0 || A->n < 0 || !s0 || (sc && (sc->grid < 0 || sc->chunk_rows < 0 || sc->passes < 0)))
         return (int)hipErrorInvalidValue;
     if (A->n == 0)
         return 0;

@@ -66,11 +66,16 @@ int cacg_k_cheb1_f32(const cacg_scsr *A, int k, const double *r, long ldr, float
  * summed by one lane group in CSR order).  grid: workgroups (0: 8192 for
  * k <= 32, 4096 above); chunk_rows: 0 deals each XCD a contiguous band of row
  * tiles, > 0 deals chunks of that many rows round-robin over the XCDs (an XCD
- * then owns a slice of every grid plane of a stencil matrix).  The _x forms
- * take one (NULL: the defaults); the forms above use the defaults. */
+ * then owns a slice of every grid plane of a stencil matrix); passes (with
+ * chunk_rows > 0; 0 or 1: one): each XCD sweeps its chunks in that many passes,
+ * pass h taking its chunks h, h + passes, ... (a narrow column of the grid,
+ * plane by plane; taken for k > 32, ignored for k = 4..32, whose kernel
+ * sweeps one pass).  The _x forms take one (NULL: the defaults); the forms
+ * above use the defaults. */
 typedef struct {
     int grid;
     int chunk_rows;
+    int passes;
 } cacg_spmm_sched;
 int cacg_k_spmm_x(const cacg_dcsr *A, int k, const double *x, long ldx, double *y, long ldy,
                   int epi, double a, double b, const double *p1, long ld1, const double *p2,

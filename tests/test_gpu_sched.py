@@ -11,10 +11,12 @@ from cacg_amd import problems as P
 pytestmark = pytest.mark.gpu
 
 SCHEDS = [(0, 0), (4096, 0), (8192, 0), (0, 1024), (0, 2048), (0, 4096), (4096, 2048),
-          (8192, 4096), (64, 64), (2048, 100)]
+          (8192, 4096), (16384, 512, 4), (32768, 512, 4), (64, 64), (2048, 100),
+          # passes: uneven chunk counts per XCD, a partial last chunk, more passes than chunks
+          (4096, 1000, 3), (64, 64, 5), (8192, 48, 7), (256, 100000, 4)]
 
 
-@pytest.mark.parametrize("k", [1, 4, 16, 20, 128])
+@pytest.mark.parametrize("k", [1, 4, 16, 20, 64, 128])
 def test_every_schedule_gives_the_same_bits(gpu_lib, k):
     A = P.laplacian_3d(64)                       # 262144 rows: above the tuning threshold
     X = P.rhs_uniform(A.n, k, 3)
