@@ -38,6 +38,8 @@
 #include <float.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "kcommon.hpp"
 
 namespace {
@@ -55,16 +57,28 @@ struct vecld<T, 1> {
 template <>
 struct vecld<double, 4> {
     double v[4];
+    // two 16-byte pieces: columns c, c + 1 and c + OFF, c + OFF + 1
+    template <int OFF = 2>
     __device__ __forceinline__ void load(const double *p)
     {
-        const double2 a = reinterpret_cast<const double2 *>(p)[0];
-        const double2 b = reinterpret_cast<const double2 *>(p)[1];
+        const double2 a = *reinterpret_cast<const double2 *>(p);
+        const double2 b = *reinterpret_cast<const double2 *>(p + OFF);
         v[0] = a.x;
         v[1] = a.y;
         v[2] = b.x;
         v[3] = b.y;
     }
 };
+// load a lane's VW elements: the FP64 pairs OFF columns apart, else contiguous
+template <int OFF, typename T, int VW>
+__device__ __forceinline__ void vload(vecld<T, VW> &d, const T *p)
+{
+    if constexpr (std::is_same<T, double>::value && VW == 4)
+        d.template load<OFF>(p);
+    else
+        d.load(p);
+}
+
 template <>
 struct vecld<float, 4> {
     float v[4];
@@ -152,7 +166,14 @@ constexpr int spmm_unroll()
     return (LPR == 4 || LPR == 32) ? 4 : 8;
 }
 
-template <typename TV, typename TX, typename TY, int LPR, int VW, int RPG, int EPI>
+// Columns of a lane: VW = 4 contiguous ones from c0 (IL = false), or with IL
+// (FP64, VW = 4, k a multiple of 4 LPR) the pairs c0, c0 + 1 and c0 + 2 LPR,
+// c0 + 2 LPR + 1 with c0 = cb + 2 lane, so that each 16-byte load or store
+// instruction of a lane group covers one contiguous half of the row's block
+// instead of every other 16 bytes of all of it (whole 64-byte runs per
+// instruction for the non-temporal stores).  Each output is still summed by one
+// lane in CSR order: the layout changes no bit.
+template <typename TV, typename TX, typename TY, int LPR, int VW, int RPG, int EPI, bool IL>
 __device__ __forceinline__ void tile_rows(const int *s_rp, const int *s_col, const TV *s_val,
                                           bool staged, int pbase, int r0, int rows, int g,
                                           int lane, const int *__restrict__ colidx,
@@ -164,10 +185,14 @@ __device__ __forceinline__ void tile_rows(const int *s_rp, const int *s_col, con
 {
     constexpr int NG = 256 / LPR;
     constexpr int UNR = spmm_unroll<LPR>();
+    static_assert(!IL || VW == 4, "interleaved columns need 4 per lane");
+    constexpr int OFF = IL ? 2 * LPR : 2;   // column distance of a lane's second pair
     for (int cb = 0; cb < k; cb += LPR * VW) {
-        const int c0 = cb + lane * VW;
+        const int c0 = cb + lane * (IL ? 2 : VW);
         if (c0 >= k)
             continue;
+        // column of a lane's element v
+        auto col = [&](int v) { return VW == 4 && v >= 2 ? c0 + OFF + v - 2 : c0 + v; };
         for (int rr = 0; rr < RPG; rr++) {
             const int li = rr * NG + g;
             if (li >= rows)
@@ -179,12 +204,12 @@ __device__ __forceinline__ void tile_rows(const int *s_rp, const int *s_col, con
             for (int v = 0; v < VW; v++) {
                 acc[v] = (TV)0;
                 if (EPI == CACG_EPI_RESID) {
-                    e1[v] = (TV)__builtin_nontemporal_load(p1 + i * ld1 + c0 + v);
+                    e1[v] = (TV)__builtin_nontemporal_load(p1 + i * ld1 + col(v));
                 } else if (EPI == CACG_EPI_CHEB1) {
-                    e1[v] = (TV)x[i * ldx + c0 + v];
+                    e1[v] = (TV)x[i * ldx + col(v)];
                 } else if (EPI == CACG_EPI_CHEB) {
-                    e1[v] = (TV)p1[i * ld1 + c0 + v];
-                    e2[v] = (TV)__builtin_nontemporal_load(p2 + i * ld2 + c0 + v);
+                    e1[v] = (TV)p1[i * ld1 + col(v)];
+                    e2[v] = (TV)__builtin_nontemporal_load(p2 + i * ld2 + col(v));
                 }
             }
             for (int p = pb; p < pe; p += UNR) {
@@ -200,7 +225,7 @@ __device__ __forceinline__ void tile_rows(const int *s_rp, const int *s_col, con
 #pragma unroll
                 for (int u = 0; u < UNR; u++)
                     if (p + u < pe)
-                        xv[u].load(x + (long)cc[u] * ldx + c0);
+                        vload<OFF>(xv[u], x + (long)cc[u] * ldx + c0);
 #pragma unroll
                 for (int u = 0; u < UNR; u++)
                     if (p + u < pe) {
@@ -218,17 +243,17 @@ __device__ __forceinline__ void tile_rows(const int *s_rp, const int *s_col, con
                     out = sub_rn(e1[v], acc[v]);
                 } else if (EPI == CACG_EPI_CHEB1) {
                     out = add_rn(mul_rn(a, acc[v]), mul_rn(b, e1[v]));
-                    __builtin_nontemporal_store((TY)e1[v], copy0 + i * ldc + c0 + v);
+                    __builtin_nontemporal_store((TY)e1[v], copy0 + i * ldc + col(v));
                 } else {
                     out = sub_rn(add_rn(mul_rn(a, acc[v]), mul_rn(b, e1[v])), e2[v]);
                 }
-                __builtin_nontemporal_store((TY)out, y + i * ldy + c0 + v);
+                __builtin_nontemporal_store((TY)out, y + i * ldy + col(v));
             }
         }
     }
 }
 
-template <typename TV, typename TX, typename TY, int LPR, int VW, int RPG, int EPI>
+template <typename TV, typename TX, typename TY, int LPR, int VW, int RPG, int EPI, bool IL>
 __global__ __launch_bounds__(256) void spmm_kernel(
     int n, const int *__restrict__ rowptr, const int *__restrict__ colidx,
     const TV *__restrict__ val, int k, const TX *__restrict__ x, long ldx, TY *__restrict__ y,
@@ -263,7 +288,7 @@ __global__ __launch_bounds__(256) void spmm_kernel(
                 s_val[e] = __builtin_nontemporal_load(val + pbase + e);
             }
         __syncthreads();
-        tile_rows<TV, TX, TY, LPR, VW, RPG, EPI>(s_rp, s_col, s_val, staged, pbase, r0, rows, g,
+        tile_rows<TV, TX, TY, LPR, VW, RPG, EPI, IL>(s_rp, s_col, s_val, staged, pbase, r0, rows, g,
                                                  lane, colidx, val, k, x, ldx, y, ldy, (TV)ad,
                                                  (TV)bd, p1, ld1, p2, ld2, copy0, ldc);
         __syncthreads();
@@ -277,7 +302,7 @@ __global__ __launch_bounds__(256) void spmm_kernel(
 // once per tile (8% on the 128^3 Laplacian at k = 16; no gain for wide rows,
 // which keep spmm_kernel, and not used for LPR <= 2, whose doubled LDS
 // staging would halve occupancy).  Two LDS buffers alternate between tiles.
-template <typename TV, typename TX, typename TY, int LPR, int VW, int RPG, int EPI>
+template <typename TV, typename TX, typename TY, int LPR, int VW, int RPG, int EPI, bool IL>
 __global__ __launch_bounds__(256) void spmm_pipe_kernel(
     int n, const int *__restrict__ rowptr, const int *__restrict__ colidx,
     const TV *__restrict__ val, int k, const TX *__restrict__ x, long ldx, TY *__restrict__ y,
@@ -355,7 +380,7 @@ __global__ __launch_bounds__(256) void spmm_pipe_kernel(
             if (tile_nn >= 0 && t <= rows_of(tile_nn))
                 rp_reg = rowptr[tile_nn * TR + t];
         }
-        tile_rows<TV, TX, TY, LPR, VW, RPG, EPI>(s_rp[cur], s_col[cur], s_val[cur], staged, pbase,
+        tile_rows<TV, TX, TY, LPR, VW, RPG, EPI, IL>(s_rp[cur], s_col[cur], s_val[cur], staged, pbase,
                                                  r0, rows, g, lane, colidx, val, k, x, ldx, y, ldy,
                                                  (TV)ad, (TV)bd, p1, ld1, p2, ld2, copy0, ldc);
         if (tile_n < 0)
@@ -402,16 +427,28 @@ int launch_spmm(int n, const int *rowptr, const int *col, const TV *val, int k, 
         P = sc->chunk_rows >= TR ? sc->chunk_rows / TR : 1;
         Q = sc->passes > 1 ? sc->passes : 1;
     }
-#define CACG_SPMM_CASE(E)                                                                        \
+#define CACG_SPMM_LAUNCH(E, IL)                                                                  \
     do {                                                                                         \
         if constexpr (LPR == 4 || LPR == 8)                                                      \
-            hipLaunchKernelGGL((spmm_pipe_kernel<TV, TX, TY, LPR, VW, RPG, E>), dim3(G), dim3(256), \
+            hipLaunchKernelGGL((spmm_pipe_kernel<TV, TX, TY, LPR, VW, RPG, E, IL>), dim3(G),     \
+                               dim3(256), 0, st, n, rowptr, col, val, k, x, ldx, y, ldy, a, b, p1, \
+                               ld1, p2, ld2, copy0, ldc, P, Q);                                  \
+        else                                                                                     \
+            hipLaunchKernelGGL((spmm_kernel<TV, TX, TY, LPR, VW, RPG, E, IL>), dim3(G), dim3(256), \
                                0, st, n, rowptr, col, val, k, x, ldx, y, ldy, a, b, p1, ld1, p2,    \
                                ld2, copy0, ldc, P, Q);                                           \
-        else                                                                                     \
-            hipLaunchKernelGGL((spmm_kernel<TV, TX, TY, LPR, VW, RPG, E>), dim3(G), dim3(256), 0, \
-                               st, n, rowptr, col, val, k, x, ldx, y, ldy, a, b, p1, ld1, p2, ld2, \
-                               copy0, ldc, P, Q);                                                \
+    } while (0)
+    // FP64 with 4 columns per lane: interleaved pairs when every lane's block is full
+#define CACG_SPMM_CASE(E)                                                                        \
+    do {                                                                                         \
+        if constexpr (f64 && VW == 4) {                                                          \
+            if (k % (LPR * VW) == 0)                                                             \
+                CACG_SPMM_LAUNCH(E, true);                                                       \
+            else                                                                                 \
+                CACG_SPMM_LAUNCH(E, false);                                                      \
+        } else {                                                                                 \
+            CACG_SPMM_LAUNCH(E, false);                                                          \
+        }                                                                                        \
     } while (0)
     constexpr bool f64 = sizeof(TV) == 8 && sizeof(TX) == 8 && sizeof(TY) == 8;
     constexpr bool f32 = sizeof(TV) == 4 && sizeof(TX) == 4 && sizeof(TY) == 4;
@@ -441,6 +478,7 @@ int launch_spmm(int n, const int *rowptr, const int *col, const TV *val, int k, 
             return (int)hipErrorInvalidValue;
     }
 #undef CACG_SPMM_CASE
+#undef CACG_SPMM_LAUNCH
     CACG_LAUNCH_CHECK();
     return 0;
 }
