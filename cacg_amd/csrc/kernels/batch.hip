@@ -94,9 +94,10 @@ __device__ __forceinline__ bool stage_tile(TileCsr<TV> &tc, const int *__restric
     return staged;
 }
 
-// 4 consecutive elements of x at p (16-byte aligned), converted to TV
+// 4 elements of x converted to TV: p[0..3] (FP32, one 16-byte load), or the
+// FP64 pairs p[0..1] and p[off..off+1] (16-byte aligned)
 template <typename TV, typename TX>
-__device__ __forceinline__ void load4(const TX *p, TV *o)
+__device__ __forceinline__ void load4(const TX *p, TV *o, int off)
 {
     if constexpr (sizeof(TX) == 4) {
         const float4 v = *reinterpret_cast<const float4 *>(p);
@@ -105,8 +106,8 @@ __device__ __forceinline__ void load4(const TX *p, TV *o)
         o[2] = (TV)v.z;
         o[3] = (TV)v.w;
     } else {
-        const double2 v0 = reinterpret_cast<const double2 *>(p)[0];
-        const double2 v1 = reinterpret_cast<const double2 *>(p)[1];
+        const double2 v0 = *reinterpret_cast<const double2 *>(p);
+        const double2 v1 = *reinterpret_cast<const double2 *>(p + off);
         o[0] = (TV)v0.x;
         o[1] = (TV)v0.y;
         o[2] = (TV)v1.x;
@@ -114,10 +115,22 @@ __device__ __forceinline__ void load4(const TX *p, TV *o)
     }
 }
 
-// acc[0..3] = sums over row r of the staged tile of val * x[col * ldx + c0 + 0..3]
+// Columns of a thread's 4 elements in an FP64 row block of width w (w % 4 == 0):
+// the pairs c0, c0 + 1 and c0 + w / 2, c0 + w / 2 + 1 with c0 = 2 t for the
+// t-th thread of the row, so each 16-byte access instruction of the threads of a
+// row covers one contiguous half of it (FP32: 4 contiguous from 4 t).  Which
+// thread computes an element changes no bit.
+struct cols4 {
+    int c0, off;
+    __device__ __forceinline__ cols4(int t, int w, bool fp64)
+        : c0(fp64 ? 2 * t : 4 * t), off(fp64 ? w / 2 : 2) {}
+    __device__ __forceinline__ int operator()(int k) const { return k < 2 ? c0 + k : c0 + off + k - 2; }
+};
+
+// acc[0..3] = sums over row r of the staged tile of val * x[col * ldx + cs(0..3)]
 template <typename TV, typename TX>
 __device__ __forceinline__ void row_dot4(const TileCsr<TV> &tc, int r, const TX *__restrict__ x,
-                                         long ldx, int c0, TV *acc)
+                                         long ldx, const cols4 &cs, TV *acc)
 {
     const int pb = tc.rp[0], p0 = tc.rp[r] - pb, p1 = tc.rp[r + 1] - pb;
 #pragma unroll
@@ -130,7 +143,7 @@ __device__ __forceinline__ void row_dot4(const TileCsr<TV> &tc, int r, const TX 
             const bool in = p + u < p1;
             vv[u] = in ? tc.val[p + u] : (TV)0;
             if (in)
-                load4<TV, TX>(x + (long)tc.col[p + u] * ldx + c0, xv[u]);
+                load4<TV, TX>(x + (long)tc.col[p + u] * ldx + cs.c0, xv[u], cs.off);
         }
 #pragma unroll
         for (int u = 0; u < 4; u++)
@@ -168,25 +181,29 @@ __global__ __launch_bounds__(256) void bbasis_kernel(const cacg_bsys *__restrict
     const bool vec = m % 4 == 0 && stage_tile<TB>(tc, d.rowptr, d.col, val, r0, rows);
     if (vec) {
         const int q = m / 4;
+        constexpr bool f64 = sizeof(TB) == 8;
         for (int e = threadIdx.x; e < rows * q; e += 256) {
-            const int r = e / q, c0 = 4 * (e - r * q);
+            const int r = e / q;
+            const cols4 cs(e - r * q, m, f64);
             const long i = r0 + r;
             TB acc[4];
             if (j == 1) {
-                row_dot4<TB, double>(tc, r, d.R, m, c0, acc);
+                // FP32 basis: the FP64 R read as 4 contiguous columns
+                const cols4 cr = f64 ? cs : cols4(e - r * q, m, false);
+                row_dot4<TB, double>(tc, r, d.R, m, cr, acc);
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
-                    const TB e1 = (TB)d.R[i * m + c0 + k];
-                    S[i * sm + c0 + k] = e1;
-                    S[i * sm + m + c0 + k] = add_rn(mul_rn(a1, acc[k]), mul_rn(b1, e1));
+                    const TB e1 = (TB)d.R[i * m + cr(k)];
+                    S[i * sm + cr(k)] = e1;
+                    S[i * sm + m + cr(k)] = add_rn(mul_rn(a1, acc[k]), mul_rn(b1, e1));
                 }
             } else {
                 const TB *xs = S + (long)(j - 1) * m;
-                row_dot4<TB, TB>(tc, r, xs, sm, c0, acc);
+                row_dot4<TB, TB>(tc, r, xs, sm, cs, acc);
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
-                    const TB e1 = xs[i * sm + c0 + k], e2 = S[i * sm + (long)(j - 2) * m + c0 + k];
-                    S[i * sm + (long)j * m + c0 + k] =
+                    const TB e1 = xs[i * sm + cs(k)], e2 = S[i * sm + (long)(j - 2) * m + cs(k)];
+                    S[i * sm + (long)j * m + cs(k)] =
                         sub_rn(add_rn(mul_rn(a2, acc[k]), mul_rn(b2, e1)), e2);
                 }
             }
@@ -235,12 +252,13 @@ __global__ __launch_bounds__(256) void baq_kernel(const cacg_bsys *__restrict__ 
     if (sm % 4 == 0 && stage_tile<double>(tc, d.rowptr, d.col, d.val, r0, rows)) {
         const int q = sm / 4;
         for (int e = threadIdx.x; e < rows * q; e += 256) {
-            const int r = e / q, c0 = 4 * (e - r * q);
+            const int r = e / q;
+            const cols4 cs(e - r * q, sm, true);
             double acc[4];
-            row_dot4<double, double>(tc, r, d.Q, sm, c0, acc);
-            double *y = d.AQ + (r0 + r) * sm + c0;
-            reinterpret_cast<double2 *>(y)[0] = double2{acc[0], acc[1]};
-            reinterpret_cast<double2 *>(y)[1] = double2{acc[2], acc[3]};
+            row_dot4<double, double>(tc, r, d.Q, sm, cs, acc);
+            double *y = d.AQ + (r0 + r) * sm;
+            *reinterpret_cast<double2 *>(y + cs.c0) = double2{acc[0], acc[1]};
+            *reinterpret_cast<double2 *>(y + cs.c0 + cs.off) = double2{acc[2], acc[3]};
         }
         return;
     }
