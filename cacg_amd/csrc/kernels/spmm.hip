@@ -253,8 +253,19 @@ __device__ __forceinline__ void tile_rows(const int *s_rp, const int *s_col, con
     }
 }
 
+// Minimum waves per SIMD asked of the register allocator: 6 for the FP64 A Q
+// form (k > 64, no epilogue: 80 VGPRs with 3 spilled instead of 82 at 5 waves,
+// 1306 against 1331 us at C2); 1 (no constraint) elsewhere.  Asking 5 of the
+// k = 16 basis step (96 VGPRs, 1 spilled) gained nothing.
+template <typename TV, int LPR, int EPI>
+constexpr int spmm_min_waves()
+{
+    return (sizeof(TV) == 8 && LPR == 32 && EPI == CACG_EPI_NONE) ? 6 : 1;
+}
+
 template <typename TV, typename TX, typename TY, int LPR, int VW, int RPG, int EPI, bool IL>
-__global__ __launch_bounds__(256) void spmm_kernel(
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(spmm_min_waves<TV, LPR, EPI>()))) void spmm_kernel(
     int n, const int *__restrict__ rowptr, const int *__restrict__ colidx,
     const TV *__restrict__ val, int k, const TX *__restrict__ x, long ldx, TY *__restrict__ y,
     long ldy, double ad, double bd, const TY *__restrict__ p1, long ld1, const TY *__restrict__ p2,
