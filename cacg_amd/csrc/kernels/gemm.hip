@@ -307,16 +307,16 @@ __global__ __launch_bounds__(256) void xrupdate_mfma_kernel(int n, int m, const 
 {
     constexpr int KQ = SM / 4;
     __shared__ double red[4][64][NTM];
+    // alpha (SM x m, zero-padded to 16 NTM columns) in LDS, not in registers: the
+    // 32 k-values x 16 columns a lane would hold cost 64 VGPRs and a wave per SIMD
+    __shared__ double s_alf[SM][16 * NTM];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int lg = lane >> 4, lc = lane & 15;
-    double alf[NTM][KQ];
-#pragma unroll
-    for (int j = 0; j < NTM; j++) {
-        const int c = j * 16 + lc;
-#pragma unroll
-        for (int tt = 0; tt < KQ; tt++)
-            alf[j][tt] = c < m ? alpha[(long)(lg * KQ + tt) * m + c] : 0.0;
+    for (int e = threadIdx.x; e < SM * 16 * NTM; e += 256) {
+        const int k = e / (16 * NTM), c = e % (16 * NTM);
+        s_alf[k][c] = c < m ? alpha[(long)k * m + c] : 0.0;
     }
+    __syncthreads();
     double nrm[NTM];
 #pragma unroll
     for (int j = 0; j < NTM; j++)
@@ -330,6 +330,9 @@ __global__ __launch_bounds__(256) void xrupdate_mfma_kernel(int n, int m, const 
     if (te > ntiles)
         te = ntiles;
     for (int tile = tb; tile < te; tile++) {
+        // alpha is re-read from LDS for every tile (a compiler barrier keeps the
+        // loop-invariant reads from being hoisted back into registers)
+        asm volatile("" ::: "memory");
         const long i0 = (long)tile * 16;
         const long arow = i0 + lc;
         double av[KQ], aq[KQ];
@@ -350,10 +353,11 @@ __global__ __launch_bounds__(256) void xrupdate_mfma_kernel(int n, int m, const 
             dbl4 ax = {0.0, 0.0, 0.0, 0.0}, ar = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int tt = 0; tt < KQ; tt++) {
+                const double b = s_alf[lg * KQ + tt][j * 16 + lc];
                 if (DOX)
-                    ax = mfma16x16x4(av[tt], alf[j][tt], ax);
+                    ax = mfma16x16x4(av[tt], b, ax);
                 if (DOR)
-                    ar = mfma16x16x4(aq[tt], alf[j][tt], ar);
+                    ar = mfma16x16x4(aq[tt], b, ar);
             }
             const int c = j * 16 + lc;
             if (c < m) {
