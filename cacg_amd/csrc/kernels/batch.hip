@@ -693,8 +693,20 @@ __global__ __launch_bounds__(64) void bnorm_kernel(const cacg_bsys *__restrict__
     const int c = threadIdx.x;
     if (c >= d.m)
         return;
+    // tile order, as before; the loads of 16 tiles go out before their adds
+    // (one dependent load per tile made a 1024-tile system take ~170 us)
     double s = 0.0;
-    for (int t = 0; t < d.ntile; t++)
+    int t = 0;
+    for (; t + 16 <= d.ntile; t += 16) {
+        double v[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++)
+            v[u] = d.npart[(long)(t + u) * d.m + c];
+#pragma unroll
+        for (int u = 0; u < 16; u++)
+            s += v[u];
+    }
+    for (; t < d.ntile; t++)
         s += d.npart[(long)t * d.m + c];
     d.norm2[c] = s;
 }
