@@ -76,7 +76,7 @@ class Batch:
     """A batch of device-resident systems advanced together (cacg_batch_*)."""
 
     def __init__(self, s=4, precision="fp32", slots=1024, hbm_bytes=0, maxiter=1000,
-                 max_large=0, graph=True, tune=True):
+                 max_large=0, graph=False, tune=True):
         o = BatchOpts()
         lib().cacg_batch_default_opts(ctypes.byref(o))
         o.hbm_bytes, o.slots, o.s = int(hbm_bytes), int(slots), int(s)

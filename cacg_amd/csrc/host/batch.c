@@ -123,6 +123,7 @@ struct cacg_batch {
     hipGraphExec_t g_exec;
     long graph_rounds;
     stune *tune;                /* per slot */
+    int *act;                   /* the batched active slots of a round */
 };
 
 #define HCHK(call)                                                                               \
@@ -275,7 +276,7 @@ void cacg_batch_default_opts(cacg_batch_opts *o)
     o->precision = CACG_PREC_FP32_REFINE;
     o->maxiter = 1000;
     o->max_large = 0;
-    o->graph = 1;
+    o->graph = 0;
     o->tune = 1;
 }
 
@@ -306,6 +307,7 @@ int cacg_batch_destroy(cacg_batch *b)
     free(b->req);
     free(b->ret);
     free(b->slot);
+    free(b->act);
     if (b->tune) {
         for (int i = 0; i < b->nslots; i++)
             if (b->tune[i].have_ev)
@@ -405,9 +407,10 @@ int cacg_batch_create(cacg_batch **out, const cacg_batch_opts *opts)
     b->fl = (fblk *)malloc(sizeof(fblk) * (size_t)b->capfl);
     b->slot = (breq **)calloc((size_t)b->nslots, sizeof(breq *));
     b->tune = (stune *)calloc((size_t)b->nslots, sizeof(stune));
+    b->act = (int *)calloc((size_t)b->nslots, sizeof(int));
     b->capreq = 1024;
     b->req = (breq **)calloc((size_t)b->capreq, sizeof(breq *));
-    if (!b->fl || !b->slot || !b->tune || !b->req) {
+    if (!b->fl || !b->slot || !b->tune || !b->act || !b->req) {
         cacg_batch_destroy(b);
         return CACG_ENOMEM;
     }
@@ -955,7 +958,11 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
     /* work lists (solo systems take no part in the batched launches) */
 #define BATCHED(i) (b->slot[i] && !is_solo(b->h_sys[i].n))
     long need = 0;
-    int nact = 0, nit = 0, max_sm = 1;
+    int nact = 0, nit = 0, max_sm = 1, na = 0;
+    /* i over the batched active slots (b->act[0 .. na - 1], ascending): the class
+     * lists below scan them, not every slot */
+#define FOR_BATCHED(i) \
+    for (int q_ = 0, i = na > 0 ? b->act[0] : 0; q_ < na; i = ++q_ < na ? b->act[q_] : 0)
     for (int i = 0; i < b->nslots; i++) {
         const breq *r = b->slot[i];
         if (!r)
@@ -964,6 +971,7 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
         nact++;
         if (is_solo(d->n))
             continue;
+        b->act[na++] = i;
         need += 1 + d->ntile;
         if (r->joined) {
             nit++;
@@ -980,12 +988,12 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
     memset(&wl, 0, sizeof(wl));   /* compared bytewise: padding included */
     long k = 0;
     wl.o_it = k;
-    for (int i = 0; i < b->nslots; i++)
+    FOR_BATCHED(i)
         if (BATCHED(i) && b->slot[i]->joined)
             k = put_range(it, k, i, b->h_sys[i].ntile);
     wl.n_it = k - wl.o_it;
     wl.o_fr = k;
-    for (int i = 0; i < b->nslots; i++)
+    FOR_BATCHED(i)
         if (BATCHED(i) && b->slot[i]->joined && b->slot[i]->fresh)
             k = put_range(it, k, i, b->h_sys[i].ntile);
     wl.n_fr = k - wl.o_fr;
@@ -994,7 +1002,7 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
     for (int c = 1; c <= 8; c++)
         for (int ex = 0; ex < 2; ex++) {
             wl.o_qc[c][ex] = k;
-            for (int i = 0; i < b->nslots; i++) {
+            FOR_BATCHED(i) {
                 const breq *r = b->slot[i];
                 const int sm = b->h_sys[i].sm;
                 if (BATCHED(i) && r->joined && !r->fresh && (sm + 15) / 16 == c &&
@@ -1004,7 +1012,7 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
             wl.n_qc[c][ex] = k - wl.o_qc[c][ex];
             for (int h = 0; h < 2; h++) {
                 wl.o_xc[c][ex][h] = k;
-                for (int i = 0; i < b->nslots; i++) {
+                FOR_BATCHED(i) {
                     const breq *r = b->slot[i];
                     const int sm = b->h_sys[i].sm;
                     if (BATCHED(i) && r->joined && (sm + 15) / 16 == c && (sm == 16 * c) == ex &&
@@ -1019,7 +1027,7 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
     for (int w = 0; w < 2; w++)
         for (int c = 0; c < 25; c++) {
             wl.o_gc[w][c] = k;
-            for (int i = 0; i < b->nslots; i++) {
+            FOR_BATCHED(i) {
                 const breq *r = b->slot[i];
                 if (!BATCHED(i) || !r->joined || (w == 0 && r->fresh))
                     continue;
@@ -1032,7 +1040,7 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
             wl.n_gc[w][c] = k - wl.o_gc[w][c];
         }
     wl.o_b0 = k;
-    for (int i = 0; i < b->nslots; i++)
+    FOR_BATCHED(i)
         if (BATCHED(i) && b->slot[i]->joined && !b->slot[i]->fresh) {
             const cacg_bsys *d = &b->h_sys[i];
             k = put_range(it, k, i, (int)(((long)d->sm * d->sm + 255) / 256));
@@ -1042,7 +1050,7 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
     for (int w = 0; w < 2; w++)
         for (int c = 0; c < 2; c++) {
             wl.o_lu[w][c] = k;
-            for (int i = 0; i < b->nslots; i++) {
+            FOR_BATCHED(i) {
                 const breq *r = b->slot[i];
                 if (BATCHED(i) && r->joined && (w == 1 || !r->fresh) &&
                     ((b->h_sys[i].sm > 64) == c))
@@ -1051,19 +1059,19 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
             wl.n_lu[w][c] = k - wl.o_lu[w][c];
         }
     wl.o_b1 = k;
-    for (int i = 0; i < b->nslots; i++)
+    FOR_BATCHED(i)
         if (BATCHED(i) && b->slot[i]->joined) {
             const cacg_bsys *d = &b->h_sys[i];
             k = put_range(it, k, i, (int)(((long)d->sm * (d->sm + d->m) + 255) / 256));
         }
     wl.n_b1 = k - wl.o_b1;
     wl.o_all = k;
-    for (int i = 0; i < b->nslots; i++)
+    FOR_BATCHED(i)
         if (BATCHED(i))
             k = put_range(it, k, i, b->h_sys[i].ntile);
     wl.n_all = k - wl.o_all;
     wl.o_sys = k;
-    for (int i = 0; i < b->nslots; i++)
+    FOR_BATCHED(i)
         if (BATCHED(i))
             k = put_range(it, k, i, 1);
     wl.n_sys = k - wl.o_sys;
@@ -1135,6 +1143,7 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
         }
     }
 #undef BATCHED
+#undef FOR_BATCHED
 
     /* one outer iteration of the joined batched systems and the true residual of
      * every batched one: replayed from the graph when the lists repeat (captured on

@@ -39,9 +39,10 @@ typedef struct {
     int max_large;   /* systems of 2^17 rows or more (each runs the single-system kernels on
                         its own stream) in flight at most; 0: no limit.  A queued large system
                         at the head waits for one to retire (FIFO: later requests wait too) */
-    int graph;       /* 1 (default): when a round's batched launches repeat the previous
-                        round's (same work lists), capture them once in a HIP graph and replay
-                        it, with no work-list upload; 0: launch them one by one.  Same bits */
+    int graph;       /* 1: when a round's batched launches repeat the previous round's (same
+                        work lists), capture them once in a HIP graph and replay it, with no
+                        work-list upload; 0 (default): launch them one by one, which measured
+                        ~2% faster on the 256-system batch.  Same bits either way */
     int tune;        /* 1 (default): the first iterations of a system of 2^17 rows or more run
                         its SpMMs under candidate schedules (tile order, grid), timed on its
                         stream, and the rest under the fastest; 0: default schedules.  Same

@@ -80,7 +80,7 @@ def test_batch_options_rejected_without_device(field, value):
     """Invalid batch options are refused (CACG_EINVAL) before any device is touched."""
     o = BT.BatchOpts()
     BT.lib().cacg_batch_default_opts(ctypes.byref(o))
-    assert o.graph == 1 and o.tune == 1 and o.max_large == 0
+    assert o.graph == 0 and o.tune == 1 and o.max_large == 0
     setattr(o, field, value)
     h = ctypes.c_void_p()
     assert BT.lib().cacg_batch_create(ctypes.byref(h), ctypes.byref(o)) == 3
