@@ -99,18 +99,22 @@ __device__ __forceinline__ double group_bcast(double v, int src)
 // descending q for U x = y (unit U).  Each element receives exactly the
 // operations of the barrier-per-step form, in the same order, so the result
 // is the same bits; nothing is shared between waves, so no barrier is needed.
+// Columns cbase + w WC + l / LPC, then + cstride, ... (w = the wave in the
+// workgroup); the single-workgroup solve passes 0 and the columns its waves
+// cover per pass.
 template <int LPC>
 __device__ __forceinline__ void lu_subst_wave(int N, const double *__restrict__ a,
                                               const double *__restrict__ dinv, int k,
                                               const double *__restrict__ rhs,
-                                              double *__restrict__ x)
+                                              double *__restrict__ x, int cbase = 0,
+                                              int cstride = (LU_T / 64) * (64 / LPC))
 {
     constexpr int RPL = 128 / LPC;  // rows per lane
     constexpr int WC = 64 / LPC;    // columns per wave
     const int R = N + 1;
     const int t = threadIdx.x, w = t >> 6, l = t & 63;
     const int lc = l % LPC;
-    for (int c0 = 0; c0 < k; c0 += (LU_T / 64) * WC) {
+    for (int c0 = cbase; c0 < k; c0 += cstride) {
         const int col = c0 + w * WC + l / LPC;
         const bool has = col < k;
         double y[RPL];
@@ -440,6 +444,32 @@ __global__ __launch_bounds__(LU_T) void lu_solve_kernel(int N, const double *__r
     lu_solve_body<IN_LDS>(N, lhs, k, rhs, x, gw, flag, mode, kc, reuse, lds);
 }
 
+// The fast-order solve with a kept factor (N <= 128), spread over workgroups of
+// LU_SUB_W waves: wave w of workgroup b solves column LU_SUB_W b + w (64 lanes,
+// 2 rows each), with its own LDS copy of the factor (gw, N x N row-major).
+// The single-workgroup solve ran every column on one CU, whose LDS served the
+// factor to all of them step after step (~106 us for 128 columns); per element
+// the operations are those of lu_subst_wave in its order, so the bits are the
+// same whichever lanes and workgroup solve a column.
+constexpr int LU_SUB_W = 4;
+__global__ __launch_bounds__(LU_SUB_W * 64) void lu_subst_multi_kernel(int N,
+                                                                      const double *__restrict__ gw,
+                                                                      int k,
+                                                                      const double *__restrict__ rhs,
+                                                                      double *__restrict__ x)
+{
+    extern __shared__ double lds[];
+    double *a = lds;                          // N x N, column-major, stride N + 1
+    double *dinv = lds + (long)N * (N + 1);   // reciprocal diagonal
+    for (long e = threadIdx.x; e < (long)N * N; e += LU_SUB_W * 64)
+        a[fidx<true>(N, (int)(e / N), (int)(e % N))] = gw[e];
+    __syncthreads();
+    for (int q = threadIdx.x; q < N; q += LU_SUB_W * 64)
+        dinv[q] = 1.0 / a[fidx<true>(N, q, q)];
+    __syncthreads();
+    lu_subst_wave<64>(N, a, dinv, k, rhs, x, LU_SUB_W * blockIdx.x, LU_SUB_W * gridDim.x);
+}
+
 // batched: workgroup b solves system items[b].x (N = sm <= 128, factor in LDS);
 // which 0: beta = QtAQ^{-1} QtAS with the factor kept from the last alpha solve,
 // which 1: alpha = QtAQ^{-1} QtR, factoring QtAQ
@@ -480,6 +510,16 @@ extern "C" int cacg_k_lu_solve(int N, const double *lhs, int k, const double *rh
     const int kc = lu_columns(N, k, in_lds);
     if (kc < 1)
         return (int)hipErrorInvalidValue;  // N > 16384 is out of range
+    if (in_lds && mode == CACG_MODE_FAST && reuse_factor) {
+        // the kept factor: one column per wave over as many workgroups as it takes
+        const size_t sub = ((size_t)N * (N + 1) + N) * sizeof(double);
+        const int G = (k + LU_SUB_W - 1) / LU_SUB_W;
+        cacg_allow_lds(lu_subst_multi_kernel, sub);
+        hipLaunchKernelGGL(lu_subst_multi_kernel, dim3(G), dim3(LU_SUB_W * 64), sub, st, N, d_work,
+                           k, rhs, x);
+        CACG_LAUNCH_CHECK();
+        return 0;
+    }
     const size_t shm = lu_lds_bytes(N, k, in_lds);
     cacg_allow_lds(in_lds ? lu_solve_kernel<true> : lu_solve_kernel<false>, shm);
     if (in_lds)

@@ -175,7 +175,7 @@ def test_lu_solve(gpu_lib, oracle_lib, N, k):
 
 @pytest.mark.parametrize("N,k", [(1, 1), (4, 4), (9, 3), (16, 16), (64, 16), (100, 7), (128, 1),
                                  (128, 16), (128, 17), (128, 32), (128, 33), (128, 64),
-                                 (128, 65), (128, 128), (128, 300)])
+                                 (128, 65), (128, 128), (128, 300), (128, 384)])
 def test_lu_solve_fast_order_bitwise(gpu_lib, N, k):
     # the fast order (wave-local substitutions with the factor in LDS) against
     # its numpy restatement, bit for bit, fresh and with the kept factor
