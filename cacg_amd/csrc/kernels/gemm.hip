@@ -86,7 +86,9 @@ __global__ __launch_bounds__(256) void qupdate_ref_kernel(int n, int sm, const T
     }
 }
 
-template <typename T>
+// ALPHA_LDS: alpha (sm x m) staged in LDS; otherwise read from global memory
+// (wide shapes whose alpha does not fit beside the row tiles)
+template <typename T, bool ALPHA_LDS>
 __global__ __launch_bounds__(256) void xrupdate_ref_kernel(int n, int sm, int m,
                                                            const T *__restrict__ Q,
                                                            const T *__restrict__ AQ,
@@ -95,12 +97,14 @@ __global__ __launch_bounds__(256) void xrupdate_ref_kernel(int n, int sm, int m,
                                                            double *__restrict__ R, int rb)
 {
     extern __shared__ double lds[];
-    double *al = lds;
-    double *qt = al + (long)sm * m;
+    double *qt = lds + (ALPHA_LDS ? (long)sm * m : 0);
     double *aqt = qt + (long)rb * sm;
+    const double *al = ALPHA_LDS ? lds : alpha;
     const int t = threadIdx.x;
-    for (long e = t; e < (long)sm * m; e += 256)
-        al[e] = alpha[e];
+    if (ALPHA_LDS) {
+        for (long e = t; e < (long)sm * m; e += 256)
+            lds[e] = alpha[e];
+    }
     __syncthreads();
     const int ntile = (n + rb - 1) / rb;
     for (int tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
@@ -127,28 +131,61 @@ __global__ __launch_bounds__(256) void xrupdate_ref_kernel(int n, int sm, int m,
     }
 }
 
+// ||R(:, c)||^2, one sequential chain per column in row order.  One wave per
+// 64 columns (W of them); the rows are staged CREF_E * 64 / W at a time: all
+// 64 lanes load a stage (coalesced when ldr == W) one stage ahead into
+// registers and write it to LDS, and the W chain lanes read their column from
+// there, so the loads never wait on the chain.
+constexpr int CREF_E = 32;   // elements per lane per stage
+
 __global__ __launch_bounds__(64) void colnorm2_ref_kernel(int n, int m, const double *__restrict__ R,
                                                          long ldr, double *__restrict__ out)
 {
-    const int c = blockIdx.x * 64 + threadIdx.x;
-    if (c >= m)
-        return;
-    const double *p = R + c;
+    __shared__ double stg[64 * CREF_E];
+    const int lane = threadIdx.x;
+    const int c0 = blockIdx.x * 64;
+    const int W = m - c0 < 64 ? m - c0 : 64;
+    const int T = 64 * CREF_E / W;   // rows per stage
+    const long last = (long)n - 1;
+    // element k of this lane in a stage: row (lane + 64 k) / W, column (lane + 64 k) % W
+    // (rows past the last one are clamped to it: loaded, never summed)
+    int rk[CREF_E], ck[CREF_E];
+#pragma unroll
+    for (int k = 0; k < CREF_E; k++) {
+        const int e = lane + 64 * k;
+        rk[k] = e / W;
+        ck[k] = c0 + e % W;
+    }
+    double nxt[CREF_E];
+    auto load_stage = [&](long i0) {
+#pragma unroll
+        for (int k = 0; k < CREF_E; k++) {
+            const long i = i0 + rk[k] < last ? i0 + rk[k] : last;
+            nxt[k] = R[i * ldr + ck[k]];
+        }
+    };
     double acc = 0.0;
-    int i = 0;
-    for (; i + 4 <= n; i += 4) {
-        const double v0 = p[(long)i * ldr], v1 = p[(long)(i + 1) * ldr];
-        const double v2 = p[(long)(i + 2) * ldr], v3 = p[(long)(i + 3) * ldr];
-        acc = add_rn(acc, mul_rn(v0, v0));
-        acc = add_rn(acc, mul_rn(v1, v1));
-        acc = add_rn(acc, mul_rn(v2, v2));
-        acc = add_rn(acc, mul_rn(v3, v3));
+    if (n > 0)
+        load_stage(0);
+    for (long i0 = 0; i0 < n; i0 += T) {
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int k = 0; k < CREF_E; k++)
+            stg[lane + 64 * k] = nxt[k];
+        __builtin_amdgcn_wave_barrier();
+        if (i0 + T < n)
+            load_stage(i0 + T);
+        const int rows = n - i0 < T ? (int)(n - i0) : T;
+        if (lane < W) {
+#pragma unroll 8
+            for (int r = 0; r < rows; r++) {
+                const double v = stg[r * W + lane];
+                acc = add_rn(acc, mul_rn(v, v));
+            }
+        }
     }
-    for (; i < n; i++) {
-        const double v = p[(long)i * ldr];
-        acc = add_rn(acc, mul_rn(v, v));
-    }
-    out[c] = acc;
+    if (lane < W)
+        out[c0 + lane] = acc;
 }
 
 // ---- fast order: MFMA -----------------------------------------------------------
@@ -587,14 +624,23 @@ int xrupdate(int n, int sm, int m, const T *Q, const T *AQ, const double *alpha,
         int rb = m >= 256 ? 1 : 256 / m;
         if ((long)rb * sm > 4096)
             rb = 4096 / sm > 0 ? 4096 / sm : 1;
-        const size_t shm = ((size_t)sm * m + 2 * (size_t)rb * sm) * sizeof(double);
+        const size_t tiles = 2 * (size_t)rb * sm * sizeof(double);
+        const size_t al_bytes = (size_t)sm * m * sizeof(double);
+        const bool al_lds = al_bytes + tiles <= 160 * 1024;
+        const size_t shm = tiles + (al_lds ? al_bytes : 0);
         if (shm > 160 * 1024)
             return (int)hipErrorInvalidValue;
         const int ntile = (n + rb - 1) / rb;
         const int G = ntile < 1024 ? ntile : 1024;
-        cacg_allow_lds(xrupdate_ref_kernel<T>, shm);
-        hipLaunchKernelGGL(xrupdate_ref_kernel<T>, dim3(G), dim3(256), shm, st, n, sm, m, Q, AQ,
-                           alpha, X, R, rb);
+        if (al_lds) {
+            cacg_allow_lds(xrupdate_ref_kernel<T, true>, shm);
+            hipLaunchKernelGGL((xrupdate_ref_kernel<T, true>), dim3(G), dim3(256), shm, st, n, sm,
+                               m, Q, AQ, alpha, X, R, rb);
+        } else {
+            cacg_allow_lds(xrupdate_ref_kernel<T, false>, shm);
+            hipLaunchKernelGGL((xrupdate_ref_kernel<T, false>), dim3(G), dim3(256), shm, st, n,
+                               sm, m, Q, AQ, alpha, X, R, rb);
+        }
         CACG_LAUNCH_CHECK();
     }
     if (mode == CACG_MODE_REFERENCE)

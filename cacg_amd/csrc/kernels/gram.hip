@@ -3,8 +3,8 @@
 //
 // Reference order: one lane per entry G[a][b], one sequential chain over all
 // rows in row order, product and sum rounded separately: the reference's
-// summation at one rank, bit for bit.  The chain makes this a verification
-// kernel (its time grows with n on sm*sm lanes only).
+// summation at one rank, bit for bit.  The chain bounds its time at n
+// dependent steps (a few ms at n = 2M) whatever the number of entries.
 //
 // Fast order: split-K.  Workgroup x owns a contiguous chunk of rows and a
 // group of 16x16 output tiles per wave; FP64 MFMA v_mfma_f64_16x16x4f64 with
@@ -19,42 +19,87 @@
 
 namespace {
 
-constexpr int GRAM_UNR = 4;   // rows per lane per chain step batch (reference kernel)
+// One wave per (row a of G, slab of 64 consecutive columns b of [R1 | R2]):
+// lane b keeps the chain of G[a][b].  The chain is sequential in the row
+// index, so the time of the kernel is n steps of one product and one
+// dependent sum per lane whatever the grid; the work of a step is one
+// coalesced 8-byte load per lane of R(i, b) and the broadcast L(i, a).  The
+// R values are loaded GREF_U rows ahead into registers (two alternating
+// batches), and the L values of 64 rows are gathered one per lane a stage
+// ahead, written to LDS and read back as broadcasts, so neither load waits
+// on the chain.
+constexpr int GREF_U = 32;   // rows per register batch (two batches in flight)
 
-__global__ __launch_bounds__(256) void gram_ref_kernel(int n, const double *__restrict__ L, long ldl,
-                                                       int kl, const double *__restrict__ R1,
-                                                       long ld1, int kr1,
-                                                       const double *__restrict__ R2, long ld2,
-                                                       int kr2, double *__restrict__ G1,
-                                                       double *__restrict__ G2)
+struct GrefSrc {
+    const double *L;
+    long ldl;
+    const double *R1;
+    long ld1;
+    int kr1;
+    const double *R2;
+    long ld2;
+    int kr2;
+};
+
+// R(i, b) of GREF_U consecutive rows from row i0, clamped to the last row
+__device__ __forceinline__ void gref_load(double *buf, const double *rp, long ldr, long i0,
+                                          long last)
 {
-    const int kr = kr1 + kr2;
-    const long e = (long)blockIdx.x * 256 + threadIdx.x;
-    if (e >= (long)kl * kr)
-        return;
-    const int a = (int)(e / kr), b = (int)(e % kr);
-    const double *rp = b < kr1 ? R1 + b : R2 + (b - kr1);
-    const long ldr = b < kr1 ? ld1 : ld2;
-    const double *lp = L + a;
-    double acc = 0.0;
-    int i = 0;
-    for (; i + GRAM_UNR <= n; i += GRAM_UNR) {
-        double lv[GRAM_UNR], rv[GRAM_UNR];
 #pragma unroll
-        for (int u = 0; u < GRAM_UNR; u++) {
-            lv[u] = lp[(long)(i + u) * ldl];
-            rv[u] = rp[(long)(i + u) * ldr];
-        }
-#pragma unroll
-        for (int u = 0; u < GRAM_UNR; u++)
-            acc = add_rn(acc, mul_rn(rv[u], lv[u]));
+    for (int u = 0; u < GREF_U; u++) {
+        const long i = i0 + u < last ? i0 + u : last;
+        buf[u] = rp[i * ldr];
     }
-    for (; i < n; i++)
-        acc = add_rn(acc, mul_rn(rp[(long)i * ldr], lp[(long)i * ldl]));
-    if (b < kr1)
-        G1[(long)a * kr1 + b] = acc;
-    else
-        G2[(long)a * kr2 + (b - kr1)] = acc;
+}
+
+// the chain over GREF_U rows whose L values sit at ls[0..GREF_U)
+__device__ __forceinline__ double gref_chain(double acc, const double *buf, const double *ls)
+{
+#pragma unroll
+    for (int u = 0; u < GREF_U; u++)
+        acc = add_rn(acc, mul_rn(buf[u], ls[u]));
+    return acc;
+}
+
+__global__ __launch_bounds__(64) void gram_ref_kernel(int n, GrefSrc s, int nslab,
+                                                      double *__restrict__ G1,
+                                                      double *__restrict__ G2)
+{
+    __shared__ double ls[64];
+    const int lane = threadIdx.x;
+    const int a = blockIdx.x / nslab;
+    const int kr = s.kr1 + s.kr2;
+    const int b0 = (blockIdx.x % nslab) * 64 + lane;
+    const int b = b0 < kr ? b0 : kr - 1;   // idle lanes repeat the last column, unstored
+    const double *rp = b < s.kr1 ? s.R1 + b : s.R2 + (b - s.kr1);
+    const long ldr = b < s.kr1 ? s.ld1 : s.ld2;
+    const double *lp = s.L + a;
+    const long last = (long)n - 1;
+    const long nfull = (long)n / 64 * 64;
+    double acc = 0.0;
+    if (nfull > 0) {
+        double bufA[GREF_U], bufB[GREF_U];
+        double lnext = lp[(long)lane * s.ldl];
+        gref_load(bufA, rp, ldr, 0, last);
+        for (long i0 = 0; i0 < nfull; i0 += 64) {
+            // this stage's L values to LDS, the next stage's into a register
+            __builtin_amdgcn_wave_barrier();
+            ls[lane] = lnext;
+            const long ln = i0 + 64 + lane;
+            lnext = lp[(ln < last ? ln : last) * s.ldl];
+            __builtin_amdgcn_wave_barrier();
+            gref_load(bufB, rp, ldr, i0 + GREF_U, last);
+            acc = gref_chain(acc, bufA, ls);
+            gref_load(bufA, rp, ldr, i0 + 2 * GREF_U, last);
+            acc = gref_chain(acc, bufB, ls + GREF_U);
+        }
+    }
+    for (long i = nfull; i < n; i++)
+        acc = add_rn(acc, mul_rn(rp[i * ldr], lp[i * s.ldl]));
+    if (b0 < s.kr1)
+        G1[(long)a * s.kr1 + b0] = acc;
+    else if (b0 < kr)
+        G2[(long)a * s.kr2 + (b0 - s.kr1)] = acc;
 }
 
 // ---- fast: split-K FP64 MFMA ------------------------------------------------
@@ -608,9 +653,12 @@ extern "C" int cacg_k_gram_ref(int n, const double *L, long ldl, int kl, const d
 {
     if (n < 0 || kl <= 0 || kr1 < 0 || kr2 < 0 || kr1 + kr2 <= 0)
         return (int)hipErrorInvalidValue;
-    const long outs = (long)kl * (kr1 + kr2);
-    hipLaunchKernelGGL(gram_ref_kernel, dim3(cacg_grid(outs, 256, 0)), dim3(256), 0, st, n, L, ldl,
-                       kl, R1, ld1, kr1, R2, ld2, kr2, G1, G2);
+    if (kr2 > 0 && !R2)
+        return (int)hipErrorInvalidValue;
+    const int nslab = (kr1 + kr2 + 63) / 64;
+    const GrefSrc s = {L, ldl, R1, ld1, kr1, R2, ld2, kr2};
+    hipLaunchKernelGGL(gram_ref_kernel, dim3((unsigned)kl * nslab), dim3(64), 0, st, n, s, nslab,
+                       G1, G2);
     CACG_LAUNCH_CHECK();
     return 0;
 }

@@ -224,21 +224,28 @@ def reference_available() -> bool:
     return os.path.exists(REF_HARNESS)
 
 
-def run_reference(A, B, X0, s, eps, solver="bcbcg", nranks=1):
+def run_reference(A, B, X0, s, eps, solver="bcbcg", nranks=1, snapshots=True, timeout=600,
+                  progress=False, workdir=None):
     """Run the reference's own bcbcg_v1 / cbcg_v1 / bcg_v1 at `nranks` MPI
     ranks (mpiexec from the image's MPICH when nranks > 1).
 
     Returns dict(iters, r2 (iters,), X (final, n x m)) and, at one rank,
     R and x_hist (iters x n x m) recorded at every outer iteration's stop test
     (bcbcg / cbcg: after the R update; bcg: the R the iteration started from,
-    with X already updated).
+    with X already updated).  With snapshots=False (one rank, large systems)
+    no R / X history is kept; instead `col_r2` (iters x m) holds every
+    column's ||R(:,c)||^2 at each stop test, summed in row order.  progress
+    passes the harness's per-iteration lines through to stderr; workdir
+    holds the temporary input / output files (default: a fresh temp dir).
     """
     if not reference_available():
         raise RuntimeError("reference build missing (make -C oracle ref)")
     B = _c64(B).reshape(A.n, -1)
     n, m = B.shape
     X0 = _c64(X0).reshape(n, m)
-    with tempfile.TemporaryDirectory() as td:
+    if not snapshots and nranks != 1:
+        raise ValueError("snapshots=False is a one-rank mode")
+    with tempfile.TemporaryDirectory(dir=workdir) as td:
         fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
         with open(fin, "wb") as f:
             f.write(b"CACGIN01")
@@ -252,16 +259,25 @@ def run_reference(A, B, X0, s, eps, solver="bcbcg", nranks=1):
         cmd = [REF_HARNESS, fin, fout]
         if nranks > 1:
             cmd = [MPIEXEC, "-n", str(nranks)] + cmd
-        subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
-                       cwd=td, timeout=600)
+        env = dict(os.environ)
+        if not snapshots:
+            env["CACG_REF_NOSNAP"] = "1"
+        if progress:
+            env["CACG_REF_PROGRESS"] = "1"
+        subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL,
+                       stderr=None if progress else subprocess.DEVNULL, cwd=td, timeout=timeout,
+                       env=env)
         raw = np.fromfile(fout, dtype=np.uint8)
+        cols = np.fromfile(fout + ".cols", dtype="<f8") if not snapshots else None
     assert raw[:8].tobytes() == b"CACGOUT2"
     iters, P, n_, m_ = (int(v) for v in raw[8:40].view("<i8"))
     assert (n_, m_, P) == (n, m, nranks)
     body = raw[40:].view("<f8")
     out = {"iters": iters, "r2": body[:iters].copy()}
     pos = iters
-    if P == 1:
+    if cols is not None:
+        out["col_r2"] = cols.reshape(iters, m).copy()
+    elif P == 1:
         rec = body[pos:pos + iters * 2 * n * m].reshape(iters, 2, n, m)
         out["R"] = rec[:, 0].copy()
         out["x_hist"] = rec[:, 1].copy()

@@ -26,9 +26,18 @@
  * Output file (rank 0): "CACGOUT2", int64 iterations, int64 P, int64 n,
  * int64 m, float64 r2[iterations], then at P == 1 per iteration float64
  * R[n*m], X[n*m], then the final float64 X[n*m].
+ *
+ * Large runs (environment, P == 1 only): CACG_REF_NOSNAP=1 keeps no R / X
+ * snapshots (the output then holds r2 and the final X only) and writes
+ * instead, to <output>.cols, float64 ||R(:,c)||^2 for every column c after
+ * every outer iteration, each a sequential row-order sum of squares of the
+ * recorded R (the reference's own norm loop, src/MatrixOperations.c:110-142,
+ * applied to every column).  CACG_REF_PROGRESS=1 prints one line per outer
+ * iteration to stderr (iteration, ||R(:,0)||^2, wall seconds).
  */
 #include <mpi.h>
 #include <stdint.h>
+#include <time.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -50,6 +59,17 @@ static double *g_X;
 static long g_nm_local;
 static long g_iters;
 static int g_np;
+static FILE *g_cols;      /* CACG_REF_NOSNAP: per-iteration column norms^2 */
+static long g_m;
+static int g_progress;
+static double g_t0;
+
+static double wall(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
 
 void __wrap_norm2square_dist_denseMat_col_n(denseType mat, double *result, long n, int myid,
                                             int numprocs)
@@ -60,6 +80,24 @@ void __wrap_norm2square_dist_denseMat_col_n(denseType mat, double *result, long 
     if (g_np == 1 && g_snap) {
         fwrite(mat.data, sizeof(double), (size_t)g_nm_local, g_snap);
         fwrite(g_X, sizeof(double), (size_t)g_nm_local, g_snap);
+    }
+    if (g_np == 1 && g_cols) {
+        const long rows = g_nm_local / g_m;
+        for (long c = 0; c < g_m; c++) {
+            double acc = 0.0;
+            for (long i = 0; i < rows; i++) {
+                const double v = mat.data[i * g_m + c];
+                const double sq = v * v;
+                acc = acc + sq;
+            }
+            fwrite(&acc, sizeof(double), 1, g_cols);
+        }
+        fflush(g_cols);
+    }
+    if (g_progress && myid == 0) {
+        fprintf(stderr, "ref_harness: iteration %ld r2 %.17g at %.1f s\n", g_iters, *result,
+                wall() - g_t0);
+        fflush(stderr);
     }
     g_iters++;
 }
@@ -140,10 +178,20 @@ int main(int argc, char **argv)
 
     g_X = X.data;
     g_nm_local = nloc * m;
+    g_m = m;
+    const char *ns = getenv("CACG_REF_NOSNAP"), *pg = getenv("CACG_REF_PROGRESS");
+    const int nosnap = ns && ns[0] == '1';
+    g_progress = pg && pg[0] == '1';
+    g_t0 = wall();
     char snap_path[4096];
-    if (np == 1) {
+    if (np == 1 && !nosnap) {
         snprintf(snap_path, sizeof(snap_path), "%s.snap", argv[2]);
         g_snap = fopen(snap_path, "wb");
+    }
+    if (np == 1 && nosnap) {
+        char cols_path[4096];
+        snprintf(cols_path, sizeof(cols_path), "%s.cols", argv[2]);
+        g_cols = fopen(cols_path, "wb");
     }
 
     if (solver == 0)
@@ -174,7 +222,9 @@ int main(int argc, char **argv)
         fwrite("CACGOUT2", 1, 8, out);
         fwrite(h, sizeof(int64_t), 4, out);
         fwrite(g_r2, sizeof(double), (size_t)(g_iters < MAXREC ? g_iters : MAXREC), out);
-        if (np == 1) {
+        if (g_cols)
+            fclose(g_cols);
+        if (np == 1 && g_snap) {
             FILE *sn = fopen(snap_path, "rb");
             char buf[1 << 16];
             size_t got;

@@ -216,3 +216,26 @@ def test_block_major_basis_gives_the_same_bits(gpu_lib, monkeypatch, grid, m, s)
     assert rb["iters"] == rr["iters"] > 1
     assert np.array_equal(rb["res_hist"], rr["res_hist"])
     assert np.array_equal(xb, xr)
+
+
+@pytest.mark.parametrize("m,s", [(32, 16), (64, 8)])
+def test_wide_shapes_reference_bitwise_and_fast(gpu_lib, oracle_lib, m, s):
+    # s m = 512: the X / R update's alpha (s m x m) no longer fits in LDS beside
+    # the row tiles and is read from global memory; m = 64 takes the
+    # non-MFMA update kernels in the fast order too
+    A = P.laplacian_2d(32)
+    B = P.rhs_uniform(A.n, m, 1)
+    X0 = np.zeros_like(B)
+    eps = 1e-8 * float(np.sqrt(oracle_lib.col_norm2sq(B, 0)))
+    print(f"wide shape m={m} s={s} n={A.n}", flush=True)
+    orc = oracle_lib.bcbcg(A, B, X0, s, eps=eps, maxiter=100)
+    assert orc["status"] == "ok"
+    ref = gpu_lib.bcbcg(A, B, X0, s, stop_mode=0, eps=eps, maxiter=100, order="reference")
+    assert ref["status"] == "ok" and ref["iters"] == orc["iters"]
+    assert np.array_equal(ref["res_hist"], orc["res_hist"])
+    assert np.array_equal(ref["X"], orc["X"])
+    fast = gpu_lib.bcbcg(A, B, X0, s, stop_mode=0, eps=eps, maxiter=100, order="fast")
+    assert fast["status"] == "ok" and abs(fast["iters"] - orc["iters"]) <= 1
+    D = A.to_dense()
+    tr = np.linalg.norm(B[:, 0] - D @ fast["X"][:, 0]) / np.linalg.norm(B[:, 0])
+    assert tr < 1e-7
