@@ -75,6 +75,7 @@ struct cacg_solver {
     hipEvent_t ev[MAXEV];
     int ev_phase[MAXEV];
     int nev;
+    int noev;   /* 1: no phase events (the solver of cacg_bcbcg_device_noalloc) */
     long footprint;
 };
 
@@ -494,7 +495,7 @@ int cacg_solver_create(cacg_solver **out, long n, const long *row_start, const l
 
 static void mark(cacg_solver *sv, int phase_done)
 {
-    if (sv->nev >= MAXEV)
+    if (sv->noev || sv->nev >= MAXEV)
         return;
     hipEventRecord(sv->ev[sv->nev], sv->st);
     sv->ev_phase[sv->nev] = phase_done;
@@ -503,6 +504,8 @@ static void mark(cacg_solver *sv, int phase_done)
 
 static void harvest(cacg_solver *sv, cacg_report *rep)
 {
+    if (sv->noev)
+        return;
     for (int i = 1; i < sv->nev; i++) {
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, sv->ev[i - 1], sv->ev[i]) == hipSuccess && rep) {
@@ -832,6 +835,131 @@ out:
         rep->iters = it;
     }
     return status;
+}
+
+/* ---- the entry point that allocates nothing -------------------------------------------- */
+
+#define NOALLOC_ALIGN 256L
+
+static long na_round(long bytes) { return (bytes + NOALLOC_ALIGN - 1) / NOALLOC_ALIGN * NOALLOC_ALIGN; }
+
+/* bytes of scratch a solve of (n, m, s) takes: the work blocks of
+ * cacg_solver_create in the same roles (FP64, Gerschgorin interval) */
+long cacg_noalloc_bytes(long n, long m, long s)
+{
+    if (n <= 0 || m <= 0 || s <= 0 || s * m > 4096 || n >= (1L << 31))
+        return -1;
+    const long sm = s * m, d = (long)sizeof(double);
+    long gws = cacg_gram_workspace_n((int)n, (int)sm, (int)(sm + m));
+    const long g2 = cacg_gram_workspace_n((int)n, (int)sm, (int)sm);
+    const long g3 = cacg_gram_workspace_n((int)n, (int)m, (int)m);
+    if (g2 > gws)
+        gws = g2;
+    if (g3 > gws)
+        gws = g3;
+    long b = 0;
+    b += na_round(n * m * d);                               /* R */
+    b += 3 * na_round(n * sm * d);                          /* S, Q, AQ */
+    b += 3 * na_round(sm * sm * d);                         /* QtAQ, QtAS, beta */
+    b += 2 * na_round(sm * m * d);                          /* QtR, alpha */
+    b += na_round(m * d) + na_round(2 * d);                 /* norms, bounds */
+    b += na_round(2L * CACG_GERSH_BLOCKS * d);              /* Gerschgorin partials */
+    b += na_round(gws * d);                                 /* Gram partials */
+    b += na_round(cacg_xrupdate_workspace((int)m) * d);     /* norm partials */
+    b += na_round(sm * sm * d);                             /* LU factor */
+    b += na_round((long)sizeof(int));                       /* breakdown flag */
+    return b;
+}
+
+static void *na_take(char **cur, long bytes)
+{
+    void *p = *cur;
+    *cur += na_round(bytes);
+    return p;
+}
+
+int cacg_bcbcg_device_noalloc(int n, const int *d_rowptr, const int *d_col, const double *d_val,
+                              long m, long s, const double *d_B, double *d_X,
+                              const cacg_opts *opts, cacg_report *report, void *stream)
+{
+    if (n <= 0 || !d_rowptr || !d_col || !d_val || !d_B || !d_X || m <= 0 ||
+        m > CACG_NOALLOC_MAX_M || s <= 0 || s * m > 4096)
+        return CACG_EINVAL;
+    cacg_opts o;
+    if (opts)
+        o = *opts;
+    else
+        cacg_default_opts(&o);
+    if (o.maxiter <= 0 || o.precision != CACG_PREC_FP64 || o.tol < 0.0 ||
+        (o.bounds != CACG_BOUNDS_GERSCHGORIN && o.bounds != CACG_BOUNDS_GIVEN) ||
+        (o.bounds == CACG_BOUNDS_GIVEN &&
+         !(isfinite(o.bound_lo) && isfinite(o.bound_hi) && o.bound_lo < o.bound_hi)))
+        return CACG_EINVAL;
+    const long need = cacg_noalloc_bytes(n, m, s);
+    void *base = NULL;
+    long cap = 0;
+    int rc = check_device();
+    if (rc != CACG_OK)
+        return rc;
+    if (cacg_noalloc_scratch(&base, &cap) != 0)
+        return CACG_EDEVICE;
+    if (need < 0 || need > cap) {
+        fprintf(stderr,
+                "cacg: cacg_bcbcg_device_noalloc: n = %d, m = %ld, s = %ld needs %ld bytes of "
+                "scratch, more than the %ld bytes it has\n",
+                n, m, s, need, cap);
+        return CACG_EINVAL;
+    }
+    /* the solver of this call: on the stack, its work blocks carved from the
+     * scratch (every block is written before it is read), no events, no SpMM
+     * tuning (the default schedules), host-side norms in a stack buffer */
+    cacg_solver sv;
+    double hbuf[2 * CACG_NOALLOC_MAX_M + 4];
+    memset(&sv, 0, sizeof(sv));
+    sv.n = n;
+    sv.m = m;
+    sv.s = s;
+    sv.sm = s * m;
+    sv.opts = o;
+    sv.st = (hipStream_t)stream;
+    sv.A.n = n;
+    sv.A.rowptr = d_rowptr;
+    sv.A.col = d_col;
+    sv.A.val = d_val;
+    sv.h_pin = hbuf;
+    sv.tuned = 1;
+    sv.noev = 1;
+    sv.basis_rows = 0;
+    const long sm = sv.sm, d = (long)sizeof(double);
+    char *cur = (char *)base;
+    sv.d_R = (double *)na_take(&cur, n * m * d);
+    sv.d_S = (double *)na_take(&cur, n * sm * d);
+    sv.d_Q = (double *)na_take(&cur, n * sm * d);
+    sv.d_AQ = (double *)na_take(&cur, n * sm * d);
+    sv.d_QtAQ = (double *)na_take(&cur, sm * sm * d);
+    sv.d_QtAS = (double *)na_take(&cur, sm * sm * d);
+    sv.d_beta = (double *)na_take(&cur, sm * sm * d);
+    sv.d_QtR = (double *)na_take(&cur, sm * m * d);
+    sv.d_alpha = (double *)na_take(&cur, sm * m * d);
+    sv.d_norm2 = (double *)na_take(&cur, m * d);
+    sv.d_bounds = (double *)na_take(&cur, 2 * d);
+    sv.d_gwork = (double *)na_take(&cur, 2L * CACG_GERSH_BLOCKS * d);
+    {
+        long gws = cacg_gram_workspace_n(n, (int)sm, (int)(sm + m));
+        const long g2 = cacg_gram_workspace_n(n, (int)sm, (int)sm);
+        const long g3 = cacg_gram_workspace_n(n, (int)m, (int)m);
+        if (g2 > gws)
+            gws = g2;
+        if (g3 > gws)
+            gws = g3;
+        sv.d_gram_work = (double *)na_take(&cur, gws * d);
+    }
+    sv.d_xr_work = (double *)na_take(&cur, cacg_xrupdate_workspace((int)m) * d);
+    sv.d_lu_work = (double *)na_take(&cur, sm * sm * d);
+    sv.d_flag = (int *)na_take(&cur, (long)sizeof(int));
+    if (cur - (char *)base != need)
+        return CACG_EDEVICE; /* the layout above and cacg_noalloc_bytes disagree */
+    return cacg_solver_solve_device(&sv, d_B, d_X, report);
 }
 
 /* ---- block CG (src/bcgKernel.c:22-462) ---------------------------------------------- */

@@ -669,6 +669,14 @@ extern "C" long cacg_gram_workspace(int kl, int kr)
     return 512L * ((kl + 15) / 16 * 16) * ((kr + 15) / 16 * 16);
 }
 
+extern "C" long cacg_gram_workspace_n(int n, int kl, int kr)
+{
+    if (n <= 0 || kl <= 0 || kr <= 0)
+        return 0;
+    const GramPlan p = plan_gram(n, kl, kr);
+    return (long)p.gx * (p.TM * 16) * (p.TN * 16);
+}
+
 extern "C" int cacg_k_gram_fast(int n, const double *L, long ldl, int kl, const double *R1,
                                 long ld1, int kr1, const double *R2, long ld2, int kr2, double *G1,
                                 double *G2, double *d_work, hipStream_t st)

@@ -208,6 +208,28 @@ int cacg_solver_destroy(cacg_solver *sv);
 /* bytes of HBM the solver holds (matrix + work blocks) */
 long cacg_solver_footprint(const cacg_solver *sv);
 
+/* ---- the entry point that allocates nothing ------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the solve of cacg_solver_solve_device (FP64, the order
+ * of opts->order, Gerschgorin or given bounds) with the matrix already in HBM
+ * as int32 CSR (d_rowptr n + 1, d_col / d_val rowptr[n] entries) and B, X
+ * device blocks (n x m row-major, X in/out), on `stream` (a hipStream_t; NULL
+ * for the null stream).  Its work blocks live in a per-device scratch array
+ * of CACG_NOALLOC_SCRATCH_BYTES (128 MiB, include/cacg_kernels.h) that the
+ * library holds on every GPU it runs on until the process ends; it calls no
+ * hipMalloc, hipFree or other allocating routine.  Limits: m <= 64 and
+ * cacg_noalloc_bytes(n, m, s) <= that size (about n (3 s m + m) 8 bytes plus
+ * the Gram partials), else CACG_EINVAL with a message; no SpMM schedule
+ * tuning and no phase times (report->phase_ms stays zero).  Every call on a
+ * GPU shares that GPU's scratch: calls that overlap on one GPU (other
+ * threads, other streams) race and must be serialised by the caller. */
+#define CACG_NOALLOC_MAX_M 64
+long cacg_noalloc_bytes(long n, long m, long s);
+int cacg_bcbcg_device_noalloc(int n, const int *d_rowptr, const int *d_col, const double *d_val,
+                              long m, long s, const double *d_B, double *d_X,
+                              const cacg_opts *opts, cacg_report *report, void *stream);
+
 /* ---- device memory helpers (for callers without their own HIP plumbing) -- */
 int cacg_device_count(void);
 int cacg_dev_alloc(void **d_ptr, long bytes);

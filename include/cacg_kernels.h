@@ -35,6 +35,12 @@ typedef struct {
 
 enum { CACG_MODE_REFERENCE = 0, CACG_MODE_FAST = 1 };
 
+/* Bytes of the per-device scratch array of cacg_bcbcg_device_noalloc. */
+#define CACG_NOALLOC_SCRATCH_BYTES (128L << 20)
+/* The calling thread's current device's copy of that array: its device
+ * address (looked up once per device) and size.  Returns 0 or a hipError_t. */
+int cacg_noalloc_scratch(void **d_ptr, long *bytes);
+
 /* Gerschgorin interval (src/common.c:53-106): d_out[0] = min_i (a_ii - sum|a_ij|),
  * d_out[1] = max_i (a_ii + sum|a_ij|), the max seeded with DBL_MIN.
  * d_work holds 2 * CACG_GERSH_BLOCKS doubles. */
@@ -100,6 +106,8 @@ int cacg_k_cvt_f32_f64(long count, const float *src, double *dst, hipStream_t st
 int cacg_k_gram_ref(int n, const double *L, long ldl, int kl, const double *R1, long ld1, int kr1,
                     const double *R2, long ld2, int kr2, double *G1, double *G2, hipStream_t st);
 long cacg_gram_workspace(int kl, int kr);
+/* the doubles of d_work a Gram of n rows actually uses (<= cacg_gram_workspace) */
+long cacg_gram_workspace_n(int n, int kl, int kr);
 int cacg_k_gram_fast(int n, const double *L, long ldl, int kl, const double *R1, long ld1, int kr1,
                      const double *R2, long ld2, int kr2, double *G1, double *G2, double *d_work,
                      hipStream_t st);
