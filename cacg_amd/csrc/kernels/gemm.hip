@@ -206,22 +206,21 @@ __global__ __launch_bounds__(512) void qupdate_mfma_kernel(int n, const TS *__re
 {
     constexpr int KQ = SM / 4, NTL = SM / 16;
     constexpr int NA = NTX > 0 ? NTX : 1;
-    extern __shared__ double bl[];  // SM x SM
+    // beta (SM x SM), then with NTX > 0 alpha (SM x m, zero-padded to 16 NTX
+    // columns): both read from LDS as MFMA B operands
+    extern __shared__ double bl[];
+    double *al = bl + SM * SM;
     for (int e = threadIdx.x; e < SM * SM; e += 512)
         bl[e] = beta[e];
+    if (NTX > 0) {
+        for (int e = threadIdx.x; e < SM * 16 * NA; e += 512) {
+            const int k = e / (16 * NA), c = e % (16 * NA);
+            al[e] = c < m ? alpha[(long)k * m + c] : 0.0;
+        }
+    }
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int lg = lane >> 4, lc = lane & 15;
-    double alf[NA][KQ];
-    if (NTX > 0) {
-#pragma unroll
-        for (int j = 0; j < NA; j++) {
-            const int c = j * 16 + lc;
-#pragma unroll
-            for (int tt = 0; tt < KQ; tt++)
-                alf[j][tt] = c < m ? alpha[(long)(lg * KQ + tt) * m + c] : 0.0;
-        }
-    }
     const int nw = gridDim.x * 8;
     const int wid = xcd_remap(blockIdx.x, gridDim.x) * 8 + wave;
     const int ntiles = (n + 15) / 16;
@@ -241,6 +240,20 @@ __global__ __launch_bounds__(512) void qupdate_mfma_kernel(int n, const TS *__re
 #pragma unroll
             for (int u = 0; u < KQ; u++)
                 av[u] = 0.0;
+        }
+        // the X values of the deferred update, loaded with the Q rows so the
+        // update after the MFMAs is a plain store (X + Q alpha, same sum)
+        double xv[NA][4];
+        if (NTX > 0) {
+#pragma unroll
+            for (int j = 0; j < NA; j++) {
+                const int c = j * 16 + lc;
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const long i = i0 + lg + 4 * r;
+                    xv[j][r] = (c < m && i < n) ? X[i * m + c] : 0.0;
+                }
+            }
         }
         // S for one 16-column tile: FP64 S by guarded loads (they stay in
         // flight until the subtraction); FP32 S from a clamped row without a
@@ -304,27 +317,35 @@ __global__ __launch_bounds__(512) void qupdate_mfma_kernel(int n, const TS *__re
                     sv[r] = svn[r];
             }
         } else {
-#pragma unroll 1
+            // FP64 S: every column tile's S values are loaded with the Q rows,
+            // so no tile's subtraction waits on a load issued just before it
+            TS sall[NTL][4];
+#pragma unroll
+            for (int j = 0; j < NTL; j++)
+                load_s(j, sall[j]);
+#pragma unroll
             for (int j = 0; j < NTL; j++) {
-                TS sv[4];
-                load_s(j, sv);
-                update_tile(j, sv);
+                // keep each tile's beta reads after the previous tile's MFMAs
+                asm volatile("" ::: "memory");
+                update_tile(j, sall[j]);
             }
         }
         if (NTX > 0) {
 #pragma unroll
             for (int j = 0; j < NA; j++) {
+                asm volatile("" ::: "memory");
                 dbl4 ax = {0.0, 0.0, 0.0, 0.0};
+                const double *acol = al + (lg * KQ) * (16 * NA) + j * 16 + lc;
 #pragma unroll
                 for (int tt = 0; tt < KQ; tt++)
-                    ax = mfma16x16x4(av[tt], alf[j][tt], ax);
+                    ax = mfma16x16x4(av[tt], acol[tt * 16 * NA], ax);
                 const int c = j * 16 + lc;
                 if (c < m) {
 #pragma unroll
                     for (int r = 0; r < 4; r++) {
                         const long i = i0 + lg + 4 * r;
                         if (i < n)
-                            X[i * m + c] += ax[r];
+                            X[i * m + c] = xv[j][r] + ax[r];
                     }
                 }
             }
@@ -385,6 +406,20 @@ __global__ __launch_bounds__(256) void xrupdate_mfma_kernel(int n, int m, const 
                 aq[u] = 0.0;
             }
         }
+        // the X and R values this lane updates, loaded with the operand rows so
+        // the updates after the MFMAs wait on nothing
+        double xo[NTM][4], ro[NTM][4];
+#pragma unroll
+        for (int j = 0; j < NTM; j++) {
+            const int c = j * 16 + lc;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const long i = i0 + lg + 4 * r;
+                const bool in = c < m && i < n;
+                xo[j][r] = (DOX && in) ? X[i * m + c] : 0.0;
+                ro[j][r] = (DOR && in) ? R[i * m + c] : 0.0;
+            }
+        }
 #pragma unroll
         for (int j = 0; j < NTM; j++) {
             dbl4 ax = {0.0, 0.0, 0.0, 0.0}, ar = {0.0, 0.0, 0.0, 0.0};
@@ -403,9 +438,9 @@ __global__ __launch_bounds__(256) void xrupdate_mfma_kernel(int n, int m, const 
                     const long i = i0 + lg + 4 * r;
                     if (i < n) {
                         if (DOX)
-                            X[i * m + c] += ax[r];
+                            X[i * m + c] = xo[j][r] + ax[r];
                         if (DOR) {
-                            const double rn = R[i * m + c] - ar[r];
+                            const double rn = ro[j][r] - ar[r];
                             R[i * m + c] = rn;
                             nrm[j] += rn * rn;
                         }
@@ -500,9 +535,10 @@ int launch_qupdate_mfma(int n, const TS *S, T *Q, const double *beta, hipStream_
         G = 256;
     if (G >= 8)
         G &= ~7;
-    cacg_allow_lds(qupdate_mfma_kernel<TS, T, SM, NTX>, (size_t)SM * SM * sizeof(double));
-    hipLaunchKernelGGL((qupdate_mfma_kernel<TS, T, SM, NTX>), dim3(G), dim3(512),
-                       (size_t)SM * SM * sizeof(double), st, n, S, Q, beta, m, alpha, X, sbs, sbw);
+    const size_t shm = ((size_t)SM * SM + (NTX > 0 ? (size_t)SM * 16 * NTX : 0)) * sizeof(double);
+    cacg_allow_lds(qupdate_mfma_kernel<TS, T, SM, NTX>, shm);
+    hipLaunchKernelGGL((qupdate_mfma_kernel<TS, T, SM, NTX>), dim3(G), dim3(512), shm, st, n, S,
+                       Q, beta, m, alpha, X, sbs, sbw);
     CACG_LAUNCH_CHECK();
     return 0;
 }

@@ -260,6 +260,15 @@ def test_deferred_x_update_kernels_match_undeferred_bitwise(gpu_lib, sm, m):
     # against plain float64 products
     assert np.allclose(Xd, X + Q @ alpha, rtol=1e-12, atol=1e-11)
     assert np.allclose(Qd, S - Q @ beta, rtol=1e-12, atol=1e-11)
+    # integers: exact in any order
+    Si, Qi = (rng.integers(-8, 9, size=(n, sm)).astype(np.float64) for _ in range(2))
+    bi = rng.integers(-8, 9, size=(sm, sm)).astype(np.float64)
+    ai = rng.integers(-8, 9, size=(sm, m)).astype(np.float64)
+    Xi = rng.integers(-8, 9, size=(n, m)).astype(np.float64)
+    Qe, Xe = gpu_lib.k_qxupdate(Si, Qi, bi, ai, Xi)
+    I64 = np.int64
+    assert np.array_equal(Qe, (Si.astype(I64) - Qi.astype(I64) @ bi.astype(I64)).astype(np.float64))
+    assert np.array_equal(Xe, (Xi.astype(I64) + Qi.astype(I64) @ ai.astype(I64)).astype(np.float64))
 
 
 @pytest.mark.parametrize("n,sm,m,gap", [(5003, 128, 16, 0), (4099, 64, 8, 2), (3001, 96, 32, 0),
