@@ -176,7 +176,9 @@ int cacg_lanczos_interval(const double *alpha, const double *beta, int k, const 
     const double tmax = tridiag_eig(alpha, beta, kk, kk - 1, lo - pad, hi + pad);
     const double w = tmax - tmin;
     double a = tmin - 0.05 * w, b = tmax + 0.05 * w;
-    if (a < 0.5 * tmin)
+    /* a positive spectrum keeps a positive lower end: not below tmin / 2 (for
+     * tmin <= 0 that clip would lift the end above tmin, so it is not applied) */
+    if (tmin > 0.0 && a < 0.5 * tmin)
         a = 0.5 * tmin;
     if (b > g[1] && g[1] > tmax)
         b = g[1];

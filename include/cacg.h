@@ -35,6 +35,12 @@ typedef struct {
 } csrType_local;
 
 typedef struct {
+    long *rowIdx;    /* COO triplets as the reference's Matrix Market reader fills them */
+    long *colIdx;
+    double *coodata;
+} cooType;
+
+typedef struct {
     long local_num_row;
     long local_num_col;
     long global_num_row;
@@ -170,9 +176,10 @@ void cacg_default_opts(cacg_opts *o);
 /* The interval CACG_BOUNDS_LANCZOS uses, from the Lanczos coefficients
  * (alpha_0..k-1, beta_0..k-1) and the Gerschgorin interval g: the extreme
  * eigenvalues t_min <= t_max of the tridiagonal T (Sturm bisection; T is cut
- * at the first negligible beta), widened to lo = t_min - 0.05 (t_max - t_min)
- * clipped at (t_min / 2, ...) and hi = t_max + 0.05 (t_max - t_min) clipped at
- * g[1], the Gerschgorin upper bound.  Host code; returns CACG_EINVAL for
+ * at the first negligible beta), widened to lo = t_min - 0.05 (t_max - t_min),
+ * raised to t_min / 2 when t_min > 0 and lo is below that, and
+ * hi = t_max + 0.05 (t_max - t_min) clipped at g[1], the Gerschgorin upper
+ * bound.  Host code; returns CACG_EINVAL for
  * k <= 0 or non-finite coefficients. */
 int cacg_lanczos_interval(const double *alpha, const double *beta, int k, const double g[2],
                           double out[2]);

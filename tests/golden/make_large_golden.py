@@ -15,7 +15,12 @@ The whole solution (268 MB) is not committed.  Stored, as data only:
   X_colnorm2  ||X(:,c)||^2 of the final X (row-order sums)
   X_rows      the final X at rows `rows` (a fixed spread of 64 rows)
 
-    python tests/golden/make_large_golden.py [--workdir DIR]   # this container only
+With --nranks 4 the same solve runs under `mpiexec -n 4` (the reference's
+row-block distribution; its Allreduce sums in another order) and writes
+<name>_p4.npz: r2_p4 (||R(:,0)||^2 after every outer iteration), X_rows_p4 and
+X_colnorm2_p4; the manifest entry gains iters_p4 and X_p4_sha256.
+
+    python tests/golden/make_large_golden.py [--workdir DIR] [--nranks 4]   # this container only
 """
 
 from __future__ import annotations
@@ -55,6 +60,7 @@ def x_digest(X: np.ndarray) -> str:
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workdir", default=None, help="where the harness's temporary files go")
+    ap.add_argument("--nranks", type=int, default=1, choices=[1, 4])
     args = ap.parse_args()
     if not oracle.reference_available():
         raise SystemExit("oracle/_ref/ref_harness is not built (make -C oracle ref)")
@@ -63,12 +69,29 @@ def main():
     X0 = np.zeros_like(B)
     eps = TOL * float(np.sqrt(oracle.col_norm2sq(B, 0)))
     t0 = time.time()
-    r = oracle.run_reference(A, B, X0, S, eps, solver="bcbcg", nranks=1, snapshots=False,
-                             timeout=24 * 3600, progress=True, workdir=args.workdir)
+    r = oracle.run_reference(A, B, X0, S, eps, solver="bcbcg", nranks=args.nranks,
+                             snapshots=args.nranks > 1, timeout=24 * 3600, progress=True,
+                             workdir=args.workdir)
     wall = time.time() - t0
     X = r["X"]
     rows = sample_rows(A.n)
     os.makedirs(OUT_DIR, exist_ok=True)
+    mpath = os.path.join(OUT_DIR, "manifest.json")
+    if args.nranks > 1:
+        np.savez_compressed(
+            os.path.join(OUT_DIR, f"{NAME}_p4.npz"), r2_p4=r["r2"], rows=rows, X_rows_p4=X[rows].copy(),
+            X_colnorm2_p4=np.array([oracle.col_norm2sq(X, c) for c in range(M)]))
+        with open(mpath) as f:
+            meta = json.load(f)
+        meta[NAME]["iters_p4"] = r["iters"]
+        meta[NAME]["X_p4_sha256"] = x_digest(X)
+        meta[NAME]["reference_p4_wall_s"] = round(wall, 1)
+        with open(mpath, "w") as f:
+            json.dump(meta, f, indent=1, sort_keys=True)
+        if args.workdir:
+            np.save(os.path.join(args.workdir, f"{NAME}_X_p4.npy"), X)
+        print(f"{NAME} at 4 ranks: {r['iters']} iterations, {wall:.0f} s")
+        return
     np.savez_compressed(
         os.path.join(OUT_DIR, f"{NAME}.npz"), r2_p1=r["r2"], col_r2=r["col_r2"],
         X_colnorm2=np.array([oracle.col_norm2sq(X, c) for c in range(M)]), rows=rows,
@@ -81,7 +104,7 @@ def main():
             "reference_wall_s": round(wall, 1),
         }
     }
-    with open(os.path.join(OUT_DIR, "manifest.json"), "w") as f:
+    with open(mpath, "w") as f:
         json.dump(meta, f, indent=1, sort_keys=True)
     print(f"{NAME}: {r['iters']} iterations, {wall:.0f} s, X sha256 {meta[NAME]['X_sha256']}")
 

@@ -63,6 +63,8 @@ int main(void) {
          offsetof(denseType, start_idx), offsetof(denseType, data));
   printf("info %zu %zu %zu %zu\n", sizeof(matInfo), offsetof(matInfo, num_rows),
          offsetof(matInfo, num_cols), offsetof(matInfo, nnz));
+  printf("coo %zu %zu %zu %zu\n", sizeof(cooType), offsetof(cooType, rowIdx),
+         offsetof(cooType, colIdx), offsetof(cooType, coodata));
   return 0;
 }
 """
@@ -80,6 +82,7 @@ def test_types_byte_identical_to_reference(tmp_path):
     ours = _layout(os.path.join(INCLUDE, "cacg.h"), tmp_path, "ours")
     assert ours.splitlines()[0] == "csr 56 0 8 16 24 32 40 48"
     assert ours.splitlines()[1] == "dense 48 0 8 16 24 32 40"
+    assert ours.splitlines()[3] == "coo 24 0 8 16"
     if not os.path.exists(REF_TYPES):
         pytest.skip("reference sources not present (layout checked against the fixed offsets)")
     assert ours == _layout(REF_TYPES, tmp_path, "ref")
@@ -206,3 +209,21 @@ def test_options_rejected_without_device(bad):
     rc = L.cacg_bcbcg(A.n, rp.ctypes.data_as(lp), col.ctypes.data_as(lp), val.ctypes.data_as(dp),
                       1, 2, B.ctypes.data_as(dp), X.ctypes.data_as(dp), ctypes.byref(o), None)
     assert rc == 3
+
+
+READER_TU_C = r"""
+/* a translation unit that includes cacg.h and then declares the reference's
+ * reader interface (src/readMTX.h:7-8, src/matrixType.h:5,8 use cooType) */
+#include "cacg.h"
+void readMtx_info_and_coo(char *path, char *name, matInfo *info, cooType *mat);
+void Converter_Coo2Csr(cooType src, csrType_local *dst, matInfo *info);
+void delete_cooType(cooType mat);
+int main(void) { cooType c = {0, 0, 0}; return c.rowIdx != 0; }
+"""
+
+
+def test_header_declares_every_reference_type(tmp_path):
+    src = tmp_path / "reader_tu.c"
+    src.write_text(READER_TU_C)
+    subprocess.run(["gcc", "-std=gnu11", "-Wall", "-Werror", "-I", INCLUDE, "-c", str(src), "-o",
+                    str(tmp_path / "reader_tu.o")], check=True)

@@ -4,20 +4,18 @@ Reference order: bit for bit against the compiled reference's golden
 fixtures (iteration count, ||R(:,c)|| after every outer iteration, final X),
 through both the extended API and the drop-in entry points.
 
-Fast order (split-K MFMA): the north-star bars, iterations within +-1 and the
-final X within 1e-10 (relative 2-norm) of the one-rank reference, and the
-per-iteration recursive residual within 1e-9 relative.  A block s-step
-recurrence amplifies any change of summation order once its residual nears
-the rounding floor; the reference run at 4 MPI ranks (golden r2_p4) shows the
-same amplification against its own 1-rank run.  Both bars are therefore
-max(stated bar, 10 x the reference's own 4-rank vs 1-rank difference)."""
+Fast order (split-K MFMA): the north-star bars where the reference itself
+meets them (conftest.check_fast_order): iterations within +-1, the
+per-iteration residual within 1e-9 relative until the reference's own 4-rank
+run departs from its 1-rank run, within 2 x that departure after it, and X
+within max(1e-10, 2 x the reference's 4-rank X difference)."""
 
 import ctypes
 
 import numpy as np
 import pytest
 
-from conftest import golden_case, golden_manifest
+from conftest import check_fast_order, golden_case, golden_manifest
 from cacg_amd import problems as P
 
 pytestmark = pytest.mark.gpu
@@ -94,18 +92,11 @@ def test_dropin_prints_reference_loop_lines(gpu_lib, capfd):
 
 
 @pytest.mark.parametrize("name", CASES)
-def test_fast_order_within_reference_envelope(gpu_lib, name):
+def test_fast_order_against_reference(gpu_lib, name):
     spec, A, B, X0, g = golden_case(name)
     out = _run(gpu_lib, spec, A, B, X0, "fast")
     assert out["status"] == "ok"
-    assert abs(out["iters"] - spec["iters_p1"]) <= 1
-    xrel = np.linalg.norm(out["X"] - g["X"]) / np.linalg.norm(g["X"])
-    assert xrel <= max(1e-10, 10 * float(g["X_p4_reldiff"])), xrel
-    r1, r4 = np.sqrt(g["r2_p1"]), np.sqrt(g["r2_p4"])
-    k = min(len(r1), len(r4), out["iters"])
-    env = np.maximum(1e-9, 10 * np.abs(r4[:k] - r1[:k]) / r1[:k])
-    rel = np.abs(out["res_hist"][:k, 0] - r1[:k]) / r1[:k]
-    assert np.all(rel <= env), (rel, env)
+    check_fast_order(name, out["iters"], out["res_hist"][:, 0], out["X"], spec, g)
 
 
 # ---- configurations of the north star at sizes the oracle finishes in seconds ------
@@ -124,7 +115,7 @@ def _config_case(which):
 
 
 @pytest.mark.parametrize("which", ["C1", "C2_16cube", "C3_spd12"])
-def test_configs_reference_bitwise_and_fast_envelope(gpu_lib, oracle_lib, which):
+def test_configs_reference_bitwise_and_fast_bars(gpu_lib, oracle_lib, which):
     A, B, s, tol, maxiter = _config_case(which)
     X0 = np.zeros_like(B)
     orc = oracle_lib.bcbcg(A, B, X0, s, stop_mode=1, tol=tol, maxiter=maxiter)
@@ -137,13 +128,21 @@ def test_configs_reference_bitwise_and_fast_envelope(gpu_lib, oracle_lib, which)
     fast = gpu_lib.bcbcg(A, B, X0, s, stop_mode=1, tol=tol, maxiter=maxiter, order="fast")
     assert fast["status"] == "ok"
     assert abs(fast["iters"] - orc["iters"]) <= 1
+    # the bars of conftest.check_fast_order, every column, against the oracle
+    # at 1 rank and its 4-rank emulation (another summation order)
     x4 = np.linalg.norm(orc4["X"] - orc["X"]) / np.linalg.norm(orc["X"])
     xrel = np.linalg.norm(fast["X"] - orc["X"]) / np.linalg.norm(orc["X"])
-    assert xrel <= max(1e-10, 10 * x4), (xrel, x4)
     k = min(fast["iters"], orc["iters"], orc4["iters"])
     r1, r4, rf = orc["res_hist"][:k], orc4["res_hist"][:k], fast["res_hist"][:k]
-    env = np.maximum(1e-9, 10 * np.abs(r4 - r1) / r1)
-    assert np.all(np.abs(rf - r1) / r1 <= env)
+    dev, dev4 = np.abs(rf - r1) / r1, np.abs(r4 - r1) / r1
+    over = np.nonzero(dev4.max(axis=1) > 1e-10)[0]
+    pre = int(over[0]) if len(over) else k
+    print(f"{which}: iterations {fast['iters']} (oracle {orc['iters']}); residual deviation max "
+          f"{dev.max():.2e} (4-rank {dev4.max():.2e}), before iteration {pre} "
+          f"{dev[:pre].max() if pre else 0.0:.2e}; X {xrel:.2e} (4-rank {x4:.2e})", flush=True)
+    assert np.all(dev[:pre] <= 1e-9)
+    assert dev.max() <= max(1e-9, 2 * dev4.max())
+    assert xrel <= max(1e-10, 2 * x4), (xrel, x4)
     # converged means every column's true residual is at the tolerance too
     D = A.to_dense() if A.n <= 5000 else None
     if D is not None:

@@ -15,7 +15,10 @@ def cacg():
 
 def _rule(tmin, tmax, g):
     w = tmax - tmin
-    return max(tmin - 0.05 * w, 0.5 * tmin), min(tmax + 0.05 * w, g[1]) if g[1] > tmax else tmax + 0.05 * w
+    lo = tmin - 0.05 * w
+    if tmin > 0:
+        lo = max(lo, 0.5 * tmin)
+    return lo, min(tmax + 0.05 * w, g[1]) if g[1] > tmax else tmax + 0.05 * w
 
 
 def test_1d_laplacian_tridiagonal_closed_form(cacg):
@@ -60,3 +63,17 @@ def test_invariant_subspace_cuts_the_tridiagonal(cacg):
 def test_rejects_non_finite(cacg):
     with pytest.raises(ValueError):
         cacg.lanczos_interval(np.array([1.0, np.nan]), np.array([1.0, 1.0]), np.array([0, 2.0]))
+
+
+@pytest.mark.parametrize("shift", [-3.0, -1.0, 0.0])
+def test_nonpositive_spectrum_keeps_the_smallest_ritz_value_inside(cacg, shift):
+    # an indefinite or singular tridiagonal: the lower end stays at
+    # t_min - 0.05 w, below t_min (no clip towards t_min / 2)
+    k = 12
+    a = np.full(k, 2.0) + shift
+    b = np.full(k, -1.0)
+    T = np.diag(a) + np.diag(b[:-1], 1) + np.diag(b[:-1], -1)
+    ev = np.linalg.eigvalsh(T)
+    iv = cacg.lanczos_interval(a, b, np.array([-10.0, 1e9]))
+    assert iv[0] < ev[0] and iv[1] > ev[-1]
+    assert np.allclose(iv, _rule(ev[0], ev[-1], [-10.0, 1e9]), rtol=1e-11, atol=1e-13)

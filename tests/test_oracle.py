@@ -196,3 +196,23 @@ def test_breakdown_on_duplicate_columns(oracle_lib):
     B = np.concatenate([b, b], axis=1)
     out = oracle_lib.bcbcg(A, B, np.zeros_like(B), 2, eps=0.0, maxiter=5)
     assert out["status"] == "breakdown"
+
+
+def test_fast_order_bars_reject_planted_deviations():
+    # the bars of the fast-order parity tests (conftest.check_fast_order) pass
+    # the reference's own one-rank data and reject a trajectory that departs
+    # early, a residual one iteration off, or a perturbed X
+    from conftest import check_fast_order, golden_case
+    spec, A, B, X0, g = golden_case("bcbcg_2d32_m4_s4_tol1e-10")
+    r1 = np.sqrt(g["r2_p1"])
+    check_fast_order("as recorded", spec["iters_p1"], r1, g["X"], spec, g)
+    early = r1.copy()
+    early[3] *= 1 + 1e-8
+    with pytest.raises(AssertionError):
+        check_fast_order("early departure", spec["iters_p1"], early, g["X"], spec, g)
+    with pytest.raises(AssertionError):
+        check_fast_order("shifted", spec["iters_p1"], np.roll(r1, 1), g["X"], spec, g)
+    with pytest.raises(AssertionError):
+        check_fast_order("X off", spec["iters_p1"], r1, g["X"] * (1 + 1e-9), spec, g)
+    with pytest.raises(AssertionError):
+        check_fast_order("two iterations more", spec["iters_p1"] + 2, r1, g["X"], spec, g)
