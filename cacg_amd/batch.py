@@ -26,12 +26,14 @@ _vp = ctypes.c_void_p
 _dp = ctypes.POINTER(ctypes.c_double)
 _lp = ctypes.POINTER(ctypes.c_long)
 PRECISION = {"fp64": 0, "fp32": 1}
+POLICY = {"fifo": 0, "smallest": 1}
 
 
 class BatchOpts(ctypes.Structure):
     _fields_ = [("hbm_bytes", ctypes.c_long), ("slots", ctypes.c_int), ("s", ctypes.c_int),
                 ("precision", ctypes.c_int), ("maxiter", ctypes.c_long),
-                ("max_large", ctypes.c_int), ("graph", ctypes.c_int), ("tune", ctypes.c_int)]
+                ("max_large", ctypes.c_int), ("graph", ctypes.c_int), ("tune", ctypes.c_int),
+                ("policy", ctypes.c_int), ("max_bypass", ctypes.c_int)]
 
 
 class BatchResult(ctypes.Structure):
@@ -76,12 +78,13 @@ class Batch:
     """A batch of device-resident systems advanced together (cacg_batch_*)."""
 
     def __init__(self, s=4, precision="fp32", slots=1024, hbm_bytes=0, maxiter=1000,
-                 max_large=0, graph=False, tune=True):
+                 max_large=0, graph=False, tune=True, policy="fifo", max_bypass=0):
         o = BatchOpts()
         lib().cacg_batch_default_opts(ctypes.byref(o))
         o.hbm_bytes, o.slots, o.s = int(hbm_bytes), int(slots), int(s)
         o.precision, o.maxiter = PRECISION[precision], int(maxiter)
         o.max_large, o.graph, o.tune = int(max_large), int(bool(graph)), int(bool(tune))
+        o.policy, o.max_bypass = POLICY[policy], int(max_bypass)
         h = _vp()
         rc = lib().cacg_batch_create(ctypes.byref(h), ctypes.byref(o))
         if rc != 0:

@@ -45,6 +45,7 @@ typedef struct {
     int verify;          /* solo: this round recomputes the true residual only */
     int xpend;           /* solo: X += Q alpha of the last iteration still to apply */
     long n, nnz, m, iters, round_submitted, rounds_queued;
+    long bypassed;       /* queued: requests submitted after this one admitted before it */
     double tol, sa, sb, true_relres;
     int *rp, *ci;        /* host copies, int32 */
     double *val, *B, *X; /* X: X0 in, the solution out */
@@ -278,6 +279,8 @@ void cacg_batch_default_opts(cacg_batch_opts *o)
     o->max_large = 0;
     o->graph = 0;
     o->tune = 1;
+    o->policy = CACG_BATCH_FIFO;
+    o->max_bypass = 0;
 }
 
 static void free_req(breq *r)
@@ -383,7 +386,8 @@ int cacg_batch_create(cacg_batch **out, const cacg_batch_opts *opts)
         o.slots = 1024;
     if (o.slots < 1 || o.slots > 4096 || o.s < 1 || o.maxiter <= 0 || o.hbm_bytes < 0 ||
         o.max_large < 0 || (o.graph != 0 && o.graph != 1) ||
-        (o.tune != 0 && o.tune != 1) ||
+        (o.tune != 0 && o.tune != 1) || o.max_bypass < 0 ||
+        (o.policy != CACG_BATCH_FIFO && o.policy != CACG_BATCH_SMALLEST_FIRST) ||
         (o.precision != CACG_PREC_FP64 && o.precision != CACG_PREC_FP32_REFINE))
         return CACG_EINVAL;
     int cnt = 0;
@@ -932,7 +936,51 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
     for (int i = 0; i < b->nslots; i++)
         if (b->slot[i] && is_solo(b->slot[i]->n))
             nlarge++;
-    while (b->qhead < b->nreq) {
+    /* smallest first: among the queued requests the one of smallest footprint
+     * that may be admitted now (a large one only below max_large) goes next,
+     * unless the oldest queued request has been overtaken max_bypass times:
+     * then it goes next, or waits, and nothing overtakes it meanwhile */
+    const long maxb = b->o.max_bypass > 0 ? b->o.max_bypass : 16;
+    while (b->o.policy == CACG_BATCH_SMALLEST_FIRST) {
+        while (b->qhead < b->nreq &&
+               (!b->req[b->qhead] || b->req[b->qhead]->state != RQ_QUEUED))
+            b->qhead++;
+        if (b->qhead >= b->nreq)
+            break;
+        breq *head = b->req[b->qhead], *pick = NULL;
+        const int large_full = b->o.max_large > 0 && nlarge >= b->o.max_large;
+        if (head->bypassed >= maxb) {
+            if (is_solo(head->n) && large_full)
+                break;
+            pick = head;
+        } else {
+            for (long i = b->qhead; i < b->nreq; i++) {
+                breq *r = b->req[i];
+                if (!r || r->state != RQ_QUEUED || (is_solo(r->n) && large_full))
+                    continue;
+                if (!pick || r->bytes < pick->bytes)
+                    pick = r;
+            }
+        }
+        if (!pick)
+            break;
+        while (free_slot < b->nslots && b->slot[free_slot])
+            free_slot++;
+        if (free_slot == b->nslots)
+            break;
+        const int ok = admit(b, pick, free_slot);
+        if (ok < 0)
+            return -ok;
+        if (ok == 0)
+            break;
+        admitted++;
+        if (is_solo(pick->n))
+            nlarge++;
+        for (long i = b->qhead; i < pick->id; i++)
+            if (b->req[i] && b->req[i]->state == RQ_QUEUED)
+                b->req[i]->bypassed++;
+    }
+    while (b->o.policy == CACG_BATCH_FIFO && b->qhead < b->nreq) {
         breq *r = b->req[b->qhead];
         if (!r || r->state != RQ_QUEUED) {
             b->qhead++;

@@ -9,8 +9,9 @@
  *
  * Requests are queued by cacg_batch_submit (the matrix, B and X0 are copied,
  * so the caller's buffers can be reused at once).  Each cacg_batch_step is one
- * round: queued requests are admitted in FIFO order while a slot and their HBM
- * footprint are free (the head waits, later requests do not overtake it);
+ * round: queued requests are admitted while a slot and their HBM footprint are
+ * free, in FIFO order (the head waits, later requests do not overtake it) or
+ * smallest footprint first (cacg_batch_opts.policy);
  * then every admitted system advances one outer iteration, in launches shared
  * by all of them; systems that converged (every column's true residual below
  * tol * ||B(:,c)||), hit maxiter or broke down are retired and their memory is
@@ -47,7 +48,16 @@ typedef struct {
                         its SpMMs under candidate schedules (tile order, grid), timed on its
                         stream, and the rest under the fastest; 0: default schedules.  Same
                         bits */
+    int policy;      /* admission order: CACG_BATCH_FIFO (default) or
+                        CACG_BATCH_SMALLEST_FIRST (smallest HBM footprint first among the
+                        queued requests that may be admitted now, with max_bypass bounding how
+                        often a request is overtaken).  Results do not depend on it */
+    int max_bypass;  /* CACG_BATCH_SMALLEST_FIRST: a queued request overtaken by this many
+                        later ones is admitted next (and nothing overtakes it while it waits
+                        for memory or a slot); 0: 16 */
 } cacg_batch_opts;
+
+enum { CACG_BATCH_FIFO = 0, CACG_BATCH_SMALLEST_FIRST = 1 };
 
 typedef struct {
     int status;          /* CACG_OK, CACG_MAXITER or CACG_BREAKDOWN */

@@ -216,3 +216,44 @@ def test_graph_replays_give_the_launch_by_launch_bits(gpu_lib, precision):
         x, y = res[0][i], res[1][i]
         assert x["status"] == "ok" and y["status"] == "ok"
         assert x["iters"] == y["iters"] and np.array_equal(x["X"], y["X"]), i
+
+
+def _sjf_order(fps, max_bypass):
+    """Admission order of the smallest-first policy with one slot, every
+    request queued before the first round (a restatement of the rule)."""
+    queued = list(range(len(fps)))
+    byp = [0] * len(fps)
+    order = []
+    while queued:
+        head = queued[0]
+        pick = head if byp[head] >= max_bypass else min(queued, key=lambda i: (fps[i], i))
+        for i in queued:
+            if i < pick:
+                byp[i] += 1
+        queued.remove(pick)
+        order.append(pick)
+    return order
+
+
+@pytest.mark.parametrize("max_bypass", [2, 100])
+def test_smallest_first_admission(gpu_lib, max_bypass):
+    """One slot, six systems queued largest first: they are admitted (so they
+    retire) smallest footprint first, a request overtaken max_bypass times goes
+    next, and every result has the bits of the FIFO run."""
+    systems = [(P.laplacian_2d(L, 0.02), P.uniform(200 + L, L * L * 2).reshape(L * L, 2))
+               for L in (60, 52, 44, 36, 28, 20)]
+    fps = [BT.footprint(A.n, A.nnz, B.shape[1], 4) for A, B in systems]
+    fifo = _run(systems, s=4, slots=1, hbm_bytes=1 << 30)
+    b = BT.Batch(s=4, slots=1, hbm_bytes=1 << 30, policy="smallest", max_bypass=max_bypass)
+    ids = [b.submit(A, B, tol=TOL) for A, B in systems]
+    order, res = [], {}
+    while len(res) < len(ids):
+        b.step()
+        for rid in b.retired():
+            order.append(ids.index(rid))
+            res[rid] = b.collect(rid)
+    b.close()
+    assert order == _sjf_order(fps, max_bypass)
+    for k, rid in enumerate(ids):
+        assert res[rid]["status"] == "ok"
+        assert np.array_equal(res[rid]["X"], fifo[k]["X"])
