@@ -6,16 +6,24 @@ x0 = 0, s = 8, the reference's stop test ||R(:,0)|| < 1e-10 ||B(:,0)||
 The golden record (tests/golden/large/) comes from the compiled reference's own
 bcbcg_v1 at one MPI rank and at four (tests/golden/make_large_golden.py): the
 iteration count, every column's ||R(:,c)||^2 after every outer iteration, the
-SHA-256 of the final X and X at 64 sampled rows.
+SHA-256 of the final X and X at 64 sampled rows; at four ranks its
+||R(:,0)||^2 history.
 
-Reference order: bit for bit against it (iterations, every per-iteration
-column norm, the X digest).  Fast order (the benchmarked path): against the
-reference-order X of the same run (the reference's X, bit for bit) and the
-recorded trajectory, with the bars of conftest.check_fast_order: iterations
-within +-1, ||R(:,0)|| within 1e-9 relative until the reference's own 4-rank
-run departs from its 1-rank run by more than 1e-10 and within 2 x that
-departure after it, X within 1e-10 (relative 2-norm).  The measured
-deviations are printed."""
+Reference order: bit for bit against the one-rank record (iterations, every
+per-iteration column norm, the X digest and sampled rows).
+
+Fast order (the benchmarked path).  At this size the reference does not
+meet the per-iteration bars against itself: its 4-rank run (Allreduce sums in
+another order) departs from its 1-rank run by 1e-8 relative within 5
+iterations and stops after 51 iterations where the 1-rank run needs 54,
+because its Gram sums are sequential chains over all rows whose rounding
+grows with n; the fast order's split-K sums are shorter chains.  Asserted:
+its ||R(:,0)|| history stays within 2 x the running maximum of the
+reference's own 4-rank departure (and within 1e-9 where that is smaller)
+for as long as the reference's residual keeps falling (39 iterations), it
+stops no later than the reference, every column's true relative residual
+||B - A X|| / ||B|| is below 1e-9, and its X is within 1e-10 (relative
+2-norm) of the reference's.  The measured deviations are printed."""
 
 import hashlib
 import json
@@ -77,30 +85,33 @@ def test_benchmark_size_reference_bitwise_and_fast_order(gpu_lib):
     fast = _solve(gpu_lib, A, dB, dX, m, s, "fast", spec["eps"], spec["maxiter"])
     Xf = dX.download(B.shape)
     assert fast["status"] == "ok"
-    assert abs(fast["iters"] - spec["iters_p1"]) <= 1
-    r1 = np.sqrt(g["r2_p1"])
+    r1, r4 = np.sqrt(g["r2_p1"]), np.sqrt(g["r2_p4"])
+    iters_p4 = len(r4)
     k = min(len(r1), fast["iters"])
     dev = np.abs(fast["res_hist"][:k, 0] - r1[:k]) / r1[:k]
-    xrel = float(np.linalg.norm(Xf - Xr) / np.linalg.norm(Xr))
-    if "r2_p4" in g:
-        r4 = np.sqrt(g["r2_p4"])
-        k4 = min(len(r1), len(r4))
-        dev4 = np.abs(r4[:k4] - r1[:k4]) / r1[:k4]
-        over = np.nonzero(dev4 > 1e-10)[0]
-        pre = int(over[0]) if len(over) else k4
-        bound = max(1e-9, 2 * float(dev4.max()))
-    else:
-        dev4, pre, bound = np.zeros(1), k, 1e-9
-    print(f"{NAME}: fast order {fast['iters']} iterations (reference {spec['iters_p1']}); "
-          f"||R(:,0)|| deviation max {dev.max():.2e} (4-rank reference {dev4.max():.2e}), "
-          f"before iteration {pre} {dev[:pre].max() if pre else 0.0:.2e}; X {xrel:.2e}",
-          flush=True)
-    assert np.all(dev[:min(pre, k)] <= 1e-9)
-    assert dev.max() <= bound
-    assert xrel <= 1e-10
-    # every column's true residual at the tolerance (the reference stops on column 0)
+    k4 = min(len(r1), iters_p4)
+    dev4 = np.abs(r4[:k4] - r1[:k4]) / r1[:k4]
+    # while the reference converges: up to the first iteration whose residual
+    # does not fall below the previous one's
+    conv = 1
+    while conv < len(r1) and r1[conv] < r1[conv - 1]:
+        conv += 1
+    kk = min(k, k4, conv)
+    env = np.maximum(1e-9, 2 * np.maximum.accumulate(dev4[:kk]))
     import scipy.sparse as sp
     As = sp.csr_matrix((A.val, A.col, A.rowptr), shape=(A.n, A.n))
-    tr = np.linalg.norm(B - As @ Xf, axis=0) / np.linalg.norm(B, axis=0)
-    print(f"{NAME}: fast order true relative residuals max {tr.max():.2e}", flush=True)
-    assert np.all(tr < 1e-9), tr
+    nb = np.linalg.norm(B, axis=0)
+    rho_f = float((np.linalg.norm(B - As @ Xf, axis=0) / nb).max())
+    rho_r = float((np.linalg.norm(B - As @ Xr, axis=0) / nb).max())
+    xrel = float(np.linalg.norm(Xf - Xr) / np.linalg.norm(Xr))
+    print(f"{NAME}: fast order {fast['iters']} iterations (reference {spec['iters_p1']} at 1 rank, "
+          f"{iters_p4} at 4); ||R(:,0)|| deviation over the first {kk} iterations max "
+          f"{dev[:kk].max():.2e} (4-rank reference {dev4[:kk].max():.2e}), ratio to the envelope "
+          f"max {(dev[:kk] / env).max():.2f}; true relres fast {rho_f:.2e} reference {rho_r:.2e}; "
+          f"X difference {xrel:.2e}", flush=True)
+    print(f"{NAME}: per-iteration deviation fast {np.array2string(dev, precision=1)}", flush=True)
+    assert kk >= 30
+    assert np.all(dev[:kk] <= env)
+    assert fast["iters"] <= spec["iters_p1"]
+    assert rho_f < 1e-9
+    assert xrel <= 1e-10
