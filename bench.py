@@ -15,9 +15,11 @@ Prints one JSON line.  Per-kernel device times come from HIP events the
 library records on its own stream around every phase of every solve in the
 timed region; the dominant phase is priced against its roofline with the
 algorithmic bytes / flops of cacg_amd.metrics.  The CPU baseline times the
-C restatement of the reference solver (oracle/) on this host: setup plus two
-outer iterations of the same workload at one row-block rank per thread,
-projected to the GPU run's iteration count.
+reference's own solver (compiled from its sources by oracle/ref/Makefile, run
+with MPICH) on this host: setup plus two outer iterations of the same
+workload at 16 ranks, projected to the reference's own iteration count, and
+full CBCG solves of the 2D 128^2 Laplacian (1 RHS, s = 4, 1e-8) at 1 rank and
+16; without the reference build or MPI, the C restatement (oracle/).
 
 Beside the headline line (unless --no-forms), the same JSON object carries
 the two further forms, each measured in this run:
@@ -88,7 +90,18 @@ def copy_probe():
 
 
 def cpu_baseline(A, B, s, iters_gpu, threads):
+    """The reference's own solver (oracle/_ref/ref_harness, compiled from the
+    reference's sources, MPICH) on this host's cores: a bounded sample of the
+    benchmark solve at `threads` ranks plus full solves of the CBCG
+    configuration; the C restatement with threads as ranks where the
+    reference build or MPI is not available."""
     import oracle
+    if oracle.reference_available() and os.path.exists(oracle.MPIEXEC):
+        try:
+            return reference_baseline(A, B, s, threads)
+        except Exception as e:  # no MPI launch possible on this host: the restatement
+            print(f"bench: reference baseline unavailable ({e}); timing the C restatement",
+                  file=sys.stderr, flush=True)
     if not os.path.exists(oracle.LIB_PATH):
         oracle.build()
     t_setup, t_it = oracle.time_sample(A, B, np.zeros_like(B), s, threads, 2)
@@ -102,6 +115,48 @@ def cpu_baseline(A, B, s, iters_gpu, threads):
                    f"on this host: setup {t_setup:.2f}s + outer iterations 0 and 1 "
                    f"({t_it[0]:.2f}s, {t_it[1]:.2f}s) of the same workload, projected to "
                    f"{iters_gpu} iterations = {proj:.1f}s per solve"),
+    }
+
+
+def reference_baseline(A, B, s, ranks):
+    """Benchmark solve at `ranks` MPI ranks, stopped after outer iterations 0
+    and 1 (epsilon just above the reference's recorded ||R(:,0)|| after
+    iteration 1), projected to the reference's own iteration count at 4 ranks
+    (51, tests/golden/large); and full CBCG solves of the 2D 128^2 Laplacian
+    (1 RHS U[0,1) seed 0, s = 4, 1e-8) at 1 rank and at `ranks`."""
+    import oracle
+    from cacg_amd import problems as P
+    gdir = os.path.join(ROOT, "tests", "golden", "large")
+    name = "bcbcg_3d128_m16_s8_tol1e-10"
+    with np.load(os.path.join(gdir, f"{name}.npz")) as z:
+        r2 = z["r2_p1"]
+    with np.load(os.path.join(gdir, f"{name}_p4.npz")) as z:
+        iters_ref = len(z["r2_p4"])
+    eps = float(np.sqrt(r2[1])) * (1 + 1e-6)
+    print(f"bench: reference solver, benchmark workload at {ranks} ranks, 2 iterations",
+          file=sys.stderr, flush=True)
+    r = oracle.time_reference(A, B, np.zeros_like(B), s, eps, "bcbcg", ranks, timeout=900)
+    if r["iters"] != 2:
+        raise RuntimeError(f"sample ran {r['iters']} iterations, not 2")
+    t0, t1 = r["t"][0], r["t"][1] - r["t"][0]
+    proj = t0 + (iters_ref - 1) * t1
+    A1 = P.laplacian_2d(128)
+    b1 = P.rhs_uniform(A1.n, 1, 0)
+    eps1 = 1e-8 * float(np.linalg.norm(b1[:, 0]))
+    c1 = {}
+    for nr in (1, ranks):
+        q = oracle.time_reference(A1, b1, np.zeros_like(b1), 4, eps1, "cbcg", nr, timeout=300)
+        c1[f"ranks_{nr}"] = {"seconds": q["t"][-1], "iters": q["iters"]}
+    return {
+        "value": 1.0 / proj,
+        "unit": "solves/s",
+        "cores": ranks,
+        "kind": "reference",
+        "sample": (f"the reference's bcbcg_v1 (compiled from its sources, -O, MPICH) at {ranks} MPI "
+                   f"ranks on this host, same workload: setup + outer iteration 0 {t0:.2f}s, "
+                   f"iteration 1 {t1:.2f}s, projected to its own {iters_ref} iterations at 1e-10 "
+                   f"(4-rank count) = {proj:.1f}s per solve"),
+        "c1_full_solve": c1,
     }
 
 
@@ -239,6 +294,7 @@ def main():
     sv.close()
     if not args.no_forms:
         out["dropin_host_arrays"] = dropin_host_arrays(A, B, m, s, args, out["ms_per_step"])
+        out["c1_cbcg"] = c1_cbcg()
         out["precision_form"] = precision_form(A, B, X, dB, dX, m, s, args, out["ms_per_step"])
         out["batching_form"] = batching_form()
         out["serving_form"] = serving_form(args.serve_requests, args.serve_rate)
@@ -252,9 +308,13 @@ def main():
 
 
 def dropin_host_arrays(A, B, m, s, args, ms_device):
-    """The same solve through the host-array API (cacg_bcbcg, what bcbcg_v1 calls): solver
-    creation (HBM allocation, int32 CSR upload, SpMM autotuning), upload of B and X, the solve,
-    download of X, teardown, timed end to end (PCIe included)."""
+    """The same solve through the host-array API (cacg_bcbcg): solver
+    creation (HBM allocation, int32 CSR upload, SpMM autotuning), upload of B
+    and X, the solve, download of X, teardown, timed end to end (PCIe
+    included); then one call of the drop-in entry point bcbcg_v1 itself as a
+    caller of the reference links it (csrType_local / denseType by value, its
+    stop test ||R(:,0)|| < 1e-10 ||B(:,0)||), in its default reference
+    summation order and with CACG_ORDER=fast."""
     import cacg_amd
     lat = []
     for k in range(2):
@@ -265,10 +325,90 @@ def dropin_host_arrays(A, B, m, s, args, ms_device):
             lat.append(time.perf_counter() - t)
     ms = float(np.mean(lat) * 1e3)
     print(f"bench: host-array drop-in {ms:.1f} ms per solve", file=sys.stderr, flush=True)
-    return {"workload": "same solve through cacg_bcbcg with host arrays: create, upload A/B/X, "
-                        "solve, download X, destroy",
-            "ms_per_solve": ms, "solves_per_s": 1e3 / ms, "device_only_ms": ms_device,
-            "status": r["status"], "iters": r["iters"]}
+    out = {"workload": "same solve through cacg_bcbcg with host arrays: create, upload A/B/X, "
+                       "solve, download X, destroy",
+           "ms_per_solve": ms, "solves_per_s": 1e3 / ms, "device_only_ms": ms_device,
+           "status": r["status"], "iters": r["iters"]}
+    eps = args.tol * float(np.linalg.norm(B[:, 0]))
+    for order in ("reference", "fast"):
+        t, it, st = dropin_call(A, B, s, eps, order)
+        out[f"bcbcg_v1_{order}_order"] = {"ms_per_solve": t * 1e3, "iters": it, "status": st}
+        print(f"bench: bcbcg_v1 ({order} order) {t * 1e3:.1f} ms, {it} iterations",
+              file=sys.stderr, flush=True)
+    return out
+
+
+def dropin_call(A, B, s, eps, order):
+    """One call of bcbcg_v1 (the reference's signature, by-value structs) with
+    host arrays; returns (seconds, iterations, status)."""
+    import ctypes
+    import cacg_amd
+
+    class CsrType(ctypes.Structure):
+        _fields_ = [("num_rows", ctypes.c_long), ("num_cols", ctypes.c_long),
+                    ("nnz", ctypes.c_long), ("start", ctypes.c_long),
+                    ("row_start", ctypes.c_void_p), ("col_idx", ctypes.c_void_p),
+                    ("csrdata", ctypes.c_void_p)]
+
+    class DenseType(ctypes.Structure):
+        _fields_ = [("local_num_row", ctypes.c_long), ("local_num_col", ctypes.c_long),
+                    ("global_num_row", ctypes.c_long), ("global_num_col", ctypes.c_long),
+                    ("start_idx", ctypes.c_long), ("data", ctypes.c_void_p)]
+
+    L = cacg_amd.lib()
+    fn = L.bcbcg_v1
+    fn.argtypes = [CsrType, DenseType, DenseType, ctypes.c_long, ctypes.c_double, ctypes.c_int,
+                   ctypes.c_int]
+    fn.restype = None
+    rp = np.ascontiguousarray(A.rowptr, dtype=np.int64)
+    ci = np.ascontiguousarray(A.col, dtype=np.int64)
+    va = np.ascontiguousarray(A.val, dtype=np.float64)
+    Bc = np.ascontiguousarray(B)
+    X = np.zeros_like(Bc)
+    n, m = Bc.shape
+    old = {k: os.environ.get(k) for k in ("CACG_ORDER", "CACG_QUIET")}
+    os.environ["CACG_QUIET"] = "1"
+    os.environ["CACG_ORDER"] = order
+    try:
+        t = time.perf_counter()
+        fn(CsrType(n, n, A.nnz, 0, rp.ctypes.data, ci.ctypes.data, va.ctypes.data),
+           DenseType(n, m, n, m, 0, Bc.ctypes.data), DenseType(n, m, n, m, 0, X.ctypes.data),
+           s, eps, 0, 1)
+        t = time.perf_counter() - t
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    return t, int(L.cacg_last_iterations()), cacg_amd.STATUS.get(L.cacg_last_status(), "?")
+
+
+def c1_cbcg():
+    """The CBCG configuration on the GPU: 2D 5-point Laplacian 128^2, 1 RHS
+    U[0,1) seed 0, x0 = 0, s = 4, Chebyshev interval from 20 Lanczos steps,
+    stop at 1e-8 relative residual; mean of 5 device-resident solves after one."""
+    import cacg_amd
+    from cacg_amd import problems as P
+    L = cacg_amd.lib()
+    A = P.laplacian_2d(128)
+    b = P.rhs_uniform(A.n, 1, 0)
+    sv = cacg_amd.Solver(A, 1, 4, tol=1e-8, maxiter=1000, order="fast", bounds="lanczos",
+                         lanczos_steps=20)
+    dB, dX = cacg_amd.DeviceBuffer.from_array(b), cacg_amd.DeviceBuffer(b.nbytes)
+    lat = []
+    for k in range(6):
+        L.cacg_k_fill_f64(dX.ptr, A.n, 0.0, None)
+        L.cacg_dev_sync()
+        t = time.perf_counter()
+        rep = sv.solve(dB, dX)
+        if k > 0:
+            lat.append(time.perf_counter() - t)
+    sv.close()
+    return {"workload": "CBCG, 2D 5-point Laplacian 128^2, 1 RHS U[0,1) seed 0, s=4, "
+                        "Lanczos interval (20 steps), 1e-8 relres",
+            "ms_per_solve": float(np.mean(lat) * 1e3), "iters": rep["iters"],
+            "status": rep["status"]}
 
 
 def precision_form(A, B, X64, dB, dX, m, s, args, ms64):
@@ -379,25 +519,40 @@ def batching_form(count=256):
 
 
 def serving_form(count, rate):
-    """A Poisson stream: continuous batching vs one request at a time."""
+    """Poisson streams at several offered loads: continuous batching (smallest
+    HBM footprint first, at most two large systems in flight) against the
+    same stream solved one request at a time (FIFO) with the FP64 path.  The
+    lowest rate is below the capacity of either server (a few solves/s on
+    this mix), so latency there is service time; the higher ones build a
+    backlog.  `rate` is the rate also reported at the top level."""
     from cacg_amd import batch as BT
-    stream = BT.serving_stream(count, rate, seed=4)
-    b = BT.Batch(s=4, precision="fp64", hbm_bytes=96 << 30, maxiter=3000)
-    res, wall = BT.serve_stream(stream, b, tol=1e-10)
-    b.close()
-    seq, wall_seq = BT.serve_sequential(stream, tol=1e-10, s=4, maxiter=3000)
-    lb = np.array([r["latency"] for r in res]) * 1e3
-    ls = np.array([r["latency"] for r in seq]) * 1e3
-    print(f"bench: serving form {count / wall:.2f} solves/s vs {count / wall_seq:.2f} sequential",
-          file=sys.stderr, flush=True)
-    return {"workload": f"{count} requests, Poisson {rate:g}/s (seed 4), 2D L 64..512 or 3D edge "
-                        "32..128, 1..32 RHS, s=4, 1e-10, FP64 basis",
-            "solves_per_s": count / wall, "p50_ms": float(np.percentile(lb, 50)),
-            "p99_ms": float(np.percentile(lb, 99)),
-            "converged": int(sum(r["status"] == "ok" for r in res)),
-            "sequential_fp64": {"solves_per_s": count / wall_seq,
-                                "p50_ms": float(np.percentile(ls, 50)),
-                                "p99_ms": float(np.percentile(ls, 99))}}
+    rates = sorted({2.0, 50.0, float(rate), 2000.0})
+    sweep = []
+    for r in rates:
+        cnt = 16 if r < 10 else count
+        stream = BT.serving_stream(cnt, r, seed=4)
+        b = BT.Batch(s=4, precision="fp64", hbm_bytes=96 << 30, maxiter=3000, policy="smallest",
+                     max_large=2)
+        res, wall = BT.serve_stream(stream, b, tol=1e-10)
+        b.close()
+        seq, wall_seq = BT.serve_sequential(stream, tol=1e-10, s=4, maxiter=3000)
+        lb = np.array([q["latency"] for q in res]) * 1e3
+        ls = np.array([q["latency"] for q in seq]) * 1e3
+        row = {"rate_per_s": r, "requests": cnt, "solves_per_s": cnt / wall,
+               "p50_ms": float(np.percentile(lb, 50)), "p99_ms": float(np.percentile(lb, 99)),
+               "converged": int(sum(q["status"] == "ok" for q in res)),
+               "sequential_fp64": {"solves_per_s": cnt / wall_seq,
+                                   "p50_ms": float(np.percentile(ls, 50)),
+                                   "p99_ms": float(np.percentile(ls, 99))}}
+        sweep.append(row)
+        print(f"bench: serving form at {r:g}/s: {row['solves_per_s']:.2f} solves/s p50 "
+              f"{row['p50_ms']:.0f} ms vs sequential {row['sequential_fp64']['solves_per_s']:.2f} "
+              f"p50 {row['sequential_fp64']['p50_ms']:.0f} ms", file=sys.stderr, flush=True)
+    top = next(x for x in sweep if x["rate_per_s"] == float(rate))
+    return dict(top, workload=(f"Poisson streams (seed 4) of 2D L 64..512 and 3D edge 32..128 "
+                               "Laplacians, 1..32 RHS, s=4, 1e-10, FP64 basis; continuous batching, "
+                               "smallest footprint first, at most 2 large systems in flight"),
+                sweep=sweep)
 
 
 def measured_traffic(kernel):

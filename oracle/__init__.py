@@ -224,6 +224,44 @@ def reference_available() -> bool:
     return os.path.exists(REF_HARNESS)
 
 
+def time_reference(A, B, X0, s, eps, solver="bcbcg", nranks=1, timeout=600, workdir=None):
+    """Wall-clock profile of the reference's own solver (the compiled
+    oracle/_ref/ref_harness) at `nranks` MPI ranks: the harness stamps every
+    outer iteration with the seconds since the solver was called (rank 0,
+    after MPI_Init and its input was read), so t[k] = setup + iterations
+    0..k.  Returns dict(iters, t (list of seconds), wall (seconds of the whole
+    run, input reading and output writing included)).  The run keeps no
+    snapshots."""
+    import re
+    import time as _time
+    if not reference_available():
+        raise RuntimeError("reference build missing (make -C oracle ref)")
+    B = _c64(B).reshape(A.n, -1)
+    n, m = B.shape
+    X0 = _c64(X0).reshape(n, m)
+    with tempfile.TemporaryDirectory(dir=workdir) as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(b"CACGIN01")
+            f.write(struct.pack("<5q", n, A.nnz, m, s, {"bcg": 0, "cbcg": 1, "bcbcg": 2}[solver]))
+            f.write(struct.pack("<d", eps))
+            f.write(np.ascontiguousarray(A.rowptr, dtype="<i8").tobytes())
+            f.write(np.ascontiguousarray(A.col, dtype="<i8").tobytes())
+            f.write(_c64(A.val).tobytes())
+            f.write(B.tobytes())
+            f.write(X0.tobytes())
+        cmd = [REF_HARNESS, fin, fout]
+        if nranks > 1:
+            cmd = [MPIEXEC, "-n", str(nranks)] + cmd
+        env = dict(os.environ, CACG_REF_NOSNAP="1", CACG_REF_PROGRESS="1")
+        t0 = _time.perf_counter()
+        r = subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE,
+                           text=True, cwd=td, timeout=timeout, env=env)
+        wall = _time.perf_counter() - t0
+    stamps = [float(x) for x in re.findall(r"iteration \d+ r2 \S+ at (\S+) s", r.stderr)]
+    return {"iters": len(stamps), "t": stamps, "wall": wall}
+
+
 def run_reference(A, B, X0, s, eps, solver="bcbcg", nranks=1, snapshots=True, timeout=600,
                   progress=False, workdir=None):
     """Run the reference's own bcbcg_v1 / cbcg_v1 / bcg_v1 at `nranks` MPI

@@ -95,7 +95,7 @@ void __wrap_norm2square_dist_denseMat_col_n(denseType mat, double *result, long 
         fflush(g_cols);
     }
     if (g_progress && myid == 0) {
-        fprintf(stderr, "ref_harness: iteration %ld r2 %.17g at %.1f s\n", g_iters, *result,
+        fprintf(stderr, "ref_harness: iteration %ld r2 %.17g at %.6f s\n", g_iters, *result,
                 wall() - g_t0);
         fflush(stderr);
     }
