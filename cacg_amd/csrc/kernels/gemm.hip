@@ -52,7 +52,10 @@ __device__ __forceinline__ void load_row(const T *p, double *out)
 
 // ---- reference order ----------------------------------------------------------
 
-template <typename TS, typename T, bool BETA_LDS>
+// RPT > 1 (sm divides 256): a thread updates RPT rows of one column, rows
+// 256 / sm apart, with one read of beta(q, c) for all of them; each entry
+// still sums q = 0, 1, ... in order.
+template <typename TS, typename T, bool BETA_LDS, int RPT>
 __global__ __launch_bounds__(256) void qupdate_ref_kernel(int n, int sm, const TS *__restrict__ S,
                                                           T *Q, const double *__restrict__ beta,
                                                           int rb)
@@ -74,13 +77,35 @@ __global__ __launch_bounds__(256) void qupdate_ref_kernel(int n, int sm, const T
         for (int e = t; e < rows * sm; e += 256)
             qt[e] = (double)Q[i0 * sm + e];
         __syncthreads();
-        for (int e = t; e < rows * sm; e += 256) {
-            const int r = e / sm, c = e % sm;
-            const double *qr = qt + (long)r * sm;
-            double acc = 0.0;
-            for (int q = 0; q < sm; q++)
-                acc = add_rn(acc, mul_rn(B[(long)q * sm + c], qr[q]));
-            Q[i0 * sm + e] = (T)sub_rn(S ? (double)S[i0 * sm + e] : 0.0, acc);
+        if (RPT > 1) {
+            const int P = 256 / sm, c = t % sm, r0 = t / sm;
+            double acc[RPT];
+#pragma unroll
+            for (int u = 0; u < RPT; u++)
+                acc[u] = 0.0;
+            for (int q = 0; q < sm; q++) {
+                const double bq = B[(long)q * sm + c];
+#pragma unroll
+                for (int u = 0; u < RPT; u++)
+                    acc[u] = add_rn(acc[u], mul_rn(bq, qt[(long)(r0 + u * P) * sm + q]));
+            }
+#pragma unroll
+            for (int u = 0; u < RPT; u++) {
+                const int r = r0 + u * P;
+                if (r < rows) {
+                    const long e = (long)r * sm + c;
+                    Q[i0 * sm + e] = (T)sub_rn(S ? (double)S[i0 * sm + e] : 0.0, acc[u]);
+                }
+            }
+        } else {
+            for (int e = t; e < rows * sm; e += 256) {
+                const int r = e / sm, c = e % sm;
+                const double *qr = qt + (long)r * sm;
+                double acc = 0.0;
+                for (int q = 0; q < sm; q++)
+                    acc = add_rn(acc, mul_rn(B[(long)q * sm + c], qr[q]));
+                Q[i0 * sm + e] = (T)sub_rn(S ? (double)S[i0 * sm + e] : 0.0, acc);
+            }
         }
         __syncthreads();
     }
@@ -593,19 +618,27 @@ int qupdate(int n, int sm, const TS *S, T *Q, const double *beta, int mode, hipS
         default: break;  // other widths: the reference-order kernel
         }
     }
-    const int rb = sm >= 256 ? 1 : 256 / sm;
+    // 4 rows per thread when sm divides 256 and beta plus the tile fit in LDS
+    const bool rows4 = sm <= 256 && 256 % sm == 0 &&
+                       (long)sm * sm + 4L * (256 / sm) * sm <= 20480 - 256;
+    const int rb = rows4 ? 4 * (256 / sm) : (sm >= 256 ? 1 : 256 / sm);
     const bool beta_lds = (long)sm * sm + (long)rb * sm <= 20480 - 256;
     const size_t shm = ((beta_lds ? (size_t)sm * sm : 0) + (size_t)rb * sm) * sizeof(double);
     const int ntile = (n + rb - 1) / rb;
     const int G = ntile < 1024 ? ntile : 1024;
-    cacg_allow_lds(beta_lds ? qupdate_ref_kernel<TS, T, true> : qupdate_ref_kernel<TS, T, false>,
-                   shm);
-    if (beta_lds)
-        hipLaunchKernelGGL((qupdate_ref_kernel<TS, T, true>), dim3(G), dim3(256), shm, st, n, sm, S,
-                           Q, beta, rb);
+#define CACG_QREF(BL, RPT)                                                                        \
+    do {                                                                                          \
+        cacg_allow_lds(qupdate_ref_kernel<TS, T, BL, RPT>, shm);                                  \
+        hipLaunchKernelGGL((qupdate_ref_kernel<TS, T, BL, RPT>), dim3(G), dim3(256), shm, st, n,  \
+                           sm, S, Q, beta, rb);                                                   \
+    } while (0)
+    if (rows4)
+        CACG_QREF(true, 4);
+    else if (beta_lds)
+        CACG_QREF(true, 1);
     else
-        hipLaunchKernelGGL((qupdate_ref_kernel<TS, T, false>), dim3(G), dim3(256), shm, st, n, sm,
-                           S, Q, beta, rb);
+        CACG_QREF(false, 1);
+#undef CACG_QREF
     CACG_LAUNCH_CHECK();
     return 0;
 }

@@ -4,7 +4,7 @@
 // Reference order: one lane per entry G[a][b], one sequential chain over all
 // rows in row order, product and sum rounded separately: the reference's
 // summation at one rank, bit for bit.  The chain bounds its time at n
-// dependent steps (a few ms at n = 2M) whatever the number of entries.
+// dependent steps whatever the number of entries.
 //
 // Fast order: split-K.  Workgroup x owns a contiguous chunk of rows and a
 // group of 16x16 output tiles per wave; FP64 MFMA v_mfma_f64_16x16x4f64 with
@@ -19,20 +19,22 @@
 
 namespace {
 
-// One wave per (row a of G, slab of 64 consecutive columns b of [R1 | R2]):
-// lane b keeps the chain of G[a][b].  The chain is sequential in the row
-// index, so the time of the kernel is n steps of one product and one
-// dependent sum per lane whatever the grid; the work of a step is one
-// coalesced 8-byte load per lane of R(i, b) and the broadcast L(i, a).  The
-// R values are loaded GREF_U rows ahead into registers (two alternating
-// batches), and the L values of 64 rows are gathered one per lane a stage
-// ahead, written to LDS and read back as broadcasts, so neither load waits
-// on the chain.
-constexpr int GREF_U = 32;   // rows per register batch (two batches in flight)
+// One wave per 8 x 8 block of G: lane l keeps the chain of G[a0 + l / 8][b0 +
+// l % 8].  The chain is sequential in the row index, so the kernel takes n
+// steps of one product and one dependent sum per lane whatever the grid; the
+// data a step needs are the 8 values of L and the 8 of [R1 | R2] in the
+// block's columns (128 bytes a row, against 520 for a wave of 64 columns of
+// one row a, so far fewer bytes travel per row).  Rows go in stages of 64:
+// lane r loads row i0 + r's 16 values (16-byte loads where the block is
+// aligned) GREF_NS stages ahead into a register ring, writes them to LDS when
+// its stage comes, and every lane reads its L and R value of each row from
+// there.  One wave per workgroup: LDS order within the wave is program order.
+constexpr int GREF_NS = 4;   // stages in flight, 64 rows each
 
 struct GrefSrc {
     const double *L;
     long ldl;
+    int kl;
     const double *R1;
     long ld1;
     int kr1;
@@ -41,65 +43,94 @@ struct GrefSrc {
     int kr2;
 };
 
-// R(i, b) of GREF_U consecutive rows from row i0, clamped to the last row
-__device__ __forceinline__ void gref_load(double *buf, const double *rp, long ldr, long i0,
-                                          long last)
+// address of [R1 | R2](i, b)
+__device__ __forceinline__ const double *gref_r(const GrefSrc &s, long i, int b)
 {
+    return b < s.kr1 ? s.R1 + i * s.ld1 + b : s.R2 + i * s.ld2 + (b - s.kr1);
+}
+
+// row i's 8 L values (columns a0..) and 8 R values (columns b0..), clamped to
+// the last column; VEC: the block lies in one source with 16-byte aligned rows
+template <bool VEC>
+__device__ __forceinline__ void gref_row(const GrefSrc &s, long i, int a0, int b0, double *v)
+{
+    if (VEC) {
+        const double2 *lp = reinterpret_cast<const double2 *>(s.L + i * s.ldl + a0);
+        const double2 *rp = reinterpret_cast<const double2 *>(gref_r(s, i, b0));
 #pragma unroll
-    for (int u = 0; u < GREF_U; u++) {
-        const long i = i0 + u < last ? i0 + u : last;
-        buf[u] = rp[i * ldr];
+        for (int q = 0; q < 4; q++) {
+            const double2 x = lp[q], y = rp[q];
+            v[2 * q] = x.x;
+            v[2 * q + 1] = x.y;
+            v[8 + 2 * q] = y.x;
+            v[8 + 2 * q + 1] = y.y;
+        }
+    } else {
+        const int kr = s.kr1 + s.kr2;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int a = a0 + q < s.kl ? a0 + q : s.kl - 1;
+            const int b = b0 + q < kr ? b0 + q : kr - 1;
+            v[q] = s.L[i * s.ldl + a];
+            v[8 + q] = *gref_r(s, i, b);
+        }
     }
 }
 
-// the chain over GREF_U rows whose L values sit at ls[0..GREF_U)
-__device__ __forceinline__ double gref_chain(double acc, const double *buf, const double *ls)
-{
-#pragma unroll
-    for (int u = 0; u < GREF_U; u++)
-        acc = add_rn(acc, mul_rn(buf[u], ls[u]));
-    return acc;
-}
-
-__global__ __launch_bounds__(64) void gram_ref_kernel(int n, GrefSrc s, int nslab,
+template <bool VEC>
+__global__ __launch_bounds__(64) void gram_ref_kernel(int n, GrefSrc s, int nbb,
                                                       double *__restrict__ G1,
                                                       double *__restrict__ G2)
 {
-    __shared__ double ls[64];
+    __shared__ double stg[2][64][17];   // row r: L values 0..7, R values 8..15 (odd stride)
     const int lane = threadIdx.x;
-    const int a = blockIdx.x / nslab;
-    const int kr = s.kr1 + s.kr2;
-    const int b0 = (blockIdx.x % nslab) * 64 + lane;
-    const int b = b0 < kr ? b0 : kr - 1;   // idle lanes repeat the last column, unstored
-    const double *rp = b < s.kr1 ? s.R1 + b : s.R2 + (b - s.kr1);
-    const long ldr = b < s.kr1 ? s.ld1 : s.ld2;
-    const double *lp = s.L + a;
+    const int a0 = (blockIdx.x / nbb) * 8, b0 = (blockIdx.x % nbb) * 8;
+    const int ai = lane >> 3, bi = lane & 7;
     const long last = (long)n - 1;
-    const long nfull = (long)n / 64 * 64;
+    const long nst = ((long)n + 63) / 64;
+    double ring[GREF_NS][16];
+    auto load = [&](double *v, long st) __attribute__((always_inline)) {
+        const long i = st * 64 + lane;
+        gref_row<VEC>(s, i < last ? i : last, a0, b0, v);
+    };
+#pragma unroll
+    for (int q = 0; q < GREF_NS; q++)
+        if (q < nst)
+            load(ring[q], q);
     double acc = 0.0;
-    if (nfull > 0) {
-        double bufA[GREF_U], bufB[GREF_U];
-        double lnext = lp[(long)lane * s.ldl];
-        gref_load(bufA, rp, ldr, 0, last);
-        for (long i0 = 0; i0 < nfull; i0 += 64) {
-            // this stage's L values to LDS, the next stage's into a register
+    for (long st0 = 0; st0 < nst; st0 += GREF_NS) {
+#pragma unroll
+        for (int q = 0; q < GREF_NS; q++) {
+            const long st = st0 + q;
+            if (st >= nst)
+                break;
+            const int buf = (int)(st & 1);
             __builtin_amdgcn_wave_barrier();
-            ls[lane] = lnext;
-            const long ln = i0 + 64 + lane;
-            lnext = lp[(ln < last ? ln : last) * s.ldl];
+#pragma unroll
+            for (int u = 0; u < 16; u++)
+                stg[buf][lane][u] = ring[q][u];
             __builtin_amdgcn_wave_barrier();
-            gref_load(bufB, rp, ldr, i0 + GREF_U, last);
-            acc = gref_chain(acc, bufA, ls);
-            gref_load(bufA, rp, ldr, i0 + 2 * GREF_U, last);
-            acc = gref_chain(acc, bufB, ls + GREF_U);
+            if (st + GREF_NS < nst)
+                load(ring[q], st + GREF_NS);
+            const long i0 = st * 64;
+            const int rows = n - i0 < 64 ? (int)(n - i0) : 64;
+            if (rows == 64) {
+#pragma unroll 16
+                for (int r = 0; r < 64; r++)
+                    acc = add_rn(acc, mul_rn(stg[buf][r][ai], stg[buf][r][8 + bi]));
+            } else {
+                for (int r = 0; r < rows; r++)
+                    acc = add_rn(acc, mul_rn(stg[buf][r][ai], stg[buf][r][8 + bi]));
+            }
         }
     }
-    for (long i = nfull; i < n; i++)
-        acc = add_rn(acc, mul_rn(rp[i * ldr], lp[i * s.ldl]));
-    if (b0 < s.kr1)
-        G1[(long)a * s.kr1 + b0] = acc;
-    else if (b0 < kr)
-        G2[(long)a * s.kr2 + (b0 - s.kr1)] = acc;
+    const int a = a0 + ai, b = b0 + bi;
+    if (a < s.kl) {
+        if (b < s.kr1)
+            G1[(long)a * s.kr1 + b] = acc;
+        else if (b < s.kr1 + s.kr2)
+            G2[(long)a * s.kr2 + (b - s.kr1)] = acc;
+    }
 }
 
 // ---- fast: split-K FP64 MFMA ------------------------------------------------
@@ -655,10 +686,27 @@ extern "C" int cacg_k_gram_ref(int n, const double *L, long ldl, int kl, const d
         return (int)hipErrorInvalidValue;
     if (kr2 > 0 && !R2)
         return (int)hipErrorInvalidValue;
-    const int nslab = (kr1 + kr2 + 63) / 64;
-    const GrefSrc s = {L, ldl, R1, ld1, kr1, R2, ld2, kr2};
-    hipLaunchKernelGGL(gram_ref_kernel, dim3((unsigned)kl * nslab), dim3(64), 0, st, n, s, nslab,
-                       G1, G2);
+    if (n == 0) {
+        // empty sums: zero blocks
+        int rc = kr1 > 0 ? cacg_k_fill_f64(G1, (long)kl * kr1, 0.0, st) : 0;
+        if (!rc && kr2 > 0)
+            rc = cacg_k_fill_f64(G2, (long)kl * kr2, 0.0, st);
+        return rc;
+    }
+    const int kr = kr1 + kr2;
+    const int nab = (kl + 7) / 8, nbb = (kr + 7) / 8;
+    const GrefSrc s = {L, ldl, kl, R1, ld1, kr1, R2, ld2, kr2};
+    // 16-byte loads when every 8 x 8 block is whole, inside one source, with
+    // 16-byte aligned rows
+    const bool vec = kl % 8 == 0 && kr1 % 8 == 0 && kr2 % 8 == 0 && ((uintptr_t)L & 15) == 0 &&
+                     ldl % 2 == 0 && (kr1 == 0 || (((uintptr_t)R1 & 15) == 0 && ld1 % 2 == 0)) &&
+                     (kr2 == 0 || (((uintptr_t)R2 & 15) == 0 && ld2 % 2 == 0));
+    if (vec)
+        hipLaunchKernelGGL(gram_ref_kernel<true>, dim3((unsigned)(nab * nbb)), dim3(64), 0, st, n,
+                           s, nbb, G1, G2);
+    else
+        hipLaunchKernelGGL(gram_ref_kernel<false>, dim3((unsigned)(nab * nbb)), dim3(64), 0, st, n,
+                           s, nbb, G1, G2);
     CACG_LAUNCH_CHECK();
     return 0;
 }
