@@ -295,6 +295,7 @@ def main():
     if not args.no_forms:
         out["dropin_host_arrays"] = dropin_host_arrays(A, B, m, s, args, out["ms_per_step"])
         out["c1_cbcg"] = c1_cbcg()
+        out["spmm_long_rows"] = spmm_long_rows()
         out["precision_form"] = precision_form(A, B, X, dB, dX, m, s, args, out["ms_per_step"])
         out["batching_form"] = batching_form()
         out["serving_form"] = serving_form(args.serve_requests, args.serve_rate)
@@ -382,6 +383,50 @@ def dropin_call(A, B, s, eps, order):
             else:
                 os.environ[k] = v
     return t, int(L.cacg_last_iterations()), cacg_amd.STATUS.get(L.cacg_last_status(), "?")
+
+
+def spmm_long_rows(reps=20):
+    """SpMM on an irregular matrix: a 1024^2-grid 5-point pattern plus 64 hub
+    rows of power-law length (up to 19,824 entries; problems.power_law_spd),
+    k = 16 (a basis step's width) and k = 128 (A Q's), every row vector-per-row
+    against the hub rows through the row-split kernel; the same bits asserted.
+    Time per launch over `reps` back-to-back launches; GB/s from the
+    algorithmic bytes 12 nnz + 4 (n + 1) + 16 n k."""
+    import ctypes
+    import cacg_amd
+    from cacg_amd import problems as P
+    L = cacg_amd.lib()
+    A = P.power_law_spd(1024, 64, 20000, 7)
+    nnz = len(A.col)
+    d = np.diff(A.rowptr)
+    out = {"matrix": f"power-law SPD, n={A.n}, nnz={nnz}, rows {int(d.min())}..{int(d.max())}, "
+                     f"mean {d.mean():.2f}"}
+    for k in (16, 128):
+        X = ((np.arange(A.n * k) % 1013) * 1e-3 - 0.5).reshape(A.n, k)
+        dX = cacg_amd.DeviceBuffer.from_array(X)
+        res, Y = {}, {}
+        for label, lm in (("vector_per_row", 0), ("row_split", None)):
+            dA = cacg_amd.DeviceCSR(A, lm)
+            dY = cacg_amd.DeviceBuffer(X.nbytes)
+            args = (ctypes.byref(dA.s), k, dX.ptr, k, dY.ptr, k, 0, 0.0, 0.0, None, k, None, k,
+                    None, k, None, None)
+            if L.cacg_k_spmm_x(*args) != 0:
+                raise RuntimeError("spmm launch failed")
+            L.cacg_dev_sync()
+            t = time.perf_counter()
+            for _ in range(reps):
+                L.cacg_k_spmm_x(*args)
+            L.cacg_dev_sync()
+            ms = (time.perf_counter() - t) / reps * 1e3
+            by = 12 * nnz + 4 * (A.n + 1) + 16 * A.n * k
+            res[label] = {"ms": ms, "gbps": by / (ms * 1e-3) / 1e9, "long_rows": dA.n_long}
+            Y[label] = dY.download((A.n, k))
+        res["same_bits"] = bool(np.array_equal(Y["vector_per_row"], Y["row_split"]))
+        res["speedup"] = res["vector_per_row"]["ms"] / res["row_split"]["ms"]
+        out[f"k{k}"] = res
+    print(f"bench: long rows k=16 {out['k16']['speedup']:.1f}x, k=128 {out['k128']['speedup']:.1f}x",
+          file=sys.stderr, flush=True)
+    return out
 
 
 def c1_cbcg():

@@ -57,7 +57,8 @@ class Report(ctypes.Structure):
 
 
 class DCsr(ctypes.Structure):
-    _fields_ = [("n", ctypes.c_int), ("rowptr", _vp), ("col", _vp), ("val", _vp)]
+    _fields_ = [("n", ctypes.c_int), ("rowptr", _vp), ("col", _vp), ("val", _vp),
+                ("long_rows", _vp), ("n_long", ctypes.c_int), ("long_min", ctypes.c_int)]
 
 
 # cacg_scsr: the same layout with FP32 values
@@ -120,6 +121,8 @@ def lib() -> ctypes.CDLL:
         "cacg_k_cheb1_f32": ([ctypes.POINTER(DCsr), c_int, _vp, c_long, _vp, c_long, _vp, c_long,
                               c_double, c_double, _vp], c_int),
         "cacg_k_cvt_f64_f32": ([c_long, _vp, _vp, _vp], c_int),
+        "cacg_csr_long_min": ([c_long, _vp], c_int),
+        "cacg_csr_long_rows": ([c_long, _vp, c_int, _vp], c_long),
         "cacg_k_qupdate_s32": ([c_int, c_int, _vp, _vp, _vp, _vp], c_int),
         "cacg_k_cvt_f32_f64": ([c_long, _vp, _vp, _vp], c_int),
         "cacg_k_qupdate_zero": ([c_int, c_int, _vp, _vp, _vp], c_int),
@@ -199,15 +202,34 @@ class DeviceBuffer:
             pass
 
 
-class DeviceCSR:
-    """The int32-index device copy of a problems.CSR (as the solver stores it)."""
+def long_rows(A, long_min=None):
+    """(long_min, ids): the rows the row-split SpMM takes, by the library's rule
+    (cacg_csr_long_min) or from a given threshold (0: none)."""
+    rp = np.ascontiguousarray(A.rowptr, dtype=np.int64)
+    L = lib()
+    lm = L.cacg_csr_long_min(A.n, rp.ctypes.data) if long_min is None else int(long_min)
+    cnt = L.cacg_csr_long_rows(A.n, rp.ctypes.data, lm, None)
+    ids = np.zeros(max(cnt, 1), dtype=np.int32)
+    L.cacg_csr_long_rows(A.n, rp.ctypes.data, lm, ids.ctypes.data)
+    return lm, ids[:cnt]
 
-    def __init__(self, A):
+
+class DeviceCSR:
+    """The int32-index device copy of a problems.CSR (as the solver stores it),
+    with its long-row list (long_min: None = the library's rule, 0 = none)."""
+
+    def __init__(self, A, long_min=None):
         self.n = A.n
         self.rowptr = DeviceBuffer.from_array(np.asarray(A.rowptr, dtype=np.int32))
         self.col = DeviceBuffer.from_array(np.asarray(A.col, dtype=np.int32))
         self.val = DeviceBuffer.from_array(np.asarray(A.val, dtype=np.float64))
-        self.s = DCsr(A.n, self.rowptr.ptr, self.col.ptr, self.val.ptr)
+        lm, ids = long_rows(A, long_min)
+        self.n_long = len(ids)
+        self.long_min = lm if self.n_long else 0
+        self.lrows = DeviceBuffer.from_array(ids) if self.n_long else None
+        lp = self.lrows.ptr if self.lrows else None
+        self.s = DCsr(A.n, self.rowptr.ptr, self.col.ptr, self.val.ptr, lp, self.n_long,
+                      self.long_min)
         self._s32 = None
 
     @property
@@ -218,7 +240,8 @@ class DeviceCSR:
             nnz = int(self.val.nbytes // 8)
             self.val32 = DeviceBuffer(4 * max(nnz, 1))
             _check(lib().cacg_k_cvt_f64_f32(nnz, self.val.ptr, self.val32.ptr, None), "cvt")
-            self._s32 = SCsr(self.n, self.rowptr.ptr, self.col.ptr, self.val32.ptr)
+            self._s32 = SCsr(self.n, self.rowptr.ptr, self.col.ptr, self.val32.ptr, self.s.long_rows,
+                             self.n_long, self.long_min)
         return self._s32
 
 
@@ -236,10 +259,12 @@ class SpmmSched(ctypes.Structure):
     _fields_ = [("grid", ctypes.c_int), ("chunk_rows", ctypes.c_int), ("passes", ctypes.c_int)]
 
 
-def k_spmm(A, X, epi="none", a=0.0, b=0.0, p1=None, p2=None, want_copy0=False, sched=None):
+def k_spmm(A, X, epi="none", a=0.0, b=0.0, p1=None, p2=None, want_copy0=False, sched=None,
+           long_min=None):
     """Y = epilogue(A X) for X n x k; returns Y (and copy0 for 'cheb1').
-    sched = (grid, chunk_rows[, passes]) selects a launch schedule (cacg_k_spmm_x)."""
-    dA = DeviceCSR(A)
+    sched = (grid, chunk_rows[, passes]) selects a launch schedule (cacg_k_spmm_x);
+    long_min the row-split threshold (None: the library's rule, 0: no row split)."""
+    dA = DeviceCSR(A, long_min)
     X = np.ascontiguousarray(X, dtype=np.float64).reshape(A.n, -1)
     k = X.shape[1]
     dX = DeviceBuffer.from_array(X)
@@ -391,9 +416,9 @@ def k_xrupdate_split(Q, AQ, alpha, X, R, which):
     return dR.download((n, m)), dn.download(m)
 
 
-def k_spmm_f32(A, X, epi="none", a=0.0, b=0.0, p1=None, p2=None):
+def k_spmm_f32(A, X, epi="none", a=0.0, b=0.0, p1=None, p2=None, long_min=None):
     """FP32 Y = epilogue(A32 X) for X n x k float32 ('none' or 'cheb')."""
-    dA = DeviceCSR(A)
+    dA = DeviceCSR(A, long_min)
     X = np.ascontiguousarray(X, dtype=np.float32).reshape(A.n, -1)
     k = X.shape[1]
     dX = DeviceBuffer.from_array(X)
@@ -406,9 +431,9 @@ def k_spmm_f32(A, X, epi="none", a=0.0, b=0.0, p1=None, p2=None):
     return dY.download((A.n, k), np.float32)
 
 
-def k_cheb1_f32(A, R, a, b):
+def k_cheb1_f32(A, R, a, b, long_min=None):
     """(S_1, S_0) of the FP32 basis from the FP64 residual R (n x k)."""
-    dA = DeviceCSR(A)
+    dA = DeviceCSR(A, long_min)
     R = np.ascontiguousarray(R, dtype=np.float64).reshape(A.n, -1)
     k = R.shape[1]
     dR = DeviceBuffer.from_array(R)

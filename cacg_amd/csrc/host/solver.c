@@ -52,6 +52,7 @@ struct cacg_solver {
     hipStream_t st;
     cacg_dcsr A;
     int *d_rowptr, *d_col;
+    int *d_long_rows;   /* rows with at least A.long_min nonzeros (row-split SpMM) */
     double *d_val;
     double *d_R, *d_S, *d_Q, *d_AQ;
     /* precision form: FP32 values and basis */
@@ -312,7 +313,7 @@ int cacg_solver_destroy(cacg_solver *sv)
                     sv->d_Q,      sv->d_AQ,    sv->d_QtAQ,  sv->d_QtAS,    sv->d_QtR,
                     sv->d_alpha,  sv->d_beta,  sv->d_norm2, sv->d_bounds,  sv->d_gwork,
                     sv->d_gram_work, sv->d_xr_work, sv->d_lu_work, sv->d_flag,
-                    sv->f_val,    sv->f_S,     sv->d_lz_work, sv->d_lz_ab};
+                    sv->f_val,    sv->f_S,     sv->d_lz_work, sv->d_lz_ab, sv->d_long_rows};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++)
         if (ptrs[i])
             hipFree(ptrs[i]);
@@ -478,6 +479,29 @@ int cacg_solver_create(cacg_solver **out, long n, const long *row_start, const l
     sv->A.rowptr = sv->d_rowptr;
     sv->A.col = sv->d_col;
     sv->A.val = sv->d_val;
+    {
+        /* rows far longer than the mean go to the row-split SpMM */
+        const int lmin = cacg_csr_long_min(n, row_start);
+        const long nl = cacg_csr_long_rows(n, row_start, lmin, NULL);
+        if (nl > 0) {
+            int *ids = (int *)malloc(sizeof(int) * (size_t)nl);
+            if (!ids || dalloc(sv, (void **)&sv->d_long_rows, nl * (long)sizeof(int)) != CACG_OK) {
+                free(ids);
+                cacg_solver_destroy(sv);
+                return CACG_ENOMEM;
+            }
+            cacg_csr_long_rows(n, row_start, lmin, ids);
+            e = hipMemcpy(sv->d_long_rows, ids, sizeof(int) * (size_t)nl, hipMemcpyHostToDevice);
+            free(ids);
+            if (e != hipSuccess) {
+                cacg_solver_destroy(sv);
+                return CACG_EDEVICE;
+            }
+            sv->A.long_rows = sv->d_long_rows;
+            sv->A.n_long = (int)nl;
+            sv->A.long_min = lmin;
+        }
+    }
     if (prec32) {
         if (cacg_k_cvt_f64_f32(sv->nnz, sv->d_val, sv->f_val, sv->st) != 0 ||
             hipStreamSynchronize(sv->st) != hipSuccess) {
@@ -488,9 +512,36 @@ int cacg_solver_create(cacg_solver **out, long n, const long *row_start, const l
         sv->A32.rowptr = sv->d_rowptr;
         sv->A32.col = sv->d_col;
         sv->A32.val = sv->f_val;
+        sv->A32.long_rows = sv->A.long_rows;
+        sv->A32.n_long = sv->A.n_long;
+        sv->A32.long_min = sv->A.long_min;
     }
     *out = sv;
     return CACG_OK;
+}
+
+/* ---- long rows ------------------------------------------------------------------ */
+
+int cacg_csr_long_min(long n, const long *row_start)
+{
+    if (n <= 0)
+        return 256;
+    const long nnz = row_start[n] - row_start[0];
+    const long mean = (nnz + n - 1) / n;
+    const long t = 16 * mean > 256 ? 16 * mean : 256;
+    return t < 0x7fffffffL ? (int)t : 0x7fffffff;
+}
+
+long cacg_csr_long_rows(long n, const long *row_start, int long_min, int *out)
+{
+    long c = 0;
+    for (long i = 0; i < n; i++)
+        if (long_min > 0 && row_start[i + 1] - row_start[i] >= long_min) {
+            if (out)
+                out[c] = (int)i;
+            c++;
+        }
+    return c;
 }
 
 /* ---- phase timing: events recorded between phases on the solver's stream ---- */

@@ -181,7 +181,7 @@ __device__ __forceinline__ void tile_rows(const int *s_rp, const int *s_col, con
                                           const TX *__restrict__ x, long ldx, TY *__restrict__ y,
                                           long ldy, TV a, TV b, const TY *__restrict__ p1,
                                           long ld1, const TY *__restrict__ p2, long ld2,
-                                          TY *__restrict__ copy0, long ldc)
+                                          TY *__restrict__ copy0, long ldc, int long_min)
 {
     constexpr int NG = 256 / LPR;
     constexpr int UNR = spmm_unroll<LPR>();
@@ -198,7 +198,10 @@ __device__ __forceinline__ void tile_rows(const int *s_rp, const int *s_col, con
             if (li >= rows)
                 continue;
             const long i = r0 + li;
-            const int pb = s_rp[li] - pbase, pe = s_rp[li + 1] - pbase;
+            const int pb = s_rp[li] - pbase, pe0 = s_rp[li + 1] - pbase;
+            // a long row is the row-split kernel's: no gathers, no stores
+            const bool lng = long_min > 0 && pe0 - pb >= long_min;
+            const int pe = lng ? pb : pe0;
             TV acc[VW], e1[VW], e2[VW];
 #pragma unroll
             for (int v = 0; v < VW; v++) {
@@ -243,11 +246,13 @@ __device__ __forceinline__ void tile_rows(const int *s_rp, const int *s_col, con
                     out = sub_rn(e1[v], acc[v]);
                 } else if (EPI == CACG_EPI_CHEB1) {
                     out = add_rn(mul_rn(a, acc[v]), mul_rn(b, e1[v]));
-                    __builtin_nontemporal_store((TY)e1[v], copy0 + i * ldc + col(v));
+                    if (!lng)
+                        __builtin_nontemporal_store((TY)e1[v], copy0 + i * ldc + col(v));
                 } else {
                     out = sub_rn(add_rn(mul_rn(a, acc[v]), mul_rn(b, e1[v])), e2[v]);
                 }
-                __builtin_nontemporal_store((TY)out, y + i * ldy + col(v));
+                if (!lng)
+                    __builtin_nontemporal_store((TY)out, y + i * ldy + col(v));
             }
         }
     }
@@ -269,7 +274,7 @@ __attribute__((amdgpu_waves_per_eu(spmm_min_waves<TV, LPR, EPI>()))) void spmm_k
     int n, const int *__restrict__ rowptr, const int *__restrict__ colidx,
     const TV *__restrict__ val, int k, const TX *__restrict__ x, long ldx, TY *__restrict__ y,
     long ldy, double ad, double bd, const TY *__restrict__ p1, long ld1, const TY *__restrict__ p2,
-    long ld2, TY *__restrict__ copy0, long ldc, int P, int Q)
+    long ld2, TY *__restrict__ copy0, long ldc, int P, int Q, int long_min)
 {
     constexpr int NG = 256 / LPR, TR = NG * RPG;
     constexpr int CAP = TR * 8 > 256 ? TR * 8 : 256;
@@ -301,7 +306,7 @@ __attribute__((amdgpu_waves_per_eu(spmm_min_waves<TV, LPR, EPI>()))) void spmm_k
         __syncthreads();
         tile_rows<TV, TX, TY, LPR, VW, RPG, EPI, IL>(s_rp, s_col, s_val, staged, pbase, r0, rows, g,
                                                  lane, colidx, val, k, x, ldx, y, ldy, (TV)ad,
-                                                 (TV)bd, p1, ld1, p2, ld2, copy0, ldc);
+                                                 (TV)bd, p1, ld1, p2, ld2, copy0, ldc, long_min);
         __syncthreads();
     }
 }
@@ -318,7 +323,7 @@ __global__ __launch_bounds__(256) void spmm_pipe_kernel(
     int n, const int *__restrict__ rowptr, const int *__restrict__ colidx,
     const TV *__restrict__ val, int k, const TX *__restrict__ x, long ldx, TY *__restrict__ y,
     long ldy, double ad, double bd, const TY *__restrict__ p1, long ld1, const TY *__restrict__ p2,
-    long ld2, TY *__restrict__ copy0, long ldc, int P, int Q)
+    long ld2, TY *__restrict__ copy0, long ldc, int P, int long_min)
 {
     constexpr int NG = 256 / LPR, TR = NG * RPG;
     constexpr int CAP = TR * 8 > 256 ? TR * 8 : 256;
@@ -329,8 +334,9 @@ __global__ __launch_bounds__(256) void spmm_pipe_kernel(
     __shared__ TV s_val[2][CAP];
     // one pass: the multi-pass order is not built into the pipelined form (with
     // it, hipcc spills SGPRs in these kernels and fails instruction selection
-    // in the FP64 k = 17..32 one)
-    Q = 1;
+    // in the FP64 k = 17..32 one; for the same reason the kernel takes no Q
+    // argument beside long_min)
+    const int Q = 1;
     const int t = threadIdx.x, g = t / LPR, lane = t % LPR;
     const int ntiles = (n + TR - 1) / TR;
     const int nx = gridDim.x >= 8 ? 8 : 1;
@@ -393,7 +399,8 @@ __global__ __launch_bounds__(256) void spmm_pipe_kernel(
         }
         tile_rows<TV, TX, TY, LPR, VW, RPG, EPI, IL>(s_rp[cur], s_col[cur], s_val[cur], staged, pbase,
                                                  r0, rows, g, lane, colidx, val, k, x, ldx, y, ldy,
-                                                 (TV)ad, (TV)bd, p1, ld1, p2, ld2, copy0, ldc);
+                                                 (TV)ad, (TV)bd, p1, ld1, p2, ld2, copy0, ldc,
+                                                 long_min);
         if (tile_n < 0)
             break;
         if (tnn <= CAP) {
@@ -415,6 +422,103 @@ __global__ __launch_bounds__(256) void spmm_pipe_kernel(
     }
 }
 
+// Row split for long rows.  A row with far more nonzeros than the mean holds
+// its lane group for one chain of batched gathers per few nonzeros (two
+// dependent global latencies per batch of 4 or 8), and the launch ends with
+// that workgroup.  Rows with at least long_min nonzeros are left out of the
+// kernels above and taken here, one workgroup per row: waves 1-3 form the
+// products val(p) * x(col(p), c) of the next segment of the row into LDS,
+// every product rounded as above, while the lanes c < kc of wave 0 add the
+// current segment's in CSR order, each keeping the chain of column cb + c.
+// Same products, same order: the same bits as the vector-per-row kernels,
+// with one LDS read per nonzero on the chain instead of a global gather.
+// Columns go 64 at a time (cb = 0, 64, ...), each block one pass over the row.
+constexpr int LR_SEGE = 4096;                  // products per LDS buffer (two buffers)
+constexpr int LR_J = (LR_SEGE + 191) / 192;    // products per producer lane and segment
+
+template <typename TV, typename TX, typename TY, int EPI>
+__global__ __launch_bounds__(256) void spmm_long_kernel(
+    const int *__restrict__ long_rows, const int *__restrict__ rowptr,
+    const int *__restrict__ colidx, const TV *__restrict__ val, int k, const TX *__restrict__ x,
+    long ldx, TY *__restrict__ y, long ldy, double ad, double bd, const TY *__restrict__ p1,
+    long ld1, const TY *__restrict__ p2, long ld2, TY *__restrict__ copy0, long ldc)
+{
+    __shared__ TV prod[2 * LR_SEGE];   // two segment buffers
+    const int t = threadIdx.x;
+    const long i = long_rows[blockIdx.x];
+    const int pb = rowptr[i], pe = rowptr[i + 1];
+    const TV a = (TV)ad, b = (TV)bd;
+    for (int cb = 0; cb < k; cb += 64) {
+        const int kc = k - cb < 64 ? k - cb : 64;
+        const int seg = LR_SEGE / kc;   // nonzeros per segment
+        const int nseg = (pe - pb + seg - 1) / seg;
+        // products of segment sg into buffer sg & 1 (producer lanes t >= 64):
+        // all (col, val) loads of the lane, then all its x gathers, then the
+        // products, so each step's loads are in flight together
+        auto produce = [&](int sg) {
+            TV *buf = prod + (sg & 1) * LR_SEGE;
+            const int p0 = pb + sg * seg;
+            const int cnt = (pe - p0 < seg ? pe - p0 : seg) * kc;
+            const int tp = t - 64;
+            int cc[LR_J];
+            TV vv[LR_J];
+#pragma unroll
+            for (int j = 0; j < LR_J; j++) {
+                const int e = tp + 192 * j;
+                cc[j] = e < cnt ? colidx[p0 + e / kc] : 0;
+                vv[j] = e < cnt ? val[p0 + e / kc] : (TV)0;
+            }
+            TX xv[LR_J];
+#pragma unroll
+            for (int j = 0; j < LR_J; j++) {
+                const int e = tp + 192 * j;
+                xv[j] = e < cnt ? x[(long)cc[j] * ldx + cb + e % kc] : (TX)0;
+            }
+#pragma unroll
+            for (int j = 0; j < LR_J; j++) {
+                const int e = tp + 192 * j;
+                if (e < cnt)
+                    buf[e] = mul_rn(vv[j], (TV)xv[j]);
+            }
+        };
+        if (t >= 64 && nseg > 0)
+            produce(0);
+        __syncthreads();
+        TV acc = (TV)0;
+        for (int sg = 0; sg < nseg; sg++) {
+            if (t >= 64) {
+                if (sg + 1 < nseg)
+                    produce(sg + 1);
+            } else if (t < kc) {
+                const TV *buf = prod + (sg & 1) * LR_SEGE;
+                const int p0 = pb + sg * seg;
+                const int cnt = pe - p0 < seg ? pe - p0 : seg;
+#pragma unroll 8
+                for (int q = 0; q < cnt; q++)
+                    acc = add_rn(acc, buf[q * kc + t]);
+            }
+            __syncthreads();
+        }
+        if (t < kc) {
+            const int c = cb + t;
+            TV out;
+            if (EPI == CACG_EPI_NONE) {
+                out = acc;
+            } else if (EPI == CACG_EPI_RESID) {
+                out = sub_rn((TV)p1[i * ld1 + c], acc);
+            } else if (EPI == CACG_EPI_CHEB1) {
+                const TV e1 = (TV)x[i * ldx + c];
+                out = add_rn(mul_rn(a, acc), mul_rn(b, e1));
+                copy0[i * ldc + c] = (TY)e1;
+            } else {
+                out = sub_rn(add_rn(mul_rn(a, acc), mul_rn(b, (TV)p1[i * ld1 + c])),
+                             (TV)p2[i * ld2 + c]);
+            }
+            y[i * ldy + c] = (TY)out;
+        }
+    }
+}
+
 // Launch shape (measured on the 128^3 Laplacian, k = 16 and k = 128): narrow
 // rows (LPR <= 8) one row per lane group and 32 workgroups per CU; wide rows
 // two rows per lane group and 16 workgroups per CU.
@@ -422,7 +526,7 @@ template <typename TV, typename TX, typename TY, int LPR, int VW>
 int launch_spmm(int n, const int *rowptr, const int *col, const TV *val, int k, const TX *x,
                 long ldx, TY *y, long ldy, int epi, double a, double b, const TY *p1, long ld1,
                 const TY *p2, long ld2, TY *copy0, long ldc, const cacg_spmm_sched *sc,
-                hipStream_t st)
+                const int *long_rows, int n_long, int long_min, hipStream_t st)
 {
     constexpr int RPG = LPR >= 16 ? 2 : 1;
     constexpr int TR = (256 / LPR) * RPG;
@@ -438,16 +542,21 @@ int launch_spmm(int n, const int *rowptr, const int *col, const TV *val, int k, 
         P = sc->chunk_rows >= TR ? sc->chunk_rows / TR : 1;
         Q = sc->passes > 1 ? sc->passes : 1;
     }
+    const int lmin = n_long > 0 ? long_min : 0;
 #define CACG_SPMM_LAUNCH(E, IL)                                                                  \
     do {                                                                                         \
+        if (lmin > 0)                                                                            \
+            hipLaunchKernelGGL((spmm_long_kernel<TV, TX, TY, E>), dim3(n_long), dim3(256), 0,    \
+                               st, long_rows, rowptr, col, val, k, x, ldx, y, ldy, a, b, p1, ld1, \
+                               p2, ld2, copy0, ldc);                                             \
         if constexpr (LPR == 4 || LPR == 8)                                                      \
             hipLaunchKernelGGL((spmm_pipe_kernel<TV, TX, TY, LPR, VW, RPG, E, IL>), dim3(G),     \
                                dim3(256), 0, st, n, rowptr, col, val, k, x, ldx, y, ldy, a, b, p1, \
-                               ld1, p2, ld2, copy0, ldc, P, Q);                                  \
+                               ld1, p2, ld2, copy0, ldc, P, lmin);                               \
         else                                                                                     \
             hipLaunchKernelGGL((spmm_kernel<TV, TX, TY, LPR, VW, RPG, E, IL>), dim3(G), dim3(256), \
                                0, st, n, rowptr, col, val, k, x, ldx, y, ldy, a, b, p1, ld1, p2,    \
-                               ld2, copy0, ldc, P, Q);                                           \
+                               ld2, copy0, ldc, P, Q, lmin);                                     \
     } while (0)
     // FP64 with 4 columns per lane: interleaved pairs when every lane's block is full
 #define CACG_SPMM_CASE(E)                                                                        \
@@ -499,9 +608,9 @@ template <typename TV, typename TX, typename TY>
 int dispatch_spmm(int n, const int *rowptr, const int *col, const TV *val, int k, const TX *x,
                   long ldx, TY *y, long ldy, int epi, double a, double b, const TY *p1, long ld1,
                   const TY *p2, long ld2, TY *copy0, long ldc, const cacg_spmm_sched *sc,
-                  hipStream_t st)
+                  const int *long_rows, int n_long, int long_min, hipStream_t st)
 {
-#define CACG_SPMM_ARGS n, rowptr, col, val, k, x, ldx, y, ldy, epi, a, b, p1, ld1, p2, ld2, copy0, ldc, sc, st
+#define CACG_SPMM_ARGS n, rowptr, col, val, k, x, ldx, y, ldy, epi, a, b, p1, ld1, p2, ld2, copy0, ldc, sc, long_rows, n_long, long_min, st
     const bool vec4 = k % 4 == 0 && ((uintptr_t)x & 15) == 0 && (ldx * (long)sizeof(TX)) % 16 == 0;
     if (vec4) {
         // LPR = k / 4 rounded up to a power of two, at most 32 (wider k loops)
@@ -599,6 +708,15 @@ __global__ __launch_bounds__(256) void gersh_final(int nblk, const double *__res
     }
 }
 
+// the long-row fields of a CSR: a negative count or threshold, or long rows
+// without their list
+template <typename CSR>
+bool bad_long_rows(const CSR *A)
+{
+    return A->n_long < 0 || A->long_min < 0 ||
+           (A->long_min > 0 && A->n_long > 0 && !A->long_rows) || A->n_long > A->n;
+}
+
 }  // namespace
 
 extern "C" int cacg_k_spmm_x(const cacg_dcsr *A, int k, const double *x, long ldx, double *y,
@@ -606,13 +724,14 @@ extern "C" int cacg_k_spmm_x(const cacg_dcsr *A, int k, const double *x, long ld
                              const double *p2, long ld2, double *copy0, long ldc,
                              const cacg_spmm_sched *sc, hipStream_t st)
 {
-    if (!A || k <= 0 || A->n < 0 || (sc && (sc->grid < 0 || sc->chunk_rows < 0 || sc->passes < 0)))
+    if (!A || k <= 0 || A->n < 0 || bad_long_rows(A) ||
+        (sc && (sc->grid < 0 || sc->chunk_rows < 0 || sc->passes < 0)))
         return (int)hipErrorInvalidValue;
     if (A->n == 0)
         return 0;
     return dispatch_spmm<double, double, double>(A->n, A->rowptr, A->col, A->val, k, x, ldx, y,
                                                  ldy, epi, a, b, p1, ld1, p2, ld2, copy0, ldc, sc,
-                                                 st);
+                                                 A->long_rows, A->n_long, A->long_min, st);
 }
 
 extern "C" int cacg_k_spmm(const cacg_dcsr *A, int k, const double *x, long ldx, double *y,
@@ -628,13 +747,14 @@ extern "C" int cacg_k_spmm_f32_x(const cacg_scsr *A, int k, const float *x, long
                                  const float *p2, long ld2, const cacg_spmm_sched *sc,
                                  hipStream_t st)
 {
-    if (!A || k <= 0 || A->n < 0 || (epi != CACG_EPI_NONE && epi != CACG_EPI_CHEB) ||
+    if (!A || k <= 0 || A->n < 0 || bad_long_rows(A) || (epi != CACG_EPI_NONE && epi != CACG_EPI_CHEB) ||
         (sc && (sc->grid < 0 || sc->chunk_rows < 0 || sc->passes < 0)))
         return (int)hipErrorInvalidValue;
     if (A->n == 0)
         return 0;
     return dispatch_spmm<float, float, float>(A->n, A->rowptr, A->col, A->val, k, x, ldx, y, ldy,
-                                              epi, a, b, p1, ld1, p2, ld2, NULL, 0, sc, st);
+                                              epi, a, b, p1, ld1, p2, ld2, NULL, 0, sc,
+                                              A->long_rows, A->n_long, A->long_min, st);
 }
 
 extern "C" int cacg_k_spmm_f32(const cacg_scsr *A, int k, const float *x, long ldx, float *y,
@@ -648,13 +768,14 @@ extern "C" int cacg_k_cheb1_f32_x(const cacg_scsr *A, int k, const double *r, lo
                                   float *s1, long ld1, float *s0, long ld0, double a, double b,
                                   const cacg_spmm_sched *sc, hipStream_t st)
 {
-    if (!A || k <= 0 || A->n < 0 || !s0 || (sc && (sc->grid < 0 || sc->chunk_rows < 0 || sc->passes < 0)))
+    if (!A || k <= 0 || A->n < 0 || !s0 || bad_long_rows(A) ||
+        (sc && (sc->grid < 0 || sc->chunk_rows < 0 || sc->passes < 0)))
         return (int)hipErrorInvalidValue;
     if (A->n == 0)
         return 0;
     return dispatch_spmm<float, double, float>(A->n, A->rowptr, A->col, A->val, k, r, ldr, s1,
                                                ld1, CACG_EPI_CHEB1, a, b, NULL, 0, NULL, 0, s0,
-                                               ld0, sc, st);
+                                               ld0, sc, A->long_rows, A->n_long, A->long_min, st);
 }
 
 extern "C" int cacg_k_cheb1_f32(const cacg_scsr *A, int k, const double *r, long ldr, float *s1,

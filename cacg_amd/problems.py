@@ -157,6 +157,47 @@ def random_spd_like(A: CSR, seed: int) -> CSR:
     return CSR(n, A.rowptr.copy(), A.col.copy(), val)
 
 
+def power_law_spd(L: int, hubs: int, dmax: int, seed: int, alpha: float = 1.0) -> CSR:
+    """An irregular SPD matrix: the 5-point pattern of an L x L grid plus `hubs`
+    hub vertices, hub h (vertex (h * n) // hubs) joined to d_h = max(1,
+    floor(dmax / (h + 1)^alpha)) other vertices, so the row lengths follow a
+    power law from dmax + 5 down to the grid's 5.  Hub neighbours: stream
+    outputs of SplitMix64(seed) taken mod n in order (duplicates and the hub
+    itself dropped).  Values as random_spd_like: off-diagonals -U(0.5, 1.5)
+    drawn per upper-triangle edge in CSR order with seed + 1 and mirrored,
+    diagonal 1.01 x the absolute off-diagonal row sum (strictly diagonally
+    dominant, so SPD)."""
+    G = laplacian_2d(L)
+    n = G.n
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(G.rowptr))
+    up = G.col > rows
+    keys = [rows[up] * n + G.col[up]]
+    off = 0
+    for h in range(hubs):
+        v = (h * n) // hubs
+        d = max(1, int(dmax / (h + 1) ** alpha))
+        nb = (splitmix64(seed, d, off) % np.uint64(n)).astype(np.int64)
+        off += d
+        nb = nb[nb != v]
+        lo, hi = np.minimum(nb, v), np.maximum(nb, v)
+        keys.append(lo * n + hi)
+    ukeys = np.unique(np.concatenate(keys))            # upper edges, CSR order
+    draws = 0.5 + uniform(seed + 1, len(ukeys))
+    ui, uj = ukeys // n, ukeys % n
+    # both triangles, then the diagonal
+    r = np.concatenate([ui, uj, np.arange(n, dtype=np.int64)])
+    c = np.concatenate([uj, ui, np.arange(n, dtype=np.int64)])
+    v = np.concatenate([-draws, -draws, np.zeros(n)])
+    order = np.lexsort((c, r))
+    r, c, v = r[order], c[order], v[order]
+    offsum = np.zeros(n)
+    np.add.at(offsum, r[r != c], np.abs(v[r != c]))
+    v[r == c] = 1.01 * offsum[r[r == c]]
+    rowptr = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(np.bincount(r, minlength=n), out=rowptr[1:])
+    return CSR(n, rowptr, c, v)
+
+
 def rhs_uniform(n: int, m: int, seed: int) -> np.ndarray:
     """n x m row-major block, entries U[0,1), filled row by row."""
     return uniform(seed, n * m).reshape(n, m)

@@ -17,12 +17,20 @@
 extern "C" {
 #endif
 
-/* device CSR, int32 row pointers and column ids (nnz < 2^31) */
+/* device CSR, int32 row pointers and column ids (nnz < 2^31).
+ * Long rows: with long_min > 0, the SpMMs leave every row with at least
+ * long_min nonzeros to a row-split kernel (one workgroup per row), and
+ * long_rows (device, n_long ids) must list exactly those rows
+ * (cacg_csr_long_rows builds it).  long_min = 0 (zero-initialised): every row
+ * vector-per-row. */
 typedef struct {
     int n;
     const int *rowptr;
     const int *col;
     const double *val;
+    const int *long_rows;
+    int n_long;
+    int long_min;
 } cacg_dcsr;
 
 /* the same pattern with FP32 values (the FP32-basis precision form) */
@@ -31,7 +39,18 @@ typedef struct {
     const int *rowptr;
     const int *col;
     const float *val;
+    const int *long_rows;
+    int n_long;
+    int long_min;
 } cacg_scsr;
+
+/* The row length from which a row of this matrix goes to the row-split
+ * kernel: max(256, 16 x the mean row length), so rows far above the mean.
+ * Host-side, from the host row pointers (n + 1 of them, int64). */
+int cacg_csr_long_min(long n, const long *row_start);
+/* The rows with at least long_min nonzeros, ascending, into out (host; may be
+ * NULL to count).  Returns their number. */
+long cacg_csr_long_rows(long n, const long *row_start, int long_min, int *out);
 
 enum { CACG_MODE_REFERENCE = 0, CACG_MODE_FAST = 1 };
 

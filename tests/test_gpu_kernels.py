@@ -38,9 +38,68 @@ MATRICES = {
     "spd_13": lambda: P.random_spd_like(P.laplacian_2d(13), seed=2),
     "arrow_10": _arrow,
     # a 1600-entry row: the first row tile holds more nonzeros than the LDS
-    # stage of any launch shape, so its rows read (col, val) from global memory
+    # stage of any launch shape, so its rows read (col, val) from global memory,
+    # and the row is long enough for the row-split kernel
     "arrow_40": lambda: _arrow(40),
+    # power-law row lengths: five hub rows of 277..989 entries go to the
+    # row-split kernel, the rest stay vector-per-row
+    "powerlaw_40": lambda: P.power_law_spd(40, 8, 1500, 5),
 }
+
+
+def test_long_row_rule(gpu_lib):
+    """Rows at or above max(256, 16 x mean length) are split; the 3D Laplacian
+    has none, the arrow its full row, the power-law matrix its five hubs."""
+    lm, ids = gpu_lib.long_rows(MATRICES["lap3d_7"]())
+    assert lm == 256 and len(ids) == 0
+    lm, ids = gpu_lib.long_rows(MATRICES["arrow_40"]())
+    assert lm == 256 and list(ids) == [0]
+    A = MATRICES["powerlaw_40"]()
+    lm, ids = gpu_lib.long_rows(A)
+    d = np.diff(A.rowptr)
+    assert list(ids) == list(np.nonzero(d >= lm)[0]) and len(ids) == 5
+    assert list(gpu_lib.long_rows(A, 300)[1]) == list(np.nonzero(d >= 300)[0])
+    assert len(gpu_lib.long_rows(A, 0)[1]) == 0
+
+
+@pytest.mark.parametrize("mat", ["arrow_40", "powerlaw_40", "lap3d_7"])
+@pytest.mark.parametrize("long_min", [1, 6, 64])
+@pytest.mark.parametrize("k", [1, 3, 16, 17, 64, 128, 144])
+def test_spmm_row_split_bitwise(gpu_lib, oracle_lib, mat, long_min, k):
+    """Row split forced down to short rows (long_min 1: every nonempty row),
+    every epilogue: the same bits as the oracle.  k = 128, 144 take column
+    blocks of 64 (and 16); the segments of a row are 4096 / min(k, 64)
+    nonzeros, so the 1600-entry row crosses several."""
+    A = MATRICES[mat]()
+    X = P.rhs_normal(A.n, k, 31)
+    AX = oracle_lib.spmm(A, X)
+    assert np.array_equal(gpu_lib.k_spmm(A, X, long_min=long_min), AX)
+    p1, p2 = P.rhs_normal(A.n, k, 32), P.rhs_normal(A.n, k, 33)
+    a, b = 0.37, -0.21
+    got = gpu_lib.k_spmm(A, X, "cheb", a, b, p1=p1, p2=p2, long_min=long_min)
+    assert np.array_equal(got, (a * AX + b * p1) - p2)
+    assert np.array_equal(gpu_lib.k_spmm(A, X, "resid", p1=p1, long_min=long_min), p1 - AX)
+    S1, S0 = gpu_lib.k_spmm(A, X, "cheb1", a, b, want_copy0=True, long_min=long_min)
+    assert np.array_equal(S0, X) and np.array_equal(S1, a * AX + b * X)
+
+
+@pytest.mark.parametrize("long_min", [1, 64])
+def test_spmm_f32_row_split_same_bits(gpu_lib, long_min):
+    """FP32 forms: row split against vector-per-row, bit for bit (the same
+    FP32 products and sums in the same order)."""
+    A = MATRICES["powerlaw_40"]()
+    for k in (4, 16, 33):
+        X = P.rhs_normal(A.n, k, 41).astype(np.float32)
+        p1, p2 = (P.rhs_normal(A.n, k, sd).astype(np.float32) for sd in (42, 43))
+        for epi in ("none", "cheb"):
+            kw = dict(p1=p1, p2=p2) if epi == "cheb" else {}
+            want = gpu_lib.k_spmm_f32(A, X, epi, 0.5, -0.25, long_min=0, **kw)
+            got = gpu_lib.k_spmm_f32(A, X, epi, 0.5, -0.25, long_min=long_min, **kw)
+            assert np.array_equal(got, want), (k, epi)
+        R = P.rhs_normal(A.n, k, 44)
+        w1, w0 = gpu_lib.k_cheb1_f32(A, R, 0.5, -0.25, long_min=0)
+        g1, g0 = gpu_lib.k_cheb1_f32(A, R, 0.5, -0.25, long_min=long_min)
+        assert np.array_equal(g0, w0) and np.array_equal(g1, w1), k
 
 
 @pytest.mark.parametrize("mat", sorted(MATRICES))
@@ -60,16 +119,17 @@ def test_spmm_bitwise(gpu_lib, oracle_lib, mat, k):
 @pytest.mark.parametrize("k,off,ld", [(16, 16, 128), (16, 3, 40), (12, 4, 20), (20, 0, 20),
                                       (256, 0, 256), (4, 2, 8), (128, 0, 144), (6, 1, 9)])
 @pytest.mark.parametrize("epi", ["none", "cheb"])
-def test_spmm_strided_bitwise(gpu_lib, oracle_lib, k, off, ld, epi):
+@pytest.mark.parametrize("long_min", [0, 5])
+def test_spmm_strided_bitwise(gpu_lib, oracle_lib, k, off, ld, epi, long_min):
     """x, p1, p2 and y as column blocks (columns off .. off+k) of wider
     row-major n x ld arrays, the layout of the basis blocks; off and ld select
     the 16-byte gather path (4 | k, 16-byte aligned x, even ld) or the scalar
-    one."""
+    one.  long_min 5: the rows of 5 or more entries through the row split."""
     A = P.laplacian_3d(9)
     n = A.n
     Xw, P1w, P2w = (P.rhs_normal(n, ld, sd) for sd in (21, 22, 23))
     a, b = 0.37, -0.21
-    dA = gpu_lib.DeviceCSR(A)
+    dA = gpu_lib.DeviceCSR(A, long_min)
     dX, d1, d2 = (gpu_lib.DeviceBuffer.from_array(M) for M in (Xw, P1w, P2w))
     Yw = np.full((n, ld), 7.0)
     dY = gpu_lib.DeviceBuffer.from_array(Yw)

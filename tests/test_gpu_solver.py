@@ -150,6 +150,43 @@ def test_configs_reference_bitwise_and_fast_bars(gpu_lib, oracle_lib, which):
         assert np.all(tr < 10 * tol)
 
 
+def test_long_rows_solver_reference_bitwise_and_fast(gpu_lib, oracle_lib):
+    """A power-law matrix whose five hub rows (277..989 entries) go through
+    the row-split SpMM inside the solver: reference order the same bits as
+    the oracle.  On this matrix (cond 1.3e4, Gerschgorin interval [0.02,
+    1991] around a spectrum in [0.08, 1001], s = 4) the reference's own
+    iteration count moves with its summation order, 37 at one rank, 51 at two,
+    60 at four (the oracle's rank emulation), and its recursive residual parts
+    from the true one: ||B - A X|| / ||B|| of its solutions is 1e-6, 2e-7 and
+    4e-6.  The fast order must stop within that iteration range, with a true
+    residual within 2 x the reference's worst."""
+    A = P.power_law_spd(40, 8, 1500, 5)
+    assert len(gpu_lib.long_rows(A)[1]) == 5
+    B = P.rhs_uniform(A.n, 4, 3)
+    X0 = np.zeros_like(B)
+    eps = 1e-10 * float(np.sqrt(oracle_lib.col_norm2sq(B, 0)))
+    orc = oracle_lib.bcbcg(A, B, X0, 4, eps=eps, maxiter=200)
+    assert orc["status"] == "ok"
+    ref = gpu_lib.bcbcg(A, B, X0, 4, stop_mode=0, eps=eps, maxiter=200, order="reference")
+    assert ref["iters"] == orc["iters"]
+    assert np.array_equal(ref["res_hist"], orc["res_hist"])
+    assert np.array_equal(ref["X"], orc["X"])
+    D = A.to_dense()
+
+    def relres(X):
+        return float((np.linalg.norm(B - D @ X, axis=0) / np.linalg.norm(B, axis=0)).max())
+
+    runs = [orc] + [oracle_lib.bcbcg(A, B, X0, 4, eps=eps, maxiter=200, nranks=r) for r in (2, 4)]
+    spread = [o["iters"] for o in runs]
+    worst = max(relres(o["X"]) for o in runs)
+    fast = gpu_lib.bcbcg(A, B, X0, 4, stop_mode=0, eps=eps, maxiter=200, order="fast")
+    tr = relres(fast["X"])
+    print(f"power-law: fast order {fast['iters']} iterations, reference at 1/2/4 ranks {spread}; "
+          f"true relres {tr:.2e}, the reference's worst {worst:.2e}", flush=True)
+    assert fast["status"] == "ok" and min(spread) - 1 <= fast["iters"] <= max(spread) + 1
+    assert tr <= 2 * worst
+
+
 # ---- statuses and edge cases ------------------------------------------------------
 
 def test_maxiter_status(gpu_lib):
