@@ -15,6 +15,7 @@ single-system FP64 path.
 from __future__ import annotations
 
 import ctypes
+import threading
 import time
 
 import numpy as np
@@ -211,35 +212,81 @@ def serving_stream(count, rate, seed=4, l2=(64, 512), e3=(32, 128), mrange=(1, 3
 # ---- drivers -----------------------------------------------------------------------------
 
 
-def serve_stream(stream, batch: Batch, tol=1e-10, timeout=600.0):
-    """Continuous batching: a request is submitted once its arrival time has
-    passed; every loop runs one round and collects what retired.  Returns
-    (results in stream order with 'latency', wall seconds from the first
-    arrival to the last retirement)."""
-    pending = list(range(len(stream)))
-    ids, t_sub, res = {}, {}, {}
+def serve_stream(stream, batch: Batch, tol=1e-10, timeout=600.0, feeder=True):
+    """Continuous batching of a stream of (t_arrival, A, B) requests.
+
+    feeder=True: a feeder thread submits each request once its arrival time has
+    passed, so its host-side copies (cacg_batch_submit converts and copies the
+    matrix and B, and computes its Gerschgorin interval) run while a round waits
+    for the GPU; the calling thread runs rounds while any submitted request is
+    unfinished, collects what retired, and waits for the feeder when idle.
+    feeder=False: submissions happen between rounds on the calling thread.
+    Returns (results in stream order with 'latency' from arrival to the end of
+    the retiring round, wall seconds from the first arrival to the last
+    retirement)."""
+    count = len(stream)
+    ids, res = {}, {}
+    cv = threading.Condition()
+    err = []
     t0 = time.perf_counter()
-    while len(res) < len(stream):
-        now = time.perf_counter() - t0
-        if now > timeout:
-            raise RuntimeError("serving stream timed out")
+
+    def submit_due(now):
         while pending and stream[pending[0]][0] <= now:
             i = pending.pop(0)
             _, A, B = stream[i]
             ids[batch.submit(A, B, tol=tol)] = i
-            t_sub[i] = stream[i][0]
-        if not ids or all(k in res for k in ids.values()):
-            if pending:   # idle until the next arrival
+
+    def feed():
+        try:
+            for i, (t_arr, A, B) in enumerate(stream):
+                d = t_arr - (time.perf_counter() - t0)
+                if d > 0:
+                    time.sleep(d)
+                rid = batch.submit(A, B, tol=tol)
+                with cv:
+                    ids[rid] = i
+                    cv.notify_all()
+        except Exception as e:  # surfaced on the calling thread
+            with cv:
+                err.append(e)
+                cv.notify_all()
+
+    pending = list(range(count))
+    th = None
+    if feeder:
+        pending = []
+        th = threading.Thread(target=feed, daemon=True)
+        th.start()
+    while len(res) < count:
+        if time.perf_counter() - t0 > timeout:
+            raise RuntimeError("serving stream timed out")
+        if not feeder:
+            submit_due(time.perf_counter() - t0)
+            if len(ids) == len(res):   # idle until the next arrival
                 time.sleep(max(0.0, stream[pending[0]][0] - (time.perf_counter() - t0)))
                 continue
+        else:
+            with cv:
+                while len(ids) == len(res) and not err:
+                    cv.wait(timeout=1.0)
+                    if time.perf_counter() - t0 > timeout:
+                        raise RuntimeError("serving stream timed out")
+                if err:
+                    raise err[0]
         batch.step()
         t_now = time.perf_counter() - t0
         for rid in batch.retired():
             r = batch.collect(rid)
-            r["latency"] = t_now - t_sub[ids[rid]]
-            res[ids[rid]] = r
+            with cv:   # the feeder may not have recorded the id yet
+                while rid not in ids:
+                    cv.wait(timeout=1.0)
+                i = ids[rid]
+            r["latency"] = t_now - stream[i][0]
+            res[i] = r
+    if th is not None:
+        th.join()
     wall = (time.perf_counter() - t0) - stream[0][0]
-    return [res[i] for i in range(len(stream))], wall
+    return [res[i] for i in range(count)], wall
 
 
 def serve_sequential(stream, tol=1e-10, s=4, maxiter=1000, precision="fp64"):

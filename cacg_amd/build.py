@@ -27,7 +27,7 @@ HIP_FLAGS = [
     "-I", os.path.join(ROOT, "include"), "-I", os.path.join(CSRC, "kernels"),
 ]
 C_FLAGS = [
-    "-O2", "-fPIC", "-std=gnu11", "-Wall", "-Wextra", "-Wno-unused-parameter",
+    "-O2", "-fPIC", "-std=gnu11", "-pthread", "-Wall", "-Wextra", "-Wno-unused-parameter",
     "-Wno-unused-macros", "-D__HIP_PLATFORM_AMD__", "-I", os.path.join(ROCM, "include"),
     "-I", os.path.join(ROOT, "include"), "-I", os.path.join(CSRC, "kernels"),
 ]
@@ -68,7 +68,7 @@ def build(verbose: bool = False) -> str:
                 print(o)
     objs = [o for _, o in jobs]
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", *objs, "-o", tmp,
+    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", *objs, "-o", tmp, "-lpthread",
           f"-Wl,-rpath,{ROCM}/lib"])
     os.replace(tmp, LIB)
     # the command line program of the reference (src/main.c), on libcacg.so

@@ -25,6 +25,7 @@
  */
 #include <float.h>
 #include <math.h>
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -44,6 +45,7 @@ typedef struct {
     int fresh;           /* the next iteration starts the recurrence (Q = S) */
     int verify;          /* solo: this round recomputes the true residual only */
     int xpend;           /* solo: X += Q alpha of the last iteration still to apply */
+    int ran;             /* solo: launched this round (admitted earlier) */
     long n, nnz, m, iters, round_submitted, rounds_queued;
     long bypassed;       /* queued: requests submitted after this one admitted before it */
     double tol, sa, sb, true_relres;
@@ -125,6 +127,11 @@ struct cacg_batch {
     long graph_rounds;
     stune *tune;                /* per slot */
     int *act;                   /* the batched active slots of a round */
+    /* guards the request table (req, nreq, capreq, rounds) and everything a
+     * round touches, so that cacg_batch_submit may run on another thread while
+     * a round waits for the GPU (the round releases it for that wait only) */
+    pthread_mutex_t mu;
+    int mu_init;
 };
 
 #define HCHK(call)                                                                               \
@@ -347,6 +354,8 @@ int cacg_batch_destroy(cacg_batch *b)
             hipEventDestroy(b->sev[i]);
     if (b->st)
         hipStreamDestroy(b->st);
+    if (b->mu_init)
+        pthread_mutex_destroy(&b->mu);
     free(b);
     return CACG_OK;
 }
@@ -396,6 +405,11 @@ int cacg_batch_create(cacg_batch **out, const cacg_batch_opts *opts)
     cacg_batch *b = (cacg_batch *)calloc(1, sizeof(*b));
     if (!b)
         return CACG_ENOMEM;
+    if (pthread_mutex_init(&b->mu, NULL) != 0) {
+        free(b);
+        return CACG_ENOMEM;
+    }
+    b->mu_init = 1;
     b->o = o;
     b->nslots = o.slots;
     if (o.hbm_bytes == 0) {
@@ -523,9 +537,12 @@ long cacg_batch_submit(cacg_batch *b, long n, const long *row_start, const long 
     }
     r->sa = 2.0 / (hi - lo);
     r->sb = -(hi + lo) / (hi - lo);
+    /* the copies above ran unlocked; only the append takes the lock */
+    pthread_mutex_lock(&b->mu);
     if (b->nreq == b->capreq) {
         breq **nr = (breq **)realloc(b->req, sizeof(breq *) * (size_t)(2 * b->capreq));
         if (!nr) {
+            pthread_mutex_unlock(&b->mu);
             free_req(r);
             return -CACG_ENOMEM;
         }
@@ -537,7 +554,9 @@ long cacg_batch_submit(cacg_batch *b, long n, const long *row_start, const long 
     r->round_submitted = b->rounds;
     r->state = RQ_QUEUED;
     b->req[b->nreq++] = r;
-    return r->id;
+    const long id = r->id;
+    pthread_mutex_unlock(&b->mu);
+    return id;
 }
 
 /* ---- rounds ------------------------------------------------------------------------ */
@@ -769,34 +788,45 @@ static int solo_iteration(const cacg_bsys *d, int fresh, int s, int f32, int *xp
         }
         *xpend = 0;
     } else {
-        double *S = (double *)d->S;
+        /* as the single-system solver: a fresh iteration writes its basis
+         * row-major straight into Q (Q = S; the pending X update, which reads
+         * the old Q, goes first), the later ones keep S block-major (S_j as
+         * n x m blocks n m apart) when the deferred X update rides on the Q
+         * update; same bits either way */
+        if (fresh && *xpend && solo_flush_x(d, st) != CACG_OK)
+            return CACG_EDEVICE;
+        const int bm = !fresh && *xpend && split && m % 4 == 0 && s >= 2 &&
+                       cacg_gram_blk_supported(n, sm, sm, m);
+        const long bs = (long)n * m;
+        double *SB = fresh ? d->Q : (double *)d->S;
+        const long bo = bm ? bs : m, lb = bm ? m : sm;
         if (s == 1) {
-            HCHK(hipMemcpyAsync(S, d->R, sizeof(double) * (size_t)n * m, hipMemcpyDeviceToDevice,
+            HCHK(hipMemcpyAsync(SB, d->R, sizeof(double) * (size_t)n * m, hipMemcpyDeviceToDevice,
                                 st));
         } else {
-            KCHK(cacg_k_spmm_x(&A, m, d->R, m, S + m, sm, CACG_EPI_CHEB1, d->sa, d->sb, NULL, 0,
-                               NULL, 0, S, sm, sb, st));
+            KCHK(cacg_k_spmm_x(&A, m, d->R, m, SB + bo, lb, CACG_EPI_CHEB1, d->sa, d->sb, NULL, 0,
+                               NULL, 0, SB, lb, sb, st));
             for (int j = 2; j < s; j++)
-                KCHK(cacg_k_spmm_x(&A, m, S + (long)(j - 1) * m, sm, S + (long)j * m, sm,
-                                   CACG_EPI_CHEB, sa2, sb2, S + (long)(j - 1) * m, sm,
-                                   S + (long)(j - 2) * m, sm, NULL, 0, sb, st));
+                KCHK(cacg_k_spmm_x(&A, m, SB + (j - 1) * bo, lb, SB + j * bo, lb, CACG_EPI_CHEB,
+                                   sa2, sb2, SB + (j - 1) * bo, lb, SB + (j - 2) * bo, lb, NULL, 0,
+                                   sb, st));
         }
         if (ev)
             HCHK(hipEventRecord(ev[1], st));
-        if (fresh) {
-            if (*xpend && solo_flush_x(d, st) != CACG_OK)
-                return CACG_EDEVICE;
-            HCHK(hipMemcpyAsync(d->Q, S, sizeof(double) * (size_t)n * sm,
-                                hipMemcpyDeviceToDevice, st));
-        } else {
-            KCHK(cacg_k_gram_fast(n, d->AQ, sm, sm, S, sm, sm, NULL, 0, 0, d->QtAS, NULL, d->gpart,
-                                  st));
+        if (!fresh) {
+            if (bm)
+                KCHK(cacg_k_gram_fast_blk(n, d->AQ, sm, sm, SB, m, bs, sm, d->QtAS, d->gpart, st));
+            else
+                KCHK(cacg_k_gram_fast(n, d->AQ, sm, sm, SB, sm, sm, NULL, 0, 0, d->QtAS, NULL,
+                                      d->gpart, st));
             KCHK(cacg_k_lu_solve(sm, d->QtAQ, sm, d->QtAS, d->beta, d->lu, d->flag,
                                  CACG_MODE_FAST, 1, st));
-            if (*xpend)
-                KCHK(cacg_k_qxupdate(n, sm, m, S, d->Q, d->beta, d->alpha, d->X, st));
+            if (bm)
+                KCHK(cacg_k_qxupdate_blk(n, sm, m, SB, bs, d->Q, d->beta, d->alpha, d->X, st));
+            else if (*xpend)
+                KCHK(cacg_k_qxupdate(n, sm, m, SB, d->Q, d->beta, d->alpha, d->X, st));
             else
-                KCHK(cacg_k_qupdate(n, sm, S, d->Q, d->beta, CACG_MODE_FAST, st));
+                KCHK(cacg_k_qupdate(n, sm, SB, d->Q, d->beta, CACG_MODE_FAST, st));
         }
         *xpend = 0;
     }
@@ -922,14 +952,70 @@ static void capture_round(cacg_batch *b, const wlists *L)
     }
 }
 
-int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
+static int step_locked(cacg_batch *b, cacg_batch_stats *stats)
 {
-    if (!b)
-        return CACG_EINVAL;
     hipStream_t st = b->st;
     const int s = b->o.s, f32 = b->o.precision == CACG_PREC_FP32_REFINE;
     int admitted = 0, retired = 0;
 
+    for (int i = 0; i < b->nslots; i++)
+        if (b->slot[i])
+            b->slot[i]->ran = 0;
+    /* solo systems admitted in earlier rounds: one iteration (when joined) or
+     * the true residual, dealt round-robin over the solo streams, launched
+     * before this round's admission so that its uploads (host-blocking copies
+     * from pageable memory) overlap their work; systems admitted now start next
+     * round */
+    int nsolo = 0;
+    for (int i = 0; i < b->nslots; i++)
+        if (b->slot[i] && is_solo(b->h_sys[i].n))
+            nsolo++;
+    if (nsolo > 0) {
+        HCHK(hipEventRecord(b->sev[NSOLO], st));
+        for (int q = 0; q < NSOLO && q < nsolo; q++)
+            HCHK(hipStreamWaitEvent(b->sst[q], b->sev[NSOLO], 0));
+        int q = 0;
+        for (int i = 0; i < b->nslots; i++) {
+            const breq *r = b->slot[i];
+            if (!r || !is_solo(b->h_sys[i].n))
+                continue;
+            hipStream_t ss = b->sst[q];
+            q = (q + 1) % NSOLO;
+            breq *rw = b->slot[i];
+            rw->ran = 1;
+            int rc;
+            if (r->joined && !r->verify) {
+                stune *t = &b->tune[i];
+                t->timing = -1;
+                if (t->next < NCAND && !t->have_ev) {
+                    int e = 0;
+                    while (e < 4 && hipEventCreate(&t->ev[e]) == hipSuccess)
+                        e++;
+                    if (e < 4) { /* no events: keep the default schedules */
+                        (void)hipGetLastError();
+                        while (e > 0)
+                            hipEventDestroy(t->ev[--e]);
+                        t->next = NCAND;
+                    } else {
+                        t->have_ev = 1;
+                    }
+                }
+                if (t->next < NCAND)
+                    t->timing = t->next;
+                const cacg_spmm_sched *sb = t->timing >= 0 ? &CAND[t->timing] : &t->sb;
+                const cacg_spmm_sched *sa = t->timing >= 0 ? &CAND[t->timing] : &t->sa;
+                rc = solo_iteration(&b->h_sys[i], r->fresh, s, f32, &rw->xpend, sb, sa,
+                                    t->timing >= 0 ? t->ev : NULL, ss);
+            } else { /* admitted, or verifying: X complete, then the true residual */
+                rc = rw->xpend ? solo_flush_x(&b->h_sys[i], ss) : CACG_OK;
+                rw->xpend = 0;
+                if (rc == CACG_OK)
+                    rc = solo_residual(&b->h_sys[i], ss);
+            }
+            if (rc != CACG_OK)
+                return rc;
+        }
+    }
     /* admission, FIFO: the head waits for a slot and memory (and, for a large
      * system, for the large systems in flight to drop below max_large) */
     int free_slot = 0, nlarge = 0;
@@ -1138,58 +1224,6 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
         keep_lists(b, &wl, it);
     }
 
-    /* solo systems: one iteration (when joined) and the true residual, dealt
-     * round-robin over the solo streams, which first wait for what the main
-     * stream queued so far (admission uploads) */
-    int nsolo = 0;
-    for (int i = 0; i < b->nslots; i++)
-        if (b->slot[i] && is_solo(b->h_sys[i].n))
-            nsolo++;
-    if (nsolo > 0) {
-        HCHK(hipEventRecord(b->sev[NSOLO], st));
-        for (int q = 0; q < NSOLO && q < nsolo; q++)
-            HCHK(hipStreamWaitEvent(b->sst[q], b->sev[NSOLO], 0));
-        int q = 0;
-        for (int i = 0; i < b->nslots; i++) {
-            const breq *r = b->slot[i];
-            if (!r || !is_solo(b->h_sys[i].n))
-                continue;
-            hipStream_t ss = b->sst[q];
-            q = (q + 1) % NSOLO;
-            breq *rw = b->slot[i];
-            int rc;
-            if (r->joined && !r->verify) {
-                stune *t = &b->tune[i];
-                t->timing = -1;
-                if (t->next < NCAND && !t->have_ev) {
-                    int e = 0;
-                    while (e < 4 && hipEventCreate(&t->ev[e]) == hipSuccess)
-                        e++;
-                    if (e < 4) { /* no events: keep the default schedules */
-                        (void)hipGetLastError();
-                        while (e > 0)
-                            hipEventDestroy(t->ev[--e]);
-                        t->next = NCAND;
-                    } else {
-                        t->have_ev = 1;
-                    }
-                }
-                if (t->next < NCAND)
-                    t->timing = t->next;
-                const cacg_spmm_sched *sb = t->timing >= 0 ? &CAND[t->timing] : &t->sb;
-                const cacg_spmm_sched *sa = t->timing >= 0 ? &CAND[t->timing] : &t->sa;
-                rc = solo_iteration(&b->h_sys[i], r->fresh, s, f32, &rw->xpend, sb, sa,
-                                    t->timing >= 0 ? t->ev : NULL, ss);
-            } else { /* admitted, or verifying: X complete, then the true residual */
-                rc = rw->xpend ? solo_flush_x(&b->h_sys[i], ss) : CACG_OK;
-                rw->xpend = 0;
-                if (rc == CACG_OK)
-                    rc = solo_residual(&b->h_sys[i], ss);
-            }
-            if (rc != CACG_OK)
-                return rc;
-        }
-    }
 #undef BATCHED
 #undef FOR_BATCHED
 
@@ -1222,7 +1256,15 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
         HCHK(hipMemcpyAsync(b->h_flags, b->d_flags, sizeof(int) * (size_t)hi_slot,
                             hipMemcpyDeviceToHost, st));
     }
-    HCHK(hipStreamSynchronize(st));
+    {   /* the wait for the GPU runs unlocked: cacg_batch_submit may append meanwhile */
+        pthread_mutex_unlock(&b->mu);
+        const hipError_t e = hipStreamSynchronize(st);
+        pthread_mutex_lock(&b->mu);
+        if (e != hipSuccess) {
+            fprintf(stderr, "cacg_batch: hipStreamSynchronize failed: %s\n", hipGetErrorString(e));
+            return CACG_EDEVICE;
+        }
+    }
     b->rounds++;
     /* the solo systems' timed candidates (their streams joined the main one) */
     for (int i = 0; i < hi_slot; i++) {
@@ -1249,7 +1291,7 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
     /* stop tests on the true residuals; retire */
     for (int i = 0; i < hi_slot; i++) {
         breq *r = b->slot[i];
-        if (!r)
+        if (!r || (is_solo(r->n) && !r->ran))   /* a solo system admitted this round */
             continue;
         const double *nr = b->h_norms + (long)i * CACG_BATCH_MAX_M;
         const int solo = is_solo(r->n);
@@ -1316,24 +1358,42 @@ int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
     return CACG_OK;
 }
 
+int cacg_batch_step(cacg_batch *b, cacg_batch_stats *stats)
+{
+    if (!b)
+        return CACG_EINVAL;
+    pthread_mutex_lock(&b->mu);
+    const int rc = step_locked(b, stats);
+    pthread_mutex_unlock(&b->mu);
+    return rc;
+}
+
 int cacg_batch_retired(cacg_batch *b, long *ids, int cap)
 {
     if (!b || cap < 0 || (cap > 0 && !ids))
         return -CACG_EINVAL;
+    pthread_mutex_lock(&b->mu);
     const int k = b->nret < cap ? (int)b->nret : cap;
     memcpy(ids, b->ret, sizeof(long) * (size_t)k);
     memmove(b->ret, b->ret + k, sizeof(long) * (size_t)(b->nret - k));
     b->nret -= k;
+    pthread_mutex_unlock(&b->mu);
     return k;
 }
 
 int cacg_batch_collect(cacg_batch *b, long id, double *X, cacg_batch_result *res)
 {
-    if (!b || id < 0 || id >= b->nreq || !b->req[id])
+    if (!b)
         return -CACG_EINVAL;
-    breq *r = b->req[id];
-    if (r->state != RQ_DONE)
-        return 0;
+    pthread_mutex_lock(&b->mu);
+    breq *r = id >= 0 && id < b->nreq ? b->req[id] : NULL;
+    if (!r || r->state != RQ_DONE) {
+        pthread_mutex_unlock(&b->mu);
+        return r ? 0 : -CACG_EINVAL;
+    }
+    /* a retired request: no round touches it again, so the copy runs unlocked */
+    b->req[id] = NULL;
+    pthread_mutex_unlock(&b->mu);
     if (X)
         memcpy(X, r->X, sizeof(double) * (size_t)(r->n * r->m));
     if (res) {
@@ -1343,6 +1403,5 @@ int cacg_batch_collect(cacg_batch *b, long id, double *X, cacg_batch_result *res
         res->true_relres = r->true_relres;
     }
     free_req(r);
-    b->req[id] = NULL;
     return 1;
 }

@@ -19,7 +19,13 @@
  *
  * Every per-system sum runs in an order fixed by that system alone, so a
  * system's result is the same bits whether it runs alone or in any mix.
- * The calls are not thread-safe: one thread drives a batch.
+ *
+ * Threads: cacg_batch_submit, cacg_batch_step, cacg_batch_retired and
+ * cacg_batch_collect may be called from different threads at once (an
+ * internal lock; a round releases it while it waits for the GPU, so a
+ * submission's host-side copies overlap the round).  A feeder thread that
+ * submits while another thread runs rounds is the intended use.
+ * cacg_batch_destroy must not overlap any other call on the same batch.
  */
 #ifndef CACG_BATCH_H
 #define CACG_BATCH_H
