@@ -563,10 +563,19 @@ def batching_form(count=256):
             "one_at_a_time_fp64_solves_per_s": count / seq, "vs_one_at_a_time": seq / wall}
 
 
+# the serving form's large-system threshold: its stream's systems of 32^3 rows
+# and more run the single-system kernels (measured: the batched launches, built
+# for many small systems, take twice as long per iteration for a lone system of
+# 40,000-130,000 rows); the batching form keeps the library default (2^17)
+SERVE_SOLO_ROWS = 32768
+
+
 def serving_form(count, rate):
     """Poisson streams at several offered loads: continuous batching (smallest
-    HBM footprint first, at most two large systems in flight) against the
-    same stream solved one request at a time (FIFO) with the FP64 path.  The
+    HBM footprint first, at most two large systems in flight, systems of
+    SERVE_SOLO_ROWS rows or more on the single-system kernels, requests
+    submitted by a feeder thread as they arrive) against the same stream
+    solved one request at a time (FIFO) with the FP64 path.  The
     lowest rate is below the capacity of either server (a few solves/s on
     this mix), so latency there is service time; the higher ones build a
     backlog.  `rate` is the rate also reported at the top level."""
@@ -577,7 +586,7 @@ def serving_form(count, rate):
         cnt = 16 if r < 10 else count
         stream = BT.serving_stream(cnt, r, seed=4)
         b = BT.Batch(s=4, precision="fp64", hbm_bytes=96 << 30, maxiter=3000, policy="smallest",
-                     max_large=2)
+                     max_large=2, solo_rows=SERVE_SOLO_ROWS)
         res, wall = BT.serve_stream(stream, b, tol=1e-10)
         b.close()
         seq, wall_seq = BT.serve_sequential(stream, tol=1e-10, s=4, maxiter=3000)
@@ -596,7 +605,9 @@ def serving_form(count, rate):
     top = next(x for x in sweep if x["rate_per_s"] == float(rate))
     return dict(top, workload=(f"Poisson streams (seed 4) of 2D L 64..512 and 3D edge 32..128 "
                                "Laplacians, 1..32 RHS, s=4, 1e-10, FP64 basis; continuous batching, "
-                               "smallest footprint first, at most 2 large systems in flight"),
+                               "smallest footprint first, at most 2 large systems in flight, "
+                               f"systems of {SERVE_SOLO_ROWS} rows or more on the single-system "
+                               "kernels, submissions from a feeder thread"),
                 sweep=sweep)
 
 

@@ -34,7 +34,8 @@ class BatchOpts(ctypes.Structure):
     _fields_ = [("hbm_bytes", ctypes.c_long), ("slots", ctypes.c_int), ("s", ctypes.c_int),
                 ("precision", ctypes.c_int), ("maxiter", ctypes.c_long),
                 ("max_large", ctypes.c_int), ("graph", ctypes.c_int), ("tune", ctypes.c_int),
-                ("policy", ctypes.c_int), ("max_bypass", ctypes.c_int)]
+                ("policy", ctypes.c_int), ("max_bypass", ctypes.c_int),
+                ("solo_rows", ctypes.c_long)]
 
 
 class BatchResult(ctypes.Structure):
@@ -79,13 +80,15 @@ class Batch:
     """A batch of device-resident systems advanced together (cacg_batch_*)."""
 
     def __init__(self, s=4, precision="fp32", slots=1024, hbm_bytes=0, maxiter=1000,
-                 max_large=0, graph=False, tune=True, policy="fifo", max_bypass=0):
+                 max_large=0, graph=False, tune=True, policy="fifo", max_bypass=0,
+                 solo_rows=0):
         o = BatchOpts()
         lib().cacg_batch_default_opts(ctypes.byref(o))
         o.hbm_bytes, o.slots, o.s = int(hbm_bytes), int(slots), int(s)
         o.precision, o.maxiter = PRECISION[precision], int(maxiter)
         o.max_large, o.graph, o.tune = int(max_large), int(bool(graph)), int(bool(tune))
         o.policy, o.max_bypass = POLICY[policy], int(max_bypass)
+        o.solo_rows = int(solo_rows)
         h = _vp()
         rc = lib().cacg_batch_create(ctypes.byref(h), ctypes.byref(o))
         if rc != 0:

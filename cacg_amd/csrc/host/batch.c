@@ -160,11 +160,15 @@ static long al256(long x) { return (x + 255) & ~255L; }
  * (vector-per-row SpMM, wide split-K Gram, MFMA tall-skinny updates) on their
  * arena block instead of joining the batched launches, which are built for
  * many small systems: alone, a 2-million-row system runs 3-4.5 x slower per
- * iteration through them.  The choice depends on the system's shape only, so a
- * result is still the same alone and in any mix. */
+ * iteration through them.  The choice depends on the system's shape and the
+ * batch's options only, so a result is still the same alone and in any mix of
+ * one batch configuration.  The threshold is cacg_batch_opts.solo_rows,
+ * SOLO_ROWS by default. */
 #define SOLO_ROWS (1L << 17)
 
-static int is_solo(long n) { return n >= SOLO_ROWS; }
+static long solo_rows(const cacg_batch_opts *o) { return o->solo_rows > 0 ? o->solo_rows : SOLO_ROWS; }
+/* a system of this batch that runs the single-system kernels on a solo stream */
+#define IS_SOLO(nrows) ((nrows) >= solo_rows(&b->o))
 
 /* ---- per-system layout in its arena block ----------------------------------------- */
 
@@ -173,7 +177,7 @@ typedef struct {
         npart, total;
 } layout;
 
-static layout make_layout(long n, long nnz, long m, int s, int precision)
+static layout make_layout(long n, long nnz, long m, int s, int precision, long solo_min)
 {
     layout L;
     const long sm = s * m, ntile = (n + CACG_BT - 1) / CACG_BT, nch = (n + CACG_GR - 1) / CACG_GR;
@@ -202,14 +206,22 @@ static layout make_layout(long n, long nnz, long m, int s, int precision)
     PUT(alpha, sm * m * 8);
     {
         long gp = nch * sm * (sm + m);
-        if (is_solo(n)) { /* the single-system Gram's split-K partials */
+        if (n >= solo_min) { /* the single-system Gram's split-K partials */
             const long ws = cacg_gram_workspace((int)sm, (int)(sm + m));
             if (ws > gp)
                 gp = ws;
         }
         PUT(gpart, gp * 8);
     }
-    PUT(npart, ntile * m * 8);
+    {
+        /* per-tile norm partials of the batched kernels; a large system's
+         * single-system norm kernels write cacg_xrupdate_workspace(m) of them
+         * whatever its row count */
+        long np = ntile * m;
+        if (n >= solo_min && cacg_xrupdate_workspace((int)m) > np)
+            np = cacg_xrupdate_workspace((int)m);
+        PUT(npart, np * 8);
+    }
 #undef PUT
     L.total = o;
     return L;
@@ -219,7 +231,7 @@ long cacg_batch_footprint(long n, long nnz, long m, int s, int precision)
 {
     if (n <= 0 || nnz < 0 || m <= 0 || s <= 0)
         return -1;
-    return make_layout(n, nnz, m, s, precision).total;
+    return make_layout(n, nnz, m, s, precision, SOLO_ROWS).total;
 }
 
 /* ---- arena: first fit over a sorted free list, neighbours merged on free ------------- */
@@ -395,7 +407,7 @@ int cacg_batch_create(cacg_batch **out, const cacg_batch_opts *opts)
         o.slots = 1024;
     if (o.slots < 1 || o.slots > 4096 || o.s < 1 || o.maxiter <= 0 || o.hbm_bytes < 0 ||
         o.max_large < 0 || (o.graph != 0 && o.graph != 1) ||
-        (o.tune != 0 && o.tune != 1) || o.max_bypass < 0 ||
+        (o.tune != 0 && o.tune != 1) || o.max_bypass < 0 || o.solo_rows < 0 ||
         (o.policy != CACG_BATCH_FIFO && o.policy != CACG_BATCH_SMALLEST_FIRST) ||
         (o.precision != CACG_PREC_FP64 && o.precision != CACG_PREC_FP32_REFINE))
         return CACG_EINVAL;
@@ -483,7 +495,7 @@ long cacg_batch_submit(cacg_batch *b, long n, const long *row_start, const long 
     for (long p = 0; p < nnz; p++)
         if (col_idx[p] < 0 || col_idx[p] >= n)
             return -CACG_EINVAL;
-    const long bytes = cacg_batch_footprint(n, nnz, m, b->o.s, b->o.precision);
+    const long bytes = make_layout(n, nnz, m, b->o.s, b->o.precision, solo_rows(&b->o)).total;
     if (bytes > b->arena_bytes)
         return -CACG_ENOMEM;
     breq *r = (breq *)calloc(1, sizeof(*r));
@@ -584,7 +596,7 @@ static int admit(cacg_batch *b, breq *r, int slot)
         return 0;
     const int s = b->o.s, f32 = b->o.precision == CACG_PREC_FP32_REFINE;
     const long n = r->n, m = r->m, sm = s * m;
-    const layout L = make_layout(n, r->nnz, m, s, b->o.precision);
+    const layout L = make_layout(n, r->nnz, m, s, b->o.precision, solo_rows(&b->o));
     char *base = b->arena + off;
     cacg_bsys *d = &b->h_sys[slot];
     memset(d, 0, sizeof(*d));
@@ -968,7 +980,7 @@ static int step_locked(cacg_batch *b, cacg_batch_stats *stats)
      * round */
     int nsolo = 0;
     for (int i = 0; i < b->nslots; i++)
-        if (b->slot[i] && is_solo(b->h_sys[i].n))
+        if (b->slot[i] && IS_SOLO(b->h_sys[i].n))
             nsolo++;
     if (nsolo > 0) {
         HCHK(hipEventRecord(b->sev[NSOLO], st));
@@ -977,7 +989,7 @@ static int step_locked(cacg_batch *b, cacg_batch_stats *stats)
         int q = 0;
         for (int i = 0; i < b->nslots; i++) {
             const breq *r = b->slot[i];
-            if (!r || !is_solo(b->h_sys[i].n))
+            if (!r || !IS_SOLO(b->h_sys[i].n))
                 continue;
             hipStream_t ss = b->sst[q];
             q = (q + 1) % NSOLO;
@@ -1020,7 +1032,7 @@ static int step_locked(cacg_batch *b, cacg_batch_stats *stats)
      * system, for the large systems in flight to drop below max_large) */
     int free_slot = 0, nlarge = 0;
     for (int i = 0; i < b->nslots; i++)
-        if (b->slot[i] && is_solo(b->slot[i]->n))
+        if (b->slot[i] && IS_SOLO(b->slot[i]->n))
             nlarge++;
     /* smallest first: among the queued requests the one of smallest footprint
      * that may be admitted now (a large one only below max_large) goes next,
@@ -1036,13 +1048,13 @@ static int step_locked(cacg_batch *b, cacg_batch_stats *stats)
         breq *head = b->req[b->qhead], *pick = NULL;
         const int large_full = b->o.max_large > 0 && nlarge >= b->o.max_large;
         if (head->bypassed >= maxb) {
-            if (is_solo(head->n) && large_full)
+            if (IS_SOLO(head->n) && large_full)
                 break;
             pick = head;
         } else {
             for (long i = b->qhead; i < b->nreq; i++) {
                 breq *r = b->req[i];
-                if (!r || r->state != RQ_QUEUED || (is_solo(r->n) && large_full))
+                if (!r || r->state != RQ_QUEUED || (IS_SOLO(r->n) && large_full))
                     continue;
                 if (!pick || r->bytes < pick->bytes)
                     pick = r;
@@ -1060,7 +1072,7 @@ static int step_locked(cacg_batch *b, cacg_batch_stats *stats)
         if (ok == 0)
             break;
         admitted++;
-        if (is_solo(pick->n))
+        if (IS_SOLO(pick->n))
             nlarge++;
         for (long i = b->qhead; i < pick->id; i++)
             if (b->req[i] && b->req[i]->state == RQ_QUEUED)
@@ -1072,7 +1084,7 @@ static int step_locked(cacg_batch *b, cacg_batch_stats *stats)
             b->qhead++;
             continue;
         }
-        if (is_solo(r->n) && b->o.max_large > 0 && nlarge >= b->o.max_large)
+        if (IS_SOLO(r->n) && b->o.max_large > 0 && nlarge >= b->o.max_large)
             break;
         while (free_slot < b->nslots && b->slot[free_slot])
             free_slot++;
@@ -1084,13 +1096,13 @@ static int step_locked(cacg_batch *b, cacg_batch_stats *stats)
         if (ok == 0)
             break;
         admitted++;
-        if (is_solo(r->n))
+        if (IS_SOLO(r->n))
             nlarge++;
         b->qhead++;
     }
 
     /* work lists (solo systems take no part in the batched launches) */
-#define BATCHED(i) (b->slot[i] && !is_solo(b->h_sys[i].n))
+#define BATCHED(i) (b->slot[i] && !IS_SOLO(b->h_sys[i].n))
     long need = 0;
     int nact = 0, nit = 0, max_sm = 1, na = 0;
     /* i over the batched active slots (b->act[0 .. na - 1], ascending): the class
@@ -1103,7 +1115,7 @@ static int step_locked(cacg_batch *b, cacg_batch_stats *stats)
             continue;
         const cacg_bsys *d = &b->h_sys[i];
         nact++;
-        if (is_solo(d->n))
+        if (IS_SOLO(d->n))
             continue;
         b->act[na++] = i;
         need += 1 + d->ntile;
@@ -1291,10 +1303,10 @@ static int step_locked(cacg_batch *b, cacg_batch_stats *stats)
     /* stop tests on the true residuals; retire */
     for (int i = 0; i < hi_slot; i++) {
         breq *r = b->slot[i];
-        if (!r || (is_solo(r->n) && !r->ran))   /* a solo system admitted this round */
+        if (!r || (IS_SOLO(r->n) && !r->ran))   /* a solo system admitted this round */
             continue;
         const double *nr = b->h_norms + (long)i * CACG_BATCH_MAX_M;
-        const int solo = is_solo(r->n);
+        const int solo = IS_SOLO(r->n);
         /* a solo system's iteration rounds report its recursive residual */
         const int recursive = solo && r->joined && !r->verify;
         if (r->joined && !(solo && r->verify)) {

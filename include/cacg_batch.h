@@ -43,14 +43,15 @@ typedef struct {
     int precision;   /* CACG_PREC_FP64, or CACG_PREC_FP32_REFINE: FP32 matrix copy, basis and
                         Chebyshev SpMMs; Q, A Q, Gram, LU, X and R FP64 */
     long maxiter;    /* outer-iteration cap per system */
-    int max_large;   /* systems of 2^17 rows or more (each runs the single-system kernels on
-                        its own stream) in flight at most; 0: no limit.  A queued large system
-                        at the head waits for one to retire (FIFO: later requests wait too) */
+    int max_large;   /* large systems (solo_rows rows or more: each runs the single-system
+                        kernels on its own stream) in flight at most; 0: no limit.  A queued
+                        large system at the head waits for one to retire (FIFO: later requests
+                        wait too) */
     int graph;       /* 1: when a round's batched launches repeat the previous round's (same
                         work lists), capture them once in a HIP graph and replay it, with no
                         work-list upload; 0 (default): launch them one by one, which measured
                         ~2% faster on the 256-system batch.  Same bits either way */
-    int tune;        /* 1 (default): the first iterations of a system of 2^17 rows or more run
+    int tune;        /* 1 (default): the first iterations of a large system run
                         its SpMMs under candidate schedules (tile order, grid), timed on its
                         stream, and the rest under the fastest; 0: default schedules.  Same
                         bits */
@@ -61,6 +62,12 @@ typedef struct {
     int max_bypass;  /* CACG_BATCH_SMALLEST_FIRST: a queued request overtaken by this many
                         later ones is admitted next (and nothing overtakes it while it waits
                         for memory or a slot); 0: 16 */
+    long solo_rows;  /* systems of at least this many rows are large: they run the
+                        single-system kernels (recursive residual, the X update deferred into
+                        the next Q update, a final round on the true residual) on streams of
+                        their own instead of joining the batched launches; 0: 2^17.  Part of
+                        the batch's configuration: a result is the same bits alone and in any
+                        mix of batches with the same solo_rows */
 } cacg_batch_opts;
 
 enum { CACG_BATCH_FIFO = 0, CACG_BATCH_SMALLEST_FIRST = 1 };
@@ -87,7 +94,8 @@ void cacg_batch_default_opts(cacg_batch_opts *o);
 int cacg_batch_create(cacg_batch **out, const cacg_batch_opts *opts);
 int cacg_batch_destroy(cacg_batch *b);
 
-/* bytes of arena one system takes (its matrix, blocks and work space) */
+/* bytes of arena one system takes (its matrix, blocks and work space) in a
+ * batch with the default solo_rows */
 long cacg_batch_footprint(long n, long nnz, long m, int s, int precision);
 
 /* Queue one system: CSR of n rows (row_start n + 1, col_idx / val nnz), B and

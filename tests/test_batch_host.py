@@ -41,12 +41,13 @@ BATCH_C = r"""
 #include "cacg_batch.h"
 int main(void)
 {
-  printf("opts %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(cacg_batch_opts),
+  printf("opts %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(cacg_batch_opts),
          offsetof(cacg_batch_opts, hbm_bytes), offsetof(cacg_batch_opts, slots),
          offsetof(cacg_batch_opts, s), offsetof(cacg_batch_opts, precision),
          offsetof(cacg_batch_opts, maxiter), offsetof(cacg_batch_opts, max_large),
          offsetof(cacg_batch_opts, graph), offsetof(cacg_batch_opts, tune),
-         offsetof(cacg_batch_opts, policy), offsetof(cacg_batch_opts, max_bypass));
+         offsetof(cacg_batch_opts, policy), offsetof(cacg_batch_opts, max_bypass),
+         offsetof(cacg_batch_opts, solo_rows));
   printf("stats %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(cacg_batch_stats),
          offsetof(cacg_batch_stats, active), offsetof(cacg_batch_stats, queued),
          offsetof(cacg_batch_stats, finished), offsetof(cacg_batch_stats, admitted),
@@ -77,7 +78,8 @@ def test_python_batch_structs_match_header(tmp_path):
 
 @pytest.mark.parametrize("field,value", [("graph", 2), ("graph", -1), ("tune", 2), ("max_large", -1),
                                          ("s", 0), ("slots", 5000), ("maxiter", 0),
-                                         ("policy", 2), ("policy", -1), ("max_bypass", -1)])
+                                         ("policy", 2), ("policy", -1), ("max_bypass", -1),
+                                         ("solo_rows", -1)])
 def test_batch_options_rejected_without_device(field, value):
     """Invalid batch options are refused (CACG_EINVAL) before any device is touched."""
     o = BT.BatchOpts()

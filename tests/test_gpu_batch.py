@@ -257,3 +257,54 @@ def test_smallest_first_admission(gpu_lib, max_bypass):
     for k, rid in enumerate(ids):
         assert res[rid]["status"] == "ok"
         assert np.array_equal(res[rid]["X"], fifo[k]["X"])
+
+
+def test_feeder_thread_submissions_give_the_same_results(gpu_lib):
+    """A stream served with submissions from a feeder thread (cacg_batch_submit
+    beside cacg_batch_step) and with submissions between rounds: every request
+    retires with the same iterations and the same bits, and converges.  Mixed
+    sizes, one large system (2^17 rows or more) among them."""
+    stream = []
+    for k, (L, m) in enumerate([(40, 4), (96, 8), (64, 16), (368, 8), (48, 1), (80, 32),
+                                (72, 2), (56, 12)]):
+        A = P.laplacian_2d(L, 0.01 * (k + 1))
+        stream.append((0.002 * k, A, P.uniform(70 + k, A.n * m).reshape(A.n, m)))
+    assert any(A.n >= 1 << 17 for _, A, _ in stream)
+    out = {}
+    for fd in (False, True):
+        b = BT.Batch(s=4, precision="fp64", hbm_bytes=8 << 30, maxiter=2000, policy="smallest",
+                     max_large=1)
+        out[fd], _ = BT.serve_stream(stream, b, tol=1e-10, feeder=fd)
+        b.close()
+    for (_, A, B), r0, r1 in zip(stream, out[False], out[True]):
+        assert r0["status"] == r1["status"] == "ok"
+        assert r0["iters"] == r1["iters"]
+        assert np.array_equal(r0["X"], r1["X"])
+        D = sp.csr_matrix((A.val, A.col, A.rowptr), shape=(A.n, A.n))
+        rr = np.linalg.norm(B - D @ r1["X"], axis=0) / np.linalg.norm(B, axis=0)
+        assert np.all(rr < 1e-10)
+
+
+def test_low_solo_threshold_small_systems(gpu_lib):
+    """solo_rows = 1024: systems of a few thousand rows run the single-system
+    kernels on solo streams (their norm partials are sized for those kernels,
+    not by their row tiles).  Alone and in a mix the same bits; converged."""
+    systems = []
+    for k, (L, m) in enumerate([(40, 4), (64, 16), (200, 8), (24, 2)]):
+        A = P.laplacian_2d(L, 0.02)
+        systems.append((A, P.uniform(90 + k, A.n * m).reshape(A.n, m)))
+    kw = dict(s=4, precision="fp64", hbm_bytes=4 << 30, maxiter=2000, solo_rows=1024)
+    b = BT.Batch(**kw)
+    ids = [b.submit(A, B, tol=1e-10) for A, B in systems]
+    mix = b.run()
+    b.close()
+    for (A, B), rid in zip(systems, ids):
+        r = mix[rid]
+        assert r["status"] == "ok"
+        D = sp.csr_matrix((A.val, A.col, A.rowptr), shape=(A.n, A.n))
+        assert np.all(np.linalg.norm(B - D @ r["X"], axis=0) / np.linalg.norm(B, axis=0) < 1e-10)
+        a = BT.Batch(**kw)
+        rid1 = a.submit(A, B, tol=1e-10)
+        alone = a.run()[rid1]
+        a.close()
+        assert alone["iters"] == r["iters"] and np.array_equal(alone["X"], r["X"])
