@@ -296,7 +296,8 @@ def main():
         out["dropin_host_arrays"] = dropin_host_arrays(A, B, m, s, args, out["ms_per_step"])
         out["c1_cbcg"] = c1_cbcg()
         out["spmm_long_rows"] = spmm_long_rows()
-        out["precision_form"] = precision_form(A, B, X, dB, dX, m, s, args, out["ms_per_step"])
+        out["precision_form"] = precision_form(A, B, X, dB, dX, m, s, args, out["ms_per_step"],
+                                               out["phase_ms_per_solve"])
         out["batching_form"] = batching_form()
         out["serving_form"] = serving_form(args.serve_requests, args.serve_rate)
     if not args.no_cpu_baseline:
@@ -456,8 +457,27 @@ def c1_cbcg():
             "status": rep["status"]}
 
 
-def precision_form(A, B, X64, dB, dX, m, s, args, ms64):
-    """The FP32-basis / FP64-refined solve of the same workload."""
+def _reference_order_solution(A, B, s, args):
+    """X of the same solve in the reference summation order (bit-identical to
+    the reference's bcbcg_v1 at one rank under its own stop test), with the
+    bench's stop test: the FP64 oracle the precision form is measured against."""
+    import cacg_amd
+    r = cacg_amd.bcbcg(A, B, np.zeros_like(B), s, tol=args.tol, maxiter=args.maxiter,
+                       order="reference")
+    return r["X"]
+
+
+def _col_err(X, Xr):
+    return float((np.linalg.norm(X - Xr, axis=0) / np.linalg.norm(Xr, axis=0)).max())
+
+
+def precision_form(A, B, X64, dB, dX, m, s, args, ms64, phase64):
+    """The FP32-basis / FP64-refined solve of the same workload (the library
+    default: FP64 basis until every column's true relative residual is below
+    1e-3, FP32 basis after); errors per column (relative 2-norm, maximum over
+    columns) against the FP64 fast path and against the reference-order
+    solution; the FP32 basis throughout beside it; device time per phase and
+    solve of both forms against the FP64 path's (phase64)."""
     import cacg_amd
     L = cacg_amd.lib()
     sv = cacg_amd.Solver(A, m, s, tol=args.tol, maxiter=3 * args.maxiter, order="fast",
@@ -478,12 +498,24 @@ def precision_form(A, B, X64, dB, dX, m, s, args, ms64):
     ms = float(np.mean(lat) * 1e3)
     print(f"bench: precision form {ms:.1f} ms per solve, {reps[-1]['iters']} iterations",
           file=sys.stderr, flush=True)
-    out = {"workload": "same system, FP32 matrix copy / basis / Chebyshev SpMM, FP64 Q, A Q, Gram, "
-                       "LU, X, R; true residual recomputed in FP64 (residual replacement)",
+    Xr = _reference_order_solution(A, B, s, args)
+    thr = _timed_solves(A, B, m, s, args, "fp32", fp32_basis_below=0.0)
+    out = {"workload": "same system, FP32 matrix copy / basis / Chebyshev SpMM once every column's "
+                       "true relres is below 1e-3 (FP64 basis before), FP64 Q, A Q, Gram, LU, X, "
+                       "R; true residual recomputed in FP64 (residual replacement)",
            "solves_per_s": 1e3 / ms, "ms_per_solve": ms, "vs_fp64_path": ms64 / ms,
            "iters": reps[-1]["iters"], "refinements": reps[-1]["refinements"],
            "status": reps[-1]["status"], "true_relres_max": reps[-1]["true_relres"],
-           "err_vs_fp64_solution": float(err.max())}
+           "err_vs_fp64_solution": float(err.max()),
+           "err_vs_reference_order": _col_err(X, Xr),
+           "fp64_fast_err_vs_reference_order": _col_err(X64, Xr),
+           "fp32_basis_throughout": {
+               "ms_per_solve": thr["ms"], "iters": thr["rep"]["iters"],
+               "vs_fp64_path": ms64 / thr["ms"], "err_vs_reference_order": _col_err(thr["X"], Xr)},
+           "phase_ms_per_solve": {
+               "fp64": phase64,
+               "precision_form": {k: v for k, v in reps[-1]["phase_ms"].items() if v},
+               "fp32_basis_throughout": {k: v for k, v in thr["rep"]["phase_ms"].items() if v}}}
     # the 128^3 Laplacian and its random SPD variant (same pattern, off-diagonals
     # -U(0.5, 1.5) seed 2, diagonal 1.01 x absolute row sum), 16 RHS N(0, 1) seed 2,
     # each against its own FP64 run
@@ -495,25 +527,28 @@ def precision_form(A, B, X64, dB, dX, m, s, args, ms64):
         r64 = _timed_solves(Am, Bn, m, s, args, "fp64")
         r32 = _timed_solves(Am, Bn, m, s, args, "fp32")
         e = np.linalg.norm(r32["X"] - r64["X"], axis=0) / np.linalg.norm(r64["X"], axis=0)
+        Xr = _reference_order_solution(Am, Bn, s, args)
         out["normal_rhs"][name] = {
             "fp64_ms_per_solve": r64["ms"], "fp64_iters": r64["rep"]["iters"],
             "ms_per_solve": r32["ms"], "iters": r32["rep"]["iters"],
             "refinements": r32["rep"]["refinements"], "vs_fp64_path": r64["ms"] / r32["ms"],
             "status": r32["rep"]["status"], "true_relres_max": r32["rep"]["true_relres"],
-            "err_vs_fp64_solution": float(e.max())}
+            "err_vs_fp64_solution": float(e.max()),
+            "err_vs_reference_order": _col_err(r32["X"], Xr),
+            "fp64_fast_err_vs_reference_order": _col_err(r64["X"], Xr)}
         print(f"bench: precision form, {name} N(0,1) RHS: FP64 {r64['ms']:.1f} ms "
               f"({r64['rep']['iters']} it), FP32/refined {r32['ms']:.1f} ms "
               f"({r32['rep']['iters']} it)", file=sys.stderr, flush=True)
     return out
 
 
-def _timed_solves(A, B, m, s, args, precision, reps=3):
+def _timed_solves(A, B, m, s, args, precision, reps=3, **kw):
     """Mean wall time of `reps` solves from x0 = 0 after one untimed solve."""
     import cacg_amd
     L = cacg_amd.lib()
     maxiter = args.maxiter if precision == "fp64" else 3 * args.maxiter
     sv = cacg_amd.Solver(A, m, s, tol=args.tol, maxiter=maxiter, order="fast",
-                         precision=precision)
+                         precision=precision, **kw)
     dB = cacg_amd.DeviceBuffer.from_array(B)
     dX = cacg_amd.DeviceBuffer(B.nbytes)
     lat = []

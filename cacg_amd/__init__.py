@@ -42,7 +42,8 @@ class Opts(ctypes.Structure):
                 ("print_period", ctypes.c_int), ("inner_tol", ctypes.c_double),
                 ("inner_maxiter", ctypes.c_long), ("refine_restart", ctypes.c_int),
                 ("bounds", ctypes.c_int), ("lanczos_steps", ctypes.c_int),
-                ("bound_lo", ctypes.c_double), ("bound_hi", ctypes.c_double)]
+                ("bound_lo", ctypes.c_double), ("bound_hi", ctypes.c_double),
+                ("fp32_basis_below", ctypes.c_double)]
 
 
 BOUNDS = {"gerschgorin": 0, "lanczos": 1, "given": 2}
@@ -526,7 +527,7 @@ PRECISION = {"fp64": 0, "fp32": 1}
 def make_opts(stop_mode=STOP_REL_PER_COLUMN, eps=0.0, tol=1e-8, maxiter=100, order="fast",
               print_period=0, precision="fp64", inner_tol=0.0, inner_maxiter=0,
               refine_restart=0, bounds="gerschgorin", lanczos_steps=20, bound_lo=0.0,
-              bound_hi=0.0) -> Opts:
+              bound_hi=0.0, fp32_basis_below=None) -> Opts:
     """precision 'fp32' is the FP32-basis form with FP64 refinement; bounds
     'gerschgorin' (the reference's), 'lanczos' or 'given' ([bound_lo, bound_hi])."""
     o = Opts()
@@ -545,6 +546,8 @@ def make_opts(stop_mode=STOP_REL_PER_COLUMN, eps=0.0, tol=1e-8, maxiter=100, ord
     o.lanczos_steps = int(lanczos_steps)
     o.bound_lo = bound_lo
     o.bound_hi = bound_hi
+    if fp32_basis_below is not None:   # else the library default
+        o.fp32_basis_below = fp32_basis_below
     return o
 
 
@@ -566,7 +569,7 @@ def _report_dict(rep: Report, hist: np.ndarray, m: int) -> dict:
 def bcbcg(A, B, X0, s, *, stop_mode=STOP_REL_PER_COLUMN, eps=0.0, tol=1e-8, maxiter=100,
           order="fast", print_period=0, precision="fp64", inner_tol=0.0, inner_maxiter=0,
           refine_restart=0, bounds="gerschgorin", lanczos_steps=20, bound_lo=0.0,
-          bound_hi=0.0) -> dict:
+          bound_hi=0.0, fp32_basis_below=None) -> dict:
     """Host-buffer solve through cacg_bcbcg().  Returns dict with X and the report."""
     B = np.ascontiguousarray(B, dtype=np.float64).reshape(A.n, -1)
     m = B.shape[1]
@@ -575,7 +578,8 @@ def bcbcg(A, B, X0, s, *, stop_mode=STOP_REL_PER_COLUMN, eps=0.0, tol=1e-8, maxi
     ci = np.ascontiguousarray(A.col, dtype=np.int64)
     va = np.ascontiguousarray(A.val, dtype=np.float64)
     o = make_opts(stop_mode, eps, tol, maxiter, order, print_period, precision, inner_tol,
-                  inner_maxiter, refine_restart, bounds, lanczos_steps, bound_lo, bound_hi)
+                  inner_maxiter, refine_restart, bounds, lanczos_steps, bound_lo, bound_hi,
+                  fp32_basis_below)
     hist = np.zeros((maxiter, m))
     rep = Report()
     rep.res_hist = hist.ctypes.data_as(_dp)

@@ -117,6 +117,7 @@ void cacg_default_opts(cacg_opts *o)
     o->lanczos_steps = 20;
     o->bound_lo = 0.0;
     o->bound_hi = 0.0;
+    o->fp32_basis_below = 1e-3;
 }
 
 /* number of eigenvalues of the tridiagonal (a, b) below x (Sturm sequence of
@@ -367,7 +368,7 @@ int cacg_solver_create(cacg_solver **out, long n, const long *row_start, const l
             opts->inner_maxiter < 0 || opts->tol < 0.0 ||
             (opts->refine_restart != 0 && opts->refine_restart != 1) || opts->bounds < 0 ||
             opts->bounds > CACG_BOUNDS_GIVEN || opts->lanczos_steps < 0 ||
-            opts->lanczos_steps > 1000 ||
+            opts->lanczos_steps > 1000 || !(opts->fp32_basis_below >= 0.0) ||
             (opts->bounds == CACG_BOUNDS_GIVEN &&
              !(isfinite(opts->bound_lo) && isfinite(opts->bound_hi) &&
                opts->bound_lo < opts->bound_hi)))
@@ -422,9 +423,9 @@ int cacg_solver_create(cacg_solver **out, long n, const long *row_start, const l
     if (prec32) {
         rc |= dalloc(sv, (void **)&sv->f_val, sv->nnz * (long)sizeof(float));
         rc |= dalloc(sv, (void **)&sv->f_S, nsm * (long)sizeof(float));
-    } else {
-        rc |= dalloc(sv, (void **)&sv->d_S, nsm * (long)sizeof(double));
     }
+    if (!prec32 || sv->opts.fp32_basis_below > 0.0)
+        rc |= dalloc(sv, (void **)&sv->d_S, nsm * (long)sizeof(double));
     rc |= dalloc(sv, (void **)&sv->d_Q, nsm * (long)sizeof(double));
     rc |= dalloc(sv, (void **)&sv->d_AQ, nsm * (long)sizeof(double));
     rc |= dalloc(sv, (void **)&sv->d_QtAQ, sm * sm * (long)sizeof(double));
@@ -1232,6 +1233,10 @@ static int solve_refine(cacg_solver *sv, const double *d_B, double *d_X, cacg_re
      * is read: before a true residual and at the end */
     const int split = cacg_k_split_supported(sm, m);
     int xpend = 0;
+    /* opts.fp32_basis_below > 0: the basis is built in FP64 (d_S, the FP64
+     * matrix) until the true residual is below it in every column, in FP32 from
+     * then on */
+    int use64 = o->fp32_basis_below > 0.0 && sv->d_S;
     for (;;) {
         if (xpend) {
             SCHK(cacg_k_xrupdate_split(n, sm, m, Q, NULL, sv->d_alpha, d_X, NULL, NULL, NULL, 1,
@@ -1263,6 +1268,14 @@ static int solve_refine(cacg_solver *sv, const double *d_B, double *d_X, cacg_re
             status = CACG_MAXITER;
             break;
         }
+        if (use64) {
+            int below = 1;
+            for (int c = 0; c < m; c++)
+                if (!(sqrt(tn2[c]) < o->fp32_basis_below * sqrt(h_bnorm2[c])))
+                    below = 0;
+            if (below)
+                use64 = 0;
+        }
         if (it > 0)
             refinements++;
         for (int c = 0; c < m; c++) {
@@ -1275,7 +1288,28 @@ static int solve_refine(cacg_solver *sv, const double *d_B, double *d_X, cacg_re
         int broke = 0;
         while (since < inner_cap && it < o->maxiter) {
             mark(sv, CACG_PHASE_OTHER);
-            if (s == 1) {
+            if (fresh && xpend) { /* Q is about to be replaced: apply its pending X update */
+                SCHK(cacg_k_xrupdate_split(n, sm, m, Q, NULL, sv->d_alpha, d_X, NULL, NULL, NULL,
+                                           1, st));
+                xpend = 0;
+            }
+            if (use64) {
+                /* FP64 basis, row-major; a fresh one is written straight into Q */
+                double *SB = fresh ? Q : sv->d_S;
+                if (s == 1) {
+                    SCHK(hipMemcpyAsync(SB, sv->d_R, sizeof(double) * (size_t)n * m,
+                                        hipMemcpyDeviceToDevice, st));
+                } else {
+                    SCHK(cacg_k_spmm_x(&sv->A, m, sv->d_R, m, SB + m, sm, CACG_EPI_CHEB1, sa, sb,
+                                       NULL, 0, NULL, 0, SB, sm, &sv->sch_basis, st));
+                    mark(sv, CACG_PHASE_BASIS1);
+                    for (int j = 2; j < s; j++)
+                        SCHK(cacg_k_spmm_x(&sv->A, m, SB + (long)(j - 1) * m, sm, SB + (long)j * m,
+                                           sm, CACG_EPI_CHEB, sa2, sb2, SB + (long)(j - 1) * m, sm,
+                                           SB + (long)(j - 2) * m, sm, NULL, 0, &sv->sch_basis,
+                                           st));
+                }
+            } else if (s == 1) {
                 SCHK(cacg_k_cvt_f64_f32((long)n * m, sv->d_R, S, st));
             } else {
                 SCHK(cacg_k_cheb1_f32_x(&sv->A32, m, sv->d_R, m, S + m, sm, S, sm, sa, sb,
@@ -1290,13 +1324,21 @@ static int solve_refine(cacg_solver *sv, const double *d_B, double *d_X, cacg_re
             if (s != 2)
                 mark(sv, s == 1 ? CACG_PHASE_BASIS1 : CACG_PHASE_BASIS);
             if (fresh) {
-                if (xpend) { /* Q is about to be replaced: apply its pending X update */
-                    SCHK(cacg_k_xrupdate_split(n, sm, m, Q, NULL, sv->d_alpha, d_X, NULL, NULL,
-                                               NULL, 1, st));
-                    xpend = 0;
-                }
-                SCHK(cacg_k_cvt_f32_f64((long)n * sm, S, Q, st));
+                if (!use64)
+                    SCHK(cacg_k_cvt_f32_f64((long)n * sm, S, Q, st));
                 fresh = 0;
+            } else if (use64) {
+                SCHK(cacg_k_gram_fast(n, AQ, sm, sm, sv->d_S, sm, sm, NULL, 0, 0, sv->d_QtAS, NULL,
+                                      sv->d_gram_work, st));
+                mark(sv, CACG_PHASE_GRAM_AS);
+                SCHK(cacg_k_lu_solve(sm, sv->d_QtAQ, sm, sv->d_QtAS, sv->d_beta, sv->d_lu_work,
+                                     sv->d_flag, CACG_MODE_FAST, 1, st));
+                mark(sv, CACG_PHASE_SOLVE);
+                if (xpend)
+                    SCHK(cacg_k_qxupdate(n, sm, m, sv->d_S, Q, sv->d_beta, sv->d_alpha, d_X, st));
+                else
+                    SCHK(cacg_k_qupdate(n, sm, sv->d_S, Q, sv->d_beta, CACG_MODE_FAST, st));
+                xpend = 0;
             } else {
                 SCHK(cacg_k_gram_fast_f64f32(n, AQ, sm, sm, S, sm, sm, NULL, 0, 0, sv->d_QtAS, NULL,
                                              sv->d_gram_work, st));

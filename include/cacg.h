@@ -139,6 +139,14 @@ typedef struct {
     int bounds;          /* CACG_BOUNDS_* (default GERSCHGORIN, the reference's) */
     int lanczos_steps;   /* CACG_BOUNDS_LANCZOS: 1..1000 steps; 0 selects 20 */
     double bound_lo, bound_hi;  /* CACG_BOUNDS_GIVEN: the interval, lo < hi */
+    /* CACG_PREC_FP32_REFINE only: > 0: the basis is built in FP64 with the FP64
+     * matrix (the solver then also holds an FP64 basis block) until every
+     * column's true relative residual ||B - A X||/||B|| at a recompute is below
+     * fp32_basis_below, in FP32 from then on (the recurrence continues).
+     * 0: the FP32 basis throughout.  cacg_default_opts: 1e-3 (an FP32 basis in
+     * the first three decades cost the 128^3 Laplacian with smooth right-hand
+     * sides 8 of its 45 iterations; after them it costs none). */
+    double fp32_basis_below;
 } cacg_opts;
 
 typedef struct {

@@ -93,13 +93,14 @@ OPTS_C = r"""
 #include <stdio.h>
 #include "cacg.h"
 int main(void) {
-  printf("opts %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(cacg_opts),
+  printf("opts %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(cacg_opts),
          offsetof(cacg_opts, stop_mode), offsetof(cacg_opts, eps), offsetof(cacg_opts, tol),
          offsetof(cacg_opts, maxiter), offsetof(cacg_opts, order), offsetof(cacg_opts, precision),
          offsetof(cacg_opts, print_period), offsetof(cacg_opts, inner_tol),
          offsetof(cacg_opts, inner_maxiter), offsetof(cacg_opts, refine_restart),
          offsetof(cacg_opts, bounds), offsetof(cacg_opts, lanczos_steps),
-         offsetof(cacg_opts, bound_lo), offsetof(cacg_opts, bound_hi));
+         offsetof(cacg_opts, bound_lo), offsetof(cacg_opts, bound_hi),
+         offsetof(cacg_opts, fp32_basis_below));
   printf("report %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(cacg_report),
          offsetof(cacg_report, status), offsetof(cacg_report, iters), offsetof(cacg_report, bounds),
          offsetof(cacg_report, res_hist), offsetof(cacg_report, res_hist_cap),

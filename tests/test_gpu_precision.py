@@ -135,17 +135,20 @@ PREC_CASES = {
 }
 
 
+@pytest.mark.parametrize("switch", [0.0, 1e-3])
 @pytest.mark.parametrize("restart", [0, 1])
 @pytest.mark.parametrize("name", sorted(PREC_CASES))
-def test_fp32_refined_solve(gpu_lib, name, restart):
+def test_fp32_refined_solve(gpu_lib, name, restart, switch):
     """restart 0: residual replacement (the recurrence continues); 1: classic
-    iterative refinement (the recurrence restarts at every true residual)."""
+    iterative refinement (the recurrence restarts at every true residual).
+    switch: fp32_basis_below (0: FP32 basis throughout; 1e-3, the default: FP64
+    basis until every column's true relative residual is below 1e-3)."""
     mk, m, s, rhs, cond = PREC_CASES[name]
     A = mk()
     B = P.rhs_normal(A.n, m, 2) if rhs == "normal" else P.rhs_uniform(A.n, m, 1)
     tol = 1e-10
     out = gpu_lib.bcbcg(A, B, np.zeros_like(B), s, tol=tol, maxiter=400, precision="fp32",
-                        refine_restart=restart)
+                        refine_restart=restart, fp32_basis_below=switch)
     assert out["status"] == "ok", out["status"]
     As = sp.csr_matrix((A.val, A.col, A.rowptr), shape=(A.n, A.n))
     X = out["X"]
@@ -162,7 +165,8 @@ def test_fp32_refined_solve(gpu_lib, name, restart):
     assert err.max() <= cond * tol
     err64 = np.linalg.norm(X - ref["X"], axis=0) / np.linalg.norm(ref["X"], axis=0)
     assert err64.max() <= 2 * cond * tol
-    print(f"{name} restart={restart}: iters {out['iters']} (fp64 {ref['iters']}), refinements "
+    print(f"{name} restart={restart} switch={switch:g}: iters {out['iters']} (fp64 "
+          f"{ref['iters']}), refinements "
           f"{out['refinements']}, true relres {relres.max():.2e}, err vs fp64 {err64.max():.2e}")
 
 
@@ -187,3 +191,22 @@ def test_fp32_refined_maxiter(gpu_lib):
     out = gpu_lib.bcbcg(A, B, np.zeros_like(B), 8, tol=1e-14, maxiter=3, precision="fp32")
     assert out["status"] == "maxiter" and out["iters"] == 3
     assert out["true_relres"] > 0
+
+
+def test_basis_switch_direction(gpu_lib):
+    """fp32_basis_below at or above the starting relative residual (1 from
+    x0 = 0) switches to the FP32 basis before the first iteration: the same bits
+    as the FP32 basis throughout.  A threshold no residual reaches keeps the
+    FP64 basis: within one iteration of the FP64 path (a 2D Laplacian of order
+    48, smooth right-hand sides, s = 8; the counts are printed)."""
+    A = P.laplacian_2d(48)
+    B = P.rhs_uniform(A.n, 8, 1)
+    kw = dict(tol=1e-10, maxiter=400, precision="fp32")
+    a = gpu_lib.bcbcg(A, B, np.zeros_like(B), 8, fp32_basis_below=0.0, **kw)
+    b = gpu_lib.bcbcg(A, B, np.zeros_like(B), 8, fp32_basis_below=2.0, **kw)
+    assert a["iters"] == b["iters"] and np.array_equal(a["X"], b["X"])
+    c = gpu_lib.bcbcg(A, B, np.zeros_like(B), 8, fp32_basis_below=1e-300, **kw)
+    ref = gpu_lib.bcbcg(A, B, np.zeros_like(B), 8, tol=1e-10, maxiter=400)
+    print(f"switch: FP32 basis {a['iters']} iterations, FP64 basis in the refine loop "
+          f"{c['iters']}, FP64 path {ref['iters']}")
+    assert c["status"] == "ok" and abs(c["iters"] - ref["iters"]) <= 1
