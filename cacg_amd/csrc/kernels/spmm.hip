@@ -426,24 +426,28 @@ __global__ __launch_bounds__(256) void spmm_pipe_kernel(
 // its lane group for one chain of batched gathers per few nonzeros (two
 // dependent global latencies per batch of 4 or 8), and the launch ends with
 // that workgroup.  Rows with at least long_min nonzeros are left out of the
-// kernels above and taken here, one workgroup per row: waves 1-3 form the
-// products val(p) * x(col(p), c) of the next segment of the row into LDS,
+// kernels above and taken here, one workgroup of LR_T lanes per row: waves
+// 1 .. LR_T / 64 - 1 form the products val(p) * x(col(p), c) of the next
+// segment of the row into LDS,
 // every product rounded as above, while the lanes c < kc of wave 0 add the
 // current segment's in CSR order, each keeping the chain of column cb + c.
 // Same products, same order: the same bits as the vector-per-row kernels,
 // with one LDS read per nonzero on the chain instead of a global gather.
 // Columns go 64 at a time (cb = 0, 64, ...), each block one pass over the row.
-constexpr int LR_SEGE = 4096;                  // products per LDS buffer (two buffers)
-constexpr int LR_J = (LR_SEGE + 191) / 192;    // products per producer lane and segment
+constexpr int LR_T = 1024;                     // lanes per workgroup: wave 0 sums, the rest produce
+constexpr int LR_P = LR_T - 64;                // producer lanes
+constexpr int LR_SEGE = 8192;                  // products per LDS buffer (two, in dynamic LDS)
+constexpr int LR_J = (LR_SEGE + LR_P - 1) / LR_P;   // products per producer lane and segment
 
 template <typename TV, typename TX, typename TY, int EPI>
-__global__ __launch_bounds__(256) void spmm_long_kernel(
+__global__ __launch_bounds__(LR_T) void spmm_long_kernel(
     const int *__restrict__ long_rows, const int *__restrict__ rowptr,
     const int *__restrict__ colidx, const TV *__restrict__ val, int k, const TX *__restrict__ x,
     long ldx, TY *__restrict__ y, long ldy, double ad, double bd, const TY *__restrict__ p1,
     long ld1, const TY *__restrict__ p2, long ld2, TY *__restrict__ copy0, long ldc)
 {
-    __shared__ TV prod[2 * LR_SEGE];   // two segment buffers
+    extern __shared__ double lr_lds[];
+    TV *prod = reinterpret_cast<TV *>(lr_lds);   // two segment buffers of LR_SEGE
     const int t = threadIdx.x;
     const long i = long_rows[blockIdx.x];
     const int pb = rowptr[i], pe = rowptr[i + 1];
@@ -464,19 +468,19 @@ __global__ __launch_bounds__(256) void spmm_long_kernel(
             TV vv[LR_J];
 #pragma unroll
             for (int j = 0; j < LR_J; j++) {
-                const int e = tp + 192 * j;
+                const int e = tp + LR_P * j;
                 cc[j] = e < cnt ? colidx[p0 + e / kc] : 0;
                 vv[j] = e < cnt ? val[p0 + e / kc] : (TV)0;
             }
             TX xv[LR_J];
 #pragma unroll
             for (int j = 0; j < LR_J; j++) {
-                const int e = tp + 192 * j;
+                const int e = tp + LR_P * j;
                 xv[j] = e < cnt ? x[(long)cc[j] * ldx + cb + e % kc] : (TX)0;
             }
 #pragma unroll
             for (int j = 0; j < LR_J; j++) {
-                const int e = tp + 192 * j;
+                const int e = tp + LR_P * j;
                 if (e < cnt)
                     buf[e] = mul_rn(vv[j], (TV)xv[j]);
             }
@@ -543,10 +547,16 @@ int launch_spmm(int n, const int *rowptr, const int *col, const TV *val, int k, 
         Q = sc->passes > 1 ? sc->passes : 1;
     }
     const int lmin = n_long > 0 ? long_min : 0;
+    // the row-split kernel's two segment buffers exceed the 64 KiB default
+    constexpr size_t lr_bytes = 2 * LR_SEGE * sizeof(TV);
 #define CACG_SPMM_LAUNCH(E, IL)                                                                  \
     do {                                                                                         \
+        if (lmin > 0 && hipFuncSetAttribute(                                                     \
+                            reinterpret_cast<const void *>(&spmm_long_kernel<TV, TX, TY, E>),    \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lr_bytes) != hipSuccess) \
+            return (int)hipErrorInvalidConfiguration;                                            \
         if (lmin > 0)                                                                            \
-            hipLaunchKernelGGL((spmm_long_kernel<TV, TX, TY, E>), dim3(n_long), dim3(256), 0,    \
+            hipLaunchKernelGGL((spmm_long_kernel<TV, TX, TY, E>), dim3(n_long), dim3(LR_T), lr_bytes, \
                                st, long_rows, rowptr, col, val, k, x, ldx, y, ldy, a, b, p1, ld1, \
                                p2, ld2, copy0, ldc);                                             \
         if constexpr (LPR == 4 || LPR == 8)                                                      \
