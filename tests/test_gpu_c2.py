@@ -23,7 +23,12 @@ reference's own 4-rank departure (and within 1e-9 where that is smaller)
 for as long as the reference's residual keeps falling (39 iterations), it
 stops no later than the reference, every column's true relative residual
 ||B - A X|| / ||B|| is below 1e-9, and its X is within 1e-10 (relative
-2-norm) of the reference's.  The measured deviations are printed."""
+2-norm) of the reference's.  The measured deviations are printed.
+
+Precision form (FP64 basis until every column is below 1e-3, FP32 after, the
+true residual recomputed in FP64) under the same stop test: no more iterations
+than the reference, X within 1e-10 of the reference's recorded rows and of the
+GPU reference-order X."""
 
 import hashlib
 import json
@@ -115,3 +120,22 @@ def test_benchmark_size_reference_bitwise_and_fast_order(gpu_lib):
     assert fast["iters"] <= spec["iters_p1"]
     assert rho_f < 1e-9
     assert xrel <= 1e-10
+
+    # the precision form (FP64 basis until 1e-3, FP32 after, FP64 refinement),
+    # the reference's stop test: X within 1e-10 of the reference's on the
+    # recorded rows and of the GPU reference order's everywhere
+    print(f"{NAME}: precision form", flush=True)
+    sv = gpu_lib.Solver(A, m, s, stop_mode=0, eps=spec["eps"], maxiter=3 * spec["maxiter"],
+                        order="fast", precision="fp32")
+    gpu_lib.lib().cacg_k_fill_f64(dX.ptr, A.n * m, 0.0, None)
+    gpu_lib.lib().cacg_dev_sync()
+    pf = sv.solve(dB, dX)
+    sv.close()
+    Xp = dX.download(B.shape)
+    rows = g["rows"]
+    prow = float(np.linalg.norm(Xp[rows] - g["X_rows"]) / np.linalg.norm(g["X_rows"]))
+    prel = float(np.linalg.norm(Xp - Xr) / np.linalg.norm(Xr))
+    print(f"{NAME}: precision form {pf['iters']} iterations, {pf['refinements']} refinements; X vs "
+          f"the reference's rows {prow:.2e}, vs the GPU reference order {prel:.2e}", flush=True)
+    assert pf["status"] == "ok" and pf["iters"] <= spec["iters_p1"]
+    assert prow <= 1e-10 and prel <= 1e-10
