@@ -229,9 +229,11 @@ def time_reference(A, B, X0, s, eps, solver="bcbcg", nranks=1, timeout=600, work
     oracle/_ref/ref_harness) at `nranks` MPI ranks: the harness stamps every
     outer iteration with the seconds since the solver was called (rank 0,
     after MPI_Init and its input was read), so t[k] = setup + iterations
-    0..k.  Returns dict(iters, t (list of seconds), wall (seconds of the whole
-    run, input reading and output writing included)).  The run keeps no
-    snapshots."""
+    0..k; and it times the whole solve as the reference's main does (barrier,
+    clock, solve, barrier, clock, the maximum over ranks; src/main.c:154-163).
+    Returns dict(iters, t (list of seconds), solve (that time), wall (seconds
+    of the whole run, input reading and output writing included)).  The run
+    keeps no snapshots."""
     import re
     import time as _time
     if not reference_available():
@@ -259,7 +261,9 @@ def time_reference(A, B, X0, s, eps, solver="bcbcg", nranks=1, timeout=600, work
                            text=True, cwd=td, timeout=timeout, env=env)
         wall = _time.perf_counter() - t0
     stamps = [float(x) for x in re.findall(r"iteration \d+ r2 \S+ at (\S+) s", r.stderr)]
-    return {"iters": len(stamps), "t": stamps, "wall": wall}
+    solve = re.findall(r"ref_harness: solve (\S+) s", r.stderr)
+    return {"iters": len(stamps), "t": stamps, "solve": float(solve[-1]) if solve else None,
+            "wall": wall}
 
 
 def run_reference(A, B, X0, s, eps, solver="bcbcg", nranks=1, snapshots=True, timeout=600,

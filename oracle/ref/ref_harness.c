@@ -182,6 +182,9 @@ int main(int argc, char **argv)
     const char *ns = getenv("CACG_REF_NOSNAP"), *pg = getenv("CACG_REF_PROGRESS");
     const int nosnap = ns && ns[0] == '1';
     g_progress = pg && pg[0] == '1';
+    /* the reference's timing convention (src/main.c:154-163): barrier, clock,
+     * solve, barrier, clock, the maximum over ranks */
+    MPI_Barrier(MPI_COMM_WORLD);
     g_t0 = wall();
     char snap_path[4096];
     if (np == 1 && !nosnap) {
@@ -200,6 +203,16 @@ int main(int argc, char **argv)
         cbcg_v1(A, B, X, s, eps, myid, np);
     else
         bcbcg_v1(A, B, X, s, eps, myid, np);
+    MPI_Barrier(MPI_COMM_WORLD);
+    {
+        const double mine = wall() - g_t0;
+        double tmax = 0.0;
+        MPI_Reduce(&mine, &tmax, 1, MPI_DOUBLE, MPI_MAX, 0, MPI_COMM_WORLD);
+        if (g_progress && myid == 0) {
+            fprintf(stderr, "ref_harness: solve %.6f s (max over %d ranks)\n", tmax, np);
+            fflush(stderr);
+        }
+    }
 
     /* gather the final X to rank 0 */
     int *cnt = (int *)malloc(sizeof(int) * np), *dsp = (int *)malloc(sizeof(int) * np);
