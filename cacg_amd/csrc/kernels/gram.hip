@@ -115,9 +115,25 @@ __global__ __launch_bounds__(64) void gram_ref_kernel(int n, GrefSrc s, int nbb,
             const long i0 = st * 64;
             const int rows = n - i0 < 64 ? (int)(n - i0) : 64;
             if (rows == 64) {
-#pragma unroll 16
-                for (int r = 0; r < 64; r++)
-                    acc = add_rn(acc, mul_rn(stg[buf][r][ai], stg[buf][r][8 + bi]));
+                // operands of the next 16 rows read while the current 16 are
+                // summed, so the chain does not wait on each LDS read
+                double lv[2][16], rv[2][16];
+                auto rd = [&](int bb, int r0) __attribute__((always_inline)) {
+#pragma unroll
+                    for (int u = 0; u < 16; u++) {
+                        lv[bb][u] = stg[buf][r0 + u][ai];
+                        rv[bb][u] = stg[buf][r0 + u][8 + bi];
+                    }
+                };
+                rd(0, 0);
+#pragma unroll
+                for (int h = 0; h < 4; h++) {
+                    if (h < 3)
+                        rd((h + 1) & 1, 16 * (h + 1));
+#pragma unroll
+                    for (int u = 0; u < 16; u++)
+                        acc = add_rn(acc, mul_rn(lv[h & 1][u], rv[h & 1][u]));
+                }
             } else {
                 for (int r = 0; r < rows; r++)
                     acc = add_rn(acc, mul_rn(stg[buf][r][ai], stg[buf][r][8 + bi]));
