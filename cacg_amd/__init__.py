@@ -509,6 +509,20 @@ def k_cvt_f32_f64(S):
     return dQ.download(S.shape)
 
 
+def k_colnorm2_ref(R, ld=None):
+    """||R(:, c)||^2 of every column, each a row-order chain (reference order);
+    R n x m, stored with row stride ld (default m)."""
+    R = np.ascontiguousarray(R, dtype=np.float64)
+    n, m = R.shape
+    ld = m if ld is None else ld
+    W = np.zeros((max(n, 1), ld))
+    W[:n, :m] = R
+    dR = DeviceBuffer.from_array(W)
+    dn = DeviceBuffer(8 * m)
+    _check(lib().cacg_k_colnorm2_ref(n, m, dR.ptr, ld, dn.ptr, None), "colnorm2_ref")
+    return dn.download(m)
+
+
 def k_colnorm2_fast(R):
     R = np.ascontiguousarray(R, dtype=np.float64)
     n, m = R.shape

@@ -156,61 +156,90 @@ __global__ __launch_bounds__(256) void xrupdate_ref_kernel(int n, int sm, int m,
     }
 }
 
-// ||R(:, c)||^2, one sequential chain per column in row order.  One wave per
-// 64 columns (W of them); the rows are staged CREF_E * 64 / W at a time: all
-// 64 lanes load a stage (coalesced when ldr == W) one stage ahead into
-// registers and write it to LDS, and the W chain lanes read their column from
-// there, so the loads never wait on the chain.
-constexpr int CREF_E = 32;   // elements per lane per stage
+// ||R(:, c)||^2, one sequential chain per column in row order, 64 columns (W
+// of them) per workgroup.  Lanes c < W of wave 0 keep the chains; waves 1-7
+// meanwhile load the next stage of CREF_SEGE / W rows (coalesced when
+// ldr == W) into the other of two LDS buffers, so a stage costs the chain one
+// barrier instead of a global latency.  The chain lanes read their column 16
+// rows ahead of the sums.
+constexpr int CREF_T = 512;                         // lanes: wave 0 sums, the rest load
+constexpr int CREF_P = CREF_T - 64;
+constexpr int CREF_SEGE = 8192;                     // values per LDS buffer (two, dynamic LDS)
+constexpr int CREF_J = (CREF_SEGE + CREF_P - 1) / CREF_P;
 
-__global__ __launch_bounds__(64) void colnorm2_ref_kernel(int n, int m, const double *__restrict__ R,
-                                                         long ldr, double *__restrict__ out)
+__global__ __launch_bounds__(CREF_T) void colnorm2_ref_kernel(int n, int m,
+                                                              const double *__restrict__ R,
+                                                              long ldr, double *__restrict__ out)
 {
-    __shared__ double stg[64 * CREF_E];
-    const int lane = threadIdx.x;
+    extern __shared__ double cbuf[];   // two stage buffers of CREF_SEGE
+    const int t = threadIdx.x;
     const int c0 = blockIdx.x * 64;
     const int W = m - c0 < 64 ? m - c0 : 64;
-    const int T = 64 * CREF_E / W;   // rows per stage
-    const long last = (long)n - 1;
-    // element k of this lane in a stage: row (lane + 64 k) / W, column (lane + 64 k) % W
-    // (rows past the last one are clamped to it: loaded, never summed)
-    int rk[CREF_E], ck[CREF_E];
+    const int T = CREF_SEGE / W;   // rows per stage
+    const long nst = ((long)n + T - 1) / T;
+    // loader lanes: stage st into buffer st & 1, all loads before the stores
+    auto produce = [&](long st) {
+        double *buf = cbuf + (st & 1) * CREF_SEGE;
+        const long i0 = st * T;
+        const int cnt = (n - i0 < T ? (int)(n - i0) : T) * W;
+        const int tp = t - 64;
+        double v[CREF_J];
 #pragma unroll
-    for (int k = 0; k < CREF_E; k++) {
-        const int e = lane + 64 * k;
-        rk[k] = e / W;
-        ck[k] = c0 + e % W;
-    }
-    double nxt[CREF_E];
-    auto load_stage = [&](long i0) {
+        for (int j = 0; j < CREF_J; j++) {
+            const int e = tp + CREF_P * j;
+            v[j] = e < cnt ? R[(i0 + e / W) * ldr + c0 + e % W] : 0.0;
+        }
 #pragma unroll
-        for (int k = 0; k < CREF_E; k++) {
-            const long i = i0 + rk[k] < last ? i0 + rk[k] : last;
-            nxt[k] = R[i * ldr + ck[k]];
+        for (int j = 0; j < CREF_J; j++) {
+            const int e = tp + CREF_P * j;
+            if (e < cnt)
+                buf[e] = v[j];
         }
     };
+    if (t >= 64 && nst > 0)
+        produce(0);
+    __syncthreads();
     double acc = 0.0;
-    if (n > 0)
-        load_stage(0);
-    for (long i0 = 0; i0 < n; i0 += T) {
-        __builtin_amdgcn_wave_barrier();
+    for (long st = 0; st < nst; st++) {
+        if (t >= 64) {
+            if (st + 1 < nst)
+                produce(st + 1);
+        } else if (t < W) {
+            const double *buf = cbuf + (st & 1) * CREF_SEGE;
+            const long i0 = st * T;
+            const int rows = n - i0 < T ? (int)(n - i0) : T;
+            int r = 0;
+            if (rows >= 32) {
+                double va[16], vb[16];
 #pragma unroll
-        for (int k = 0; k < CREF_E; k++)
-            stg[lane + 64 * k] = nxt[k];
-        __builtin_amdgcn_wave_barrier();
-        if (i0 + T < n)
-            load_stage(i0 + T);
-        const int rows = n - i0 < T ? (int)(n - i0) : T;
-        if (lane < W) {
-#pragma unroll 8
-            for (int r = 0; r < rows; r++) {
-                const double v = stg[r * W + lane];
+                for (int u = 0; u < 16; u++)
+                    va[u] = buf[u * W + t];
+                for (; r + 32 <= rows; r += 32) {
+#pragma unroll
+                    for (int u = 0; u < 16; u++)
+                        vb[u] = buf[(r + 16 + u) * W + t];
+#pragma unroll
+                    for (int u = 0; u < 16; u++)
+                        acc = add_rn(acc, mul_rn(va[u], va[u]));
+                    if (r + 64 <= rows) {
+#pragma unroll
+                        for (int u = 0; u < 16; u++)
+                            va[u] = buf[(r + 32 + u) * W + t];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 16; u++)
+                        acc = add_rn(acc, mul_rn(vb[u], vb[u]));
+                }
+            }
+            for (; r < rows; r++) {
+                const double v = buf[r * W + t];
                 acc = add_rn(acc, mul_rn(v, v));
             }
         }
+        __syncthreads();
     }
-    if (lane < W)
-        out[c0 + lane] = acc;
+    if (t < W)
+        out[c0 + t] = acc;
 }
 
 // ---- fast order: MFMA -----------------------------------------------------------
@@ -590,8 +619,10 @@ extern "C" int cacg_k_colnorm2_ref(int n, int m, const double *R, long ldr, doub
 {
     if (n < 0 || m <= 0)
         return (int)hipErrorInvalidValue;
-    hipLaunchKernelGGL(colnorm2_ref_kernel, dim3((m + 63) / 64), dim3(64), 0, st, n, m, R, ldr,
-                       d_norm2);
+    const size_t lds = 2 * CREF_SEGE * sizeof(double);
+    cacg_allow_lds(colnorm2_ref_kernel, lds);
+    hipLaunchKernelGGL(colnorm2_ref_kernel, dim3((m + 63) / 64), dim3(CREF_T), lds, st, n, m, R,
+                       ldr, d_norm2);
     CACG_LAUNCH_CHECK();
     return 0;
 }

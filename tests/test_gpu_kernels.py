@@ -352,3 +352,16 @@ def test_deferred_forms_refuse_unsupported_widths(gpu_lib):
     L = gpu_lib.lib()
     assert L.cacg_k_split_supported(40, 4) == 0 and L.cacg_k_split_supported(128, 33) == 0
     assert L.cacg_k_qxupdate(10, 40, 4, None, None, None, None, None, None) != 0
+
+
+@pytest.mark.parametrize("n,m,ld", [(0, 3, 3), (1, 1, 1), (31, 2, 2), (100, 1, 1), (1000, 3, 5),
+                                    (5000, 16, 16), (777, 64, 64), (3000, 65, 65),
+                                    (20000, 130, 136), (70000, 16, 16), (8193, 1, 1)])
+def test_colnorm2_reference_bitwise(gpu_lib, oracle_lib, n, m, ld):
+    """Reference-order column norms (one row-order chain per column, columns 64
+    per workgroup, rows staged through LDS): the oracle's sums bit for bit, for
+    partial stages, several column blocks and a row stride wider than m."""
+    R = P.rhs_normal(max(n, 1), m, n + m)[:n]
+    got = gpu_lib.k_colnorm2_ref(R, ld)
+    want = np.array([oracle_lib.col_norm2sq(R, c) if n else 0.0 for c in range(m)])
+    assert np.array_equal(got, want)
