@@ -19,17 +19,16 @@
 
 namespace {
 
-// One wave per 8 x 8 block of G: lane l keeps the chain of G[a0 + l / 8][b0 +
-// l % 8].  The chain is sequential in the row index, so the kernel takes n
-// steps of one product and one dependent sum per lane whatever the grid; the
-// data a step needs are the 8 values of L and the 8 of [R1 | R2] in the
-// block's columns (128 bytes a row, against 520 for a wave of 64 columns of
-// one row a, so far fewer bytes travel per row).  Rows go in stages of 64:
-// lane r loads row i0 + r's 16 values (16-byte loads where the block is
-// aligned) GREF_NS stages ahead into a register ring, writes them to LDS when
-// its stage comes, and every lane reads its L and R value of each row from
-// there.  One wave per workgroup: LDS order within the wave is program order.
-constexpr int GREF_NS = 4;   // stages in flight, 64 rows each
+// One workgroup per 8 x 8 block of G: lane l of wave 0 keeps the chain of
+// G[a0 + l / 8][b0 + l % 8], sequential in the row index (the reference's
+// summation, bit for bit), so the kernel takes n steps of one product and one
+// dependent sum whatever the grid.  Waves 1-3 meanwhile stage the block's
+// columns of the next GREF_ROWS rows (8 values of L and 8 of [R1 | R2] a row,
+// columns past the last clamped to it) into the other of two LDS buffers, so
+// a stage costs the chain one barrier instead of a global latency, and the
+// chain lanes read their two operands 16 rows ahead of the sums.
+constexpr int GREF_ROWS = 256;                          // rows per stage (two 32 KiB buffers)
+constexpr int GREF_J = (GREF_ROWS * 16 + 191) / 192;    // values per loader lane and stage
 
 struct GrefSrc {
     const double *L;
@@ -43,109 +42,96 @@ struct GrefSrc {
     int kr2;
 };
 
-// address of [R1 | R2](i, b)
-__device__ __forceinline__ const double *gref_r(const GrefSrc &s, long i, int b)
+__global__ __launch_bounds__(256) void gram_ref_kernel(int n, GrefSrc s, int nbb,
+                                                       double *__restrict__ G1,
+                                                       double *__restrict__ G2)
 {
-    return b < s.kr1 ? s.R1 + i * s.ld1 + b : s.R2 + i * s.ld2 + (b - s.kr1);
-}
-
-// row i's 8 L values (columns a0..) and 8 R values (columns b0..), clamped to
-// the last column; VEC: the block lies in one source with 16-byte aligned rows
-template <bool VEC>
-__device__ __forceinline__ void gref_row(const GrefSrc &s, long i, int a0, int b0, double *v)
-{
-    if (VEC) {
-        const double2 *lp = reinterpret_cast<const double2 *>(s.L + i * s.ldl + a0);
-        const double2 *rp = reinterpret_cast<const double2 *>(gref_r(s, i, b0));
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const double2 x = lp[q], y = rp[q];
-            v[2 * q] = x.x;
-            v[2 * q + 1] = x.y;
-            v[8 + 2 * q] = y.x;
-            v[8 + 2 * q + 1] = y.y;
-        }
-    } else {
-        const int kr = s.kr1 + s.kr2;
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const int a = a0 + q < s.kl ? a0 + q : s.kl - 1;
-            const int b = b0 + q < kr ? b0 + q : kr - 1;
-            v[q] = s.L[i * s.ldl + a];
-            v[8 + q] = *gref_r(s, i, b);
-        }
-    }
-}
-
-template <bool VEC>
-__global__ __launch_bounds__(64) void gram_ref_kernel(int n, GrefSrc s, int nbb,
-                                                      double *__restrict__ G1,
-                                                      double *__restrict__ G2)
-{
-    __shared__ double stg[2][64][17];   // row r: L values 0..7, R values 8..15 (odd stride)
-    const int lane = threadIdx.x;
+    __shared__ double stg[2][GREF_ROWS * 16];   // row r: L values 0..7, R values 8..15
+    const int t = threadIdx.x;
     const int a0 = (blockIdx.x / nbb) * 8, b0 = (blockIdx.x % nbb) * 8;
-    const int ai = lane >> 3, bi = lane & 7;
-    const long last = (long)n - 1;
-    const long nst = ((long)n + 63) / 64;
-    double ring[GREF_NS][16];
-    auto load = [&](double *v, long st) __attribute__((always_inline)) {
-        const long i = st * 64 + lane;
-        gref_row<VEC>(s, i < last ? i : last, a0, b0, v);
+    const int kr = s.kr1 + s.kr2;
+    const long nst = ((long)n + GREF_ROWS - 1) / GREF_ROWS;
+    // loader lanes: stage st into buffer st & 1, all loads before the stores
+    auto produce = [&](long st) {
+        double *buf = stg[st & 1];
+        const long i0 = st * GREF_ROWS;
+        const int cnt = (n - i0 < GREF_ROWS ? (int)(n - i0) : GREF_ROWS) * 16;
+        const int tp = t - 64;
+        double v[GREF_J];
+#pragma unroll
+        for (int j = 0; j < GREF_J; j++) {
+            const int e = tp + 192 * j;
+            // rows and columns past the last are clamped to it (never summed)
+            const long i = i0 + (e >> 4) < n ? i0 + (e >> 4) : (long)n - 1;
+            const int q = e & 15, qb = q < 8 ? 0 : q - 8;
+            const int a = a0 + q < s.kl ? a0 + q : s.kl - 1;
+            const int bb = b0 + qb < kr ? b0 + qb : kr - 1;
+            const double *pr = bb < s.kr1 ? s.R1 + i * s.ld1 + bb : s.R2 + i * s.ld2 + (bb - s.kr1);
+            const double *p = q < 8 ? s.L + i * s.ldl + a : pr;
+            v[j] = e < cnt ? *p : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < GREF_J; j++) {
+            const int e = tp + 192 * j;
+            if (e < cnt)
+                buf[e] = v[j];
+        }
     };
-#pragma unroll
-    for (int q = 0; q < GREF_NS; q++)
-        if (q < nst)
-            load(ring[q], q);
+    if (t >= 64 && nst > 0)
+        produce(0);
+    __syncthreads();
+    const int ai = t >> 3, bi = t & 7;
     double acc = 0.0;
-    for (long st0 = 0; st0 < nst; st0 += GREF_NS) {
+    for (long st = 0; st < nst; st++) {
+        if (t >= 64) {
+            if (st + 1 < nst)
+                produce(st + 1);
+        } else {
+            const double *buf = stg[st & 1];
+            const long i0 = st * GREF_ROWS;
+            const int rows = n - i0 < GREF_ROWS ? (int)(n - i0) : GREF_ROWS;
+            int r = 0;
+            if (rows >= 32) {
+                double la[16], ra[16], lb[16], rb[16];
 #pragma unroll
-        for (int q = 0; q < GREF_NS; q++) {
-            const long st = st0 + q;
-            if (st >= nst)
-                break;
-            const int buf = (int)(st & 1);
-            __builtin_amdgcn_wave_barrier();
-#pragma unroll
-            for (int u = 0; u < 16; u++)
-                stg[buf][lane][u] = ring[q][u];
-            __builtin_amdgcn_wave_barrier();
-            if (st + GREF_NS < nst)
-                load(ring[q], st + GREF_NS);
-            const long i0 = st * 64;
-            const int rows = n - i0 < 64 ? (int)(n - i0) : 64;
-            if (rows == 64) {
-                // operands of the next 16 rows read while the current 16 are
-                // summed, so the chain does not wait on each LDS read
-                double lv[2][16], rv[2][16];
-                auto rd = [&](int bb, int r0) __attribute__((always_inline)) {
+                for (int u = 0; u < 16; u++) {
+                    la[u] = buf[u * 16 + ai];
+                    ra[u] = buf[u * 16 + 8 + bi];
+                }
+                for (; r + 32 <= rows; r += 32) {
 #pragma unroll
                     for (int u = 0; u < 16; u++) {
-                        lv[bb][u] = stg[buf][r0 + u][ai];
-                        rv[bb][u] = stg[buf][r0 + u][8 + bi];
+                        lb[u] = buf[(r + 16 + u) * 16 + ai];
+                        rb[u] = buf[(r + 16 + u) * 16 + 8 + bi];
                     }
-                };
-                rd(0, 0);
-#pragma unroll
-                for (int h = 0; h < 4; h++) {
-                    if (h < 3)
-                        rd((h + 1) & 1, 16 * (h + 1));
 #pragma unroll
                     for (int u = 0; u < 16; u++)
-                        acc = add_rn(acc, mul_rn(lv[h & 1][u], rv[h & 1][u]));
+                        acc = add_rn(acc, mul_rn(la[u], ra[u]));
+                    if (r + 64 <= rows) {
+#pragma unroll
+                        for (int u = 0; u < 16; u++) {
+                            la[u] = buf[(r + 32 + u) * 16 + ai];
+                            ra[u] = buf[(r + 32 + u) * 16 + 8 + bi];
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < 16; u++)
+                        acc = add_rn(acc, mul_rn(lb[u], rb[u]));
                 }
-            } else {
-                for (int r = 0; r < rows; r++)
-                    acc = add_rn(acc, mul_rn(stg[buf][r][ai], stg[buf][r][8 + bi]));
             }
+            for (; r < rows; r++)
+                acc = add_rn(acc, mul_rn(buf[r * 16 + ai], buf[r * 16 + 8 + bi]));
         }
+        __syncthreads();
     }
-    const int a = a0 + ai, b = b0 + bi;
-    if (a < s.kl) {
-        if (b < s.kr1)
-            G1[(long)a * s.kr1 + b] = acc;
-        else if (b < s.kr1 + s.kr2)
-            G2[(long)a * s.kr2 + (b - s.kr1)] = acc;
+    if (t < 64) {
+        const int a = a0 + ai, b = b0 + bi;
+        if (a < s.kl) {
+            if (b < s.kr1)
+                G1[(long)a * s.kr1 + b] = acc;
+            else if (b < kr)
+                G2[(long)a * s.kr2 + (b - s.kr1)] = acc;
+        }
     }
 }
 
@@ -712,17 +698,8 @@ extern "C" int cacg_k_gram_ref(int n, const double *L, long ldl, int kl, const d
     const int kr = kr1 + kr2;
     const int nab = (kl + 7) / 8, nbb = (kr + 7) / 8;
     const GrefSrc s = {L, ldl, kl, R1, ld1, kr1, R2, ld2, kr2};
-    // 16-byte loads when every 8 x 8 block is whole, inside one source, with
-    // 16-byte aligned rows
-    const bool vec = kl % 8 == 0 && kr1 % 8 == 0 && kr2 % 8 == 0 && ((uintptr_t)L & 15) == 0 &&
-                     ldl % 2 == 0 && (kr1 == 0 || (((uintptr_t)R1 & 15) == 0 && ld1 % 2 == 0)) &&
-                     (kr2 == 0 || (((uintptr_t)R2 & 15) == 0 && ld2 % 2 == 0));
-    if (vec)
-        hipLaunchKernelGGL(gram_ref_kernel<true>, dim3((unsigned)(nab * nbb)), dim3(64), 0, st, n,
-                           s, nbb, G1, G2);
-    else
-        hipLaunchKernelGGL(gram_ref_kernel<false>, dim3((unsigned)(nab * nbb)), dim3(64), 0, st, n,
-                           s, nbb, G1, G2);
+    hipLaunchKernelGGL(gram_ref_kernel, dim3((unsigned)(nab * nbb)), dim3(256), 0, st, n, s, nbb,
+                       G1, G2);
     CACG_LAUNCH_CHECK();
     return 0;
 }
