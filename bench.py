@@ -70,6 +70,8 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=400.0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run 3 device-to-device copies of 2 GiB (HBM counter calibration)")
+    ap.add_argument("--form", choices=["batching", "serving", "long-rows"],
+                    help="only run this form and print its JSON (for profiling it alone)")
     return ap.parse_args()
 
 
@@ -166,6 +168,11 @@ def main():
     args = parse()
     if args.copy_probe:
         copy_probe()
+        return
+    if args.form:
+        run = {"batching": batching_form, "long-rows": spmm_long_rows,
+               "serving": lambda: serving_form(args.serve_requests, args.serve_rate)}[args.form]
+        print(json.dumps(run()))
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -568,22 +575,19 @@ def _timed_solves(A, B, m, s, args, precision, reps=3, **kw):
 
 
 def batching_form(count=256):
-    """256 shifted 2D Laplacians in one batch vs one at a time (FP64 path)."""
+    """256 shifted 2D Laplacians in one batch vs one at a time (FP64 path).
+    All arrive at t = 0; four feeder threads submit them (host-side copies of
+    ~0.75 GB in all) while the first rounds already run."""
     import cacg_amd
     from cacg_amd import batch as BT
     systems = BT.batch_systems(count, 3)
     b = BT.Batch(s=4, precision="fp32", hbm_bytes=48 << 30, maxiter=2000)
-    t0 = time.perf_counter()
-    ids = [b.submit(A, Bm, tol=1e-10) for A, Bm in systems]
-    lat, res, rounds = {}, {}, 0
-    while len(res) < len(ids):
-        rounds = b.step()["rounds"]
-        now = time.perf_counter() - t0
-        for rid in b.retired():
-            res[rid] = b.collect(rid)
-            lat[rid] = now
-    wall = time.perf_counter() - t0
+    out, wall = BT.serve_stream([(0.0, A, Bm) for A, Bm in systems], b, tol=1e-10, feeders=4)
+    rounds = b.stats.rounds
     b.close()
+    ids = list(range(count))
+    res = dict(enumerate(out))
+    lat = {i: r["latency"] for i, r in res.items()}
     t1 = time.perf_counter()
     for A, Bm in systems:
         cacg_amd.bcbcg(A, Bm, np.zeros_like(Bm), 4, tol=1e-10, maxiter=2000)
@@ -592,7 +596,8 @@ def batching_form(count=256):
     print(f"bench: batching form {count / wall:.1f} solves/s vs {count / seq:.1f} one at a time",
           file=sys.stderr, flush=True)
     return {"workload": f"{count} shifted 2D Laplacians (L log-uniform 32..256, shift U(0,0.1), "
-                        "m in {1,2,4,8,16,32}, seed 3), s=4, 1e-10, FP32 basis, all submitted at t=0",
+                        "m in {1,2,4,8,16,32}, seed 3), s=4, 1e-10, FP32 basis, all arriving at t=0, "
+                        "submitted by four feeder threads",
             "solves_per_s": count / wall, "p50_ms": float(np.percentile(L, 50)),
             "p99_ms": float(np.percentile(L, 99)), "rounds": rounds,
             "converged": int(sum(res[i]["status"] == "ok" for i in ids)),

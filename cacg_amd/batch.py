@@ -215,7 +215,7 @@ def serving_stream(count, rate, seed=4, l2=(64, 512), e3=(32, 128), mrange=(1, 3
 # ---- drivers -----------------------------------------------------------------------------
 
 
-def serve_stream(stream, batch: Batch, tol=1e-10, timeout=600.0, feeder=True):
+def serve_stream(stream, batch: Batch, tol=1e-10, timeout=600.0, feeder=True, feeders=1):
     """Continuous batching of a stream of (t_arrival, A, B) requests.
 
     feeder=True: a feeder thread submits each request once its arrival time has
@@ -223,6 +223,10 @@ def serve_stream(stream, batch: Batch, tol=1e-10, timeout=600.0, feeder=True):
     matrix and B, and computes its Gerschgorin interval) run while a round waits
     for the GPU; the calling thread runs rounds while any submitted request is
     unfinished, collects what retired, and waits for the feeder when idle.
+    feeders > 1: requests are dealt round-robin to that many feeder threads, so
+    several submissions' host-side copies run at once (requests arriving
+    together may then be submitted in another order; every result is the same
+    bits whatever the order).
     feeder=False: submissions happen between rounds on the calling thread.
     Returns (results in stream order with 'latency' from arrival to the end of
     the retiring round, wall seconds from the first arrival to the last
@@ -239,9 +243,10 @@ def serve_stream(stream, batch: Batch, tol=1e-10, timeout=600.0, feeder=True):
             _, A, B = stream[i]
             ids[batch.submit(A, B, tol=tol)] = i
 
-    def feed():
+    def feed(first=0, step=1):
         try:
-            for i, (t_arr, A, B) in enumerate(stream):
+            for i in range(first, count, step):
+                t_arr, A, B = stream[i]
                 d = t_arr - (time.perf_counter() - t0)
                 if d > 0:
                     time.sleep(d)
@@ -255,11 +260,13 @@ def serve_stream(stream, batch: Batch, tol=1e-10, timeout=600.0, feeder=True):
                 cv.notify_all()
 
     pending = list(range(count))
-    th = None
+    ths = []
     if feeder:
         pending = []
-        th = threading.Thread(target=feed, daemon=True)
-        th.start()
+        k = max(1, int(feeders))
+        ths = [threading.Thread(target=feed, args=(j, k), daemon=True) for j in range(k)]
+        for th in ths:
+            th.start()
     while len(res) < count:
         if time.perf_counter() - t0 > timeout:
             raise RuntimeError("serving stream timed out")
@@ -286,7 +293,7 @@ def serve_stream(stream, batch: Batch, tol=1e-10, timeout=600.0, feeder=True):
                 i = ids[rid]
             r["latency"] = t_now - stream[i][0]
             res[i] = r
-    if th is not None:
+    for th in ths:
         th.join()
     wall = (time.perf_counter() - t0) - stream[0][0]
     return [res[i] for i in range(count)], wall
