@@ -610,13 +610,15 @@ def batching_form(count=256):
 # for many small systems, take twice as long per iteration for a lone system of
 # 40,000-130,000 rows); the batching form keeps the library default (2^17)
 SERVE_SOLO_ROWS = 32768
+# feeder threads submitting the serving stream's requests as they arrive
+SERVE_FEEDERS = 4
 
 
 def serving_form(count, rate):
     """Poisson streams at several offered loads: continuous batching (smallest
     HBM footprint first, at most two large systems in flight, systems of
     SERVE_SOLO_ROWS rows or more on the single-system kernels, requests
-    submitted by a feeder thread as they arrive) against the same stream
+    submitted by SERVE_FEEDERS feeder threads as they arrive) against the same stream
     solved one request at a time (FIFO) with the FP64 path.  The
     lowest rate is below the capacity of either server (a few solves/s on
     this mix), so latency there is service time; the higher ones build a
@@ -629,7 +631,7 @@ def serving_form(count, rate):
         stream = BT.serving_stream(cnt, r, seed=4)
         b = BT.Batch(s=4, precision="fp64", hbm_bytes=96 << 30, maxiter=3000, policy="smallest",
                      max_large=2, solo_rows=SERVE_SOLO_ROWS)
-        res, wall = BT.serve_stream(stream, b, tol=1e-10)
+        res, wall = BT.serve_stream(stream, b, tol=1e-10, feeders=SERVE_FEEDERS)
         b.close()
         seq, wall_seq = BT.serve_sequential(stream, tol=1e-10, s=4, maxiter=3000)
         lb = np.array([q["latency"] for q in res]) * 1e3
@@ -649,7 +651,7 @@ def serving_form(count, rate):
                                "Laplacians, 1..32 RHS, s=4, 1e-10, FP64 basis; continuous batching, "
                                "smallest footprint first, at most 2 large systems in flight, "
                                f"systems of {SERVE_SOLO_ROWS} rows or more on the single-system "
-                               "kernels, submissions from a feeder thread"),
+                               f"kernels, submissions from {SERVE_FEEDERS} feeder threads"),
                 sweep=sweep)
 
 
