@@ -230,11 +230,13 @@ def serve_stream(stream, batch: Batch, tol=1e-10, timeout=600.0, feeder=True, fe
     feeder=False: submissions happen between rounds on the calling thread.
     Returns (results in stream order with 'latency' from arrival to the end of
     the retiring round, wall seconds from the first arrival to the last
-    retirement)."""
+    retirement).  On an error or the timeout the feeders are stopped and joined
+    before the exception propagates, so the caller may close the batch."""
     count = len(stream)
     ids, res = {}, {}
     cv = threading.Condition()
     err = []
+    stop = threading.Event()
     t0 = time.perf_counter()
 
     def submit_due(now):
@@ -248,8 +250,10 @@ def serve_stream(stream, batch: Batch, tol=1e-10, timeout=600.0, feeder=True, fe
             for i in range(first, count, step):
                 t_arr, A, B = stream[i]
                 d = t_arr - (time.perf_counter() - t0)
-                if d > 0:
-                    time.sleep(d)
+                if d > 0 and stop.wait(d):
+                    return
+                if stop.is_set():
+                    return
                 rid = batch.submit(A, B, tol=tol)
                 with cv:
                     ids[rid] = i
@@ -267,6 +271,22 @@ def serve_stream(stream, batch: Batch, tol=1e-10, timeout=600.0, feeder=True, fe
         ths = [threading.Thread(target=feed, args=(j, k), daemon=True) for j in range(k)]
         for th in ths:
             th.start()
+    try:
+        _serve_rounds(stream, batch, feeder, count, ids, res, cv, err, t0, timeout, pending,
+                      submit_due)
+    finally:
+        stop.set()
+        with cv:
+            cv.notify_all()
+        for th in ths:
+            th.join()
+    wall = (time.perf_counter() - t0) - stream[0][0]
+    return [res[i] for i in range(count)], wall
+
+
+def _serve_rounds(stream, batch, feeder, count, ids, res, cv, err, t0, timeout, pending,
+                  submit_due):
+    """The round loop of serve_stream on the calling thread."""
     while len(res) < count:
         if time.perf_counter() - t0 > timeout:
             raise RuntimeError("serving stream timed out")
@@ -293,10 +313,6 @@ def serve_stream(stream, batch: Batch, tol=1e-10, timeout=600.0, feeder=True, fe
                 i = ids[rid]
             r["latency"] = t_now - stream[i][0]
             res[i] = r
-    for th in ths:
-        th.join()
-    wall = (time.perf_counter() - t0) - stream[0][0]
-    return [res[i] for i in range(count)], wall
 
 
 def serve_sequential(stream, tol=1e-10, s=4, maxiter=1000, precision="fp64"):

@@ -4,6 +4,7 @@ without a device."""
 
 import ctypes
 
+import numpy as np
 import pytest
 
 from cacg_amd import batch as BT
@@ -89,3 +90,58 @@ def test_batch_options_rejected_without_device(field, value):
     h = ctypes.c_void_p()
     assert BT.lib().cacg_batch_create(ctypes.byref(h), ctypes.byref(o)) == 3
     assert not h.value
+
+
+class _FakeBatch:
+    """A stand-in for Batch with its call pattern: a request retires two
+    rounds after it is submitted.  submit may be called from several threads."""
+
+    def __init__(self, fail_at=None):
+        import threading
+        self.lock = threading.Lock()
+        self.sub, self.ret, self.done = [], [], {}
+        self.rounds, self.fail_at = 0, fail_at
+
+    def submit(self, A, B, tol=1e-10):
+        with self.lock:
+            if self.fail_at is not None and len(self.sub) == self.fail_at:
+                raise ValueError("refused")
+            rid = len(self.sub)
+            self.sub.append((rid, self.rounds, B))
+            return rid
+
+    def step(self):
+        with self.lock:
+            self.rounds += 1
+            for rid, r0, B in self.sub:
+                if rid not in self.done and self.rounds - r0 >= 2:
+                    self.done[rid] = B
+                    self.ret.append(rid)
+        return {}
+
+    def retired(self):
+        with self.lock:
+            out, self.ret = self.ret, []
+            return out
+
+    def collect(self, rid):
+        with self.lock:
+            return {"status": "ok", "X": self.done[rid], "iters": 1}
+
+
+@pytest.mark.parametrize("feeders", [1, 3])
+@pytest.mark.parametrize("gap", [0.0, 0.002])
+def test_serve_stream_feeders_return_every_request_in_order(feeders, gap):
+    stream = [(gap * k, None, np.full((2, 1), float(k))) for k in range(20)]
+    res, wall = BT.serve_stream(stream, _FakeBatch(), feeders=feeders)
+    assert [float(r["X"][0, 0]) for r in res] == [float(k) for k in range(20)]
+    assert all(r["latency"] >= 0 for r in res) and wall >= 0
+
+
+def test_serve_stream_stops_its_feeders_on_an_error():
+    import threading
+    before = threading.active_count()
+    stream = [(0.001 * k, None, np.zeros((2, 1))) for k in range(30)]
+    with pytest.raises(ValueError):
+        BT.serve_stream(stream, _FakeBatch(fail_at=5), feeders=3)
+    assert threading.active_count() == before
