@@ -175,7 +175,7 @@ def main():
         print(json.dumps(run()))
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: the solver runs on one GPU; --gpus {args.gpus} is not supported "
               f"(multi-GPU sharding is a design, not a measured path)", file=sys.stderr)
         sys.exit(2)
 
