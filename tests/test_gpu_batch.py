@@ -259,11 +259,13 @@ def test_smallest_first_admission(gpu_lib, max_bypass):
         assert np.array_equal(res[rid]["X"], fifo[k]["X"])
 
 
-def test_feeder_thread_submissions_give_the_same_results(gpu_lib):
-    """A stream served with submissions from a feeder thread (cacg_batch_submit
-    beside cacg_batch_step) and with submissions between rounds: every request
-    retires with the same iterations and the same bits, and converges.  Mixed
-    sizes, one large system (2^17 rows or more) among them."""
+@pytest.mark.parametrize("feeders", [1, 3])
+def test_feeder_thread_submissions_give_the_same_results(gpu_lib, feeders):
+    """A stream served with submissions from feeder threads (cacg_batch_submit
+    beside cacg_batch_step, and with three feeders beside one another) and with
+    submissions between rounds: every request retires with the same iterations
+    and the same bits, and converges.  Mixed sizes, one large system (2^17 rows
+    or more) among them."""
     stream = []
     for k, (L, m) in enumerate([(40, 4), (96, 8), (64, 16), (368, 8), (48, 1), (80, 32),
                                 (72, 2), (56, 12)]):
@@ -274,7 +276,7 @@ def test_feeder_thread_submissions_give_the_same_results(gpu_lib):
     for fd in (False, True):
         b = BT.Batch(s=4, precision="fp64", hbm_bytes=8 << 30, maxiter=2000, policy="smallest",
                      max_large=1)
-        out[fd], _ = BT.serve_stream(stream, b, tol=1e-10, feeder=fd)
+        out[fd], _ = BT.serve_stream(stream, b, tol=1e-10, feeder=fd, feeders=feeders)
         b.close()
     for (_, A, B), r0, r1 in zip(stream, out[False], out[True]):
         assert r0["status"] == r1["status"] == "ok"
