@@ -15,6 +15,8 @@
 // gives the same product; only rounding differs.
 #include "kcommon.hpp"
 
+#include <type_traits>
+
 namespace {
 
 __device__ __forceinline__ int xcd_remap(int g, int G)
@@ -158,42 +160,72 @@ __global__ __launch_bounds__(256) void xrupdate_ref_kernel(int n, int sm, int m,
 
 // ||R(:, c)||^2, one sequential chain per column in row order, 64 columns (W
 // of them) per workgroup.  Lanes c < W of wave 0 keep the chains; waves 1-7
-// meanwhile load the next stage of CREF_SEGE / W rows (coalesced when
-// ldr == W) into the other of two LDS buffers, so a stage costs the chain one
-// barrier instead of a global latency.  The chain lanes read their column 16
-// rows ahead of the sums.
-constexpr int CREF_T = 512;                         // lanes: wave 0 sums, the rest load
+// meanwhile load the next stage of CREF_SEGE / W rows into the other of two
+// LDS buffers, column by column (stride S, even), so a chain lane reads two
+// rows of its column with one 16-byte LDS read, and reads the next batch of
+// rows while it adds the previous one.  The loader lanes walk their elements with
+// increments (no division per element): they share the chain wave's SIMD.
+// VEC = 2: a loader lane loads two adjacent columns of a row with one 16-byte
+// load (m and ldr even, R 16-byte aligned).
+constexpr int CREF_T = 512;                  // lanes: wave 0 sums, the rest load
 constexpr int CREF_P = CREF_T - 64;
-constexpr int CREF_SEGE = 8192;                     // values per LDS buffer (two, dynamic LDS)
-constexpr int CREF_J = (CREF_SEGE + CREF_P - 1) / CREF_P;
+constexpr int CREF_SEGE = 8192;              // values per stage
+constexpr int CREF_BUF = CREF_SEGE + 256;    // doubles per LDS buffer (two, dynamic LDS)
 
+template <int VEC>
 __global__ __launch_bounds__(CREF_T) void colnorm2_ref_kernel(int n, int m,
                                                               const double *__restrict__ R,
                                                               long ldr, double *__restrict__ out)
 {
-    extern __shared__ double cbuf[];   // two stage buffers of CREF_SEGE
+    constexpr int J = (CREF_SEGE / VEC + CREF_P - 1) / CREF_P;   // loads per loader lane and stage
+    extern __shared__ double cbuf[];
     const int t = threadIdx.x;
     const int c0 = blockIdx.x * 64;
     const int W = m - c0 < 64 ? m - c0 : 64;
-    const int T = CREF_SEGE / W;   // rows per stage
+    const int T = CREF_SEGE / W;           // rows per stage
+    const int S = ((T + 1) & ~1) + 2;      // LDS column stride; W * S <= CREF_BUF
     const long nst = ((long)n + T - 1) / T;
-    // loader lanes: stage st into buffer st & 1, all loads before the stores
+    // loader lane: first element VEC * (t - 64) of a stage (row r0, column q0),
+    // each next one VEC * CREF_P further (dr rows and dq columns on)
+    const int e0 = VEC * (t - 64), de = VEC * CREF_P;
+    const int r0 = e0 / W, q0 = e0 % W, dr = de / W, dq = de % W;
     auto produce = [&](long st) {
-        double *buf = cbuf + (st & 1) * CREF_SEGE;
+        double *buf = cbuf + (st & 1) * CREF_BUF;
         const long i0 = st * T;
-        const int cnt = (n - i0 < T ? (int)(n - i0) : T) * W;
-        const int tp = t - 64;
-        double v[CREF_J];
+        const int rows = n - i0 < T ? (int)(n - i0) : T;
+        using V = typename std::conditional<VEC == 2, double2, double>::type;
+        V v[J];
+        int r = r0, q = q0;
 #pragma unroll
-        for (int j = 0; j < CREF_J; j++) {
-            const int e = tp + CREF_P * j;
-            v[j] = e < cnt ? R[(i0 + e / W) * ldr + c0 + e % W] : 0.0;
+        for (int j = 0; j < J; j++) {
+            const double *p = R + (i0 + r) * ldr + c0 + q;
+            if (r < rows)
+                v[j] = *(const V *)p;
+            r += dr;
+            q += dq;
+            if (q >= W) {
+                q -= W;
+                r++;
+            }
         }
+        r = r0;
+        q = q0;
 #pragma unroll
-        for (int j = 0; j < CREF_J; j++) {
-            const int e = tp + CREF_P * j;
-            if (e < cnt)
-                buf[e] = v[j];
+        for (int j = 0; j < J; j++) {
+            if (r < rows) {
+                if constexpr (VEC == 2) {
+                    buf[q * S + r] = v[j].x;
+                    buf[(q + 1) * S + r] = v[j].y;
+                } else {
+                    buf[q * S + r] = v[j];
+                }
+            }
+            r += dr;
+            q += dq;
+            if (q >= W) {
+                q -= W;
+                r++;
+            }
         }
     };
     if (t >= 64 && nst > 0)
@@ -205,36 +237,48 @@ __global__ __launch_bounds__(CREF_T) void colnorm2_ref_kernel(int n, int m,
             if (st + 1 < nst)
                 produce(st + 1);
         } else if (t < W) {
-            const double *buf = cbuf + (st & 1) * CREF_SEGE;
+            const double *col = cbuf + (st & 1) * CREF_BUF + t * S;
             const long i0 = st * T;
             const int rows = n - i0 < T ? (int)(n - i0) : T;
+            // NB rows a batch (fewer with VEC 1, whose loader holds more registers)
+            constexpr int NB = VEC == 2 ? 32 : 16, NV = NB / 2;
+            double2 a[NV], b[NV];
             int r = 0;
-            if (rows >= 32) {
-                double va[16], vb[16];
+            if (rows >= NB) {
 #pragma unroll
-                for (int u = 0; u < 16; u++)
-                    va[u] = buf[u * W + t];
-                for (; r + 32 <= rows; r += 32) {
+                for (int u = 0; u < NV; u++)
+                    a[u] = *(const double2 *)(col + 2 * u);
+            }
+            for (; r + 2 * NB <= rows; r += 2 * NB) {
 #pragma unroll
-                    for (int u = 0; u < 16; u++)
-                        vb[u] = buf[(r + 16 + u) * W + t];
+                for (int u = 0; u < NV; u++)
+                    b[u] = *(const double2 *)(col + r + NB + 2 * u);
 #pragma unroll
-                    for (int u = 0; u < 16; u++)
-                        acc = add_rn(acc, mul_rn(va[u], va[u]));
-                    if (r + 64 <= rows) {
+                for (int u = 0; u < NV; u++) {
+                    acc = add_rn(acc, mul_rn(a[u].x, a[u].x));
+                    acc = add_rn(acc, mul_rn(a[u].y, a[u].y));
+                }
+                if (r + 3 * NB <= rows) {
 #pragma unroll
-                        for (int u = 0; u < 16; u++)
-                            va[u] = buf[(r + 32 + u) * W + t];
-                    }
+                    for (int u = 0; u < NV; u++)
+                        a[u] = *(const double2 *)(col + r + 2 * NB + 2 * u);
+                }
 #pragma unroll
-                    for (int u = 0; u < 16; u++)
-                        acc = add_rn(acc, mul_rn(vb[u], vb[u]));
+                for (int u = 0; u < NV; u++) {
+                    acc = add_rn(acc, mul_rn(b[u].x, b[u].x));
+                    acc = add_rn(acc, mul_rn(b[u].y, b[u].y));
                 }
             }
-            for (; r < rows; r++) {
-                const double v = buf[r * W + t];
-                acc = add_rn(acc, mul_rn(v, v));
+            if (r + NB <= rows) {
+#pragma unroll
+                for (int u = 0; u < NV; u++) {
+                    acc = add_rn(acc, mul_rn(a[u].x, a[u].x));
+                    acc = add_rn(acc, mul_rn(a[u].y, a[u].y));
+                }
+                r += NB;
             }
+            for (; r < rows; r++)
+                acc = add_rn(acc, mul_rn(col[r], col[r]));
         }
         __syncthreads();
     }
@@ -619,10 +663,16 @@ extern "C" int cacg_k_colnorm2_ref(int n, int m, const double *R, long ldr, doub
 {
     if (n < 0 || m <= 0)
         return (int)hipErrorInvalidValue;
-    const size_t lds = 2 * CREF_SEGE * sizeof(double);
-    cacg_allow_lds(colnorm2_ref_kernel, lds);
-    hipLaunchKernelGGL(colnorm2_ref_kernel, dim3((m + 63) / 64), dim3(CREF_T), lds, st, n, m, R,
-                       ldr, d_norm2);
+    const size_t lds = 2 * CREF_BUF * sizeof(double);
+    if (m % 2 == 0 && ldr % 2 == 0 && (uintptr_t)R % 16 == 0) {
+        cacg_allow_lds(colnorm2_ref_kernel<2>, lds);
+        hipLaunchKernelGGL(colnorm2_ref_kernel<2>, dim3((m + 63) / 64), dim3(CREF_T), lds, st, n,
+                           m, R, ldr, d_norm2);
+    } else {
+        cacg_allow_lds(colnorm2_ref_kernel<1>, lds);
+        hipLaunchKernelGGL(colnorm2_ref_kernel<1>, dim3((m + 63) / 64), dim3(CREF_T), lds, st, n,
+                           m, R, ldr, d_norm2);
+    }
     CACG_LAUNCH_CHECK();
     return 0;
 }
