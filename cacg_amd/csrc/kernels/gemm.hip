@@ -113,6 +113,77 @@ __global__ __launch_bounds__(256) void qupdate_ref_kernel(int n, int sm, const T
     }
 }
 
+// 512 lanes (two waves per SIMD), sm even and dividing 512: a thread updates
+// QR2_RPT rows of one column c, rows 512 / sm apart.  beta(q, c) and
+// beta(q + 1, c) come with one paired LDS read, each row's Q(r, q), Q(r, q + 1)
+// with one 16-byte LDS read (all lanes of a wave read the same row), so two
+// steps cost 1 + QR2_RPT LDS reads for 4 QR2_RPT FP64 operations.  The next
+// tile's Q values (each thread's own four, coalesced) and the tile's S values
+// are in flight while the tile is summed.  Each entry sums q = 0, 1, ... in
+// order, as the other reference-order kernel does.
+constexpr int QR2_T = 512, QR2_RPT = 4;
+
+template <typename TS, typename T>
+__global__ __launch_bounds__(QR2_T) void qupdate_ref2_kernel(int n, int sm,
+                                                             const TS *__restrict__ S, T *Q,
+                                                             const double *__restrict__ beta)
+{
+    extern __shared__ double lds[];
+    double *bl = lds;                     // sm x sm
+    double *qt = lds + (long)sm * sm;     // rb x sm
+    const int t = threadIdx.x;
+    const int P = QR2_T / sm, rb = QR2_RPT * P;
+    const int c = t % sm, r0 = t / sm;
+    for (int e = t; e < sm * sm; e += QR2_T)
+        bl[e] = beta[e];
+    const int ntile = (n + rb - 1) / rb;
+    double v[QR2_RPT];
+    auto load = [&](int tile) {
+        const long i0 = (long)tile * rb;
+#pragma unroll
+        for (int u = 0; u < QR2_RPT; u++) {
+            const long i = i0 + r0 + u * P;
+            v[u] = i < n ? (double)Q[i * sm + c] : 0.0;
+        }
+    };
+    int tile = blockIdx.x;
+    if (tile < ntile)
+        load(tile);
+    for (; tile < ntile; tile += gridDim.x) {
+#pragma unroll
+        for (int u = 0; u < QR2_RPT; u++)
+            qt[(r0 + u * P) * sm + c] = v[u];
+        __syncthreads();
+        if (tile + (int)gridDim.x < ntile)
+            load(tile + gridDim.x);
+        const long i0 = (long)tile * rb;
+        double sv[QR2_RPT], acc[QR2_RPT];
+#pragma unroll
+        for (int u = 0; u < QR2_RPT; u++) {
+            const long i = i0 + r0 + u * P;
+            sv[u] = (i < n && S) ? (double)S[i * sm + c] : 0.0;
+            acc[u] = 0.0;
+        }
+        const double *bc = bl + c, *qr = qt + (long)r0 * sm;
+        for (int q = 0; q < sm; q += 2) {
+            const double b0 = bc[(long)q * sm], b1 = bc[(long)(q + 1) * sm];
+#pragma unroll
+            for (int u = 0; u < QR2_RPT; u++) {
+                const double2 x = *(const double2 *)(qr + (long)u * P * sm + q);
+                acc[u] = add_rn(acc[u], mul_rn(b0, x.x));
+                acc[u] = add_rn(acc[u], mul_rn(b1, x.y));
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < QR2_RPT; u++) {
+            const long i = i0 + r0 + u * P;
+            if (i < n)
+                Q[i * sm + c] = (T)sub_rn(sv[u], acc[u]);
+        }
+        __syncthreads();   // the tile buffer is refilled next
+    }
+}
+
 // ALPHA_LDS: alpha (sm x m) staged in LDS; otherwise read from global memory
 // (wide shapes whose alpha does not fit beside the row tiles)
 template <typename T, bool ALPHA_LDS>
@@ -698,6 +769,20 @@ int qupdate(int n, int sm, const TS *S, T *Q, const double *beta, int mode, hipS
         case 16: return launch_qupdate_mfma<TS, T, 16>(n, S, Q, beta, st);
         default: break;  // other widths: the reference-order kernel
         }
+    }
+    // 512 lanes when sm is even, divides 512 and beta plus the tile fit in LDS
+    if (sm % 2 == 0 && sm <= QR2_T && QR2_T % sm == 0 &&
+        (long)sm * sm + (long)QR2_RPT * QR2_T <= 20480 - 256) {
+        const size_t shm = ((size_t)sm * sm + (size_t)QR2_RPT * QR2_T) * sizeof(double);
+        const int rb = QR2_RPT * (QR2_T / sm);
+        const int ntile = (n + rb - 1) / rb;
+        const int cap = shm > 80 * 1024 ? 256 : 1024;   // one workgroup per CU above 80 KiB
+        const int G = ntile < cap ? ntile : cap;
+        cacg_allow_lds(qupdate_ref2_kernel<TS, T>, shm);
+        hipLaunchKernelGGL((qupdate_ref2_kernel<TS, T>), dim3(G), dim3(QR2_T), shm, st, n, sm, S,
+                           Q, beta);
+        CACG_LAUNCH_CHECK();
+        return 0;
     }
     // 4 rows per thread when sm divides 256 and beta plus the tile fit in LDS
     const bool rows4 = sm <= 256 && 256 % sm == 0 &&
