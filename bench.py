@@ -70,7 +70,7 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=400.0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run 3 device-to-device copies of 2 GiB (HBM counter calibration)")
-    ap.add_argument("--form", choices=["batching", "serving", "long-rows"],
+    ap.add_argument("--form", choices=["batching", "serving", "long-rows", "reference-order"],
                     help="only run this form and print its JSON (for profiling it alone)")
     return ap.parse_args()
 
@@ -171,6 +171,7 @@ def main():
         return
     if args.form:
         run = {"batching": batching_form, "long-rows": spmm_long_rows,
+               "reference-order": reference_order_form,
                "serving": lambda: serving_form(args.serve_requests, args.serve_rate)}[args.form]
         print(json.dumps(run()))
         return
@@ -393,6 +394,39 @@ def dropin_call(A, B, s, eps, order):
             else:
                 os.environ[k] = v
     return t, int(L.cacg_last_iterations()), cacg_amd.STATUS.get(L.cacg_last_status(), "?")
+
+
+def reference_order_form(iters=3):
+    """A few outer iterations of the reference summation order (the drop-in's
+    default: every Gram entry and column norm is one chain over all n rows)
+    at the benchmark workload, 128^3 Laplacian, 16 right-hand sides, s = 8:
+    one untimed solve of `iters` iterations (a solver's first solve tunes its
+    SpMM schedule), then the same solve timed; wall time and per-phase ms per
+    iteration, for profiling the reference-order kernels on their own."""
+    import cacg_amd
+    from cacg_amd import problems as P
+    L = cacg_amd.lib()
+    A = P.laplacian_3d(128)
+    B = P.rhs_uniform(A.n, 16, 1)
+    eps = 1e-10 * float(np.sqrt(np.sum(B[:, 0] ** 2)))
+    dB = cacg_amd.DeviceBuffer.from_array(B)
+    dX = cacg_amd.DeviceBuffer(B.nbytes)
+    out = {"workload": "3D Laplacian 128^3, m=16, s=8, reference order"}
+    sv = cacg_amd.Solver(A, 16, 8, stop_mode=0, eps=eps, maxiter=iters, order="reference")
+    for _ in range(2):   # the second solve is the timed one
+        L.cacg_k_fill_f64(dX.ptr, A.n * 16, 0.0, None)
+        L.cacg_dev_sync()
+        t = time.perf_counter()
+        rep = sv.solve(dB, dX)
+        L.cacg_dev_sync()
+        dt = time.perf_counter() - t
+    it = max(rep["iters"], 1)
+    out.update({"iters": rep["iters"], "ms_per_iteration": dt * 1e3 / it,
+                "phase_ms_per_iteration": {k: round(v / it, 3) for k, v in rep["phase_ms"].items()
+                                           if v}})
+    print(f"bench: reference order {out['ms_per_iteration']:.1f} ms per iteration",
+          file=sys.stderr, flush=True)
+    return out
 
 
 def spmm_long_rows(reps=20):
