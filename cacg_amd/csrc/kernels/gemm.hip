@@ -359,6 +359,37 @@ __global__ __launch_bounds__(CREF_T) void colnorm2_ref_kernel(int n, int m,
 
 // ---- fast order: MFMA -----------------------------------------------------------
 
+// acc after the MFMAs of a[tt] x b[tt * stride], tt = 0 .. KQ - 1 in order, the
+// B values read from LDS one chunk of 4 steps ahead into alternating
+// registers: the scheduling barriers keep each chunk's reads ahead of the
+// previous chunk's MFMAs (left alone, the compiler reads each pair of steps
+// into one register pair just before its MFMAs and waits on it there).
+template <int KQ>
+__device__ __forceinline__ dbl4 mfma_chain_lds(const double *a, const double *b, int stride,
+                                               dbl4 acc)
+{
+    constexpr int CH = 4;
+    static_assert(KQ % CH == 0, "chunks of 4 steps");
+    double bq[2][CH];
+#pragma unroll
+    for (int u = 0; u < CH; u++)
+        bq[0][u] = b[u * stride];
+#pragma unroll
+    for (int c = 0; c < KQ / CH; c++) {
+        if (c + 1 < KQ / CH) {
+#pragma unroll
+            for (int u = 0; u < CH; u++)
+                bq[(c + 1) & 1][u] = b[((c + 1) * CH + u) * stride];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < CH; u++)
+            acc = mfma16x16x4(a[c * CH + u], bq[c & 1][u], acc);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    return acc;
+}
+
 // rows per 16-row tile handled by one wave; K = SM, KQ = SM / 4 per lane group
 // TS, T = storage types of S and Q (double, or float for an FP32 basis); the
 // products and sums are FP64 either way
@@ -446,11 +477,8 @@ __global__ __launch_bounds__(512) void qupdate_mfma_kernel(int n, const TS *__re
         };
         // Q(:, tile j) = S - Q beta for the rows of this tile
         auto update_tile = [&](int j, const TS *sv) {
-            dbl4 acc = {0.0, 0.0, 0.0, 0.0};
             const double *bcol = bl + (lg * KQ) * SM + j * 16 + lc;
-#pragma unroll
-            for (int tt = 0; tt < KQ; tt++)
-                acc = mfma16x16x4(av[tt], bcol[tt * SM], acc);
+            const dbl4 acc = mfma_chain_lds<KQ>(av, bcol, SM, dbl4{0.0, 0.0, 0.0, 0.0});
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const long i = i0 + lg + 4 * r;
@@ -503,11 +531,9 @@ __global__ __launch_bounds__(512) void qupdate_mfma_kernel(int n, const TS *__re
 #pragma unroll
             for (int j = 0; j < NA; j++) {
                 asm volatile("" ::: "memory");
-                dbl4 ax = {0.0, 0.0, 0.0, 0.0};
                 const double *acol = al + (lg * KQ) * (16 * NA) + j * 16 + lc;
-#pragma unroll
-                for (int tt = 0; tt < KQ; tt++)
-                    ax = mfma16x16x4(av[tt], acol[tt * 16 * NA], ax);
+                const dbl4 ax =
+                    mfma_chain_lds<KQ>(av, acol, 16 * NA, dbl4{0.0, 0.0, 0.0, 0.0});
                 const int c = j * 16 + lc;
                 if (c < m) {
 #pragma unroll

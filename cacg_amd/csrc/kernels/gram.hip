@@ -505,28 +505,48 @@ __global__ __launch_bounds__(NW * 64) void gram_lds_kernel(int n, SrcT<TL, T1> s
 #pragma unroll
         for (int j = 0; j < NT; j++)
             acc[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
+    // the wide shapes (NW >= 12) read the operands of k-step ks + 1 while the
+    // MFMAs of ks run (two register sets; the scheduling barriers keep the
+    // reads ahead, which the compiler otherwise places just before their MFMAs
+    // and waits on); the general shape keeps one set (a second one would cost
+    // its 8-tile form a wave per SIMD)
+    constexpr bool PF = NW >= 12;
     auto compute = [&](int buf) {
         const double *bl = lds + buf * BUF;
         const double *br = bl + SR * LDP;
-#pragma unroll
-        for (int ks = 0; ks < SR / 4; ks++) {
+        double av[PF ? 2 : 1][MT], bv[PF ? 2 : 1][NT];
+        auto operands = [&](int ks, int p) {
             const int row = ks * 4 + lr;
-            double av[MT], bv[NT];
 #pragma unroll
             for (int t = 0; t < MT; t++) {
                 const int a = ca0 + 16 * t;
-                av[t] = a < s.kl ? bl[row * LDP + a] : 0.0;
+                av[p][t] = a < s.kl ? bl[row * LDP + a] : 0.0;
             }
 #pragma unroll
             for (int t = 0; t < NT; t++) {
                 const int b = cb0 + 16 * t;
-                bv[t] = b < kr ? br[row * RDP + b] : 0.0;
+                bv[p][t] = b < kr ? br[row * RDP + b] : 0.0;
+            }
+        };
+        if constexpr (PF)
+            operands(0, 0);
+#pragma unroll
+        for (int ks = 0; ks < SR / 4; ks++) {
+            const int p = PF ? ks & 1 : 0;
+            if constexpr (PF) {
+                if (ks + 1 < SR / 4)
+                    operands(ks + 1, (ks + 1) & 1);
+                __builtin_amdgcn_sched_barrier(0);
+            } else {
+                operands(ks, 0);
             }
 #pragma unroll
             for (int i = 0; i < MT; i++)
 #pragma unroll
                 for (int j = 0; j < NT; j++)
-                    acc[i][j] = mfma16x16x4(av[i], bv[j], acc[i][j]);
+                    acc[i][j] = mfma16x16x4(av[p][i], bv[p][j], acc[i][j]);
+            if constexpr (PF)
+                __builtin_amdgcn_sched_barrier(0);
         }
     };
 
