@@ -34,6 +34,7 @@
  * in FP64 at refinement points and replaces the recursive residual.
  */
 #include <math.h>
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -66,6 +67,7 @@ struct cacg_solver {
      * first solve (tuned = 1) */
     cacg_spmm_sched sch_basis, sch_aq;
     int tuned;
+    unsigned long long pattern_key;   /* key of the process-wide schedule cache */
     hipEvent_t tev[2];
     /* 1: keep the fast order's basis row-major (CACG_BASIS_ROWS=1 at creation) */
     int basis_rows;
@@ -351,6 +353,9 @@ static int validate_csr(long n, const long *rp, const long *ci)
     return CACG_OK;
 }
 
+static unsigned long long pattern_key(long n, const long *row_start, const long *col_idx, long m,
+                                      long sm, int prec32);
+
 int cacg_solver_create(cacg_solver **out, long n, const long *row_start, const long *col_idx,
                        const double *val, long m, long s, const cacg_opts *opts)
 {
@@ -390,6 +395,7 @@ int cacg_solver_create(cacg_solver **out, long n, const long *row_start, const l
     else
         cacg_default_opts(&sv->opts);
     const int prec32 = sv->opts.precision == CACG_PREC_FP32_REFINE;
+    sv->pattern_key = pattern_key(n, row_start, col_idx, m, sv->sm, prec32);
     if (hipHostMalloc((void **)&sv->h_pin, sizeof(double) * (size_t)(2 * m + 4), 0) != hipSuccess) {
         sv->h_pin = NULL;
         cacg_solver_destroy(sv);
@@ -619,6 +625,53 @@ static float time_spmm(cacg_solver *sv, int f32, int k, long ld, const void *x, 
     return ms;
 }
 
+/* Schedules tuned in this process, keyed by device, shape and a sample of the
+ * sparsity pattern: a solver created for a matrix already tuned (the drop-in
+ * entry points create one per call) takes its schedules instead of timing the
+ * candidates again.  A schedule changes no result, so a key shared by two
+ * different patterns can cost speed, never bits. */
+#define SCHED_CACHE 8
+static struct {
+    unsigned long long key;
+    int used;
+    cacg_spmm_sched basis, aq;
+} g_sched[SCHED_CACHE];
+static int g_sched_next;
+static pthread_mutex_t g_sched_mu = PTHREAD_MUTEX_INITIALIZER;
+
+static unsigned long long key_mix(unsigned long long h, unsigned long long v)
+{
+    for (int b = 0; b < 8; b++) {   /* FNV-1a over the 8 bytes of v */
+        h ^= (v >> (8 * b)) & 0xffu;
+        h *= 1099511628211ULL;
+    }
+    return h;
+}
+
+/* device, n, nnz, m, s m, precision and 257 evenly spaced entries of the row
+ * starts and of the column indices */
+static unsigned long long pattern_key(long n, const long *row_start, const long *col_idx, long m,
+                                      long sm, int prec32)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess)
+        dev = -1;
+    const long nnz = row_start[n];
+    unsigned long long h = 1469598103934665603ULL;
+    h = key_mix(h, (unsigned long long)dev);
+    h = key_mix(h, (unsigned long long)n);
+    h = key_mix(h, (unsigned long long)nnz);
+    h = key_mix(h, (unsigned long long)m);
+    h = key_mix(h, (unsigned long long)sm);
+    h = key_mix(h, (unsigned long long)prec32);
+    for (int i = 0; i <= 256; i++) {
+        h = key_mix(h, (unsigned long long)row_start[(long)((double)n * i / 256.0)]);
+        if (nnz > 0)
+            h = key_mix(h, (unsigned long long)col_idx[(long)((double)(nnz - 1) * i / 256.0)]);
+    }
+    return h;
+}
+
 /* Pick the tile order and grid of the two SpMM shapes a solve runs (a basis
  * step: k = m columns of the n x sm basis with the Chebyshev epilogue; A Q:
  * k = sm) by timing a few schedules on this matrix, once per solver.  The
@@ -634,6 +687,16 @@ static int tune_spmm(cacg_solver *sv, int blk)
     sv->tuned = 1;
     if (sv->n < TUNE_MIN_ROWS)
         return 0;
+    const unsigned long long key = sv->pattern_key ^ (blk ? 0x9e3779b97f4a7c15ULL : 0ULL);
+    pthread_mutex_lock(&g_sched_mu);
+    for (int i = 0; i < SCHED_CACHE; i++)
+        if (g_sched[i].used && g_sched[i].key == key) {
+            sv->sch_basis = g_sched[i].basis;
+            sv->sch_aq = g_sched[i].aq;
+            pthread_mutex_unlock(&g_sched_mu);
+            return 0;
+        }
+    pthread_mutex_unlock(&g_sched_mu);
     const int m = (int)sv->m, sm = (int)sv->sm;
     const int f32 = sv->opts.precision == CACG_PREC_FP32_REFINE;
     const long nsm = sv->n * sv->sm;
@@ -666,7 +729,16 @@ static int tune_spmm(cacg_solver *sv, int blk)
             sv->sch_aq = cand[c];
         }
     }
-    return (best_b < 0.f || best_a < 0.f) ? (int)hipErrorLaunchFailure : 0;
+    if (best_b < 0.f || best_a < 0.f)
+        return (int)hipErrorLaunchFailure;
+    pthread_mutex_lock(&g_sched_mu);
+    g_sched[g_sched_next].key = key;
+    g_sched[g_sched_next].used = 1;
+    g_sched[g_sched_next].basis = sv->sch_basis;
+    g_sched[g_sched_next].aq = sv->sch_aq;
+    g_sched_next = (g_sched_next + 1) % SCHED_CACHE;
+    pthread_mutex_unlock(&g_sched_mu);
+    return 0;
 }
 
 /* enqueue what the Chebyshev interval needs beyond Gerschgorin: the Lanczos

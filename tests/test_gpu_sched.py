@@ -39,3 +39,34 @@ def test_tuned_solver_reference_order_matches_oracle(gpu_lib, oracle_lib):
     assert out["iters"] == orc["iters"]
     assert np.array_equal(out["res_hist"], orc["res_hist"])
     assert np.array_equal(out["X"], orc["X"])
+
+
+def test_schedule_cache_keeps_the_bits(gpu_lib):
+    """The drop-in entry points create a solver per call; a second solver on a
+    matrix already tuned in this process takes the cached schedules instead of
+    timing the candidates.  Each call (tuned, cached, and a same-pattern matrix
+    with other values, which shares the key) gives the bits of the first."""
+    import time
+    A = P.laplacian_3d(64)                       # 262144 rows: tuned
+    B = P.rhs_uniform(A.n, 8, 11)
+    X0 = np.zeros_like(B)
+    out, secs = [], []
+    for call in range(3):
+        print(f"bcbcg call {call} (schedule cache)", flush=True)
+        t = time.perf_counter()
+        out.append(gpu_lib.bcbcg(A, B, X0, 4, tol=1e-8, maxiter=200, order="fast"))
+        secs.append(time.perf_counter() - t)
+    print("first-solve seconds per call:", [round(x, 4) for x in secs], flush=True)
+    for o in out:
+        assert o["status"] == "ok"
+        assert o["iters"] == out[0]["iters"]
+        assert np.array_equal(o["X"], out[0]["X"])
+    # same pattern, other values: same key, and a solution of its own system
+    import scipy.sparse as sp
+    A2 = P.laplacian_3d(64)
+    A2.val = A2.val * 2.0
+    o2 = gpu_lib.bcbcg(A2, B, X0, 4, tol=1e-8, maxiter=200, order="fast")
+    assert o2["status"] == "ok"
+    M2 = sp.csr_matrix((A2.val, A2.col, A2.rowptr), shape=(A2.n, A2.n))
+    res = np.linalg.norm(B - M2 @ o2["X"], axis=0) / np.linalg.norm(B, axis=0)
+    assert res.max() < 1e-7
