@@ -2,7 +2,7 @@
 
 The oracle is a plain-C restatement of sclc/cacg's s-step Chebyshev (block)
 CG (oracle/oracle.c).  Only tests/, __graft_entry__.smoke() and bench.py's
-cpu_baseline leg import this module, as the checker or as the timed CPU
+cpu_baseline leg import this module, to check results or as the timed CPU
 baseline.  The product library never does.
 
 `run_reference` drives the reference's own solver objects, compiled from

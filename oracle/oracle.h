@@ -6,7 +6,7 @@
  * the kernels they call), kept in the reference's loop and summation orders
  * so that, at one rank, its floating-point results are the reference's bit
  * for bit.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
- * leg may load it, and only as the checker or the timed CPU baseline.  The
+ * leg may load it, and only to check results or as the timed CPU baseline.  The
  * product library (cacg_amd/libcacg.so) never calls into it.
  *
  * Every function cites the reference file:line it restates.  All dense

@@ -49,7 +49,7 @@ def check_fast_order(name, iters, res_col0, X, spec, g):
     fixture.  Its sums run in another order, as the reference's own run at 4
     MPI ranks does (golden r2_p4, X_p4_reldiff), and an s-step recurrence
     amplifies such differences once its residual nears the rounding floor.
-    Bars (the north star's, where the reference itself meets them):
+    Bars (the solver's accuracy targets, where the reference itself meets them):
       iterations within +-1 of the reference's;
       ||R(:,0)|| within 1e-9 relative on every iteration before the
       reference's 4-rank run departs from its 1-rank run by more than 1e-10;

@@ -4,7 +4,7 @@ Reference order: bit for bit against the compiled reference's golden
 fixtures (iteration count, ||R(:,c)|| after every outer iteration, final X),
 through both the extended API and the drop-in entry points.
 
-Fast order (split-K MFMA): the north-star bars where the reference itself
+Fast order (split-K MFMA): the accuracy targets where the reference itself
 meets them (conftest.check_fast_order): iterations within +-1, the
 per-iteration residual within 1e-9 relative until the reference's own 4-rank
 run departs from its 1-rank run, within 2 x that departure after it, and X
@@ -99,7 +99,7 @@ def test_fast_order_against_reference(gpu_lib, name):
     check_fast_order(name, out["iters"], out["res_hist"][:, 0], out["X"], spec, g)
 
 
-# ---- configurations of the north star at sizes the oracle finishes in seconds ------
+# ---- the benchmark configurations at sizes the oracle finishes in seconds ------------
 
 def _config_case(which):
     if which == "C1":        # 2D 128^2, 1 RHS U[0,1) seed 0, s = 4, 1e-8 relres
