@@ -229,6 +229,93 @@ __global__ __launch_bounds__(256) void xrupdate_ref_kernel(int n, int sm, int m,
     }
 }
 
+// 512 lanes (two waves per SIMD), m even and dividing 512, sm even: a thread
+// updates XR2_RPT rows of one column c of X and R, rows 512 / m apart.
+// alpha(q, c) and alpha(q + 1, c) come with one paired LDS read and each
+// row's Q(r, q), Q(r, q + 1) and AQ(r, q), AQ(r, q + 1) with one 16-byte read
+// each; the next tile's Q and AQ values are in flight during the sums.  Each
+// entry sums q = 0, 1, ... in order and X += (the Q sum), R -= (the AQ sum)
+// as the other reference-order kernel does.
+constexpr int XR2_T = 512, XR2_RPT = 2;
+
+template <typename T, int PER>
+__global__ __launch_bounds__(XR2_T) void xrupdate_ref2_kernel(int n, int sm, int m,
+                                                              const T *__restrict__ Q,
+                                                              const T *__restrict__ AQ,
+                                                              const double *__restrict__ alpha,
+                                                              double *__restrict__ X,
+                                                              double *__restrict__ R)
+{
+    extern __shared__ double lds[];
+    const int P = XR2_T / m, rb = XR2_RPT * P;
+    double *al = lds;                          // sm x m
+    double *qt = lds + (long)sm * m;           // rb x sm
+    double *aqt = qt + (long)rb * sm;          // rb x sm
+    const int t = threadIdx.x;
+    const int c = t % m, r0 = t / m;
+    for (int e = t; e < sm * m; e += XR2_T)
+        al[e] = alpha[e];
+    const int ntile = (n + rb - 1) / rb;
+    // PER = rb sm / 512 = XR2_RPT sm / m Q (and AQ) values per thread and tile
+    double vq[PER], va[PER];
+    auto load = [&](int tile) {
+        const long base = (long)tile * rb * sm;
+        const long lim = (long)n * sm;
+#pragma unroll
+        for (int j = 0; j < PER; j++) {
+            const long e = base + t + (long)j * XR2_T;
+            vq[j] = e < lim ? (double)Q[e] : 0.0;
+            va[j] = e < lim ? (double)AQ[e] : 0.0;
+        }
+    };
+    int tile = blockIdx.x;
+    if (tile < ntile)
+        load(tile);
+    for (; tile < ntile; tile += gridDim.x) {
+#pragma unroll
+        for (int j = 0; j < PER; j++) {
+            qt[t + j * XR2_T] = vq[j];
+            aqt[t + j * XR2_T] = va[j];
+        }
+        __syncthreads();
+        if (tile + (int)gridDim.x < ntile)
+            load(tile + gridDim.x);
+        const long i0 = (long)tile * rb;
+        double xo[XR2_RPT], ro[XR2_RPT], accq[XR2_RPT], acca[XR2_RPT];
+#pragma unroll
+        for (int u = 0; u < XR2_RPT; u++) {
+            const long i = i0 + r0 + u * P;
+            xo[u] = i < n ? X[i * m + c] : 0.0;
+            ro[u] = i < n ? R[i * m + c] : 0.0;
+            accq[u] = 0.0;
+            acca[u] = 0.0;
+        }
+        const double *ac = al + c;
+        for (int q = 0; q < sm; q += 2) {
+            const double w0 = ac[(long)q * m], w1 = ac[(long)(q + 1) * m];
+#pragma unroll
+            for (int u = 0; u < XR2_RPT; u++) {
+                const long ro_ = (long)(r0 + u * P) * sm + q;
+                const double2 x = *(const double2 *)(qt + ro_);
+                const double2 y = *(const double2 *)(aqt + ro_);
+                accq[u] = add_rn(accq[u], mul_rn(w0, x.x));
+                acca[u] = add_rn(acca[u], mul_rn(w0, y.x));
+                accq[u] = add_rn(accq[u], mul_rn(w1, x.y));
+                acca[u] = add_rn(acca[u], mul_rn(w1, y.y));
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < XR2_RPT; u++) {
+            const long i = i0 + r0 + u * P;
+            if (i < n) {
+                X[i * m + c] = add_rn(xo[u], accq[u]);
+                R[i * m + c] = sub_rn(ro[u], acca[u]);
+            }
+        }
+        __syncthreads();   // the tile buffers are refilled next
+    }
+}
+
 // ||R(:, c)||^2, one sequential chain per column in row order, 64 columns (W
 // of them) per workgroup.  Lanes c < W of wave 0 keep the chains; waves 1-7
 // meanwhile load the next stage of CREF_SEGE / W rows into the other of two
@@ -881,7 +968,36 @@ int xrupdate(int n, int sm, int m, const T *Q, const T *AQ, const double *alpha,
         }
 #undef CACG_XR
     }
-    if (n > 0) {
+    // 512 lanes when m is even and divides 512, s <= 8 (a thread holds 2 s values
+    // of the next tile) and alpha plus the two tiles fit in LDS
+    const int per2 = sm % m == 0 ? XR2_RPT * (sm / m) : 0;
+    if (n > 0 && m % 2 == 0 && XR2_T % m == 0 && per2 >= 2 && per2 <= 16 &&
+        ((long)sm * m + 2L * XR2_RPT * (XR2_T / m) * sm) * (long)sizeof(double) <= 160 * 1024) {
+        const int rb = XR2_RPT * (XR2_T / m);
+        const size_t shm = ((size_t)sm * m + 2 * (size_t)rb * sm) * sizeof(double);
+        const int ntile = (n + rb - 1) / rb;
+        const int cap = shm > 80 * 1024 ? 256 : 1024;   // one workgroup per CU above 80 KiB
+        const int G = ntile < cap ? ntile : cap;
+#define CACG_XR2(PER)                                                                             \
+    case PER:                                                                                     \
+        cacg_allow_lds(xrupdate_ref2_kernel<T, PER>, shm);                                        \
+        hipLaunchKernelGGL((xrupdate_ref2_kernel<T, PER>), dim3(G), dim3(XR2_T), shm, st, n, sm,  \
+                           m, Q, AQ, alpha, X, R);                                                \
+        break;
+        switch (per2) {
+            CACG_XR2(2)
+            CACG_XR2(4)
+            CACG_XR2(6)
+            CACG_XR2(8)
+            CACG_XR2(10)
+            CACG_XR2(12)
+            CACG_XR2(14)
+            CACG_XR2(16)
+        default: return (int)hipErrorInvalidValue;
+        }
+#undef CACG_XR2
+        CACG_LAUNCH_CHECK();
+    } else if (n > 0) {
         int rb = m >= 256 ? 1 : 256 / m;
         if ((long)rb * sm > 4096)
             rb = 4096 / sm > 0 ? 4096 / sm : 1;

@@ -275,7 +275,8 @@ def test_qupdate(gpu_lib, oracle_lib, n, sm):
 
 
 @pytest.mark.parametrize("n,sm,m", [(100, 4, 1), (333, 9, 3), (1000, 16, 4), (1001, 64, 16),
-                                    (2003, 128, 16), (700, 128, 32), (300, 96, 24)])
+                                    (2003, 128, 16), (700, 128, 32), (300, 96, 24),
+                                    (70001, 128, 16), (37, 16, 2), (999, 48, 16), (503, 256, 32)])
 def test_xrupdate(gpu_lib, oracle_lib, n, sm, m):
     rng = np.random.default_rng(sm + m)
     Q, AQ = rng.standard_normal((n, sm)), rng.standard_normal((n, sm))
