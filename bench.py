@@ -68,6 +68,9 @@ def parse():
                     help="skip the precision / batching / serving measurements")
     ap.add_argument("--serve-requests", type=int, default=40)
     ap.add_argument("--serve-rate", type=float, default=400.0)
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="only time one full solve of the benchmark workload by the reference's "
+                         "own solver on the host (minutes) and print its JSON")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run 3 device-to-device copies of 2 GiB (HBM counter calibration)")
     ap.add_argument("--form", choices=["batching", "serving", "long-rows", "reference-order"],
@@ -120,6 +123,30 @@ def cpu_baseline(A, B, s, iters_gpu, threads):
     }
 
 
+def reference_full_solve(grid, m, s, tol, ranks):
+    """One full solve of the benchmark workload by the reference's own
+    bcbcg_v1 (oracle/_ref/ref_harness, compiled from its sources, MPICH) at
+    `ranks` MPI ranks on this host, stopped as the reference stops
+    (||R(:,0)|| < tol ||B(:,0)||), timed as its main times a solve; the
+    harness's per-iteration lines are passed through as they come."""
+    import oracle
+    from cacg_amd import problems as P
+    if not (oracle.reference_available() and os.path.exists(oracle.MPIEXEC)):
+        raise SystemExit("bench.py --cpu-full: the reference build or MPI is missing")
+    A = P.laplacian_3d(grid)
+    B = P.rhs_uniform(A.n, m, 1)
+    eps = tol * float(np.linalg.norm(B[:, 0]))
+    print(f"bench: reference solver, full solve of the benchmark workload at {ranks} ranks",
+          file=sys.stderr, flush=True)
+    r = oracle.time_reference(A, B, np.zeros_like(B), s, eps, "bcbcg", ranks, timeout=1500,
+                              progress=True)
+    secs = r["solve"] if r["solve"] is not None else r["t"][-1]
+    return {"workload": f"3D Laplacian {grid}^3, m={m}, s={s}, stop at {tol} relres of "
+                        f"column 0", "ranks": ranks, "iters": r["iters"], "seconds": secs,
+            "solves_per_s": 1.0 / secs, "setup_plus_iteration0_s": r["t"][0] if r["t"] else None,
+            "timing": "barrier, clock, solve, barrier, clock, max over ranks"}
+
+
 def reference_baseline(A, B, s, ranks):
     """Benchmark solve at `ranks` MPI ranks, stopped after outer iterations 0
     and 1 (epsilon just above the reference's recorded ||R(:,0)|| after
@@ -168,6 +195,10 @@ def main():
     args = parse()
     if args.copy_probe:
         copy_probe()
+        return
+    if args.cpu_full:
+        print(json.dumps(reference_full_solve(args.grid, args.m, args.s, args.tol,
+                                              args.cpu_threads or min(16, os.cpu_count() or 1))))
         return
     if args.form:
         run = {"batching": batching_form, "long-rows": spmm_long_rows,

@@ -16,6 +16,7 @@ from __future__ import annotations
 import ctypes
 import os
 import struct
+import sys
 import subprocess
 import tempfile
 
@@ -224,7 +225,8 @@ def reference_available() -> bool:
     return os.path.exists(REF_HARNESS)
 
 
-def time_reference(A, B, X0, s, eps, solver="bcbcg", nranks=1, timeout=600, workdir=None):
+def time_reference(A, B, X0, s, eps, solver="bcbcg", nranks=1, timeout=600, workdir=None,
+                   progress=False):
     """Wall-clock profile of the reference's own solver (the compiled
     oracle/_ref/ref_harness) at `nranks` MPI ranks: the harness stamps every
     outer iteration with the seconds since the solver was called (rank 0,
@@ -233,7 +235,8 @@ def time_reference(A, B, X0, s, eps, solver="bcbcg", nranks=1, timeout=600, work
     clock, solve, barrier, clock, the maximum over ranks; src/main.c:154-163).
     Returns dict(iters, t (list of seconds), solve (that time), wall (seconds
     of the whole run, input reading and output writing included)).  The run
-    keeps no snapshots."""
+    keeps no snapshots.  progress passes the harness's per-iteration lines
+    through to stderr as they come."""
     import re
     import time as _time
     if not reference_available():
@@ -257,11 +260,27 @@ def time_reference(A, B, X0, s, eps, solver="bcbcg", nranks=1, timeout=600, work
             cmd = [MPIEXEC, "-n", str(nranks)] + cmd
         env = dict(os.environ, CACG_REF_NOSNAP="1", CACG_REF_PROGRESS="1")
         t0 = _time.perf_counter()
-        r = subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE,
-                           text=True, cwd=td, timeout=timeout, env=env)
+        if progress:
+            lines = []
+            with subprocess.Popen(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE,
+                                  text=True, cwd=td, env=env) as proc:
+                for line in proc.stderr:
+                    lines.append(line)
+                    sys.stderr.write(line)
+                    sys.stderr.flush()
+                    if _time.perf_counter() - t0 > timeout:
+                        proc.kill()
+                        raise subprocess.TimeoutExpired(cmd, timeout)
+            if proc.returncode != 0:
+                raise subprocess.CalledProcessError(proc.returncode, cmd)
+            err = "".join(lines)
+        else:
+            err = subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL,
+                                 stderr=subprocess.PIPE, text=True, cwd=td, timeout=timeout,
+                                 env=env).stderr
         wall = _time.perf_counter() - t0
-    stamps = [float(x) for x in re.findall(r"iteration \d+ r2 \S+ at (\S+) s", r.stderr)]
-    solve = re.findall(r"ref_harness: solve (\S+) s", r.stderr)
+    stamps = [float(x) for x in re.findall(r"iteration \d+ r2 \S+ at (\S+) s", err)]
+    solve = re.findall(r"ref_harness: solve (\S+) s", err)
     return {"iters": len(stamps), "t": stamps, "solve": float(solve[-1]) if solve else None,
             "wall": wall}
 
