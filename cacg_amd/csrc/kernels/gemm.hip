@@ -321,8 +321,12 @@ __global__ __launch_bounds__(XR2_T) void xrupdate_ref2_kernel(int n, int sm, int
 // meanwhile load the next stage of CREF_SEGE / W rows into the other of two
 // LDS buffers, column by column (stride S, even), so a chain lane reads two
 // rows of its column with one 16-byte LDS read, and reads the next batch of
-// rows while it adds the previous one.  The loader lanes walk their elements with
-// increments (no division per element): they share the chain wave's SIMD.
+// rows while it adds the previous one.  The loaders store each value's square
+// (the same product the chain would form), so a chain lane issues one FP64
+// add per row: a wave issues an FP64 instruction about every 8.5 cycles, and
+// the chain's time per row is its instruction count.  The loader lanes walk
+// their elements with increments (no division per element): they share the
+// chain wave's SIMD.
 // VEC = 2: a loader lane loads two adjacent columns of a row with one 16-byte
 // load (m and ldr even, R 16-byte aligned).
 constexpr int CREF_T = 512;                  // lanes: wave 0 sums, the rest load
@@ -372,10 +376,10 @@ __global__ __launch_bounds__(CREF_T) void colnorm2_ref_kernel(int n, int m,
         for (int j = 0; j < J; j++) {
             if (r < rows) {
                 if constexpr (VEC == 2) {
-                    buf[q * S + r] = v[j].x;
-                    buf[(q + 1) * S + r] = v[j].y;
+                    buf[q * S + r] = mul_rn(v[j].x, v[j].x);
+                    buf[(q + 1) * S + r] = mul_rn(v[j].y, v[j].y);
                 } else {
-                    buf[q * S + r] = v[j];
+                    buf[q * S + r] = mul_rn(v[j], v[j]);
                 }
             }
             r += dr;
@@ -413,8 +417,8 @@ __global__ __launch_bounds__(CREF_T) void colnorm2_ref_kernel(int n, int m,
                     b[u] = *(const double2 *)(col + r + NB + 2 * u);
 #pragma unroll
                 for (int u = 0; u < NV; u++) {
-                    acc = add_rn(acc, mul_rn(a[u].x, a[u].x));
-                    acc = add_rn(acc, mul_rn(a[u].y, a[u].y));
+                    acc = add_rn(acc, a[u].x);
+                    acc = add_rn(acc, a[u].y);
                 }
                 if (r + 3 * NB <= rows) {
 #pragma unroll
@@ -423,20 +427,20 @@ __global__ __launch_bounds__(CREF_T) void colnorm2_ref_kernel(int n, int m,
                 }
 #pragma unroll
                 for (int u = 0; u < NV; u++) {
-                    acc = add_rn(acc, mul_rn(b[u].x, b[u].x));
-                    acc = add_rn(acc, mul_rn(b[u].y, b[u].y));
+                    acc = add_rn(acc, b[u].x);
+                    acc = add_rn(acc, b[u].y);
                 }
             }
             if (r + NB <= rows) {
 #pragma unroll
                 for (int u = 0; u < NV; u++) {
-                    acc = add_rn(acc, mul_rn(a[u].x, a[u].x));
-                    acc = add_rn(acc, mul_rn(a[u].y, a[u].y));
+                    acc = add_rn(acc, a[u].x);
+                    acc = add_rn(acc, a[u].y);
                 }
                 r += NB;
             }
             for (; r < rows; r++)
-                acc = add_rn(acc, mul_rn(col[r], col[r]));
+                acc = add_rn(acc, col[r]);
         }
         __syncthreads();
     }
