@@ -216,3 +216,28 @@ def test_fast_order_bars_reject_planted_deviations():
         check_fast_order("X off", spec["iters_p1"], r1, g["X"] * (1 + 1e-9), spec, g)
     with pytest.raises(AssertionError):
         check_fast_order("two iterations more", spec["iters_p1"] + 2, r1, g["X"], spec, g)
+
+
+def _reference_mpi():
+    import os
+    import oracle
+    return oracle.reference_available() and os.path.exists(oracle.MPIEXEC)
+
+
+@pytest.mark.skipif(not _reference_mpi(), reason="reference build or MPI not available here")
+def test_reference_timing_streamed_and_captured_agree():
+    """time_reference(progress=True) passes the harness's per-iteration lines
+    through as they come (a long CPU solve stays visibly alive) and parses the
+    same stamps as the captured form: same iteration count, one stamp per
+    iteration, and the solve time the harness reports."""
+    import oracle
+    A = P.laplacian_3d(10)
+    B = P.rhs_uniform(A.n, 4, 1)
+    eps = 1e-10 * float(np.linalg.norm(B[:, 0]))
+    streamed = oracle.time_reference(A, B, np.zeros_like(B), 4, eps, "bcbcg", 2, timeout=120,
+                                     progress=True)
+    captured = oracle.time_reference(A, B, np.zeros_like(B), 4, eps, "bcbcg", 2, timeout=120)
+    assert streamed["iters"] == captured["iters"] > 0
+    assert len(streamed["t"]) == streamed["iters"]
+    assert streamed["solve"] is not None and streamed["solve"] > 0.0
+    assert all(b >= a for a, b in zip(streamed["t"], streamed["t"][1:]))
