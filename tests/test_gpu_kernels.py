@@ -260,7 +260,8 @@ def test_lu_zero_pivot_sets_flag(gpu_lib):
 
 
 @pytest.mark.parametrize("n,sm", [(100, 4), (333, 9), (1000, 16), (999, 48), (2000, 128), (50, 256),
-                                  (1003, 128), (37, 16), (5001, 64), (777, 32), (70000, 128)])
+                                  (1003, 128), (37, 16), (5001, 64), (777, 32), (70000, 128),
+                                  (1003, 8), (4099, 2)])
 def test_qupdate(gpu_lib, oracle_lib, n, sm):
     rng = np.random.default_rng(sm)
     S, Q, beta = rng.standard_normal((n, sm)), rng.standard_normal((n, sm)), rng.standard_normal((sm, sm))

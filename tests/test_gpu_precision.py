@@ -98,7 +98,8 @@ def test_gram_f64f32_exact_on_integers(gpu_lib, n, kl, kr):
     assert np.array_equal(gpu_lib.k_gram_f64f32(L, R1), L.T @ R1.astype(np.float64))
 
 
-@pytest.mark.parametrize("n,sm", [(333, 9), (1000, 16), (999, 48), (2000, 128)])
+@pytest.mark.parametrize("n,sm", [(333, 9), (1000, 16), (999, 48), (2000, 128), (1003, 8), (777, 4),
+                                  (4099, 2)])
 def test_qupdate_s32_exact_on_integers(gpu_lib, n, sm):
     rng = np.random.default_rng(sm + 1)
     S = rng.integers(-8, 9, size=(n, sm)).astype(f32)
